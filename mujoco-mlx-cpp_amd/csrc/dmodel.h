@@ -1,0 +1,162 @@
+// Flat FP32 model the kernels read, and the per-env LDS workspace layout.
+//
+// The reference bakes the same model data into its GLSL kernels as constant
+// arrays (reference src/compat/mkx_kernels/forward.hpp:108-141,
+// collision.hpp:506-517) and passes the pair/limit tables as float buffers
+// (reference src/batched.cpp:317-561). Here the tables live in one device
+// buffer; integer tables stay integers.
+#pragma once
+#include <stdint.h>
+
+namespace mjk {
+
+enum { JFREE = 0, JBALL = 1, JSLIDE = 2, JHINGE = 3 };
+enum { GPLANE = 0, GSPHERE = 2, GCAPSULE = 3, GELLIPSOID = 4, GCYLINDER = 5, GBOX = 6 };
+enum { EFC_EQUALITY = 0, EFC_FRICTION_DOF = 1, EFC_LIMIT_JOINT = 3, EFC_CONTACT_FRICTIONLESS = 5,
+       EFC_CONTACT_PYRAMIDAL = 6, EFC_CONTACT_ELLIPTIC = 7 };
+enum {
+  DS_CONSTRAINT = 1 << 0, DS_FRICTIONLOSS = 1 << 2, DS_LIMIT = 1 << 3, DS_CONTACT = 1 << 4,
+  DS_PASSIVE = 1 << 5, DS_GRAVITY = 1 << 6, DS_CLAMPCTRL = 1 << 7, DS_WARMSTART = 1 << 8,
+  DS_ACTUATION = 1 << 10, DS_REFSAFE = 1 << 11, DS_EULERDAMP = 1 << 15,
+};
+// per-env status bits
+enum { ST_CON_OVERFLOW = 1, ST_EFC_OVERFLOW = 2, ST_BAD_PAIR = 4, ST_NOT_PD = 16, ST_M_NOT_PD = 32,
+       ST_NONFINITE = 64 };
+
+// contact record in the workspace (floats)
+enum { CON_POS = 0, CON_FRAME = 3, CON_DIST = 12, CON_PAIR = 13, CON_NROW = 14, CON_ROW0 = 15, CON_STRIDE = 16 };
+// max contacts one narrow-phase call emits
+enum { MAXC_PAIR = 4 };
+enum { MAX_DOF_DEPTH = 64 };
+
+struct DModel {
+  int nq, nv, nu, nbody, njnt, ngeom, npair;
+  int nlevel, ndoflevel, maxdepth, nfloss, nlim;
+  int iterations, ls_iterations, solver, cone, integrator, disableflags;
+  int has_damping;
+  float timestep, gravity[3], tolerance, ls_tolerance, impratio, meaninertia;
+  // bodies
+  const int *body_parentid, *body_rootid, *body_jntnum, *body_jntadr, *body_dofnum, *body_dofadr,
+      *body_subtree_end, *level_adr, *level_body, *body_lastdof;
+  const uint64_t* body_dofmask;
+  const float *body_pos, *body_quat, *body_ipos, *body_iquat, *body_mass, *body_inertia,
+      *body_subtreemass, *body_invweight0;
+  // joints
+  const int *jnt_type, *jnt_qposadr, *jnt_dofadr, *jnt_bodyid;
+  const float *jnt_pos, *jnt_axis, *jnt_range, *jnt_margin, *jnt_solref, *jnt_solimp, *jnt_stiffness;
+  const float *qpos0, *qpos_spring;
+  // dofs
+  const int *dof_bodyid, *dof_jntid, *dof_parentid, *dof_depth, *dof_subtree_end, *doflevel_adr,
+      *doflevel_dof, *dof_chain, *dof_nchain, *dof_act_adr, *dof_act_num, *dof_act_list;
+  const float *dof_armature, *dof_damping, *dof_frictionloss, *dof_invweight0, *dof_solref, *dof_solimp;
+  // geoms
+  const int *geom_type, *geom_bodyid;
+  const float *geom_size, *geom_pos, *geom_quat, *geom_rbound;
+  // static pair table (geom1 has the lower type)
+  const int *pair_g1, *pair_g2, *pair_b1, *pair_b2, *pair_condim;
+  const float *pair_friction, *pair_solref, *pair_solimp, *pair_margin, *pair_gap;
+  // actuators
+  const int *act_dof, *act_qadr, *act_ctrllimited, *act_forcelimited;
+  const float *act_gear, *act_gain, *act_bias, *act_ctrlrange, *act_forcerange;
+  // constraint source tables
+  const int *floss_dof, *lim_jnt;
+};
+
+// Per-env workspace layout in floats. Region A (kinematics / dynamics
+// intermediates) is dead once the constraint Jacobian is built, so the FP64
+// Newton Hessian overlays it.
+struct Layout {
+  int maxcon, maxefc, nv, total;
+  int qpos, qvel, ctrl, qacc_ws, qacc, qM, qLD, qfrc_smooth, qacc_smooth, qfrc_constraint, act_force, red, misc;
+  int xpos, xquat, xmat, xipos, xanchor, xaxis, gxpos, gxmat, scom, cinert, cdof, cvel, cdofdot, crb, cacc, cfrc, buf6;
+  int H;
+  int con, efc_J, efc_D, efc_R, efc_aref, efc_Jaref, efc_Jv, efc_force, efc_floss, efc_pos, efc_margin,
+      efc_type, efc_act, efc_id;
+  int Ma, grad, Mgrad, search, Mv, grad_old, Mgrad_old, tmpv, dvec;
+};
+
+// misc ints (stored as int bits in the float workspace)
+enum { MISC_NCON = 0, MISC_NEFC = 1, MISC_STATUS = 2, MISC_NITER = 3, MISC_NFLOSSROWS = 4, MISC_NLIMROWS = 5,
+       MISC_CONSUB = 6, MISC_SIZE = 8 };
+
+#if defined(__HIPCC__) || defined(__CUDACC__)
+#define MJK_HD __host__ __device__
+#else
+#define MJK_HD
+#endif
+
+MJK_HD inline int al2(int x) { return (x + 1) & ~1; }
+
+MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, int ngeom, int maxcon, int maxefc) {
+  Layout L;
+  L.maxcon = maxcon; L.maxefc = maxefc; L.nv = nv;
+  int nvp = nv > 0 ? nv : 1, nup = nu > 0 ? nu : 1, njp = njnt > 0 ? njnt : 1, ngp = ngeom > 0 ? ngeom : 1;
+  int o = 0;
+  L.qpos = o; o += al2(nq > 0 ? nq : 1);
+  L.qvel = o; o += al2(nvp);
+  L.ctrl = o; o += al2(nup);
+  L.qacc_ws = o; o += al2(nvp);
+  L.qacc = o; o += al2(nvp);
+  L.qM = o; o += al2(nvp * nvp);
+  L.qLD = o; o += al2(nvp * nvp);
+  L.qfrc_smooth = o; o += al2(nvp);
+  L.qacc_smooth = o; o += al2(nvp);
+  L.qfrc_constraint = o; o += al2(nvp);
+  L.act_force = o; o += al2(nup);
+  L.red = o; o += 64;
+  L.misc = o; o += MISC_SIZE;
+  // region A
+  int a0 = o;
+  L.xpos = o; o += 3 * nbody;
+  L.xquat = o; o += 4 * nbody;
+  L.xmat = o; o += 9 * nbody;
+  L.xipos = o; o += 3 * nbody;
+  L.xanchor = o; o += 3 * njp;
+  L.xaxis = o; o += 3 * njp;
+  L.gxpos = o; o += 3 * ngp;
+  L.gxmat = o; o += 9 * ngp;
+  L.scom = o; o += 3 * nbody;
+  L.cinert = o; o += 10 * nbody;
+  L.cdof = o; o += 6 * nvp;
+  L.cvel = o; o += 6 * nbody;
+  L.cdofdot = o; o += 6 * nvp;
+  L.crb = o; o += 10 * nbody;
+  L.cacc = o; o += 6 * nbody;
+  L.cfrc = o; o += 6 * nbody;
+  L.buf6 = o; o += 6 * nvp;
+  o = al2(o);
+  int aend = o;
+  L.H = al2(a0);
+  int hend = L.H + 2 * nvp * nvp;
+  o = aend > hend ? aend : hend;
+  o = al2(o);
+  // region B
+  L.con = o; o += CON_STRIDE * (maxcon > 0 ? maxcon : 1);
+  L.efc_J = o; o += maxefc * nvp;
+  L.efc_D = o; o += maxefc;
+  L.efc_R = o; o += maxefc;
+  L.efc_aref = o; o += maxefc;
+  L.efc_Jaref = o; o += maxefc;
+  L.efc_Jv = o; o += maxefc;
+  L.efc_force = o; o += maxefc;
+  L.efc_floss = o; o += maxefc;
+  L.efc_pos = o; o += maxefc;
+  L.efc_margin = o; o += maxefc;
+  L.efc_type = o; o += maxefc;
+  L.efc_act = o; o += maxefc;
+  L.efc_id = o; o += maxefc;
+  L.Ma = o; o += nvp;
+  L.grad = o; o += nvp;
+  L.Mgrad = o; o += nvp;
+  L.search = o; o += nvp;
+  L.Mv = o; o += nvp;
+  L.grad_old = o; o += nvp;
+  L.Mgrad_old = o; o += nvp;
+  L.tmpv = o; o += nvp;
+  o = al2(o);
+  L.dvec = o; o += 2 * nvp;
+  L.total = al2(o);
+  return L;
+}
+
+}  // namespace mjk

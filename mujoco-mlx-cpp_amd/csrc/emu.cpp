@@ -1,0 +1,125 @@
+// TEST-ONLY serial interpreter of the kernel source (kernels.h with WAVE == 1).
+//
+// It runs the exact stage sequence the gfx950 kernel runs, one work item at a
+// time on the host, so the CPU test suite can check the kernel algorithm
+// against the FP64 oracle without a GPU. It is built into its own library
+// (libmjmlx_emu.so) that only tests load; the engine library libmjmlx.so never
+// contains or calls it.
+#define MJ_EMU 1
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hostmodel.h"
+#include "kernels.h"
+
+namespace {
+struct EmuSim {
+  mjh::Model model;
+  mjh::Blob blob;
+  mjk::DModel dm;
+  mjk::Layout L;
+  std::vector<mjh::LayoutField> fields;
+  int B;
+  std::vector<float> qpos, qvel, qacc_ws, ws, dump1, dump2;
+  std::vector<int> niter, status, ncon, nefc;
+};
+}  // namespace
+
+extern "C" {
+
+__attribute__((visibility("default"))) void* mjemu_create(const char* xml, int self_collision, int max_condim, int B,
+                                                          int maxcon, int maxefc) {
+  try {
+    auto* s = new EmuSim();
+    mjh::LoadOptions o;
+    o.self_collision = self_collision;
+    o.max_condim = max_condim;
+    s->model = mjh::compile_mjcf(xml, o);
+    std::vector<mjh::CollisionPair> pairs;
+    s->blob = mjh::build_blob(s->model, &pairs);
+    s->dm = s->blob.bind(s->blob.bytes.data());
+    if (maxcon <= 0) maxcon = std::min(mjh::max_contacts(s->model, pairs), 64);
+    if (maxefc <= 0) maxefc = std::min(mjh::max_rows(s->model, pairs), 128);
+    if (maxcon < 1) maxcon = 1;
+    if (maxefc < 1) maxefc = 1;
+    auto& m = s->model;
+    s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, maxcon, maxefc);
+    s->fields = mjh::layout_fields(m, s->L);
+    s->B = B;
+    s->qpos.resize((size_t)B * m.nq);
+    for (int e = 0; e < B; e++) for (int i = 0; i < m.nq; i++) s->qpos[(size_t)e * m.nq + i] = (float)m.qpos0[i];
+    s->qvel.assign((size_t)B * m.nv, 0.f);
+    s->qacc_ws.assign((size_t)B * m.nv, 0.f);
+    s->ws.assign(s->L.total, 0.f);
+    s->niter.assign(B, 0); s->status.assign(B, 0); s->ncon.assign(B, 0); s->nefc.assign(B, 0);
+    return s;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "mjemu_create: %s\n", e.what());
+    return nullptr;
+  }
+}
+
+__attribute__((visibility("default"))) void mjemu_free(void* p) { delete static_cast<EmuSim*>(p); }
+
+__attribute__((visibility("default"))) void mjemu_set_state(void* p, const float* qpos, const float* qvel, const float* qacc_ws) {
+  auto* s = static_cast<EmuSim*>(p);
+  if (qpos) std::memcpy(s->qpos.data(), qpos, s->qpos.size() * sizeof(float));
+  if (qvel) std::memcpy(s->qvel.data(), qvel, s->qvel.size() * sizeof(float));
+  if (qacc_ws) std::memcpy(s->qacc_ws.data(), qacc_ws, s->qacc_ws.size() * sizeof(float));
+}
+
+__attribute__((visibility("default"))) void mjemu_get_state(void* p, float* qpos, float* qvel, float* qacc_ws) {
+  auto* s = static_cast<EmuSim*>(p);
+  if (qpos) std::memcpy(qpos, s->qpos.data(), s->qpos.size() * sizeof(float));
+  if (qvel) std::memcpy(qvel, s->qvel.data(), s->qvel.size() * sizeof(float));
+  if (qacc_ws) std::memcpy(qacc_ws, s->qacc_ws.data(), s->qacc_ws.size() * sizeof(float));
+}
+
+__attribute__((visibility("default"))) int mjemu_step(void* p, const float* ctrl, int nsub, int dump) {
+  auto* s = static_cast<EmuSim*>(p);
+  mjk::StepIO io{};
+  io.qpos = s->qpos.data(); io.qvel = s->qvel.data(); io.qacc_ws = s->qacc_ws.data(); io.ctrl = ctrl;
+  io.niter = s->niter.data(); io.status = s->status.data(); io.ncon = s->ncon.data(); io.nefc = s->nefc.data();
+  if (dump) {
+    s->dump1.assign((size_t)s->B * s->L.total, 0.f);
+    s->dump2.assign((size_t)s->B * s->L.total, 0.f);
+    io.dump1 = s->dump1.data(); io.dump2 = s->dump2.data();
+  }
+  int bad = 0;
+  for (int e = 0; e < s->B; e++) {
+    std::fill(s->ws.begin(), s->ws.end(), 0.f);
+    mjk::run_env(s->dm, s->L, s->ws.data(), io, e, nsub);
+    if (s->status[e]) bad++;
+  }
+  return bad;
+}
+
+__attribute__((visibility("default"))) int mjemu_layout(void* p, const char* name, int* off, int* len, int* kind) {
+  auto* s = static_cast<EmuSim*>(p);
+  if (std::strcmp(name, "total") == 0) { *off = 0; *len = s->L.total; *kind = 0; return 0; }
+  for (auto& f : s->fields)
+    if (std::strcmp(f.name, name) == 0) { *off = f.off; *len = f.len; *kind = f.kind; return 0; }
+  return -1;
+}
+
+__attribute__((visibility("default"))) const float* mjemu_dump(void* p, int which) {
+  auto* s = static_cast<EmuSim*>(p);
+  return which == 1 ? s->dump1.data() : s->dump2.data();
+}
+
+__attribute__((visibility("default"))) void mjemu_stats(void* p, int* niter, int* status, int* ncon, int* nefc) {
+  auto* s = static_cast<EmuSim*>(p);
+  if (niter) std::memcpy(niter, s->niter.data(), s->B * sizeof(int));
+  if (status) std::memcpy(status, s->status.data(), s->B * sizeof(int));
+  if (ncon) std::memcpy(ncon, s->ncon.data(), s->B * sizeof(int));
+  if (nefc) std::memcpy(nefc, s->nefc.data(), s->B * sizeof(int));
+}
+
+}  // extern "C"
+
+extern "C" __attribute__((visibility("default"))) int mjemu_model_field(void* p, const char* name, double* out, int cap) {
+  return mjh::model_field(static_cast<EmuSim*>(p)->model, name, out, cap);
+}

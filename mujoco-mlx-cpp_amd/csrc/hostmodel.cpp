@@ -1,0 +1,324 @@
+// Kernel tables from a compiled model. Static derived data mirrors what the
+// reference precomputes on the host: the collision pair table
+// (reference src/batched.cpp:317-367), per-pair solver properties (:529-548),
+// limit/friction tables (:378-409,458-478) and per-body dof masks (:551-561).
+#include "hostmodel.h"
+
+#include <algorithm>
+#include <cstddef>
+#include <cstring>
+#include <stdexcept>
+
+namespace mjh {
+
+namespace {
+
+struct Builder {
+  std::vector<char> bytes;
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    size_t o = (bytes.size() + 15) & ~size_t(15);
+    bytes.resize(o + std::max<size_t>(1, v.size()) * sizeof(T), 0);
+    if (!v.empty()) std::memcpy(bytes.data() + o, v.data(), v.size() * sizeof(T));
+    return o;
+  }
+};
+
+std::vector<float> tof(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
+
+bool pair_supported(int t1, int t2) {
+  using namespace mjk;
+  if (t1 == GPLANE) return t2 == GSPHERE || t2 == GCAPSULE || t2 == GBOX;
+  if (t1 == GSPHERE) return t2 == GSPHERE || t2 == GCAPSULE || t2 == GBOX;
+  if (t1 == GCAPSULE) return t2 == GCAPSULE;
+  return false;
+}
+
+int pair_maxc(int t1, int t2) {
+  using namespace mjk;
+  if (t1 == GPLANE && t2 == GCAPSULE) return 2;
+  if (t1 == GPLANE && t2 == GBOX) return 4;
+  return 1;
+}
+
+}  // namespace
+
+mjk::DModel Blob::bind(const char* base) const {
+  mjk::DModel d = proto;
+  char* p0 = reinterpret_cast<char*>(&d) + offsetof(mjk::DModel, body_parentid);
+  char* p1 = reinterpret_cast<char*>(&d) + sizeof(mjk::DModel);
+  for (char* p = p0; p + sizeof(uintptr_t) <= p1; p += sizeof(uintptr_t)) {
+    uintptr_t v;
+    std::memcpy(&v, p, sizeof(v));
+    v += reinterpret_cast<uintptr_t>(base);
+    std::memcpy(p, &v, sizeof(v));
+  }
+  return d;
+}
+
+int max_contacts(const Model& m, const std::vector<CollisionPair>& pairs) {
+  int s = 0;
+  for (auto& p : pairs) s += pair_maxc(m.geom_type[p.g1], m.geom_type[p.g2]);
+  return s;
+}
+
+int max_rows(const Model& m, const std::vector<CollisionPair>& pairs) {
+  int s = 0;
+  for (int k = 0; k < m.nv; k++) if (m.dof_frictionloss[k] > 0) s++;
+  for (int j = 0; j < m.njnt; j++) if (m.jnt_limited[j]) s += 2;
+  for (auto& p : pairs) {
+    int per = p.condim == 1 ? 1 : (m.cone == CONE_PYRAMIDAL ? 2 * (p.condim - 1) : p.condim);
+    s += per * pair_maxc(m.geom_type[p.g1], m.geom_type[p.g2]);
+  }
+  return s;
+}
+
+Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
+  using namespace mjk;
+  if (m.nv > 64) throw std::runtime_error("HIP engine: models with nv > 64 are not supported by the dense-row kernel");
+  if (m.cone != CONE_PYRAMIDAL) throw std::runtime_error("HIP engine: elliptic friction cones are not implemented yet");
+  if (m.solver == SOL_PGS) throw std::runtime_error("HIP engine: the PGS solver is not implemented (use Newton or CG)");
+  if (m.integrator == INT_RK4) throw std::runtime_error("HIP engine: the RK4 integrator is not implemented");
+  std::vector<CollisionPair> pairs = build_pairs(m);
+  for (auto& p : pairs)
+    if (!pair_supported(m.geom_type[p.g1], m.geom_type[p.g2]))
+      throw std::runtime_error("HIP engine: geom pair types " + std::to_string(m.geom_type[p.g1]) + "/" +
+                               std::to_string(m.geom_type[p.g2]) + " have no narrow phase yet");
+  for (int j = 0; j < m.njnt; j++)
+    if (m.jnt_limited[j] && (m.jnt_type[j] == JNT_BALL)) throw std::runtime_error("HIP engine: ball joint limits are not implemented");
+  int nb = m.nbody, nv = m.nv;
+  // subtree ranges (depth-first numbering)
+  std::vector<int> send(nb);
+  for (int b = 0; b < nb; b++) send[b] = b + 1;
+  for (int b = nb - 1; b > 0; b--) send[m.body_parentid[b]] = std::max(send[m.body_parentid[b]], send[b]);
+  for (int b = 1; b < nb; b++)
+    if (m.body_parentid[b] >= b) throw std::runtime_error("HIP engine: bodies must be numbered parent-first");
+  // tree levels (depth 1..)
+  std::vector<int> depth(nb, 0);
+  int maxd = 0;
+  for (int b = 1; b < nb; b++) { depth[b] = depth[m.body_parentid[b]] + 1; maxd = std::max(maxd, depth[b]); }
+  std::vector<int> level_adr(maxd + 1, 0), level_body;
+  for (int lv = 1; lv <= maxd; lv++) {
+    level_adr[lv - 1] = (int)level_body.size();
+    for (int b = 1; b < nb; b++) if (depth[b] == lv) level_body.push_back(b);
+  }
+  level_adr[maxd] = (int)level_body.size();
+  // last dof of each body's kinematic chain and the dof mask
+  std::vector<int> lastdof(nb, -1);
+  std::vector<uint64_t> mask(nb, 0);
+  for (int b = 1; b < nb; b++) {
+    int bb = b;
+    while (bb > 0 && m.body_dofnum[bb] == 0) bb = m.body_parentid[bb];
+    if (bb > 0) {
+      lastdof[b] = m.body_dofadr[bb] + m.body_dofnum[bb] - 1;
+      for (int d = lastdof[b]; d >= 0; d = m.dof_parentid[d]) mask[b] |= (uint64_t)1 << d;
+    }
+  }
+  // dof tree
+  std::vector<int> ddepth(nv, 0), dsend(nv), nchain(nv), chain((size_t)std::max(nv, 1) * MAX_DOF_DEPTH, -1);
+  int maxdd = 0;
+  for (int k = 0; k < nv; k++) {
+    int p = m.dof_parentid[k];
+    if (p >= k) throw std::runtime_error("HIP engine: dof parents must precede their children");
+    ddepth[k] = p < 0 ? 0 : ddepth[p] + 1;
+    maxdd = std::max(maxdd, ddepth[k]);
+    int n = 0;
+    for (int a = k; a >= 0; a = m.dof_parentid[a]) {
+      if (n >= MAX_DOF_DEPTH) throw std::runtime_error("HIP engine: dof tree deeper than 64");
+      chain[(size_t)k * MAX_DOF_DEPTH + n++] = a;
+    }
+    nchain[k] = n;
+  }
+  for (int k = 0; k < nv; k++) dsend[k] = k + 1;
+  for (int k = nv - 1; k >= 0; k--) { int p = m.dof_parentid[k]; if (p >= 0) dsend[p] = std::max(dsend[p], dsend[k]); }
+  for (int k = 0; k < nv; k++)
+    for (int q = k + 1; q < dsend[k]; q++) {
+      bool desc = false;
+      for (int a = q; a >= 0; a = m.dof_parentid[a]) if (a == k) { desc = true; break; }
+      if (!desc) throw std::runtime_error("HIP engine: dof subtree is not contiguous");
+    }
+  int ndl = nv > 0 ? maxdd + 1 : 0;
+  std::vector<int> dl_adr(ndl + 1, 0), dl_dof;
+  for (int lv = 0; lv < ndl; lv++) {
+    dl_adr[lv] = (int)dl_dof.size();
+    for (int k = 0; k < nv; k++) if (ddepth[k] == lv) dl_dof.push_back(k);
+  }
+  dl_adr[ndl] = (int)dl_dof.size();
+  // actuators per dof
+  std::vector<int> act_dof(m.nu), act_qadr(m.nu), dof_act_adr(nv, 0), dof_act_num(nv, 0), dof_act_list;
+  for (int u = 0; u < m.nu; u++) { act_dof[u] = m.jnt_dofadr[m.actuator_trnid[u]]; act_qadr[u] = m.jnt_qposadr[m.actuator_trnid[u]]; }
+  for (int k = 0; k < nv; k++) {
+    dof_act_adr[k] = (int)dof_act_list.size();
+    for (int u = 0; u < m.nu; u++) if (act_dof[u] == k) dof_act_list.push_back(u);
+    dof_act_num[k] = (int)dof_act_list.size() - dof_act_adr[k];
+  }
+  std::vector<int> floss_dof, lim_jnt;
+  for (int k = 0; k < nv; k++) if (m.dof_frictionloss[k] > 0) floss_dof.push_back(k);
+  for (int j = 0; j < m.njnt; j++)
+    if (m.jnt_limited[j] && (m.jnt_type[j] == JNT_HINGE || m.jnt_type[j] == JNT_SLIDE)) lim_jnt.push_back(j);
+  // pairs
+  int np = (int)pairs.size();
+  std::vector<int> pg1(np), pg2(np), pb1(np), pb2(np), pcd(np);
+  std::vector<float> pfr(5 * np), psr(2 * np), psi(5 * np), pmg(np), pgp(np);
+  for (int i = 0; i < np; i++) {
+    auto& p = pairs[i];
+    pg1[i] = p.g1; pg2[i] = p.g2; pb1[i] = p.b1; pb2[i] = p.b2; pcd[i] = p.condim;
+    for (int k = 0; k < 5; k++) { pfr[5 * i + k] = (float)p.friction[k]; psi[5 * i + k] = (float)p.solimp[k]; }
+    for (int k = 0; k < 2; k++) psr[2 * i + k] = (float)p.solref[k];
+    pmg[i] = (float)p.margin; pgp[i] = (float)p.gap;
+  }
+  std::vector<float> gain(3 * m.nu), bias(3 * m.nu);
+  for (int i = 0; i < 3 * m.nu; i++) { gain[i] = (float)m.actuator_gainprm[i]; bias[i] = (float)m.actuator_biasprm[i]; }
+
+  Blob blob;
+  Builder B;
+  DModel& d = blob.proto;
+  std::memset(&d, 0, sizeof(d));
+  d.nq = m.nq; d.nv = nv; d.nu = m.nu; d.nbody = nb; d.njnt = m.njnt; d.ngeom = m.ngeom; d.npair = np;
+  d.nlevel = maxd; d.ndoflevel = ndl; d.maxdepth = maxdd; d.nfloss = (int)floss_dof.size(); d.nlim = (int)lim_jnt.size();
+  d.iterations = m.iterations; d.ls_iterations = m.ls_iterations; d.solver = m.solver; d.cone = m.cone;
+  d.integrator = m.integrator; d.disableflags = m.disableflags;
+  d.has_damping = 0;
+  for (int k = 0; k < nv; k++) if (m.dof_damping[k] > 0) d.has_damping = 1;
+  d.timestep = (float)m.timestep;
+  for (int i = 0; i < 3; i++) d.gravity[i] = (float)m.gravity[i];
+  d.tolerance = (float)m.tolerance; d.ls_tolerance = (float)m.ls_tolerance; d.impratio = (float)m.impratio;
+  d.meaninertia = (float)m.meaninertia;
+  std::vector<float> subtreemass = tof(m.body_subtreemass);
+#define ADDP(field, vec) d.field = reinterpret_cast<decltype(d.field)>(B.add(vec))
+  ADDP(body_parentid, m.body_parentid); ADDP(body_rootid, m.body_rootid); ADDP(body_jntnum, m.body_jntnum);
+  ADDP(body_jntadr, m.body_jntadr); ADDP(body_dofnum, m.body_dofnum); ADDP(body_dofadr, m.body_dofadr);
+  ADDP(body_subtree_end, send); ADDP(level_adr, level_adr); ADDP(level_body, level_body); ADDP(body_lastdof, lastdof);
+  ADDP(body_dofmask, mask);
+  ADDP(body_pos, tof(m.body_pos)); ADDP(body_quat, tof(m.body_quat)); ADDP(body_ipos, tof(m.body_ipos));
+  ADDP(body_iquat, tof(m.body_iquat)); ADDP(body_mass, tof(m.body_mass)); ADDP(body_inertia, tof(m.body_inertia));
+  ADDP(body_subtreemass, subtreemass); ADDP(body_invweight0, tof(m.body_invweight0));
+  ADDP(jnt_type, m.jnt_type); ADDP(jnt_qposadr, m.jnt_qposadr); ADDP(jnt_dofadr, m.jnt_dofadr); ADDP(jnt_bodyid, m.jnt_bodyid);
+  ADDP(jnt_pos, tof(m.jnt_pos)); ADDP(jnt_axis, tof(m.jnt_axis)); ADDP(jnt_range, tof(m.jnt_range));
+  ADDP(jnt_margin, tof(m.jnt_margin)); ADDP(jnt_solref, tof(m.jnt_solref)); ADDP(jnt_solimp, tof(m.jnt_solimp));
+  ADDP(jnt_stiffness, tof(m.jnt_stiffness)); ADDP(qpos0, tof(m.qpos0)); ADDP(qpos_spring, tof(m.qpos_spring));
+  ADDP(dof_bodyid, m.dof_bodyid); ADDP(dof_jntid, m.dof_jntid); ADDP(dof_parentid, m.dof_parentid);
+  ADDP(dof_depth, ddepth); ADDP(dof_subtree_end, dsend); ADDP(doflevel_adr, dl_adr); ADDP(doflevel_dof, dl_dof);
+  ADDP(dof_chain, chain); ADDP(dof_nchain, nchain); ADDP(dof_act_adr, dof_act_adr); ADDP(dof_act_num, dof_act_num);
+  ADDP(dof_act_list, dof_act_list);
+  ADDP(dof_armature, tof(m.dof_armature)); ADDP(dof_damping, tof(m.dof_damping)); ADDP(dof_frictionloss, tof(m.dof_frictionloss));
+  ADDP(dof_invweight0, tof(m.dof_invweight0)); ADDP(dof_solref, tof(m.dof_solref)); ADDP(dof_solimp, tof(m.dof_solimp));
+  ADDP(geom_type, m.geom_type); ADDP(geom_bodyid, m.geom_bodyid);
+  ADDP(geom_size, tof(m.geom_size)); ADDP(geom_pos, tof(m.geom_pos)); ADDP(geom_quat, tof(m.geom_quat));
+  ADDP(geom_rbound, tof(m.geom_rbound));
+  ADDP(pair_g1, pg1); ADDP(pair_g2, pg2); ADDP(pair_b1, pb1); ADDP(pair_b2, pb2); ADDP(pair_condim, pcd);
+  ADDP(pair_friction, pfr); ADDP(pair_solref, psr); ADDP(pair_solimp, psi); ADDP(pair_margin, pmg); ADDP(pair_gap, pgp);
+  ADDP(act_dof, act_dof); ADDP(act_qadr, act_qadr); ADDP(act_ctrllimited, m.actuator_ctrllimited);
+  ADDP(act_forcelimited, m.actuator_forcelimited);
+  ADDP(act_gear, tof(m.actuator_gear)); ADDP(act_gain, gain); ADDP(act_bias, bias);
+  ADDP(act_ctrlrange, tof(m.actuator_ctrlrange)); ADDP(act_forcerange, tof(m.actuator_forcerange));
+  ADDP(floss_dof, floss_dof); ADDP(lim_jnt, lim_jnt);
+#undef ADDP
+  blob.bytes = std::move(B.bytes);
+  if (pairs_out) *pairs_out = pairs;
+  return blob;
+}
+
+std::vector<LayoutField> layout_fields(const Model& m, const mjk::Layout& L) {
+  int nb = m.nbody, nv = m.nv, nj = m.njnt, ng = m.ngeom;
+  std::vector<LayoutField> f = {
+      {"qpos", L.qpos, m.nq, 0}, {"qvel", L.qvel, nv, 0}, {"ctrl", L.ctrl, m.nu, 0},
+      {"qacc_warmstart", L.qacc_ws, nv, 0}, {"qacc", L.qacc, nv, 0}, {"qM", L.qM, nv * nv, 0},
+      {"qLD", L.qLD, nv * nv, 0}, {"qfrc_smooth", L.qfrc_smooth, nv, 0}, {"qacc_smooth", L.qacc_smooth, nv, 0},
+      {"qfrc_constraint", L.qfrc_constraint, nv, 0}, {"actuator_force", L.act_force, m.nu, 0},
+      {"misc", L.misc, mjk::MISC_SIZE, 1},
+      {"xpos", L.xpos, 3 * nb, 0}, {"xquat", L.xquat, 4 * nb, 0}, {"xmat", L.xmat, 9 * nb, 0},
+      {"xipos", L.xipos, 3 * nb, 0}, {"xanchor", L.xanchor, 3 * nj, 0}, {"xaxis", L.xaxis, 3 * nj, 0},
+      {"geom_xpos", L.gxpos, 3 * ng, 0}, {"geom_xmat", L.gxmat, 9 * ng, 0}, {"subtree_com", L.scom, 3 * nb, 0},
+      {"cinert", L.cinert, 10 * nb, 0}, {"cdof", L.cdof, 6 * nv, 0}, {"cvel", L.cvel, 6 * nb, 0},
+      {"cdof_dot", L.cdofdot, 6 * nv, 0}, {"crb", L.crb, 10 * nb, 0},
+      {"contact", L.con, mjk::CON_STRIDE * L.maxcon, 0},
+      {"efc_J", L.efc_J, L.maxefc * nv, 0}, {"efc_D", L.efc_D, L.maxefc, 0}, {"efc_R", L.efc_R, L.maxefc, 0},
+      {"efc_aref", L.efc_aref, L.maxefc, 0}, {"efc_force", L.efc_force, L.maxefc, 0},
+      {"efc_pos", L.efc_pos, L.maxefc, 0}, {"efc_margin", L.efc_margin, L.maxefc, 0},
+      {"efc_type", L.efc_type, L.maxefc, 1}, {"efc_id", L.efc_id, L.maxefc, 1}, {"efc_floss", L.efc_floss, L.maxefc, 0},
+  };
+  return f;
+}
+
+}  // namespace mjh
+
+namespace mjh {
+
+// Raw model field by MuJoCo name (integers are returned as doubles). Scalars:
+// nq nv nu nbody njnt ngeom timestep gravity tolerance ls_tolerance impratio
+// iterations ls_iterations solver cone integrator disableflags meaninertia
+// npair nexclude. Compiled pair table: cpair_* (g1 g2 b1 b2 condim friction
+// solref solimp margin gap). Returns the length, or -1 for an unknown name.
+int model_field(const Model& m, const char* name, double* out, int cap) {
+  std::string n(name);
+  auto put = [&](const std::vector<double>& v) {
+    for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = v[i];
+    return (int)v.size();
+  };
+  auto puti = [&](const std::vector<int>& v) {
+    for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = v[i];
+    return (int)v.size();
+  };
+  auto scal = [&](double x) { if (cap > 0) out[0] = x; return 1; };
+  if (n == "nq") return scal(m.nq);
+  if (n == "nv") return scal(m.nv);
+  if (n == "nu") return scal(m.nu);
+  if (n == "nbody") return scal(m.nbody);
+  if (n == "njnt") return scal(m.njnt);
+  if (n == "ngeom") return scal(m.ngeom);
+  if (n == "timestep") return scal(m.timestep);
+  if (n == "gravity") return put({m.gravity[0], m.gravity[1], m.gravity[2]});
+  if (n == "tolerance") return scal(m.tolerance);
+  if (n == "ls_tolerance") return scal(m.ls_tolerance);
+  if (n == "impratio") return scal(m.impratio);
+  if (n == "iterations") return scal(m.iterations);
+  if (n == "ls_iterations") return scal(m.ls_iterations);
+  if (n == "solver") return scal(m.solver);
+  if (n == "cone") return scal(m.cone);
+  if (n == "integrator") return scal(m.integrator);
+  if (n == "disableflags") return scal(m.disableflags);
+  if (n == "meaninertia") return scal(m.meaninertia);
+  if (n == "npair") return scal((double)m.pair_geom1.size());
+  if (n == "nexclude") return scal((double)m.exclude_signature.size());
+#define F(x) if (n == #x) return put(m.x);
+#define I(x) if (n == #x) return puti(m.x);
+  I(body_parentid) I(body_rootid) I(body_weldid) I(body_jntnum) I(body_jntadr) I(body_dofnum) I(body_dofadr)
+  I(body_geomnum) I(body_geomadr)
+  F(body_pos) F(body_quat) F(body_ipos) F(body_iquat) F(body_mass) F(body_inertia) F(body_invweight0) F(body_subtreemass)
+  I(jnt_type) I(jnt_qposadr) I(jnt_dofadr) I(jnt_bodyid) I(jnt_limited)
+  F(jnt_pos) F(jnt_axis) F(jnt_range) F(jnt_margin) F(jnt_solref) F(jnt_solimp) F(jnt_stiffness)
+  I(dof_bodyid) I(dof_jntid) I(dof_parentid)
+  F(dof_armature) F(dof_damping) F(dof_frictionloss) F(dof_invweight0) F(dof_solref) F(dof_solimp)
+  I(geom_type) I(geom_contype) I(geom_conaffinity) I(geom_condim) I(geom_bodyid) I(geom_priority)
+  F(geom_size) F(geom_pos) F(geom_quat) F(geom_friction) F(geom_solref) F(geom_solimp) F(geom_margin) F(geom_gap)
+  F(geom_solmix) F(geom_rbound)
+  I(actuator_trnid) I(actuator_ctrllimited) I(actuator_forcelimited)
+  F(actuator_gear) F(actuator_gainprm) F(actuator_biasprm) F(actuator_ctrlrange) F(actuator_forcerange)
+  I(pair_geom1) I(pair_geom2) I(pair_dim) F(pair_friction) F(pair_solref) F(pair_solimp) F(pair_margin) F(pair_gap)
+  I(exclude_signature) F(qpos0) F(qpos_spring)
+#undef F
+#undef I
+  if (n.rfind("cpair_", 0) == 0) {
+    std::vector<CollisionPair> ps = build_pairs(m);
+    std::vector<double> v;
+    std::string f = n.substr(6);
+    for (auto& p : ps) {
+      if (f == "g1") v.push_back(p.g1);
+      else if (f == "g2") v.push_back(p.g2);
+      else if (f == "b1") v.push_back(p.b1);
+      else if (f == "b2") v.push_back(p.b2);
+      else if (f == "condim") v.push_back(p.condim);
+      else if (f == "friction") v.insert(v.end(), p.friction, p.friction + 5);
+      else if (f == "solref") v.insert(v.end(), p.solref, p.solref + 2);
+      else if (f == "solimp") v.insert(v.end(), p.solimp, p.solimp + 5);
+      else if (f == "margin") v.push_back(p.margin);
+      else if (f == "gap") v.push_back(p.gap);
+      else return -1;
+    }
+    return put(v);
+  }
+  return -1;
+}
+
+}  // namespace mjh

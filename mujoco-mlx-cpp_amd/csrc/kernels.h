@@ -1,0 +1,1400 @@
+// Physics of one env, written for one 64-lane wavefront per env.
+//
+// Every stage is a lane-strided loop over independent work items (bodies of
+// one tree level, dofs, pairs, rows, matrix entries) followed by KSYNC().
+// Values that live across a KSYNC() are either in the LDS workspace or
+// wave-uniform (results of wave reductions), so the same source also runs
+// with WAVE == 1 as a serial interpreter (MJ_EMU builds; test-only).
+//
+// Stage order per substep follows the reference pipeline
+// (reference src/batched.cpp:980-1228: kinematics, forward, collision, solver,
+// Euler) with MuJoCo semantics where the reference departs from them
+// (multi-contact narrow phase, warm start, MuJoCo termination test).
+#pragma once
+#include <math.h>
+#include <stdint.h>
+
+#include "dmodel.h"
+
+#if defined(MJ_EMU)
+#define KDEV static inline
+#define WAVE 1
+#define LANE 0
+#define KSYNC() ((void)0)
+#else
+#define KDEV __device__ __forceinline__
+#define WAVE 64
+#define LANE ((int)threadIdx.x)
+#define KSYNC() __syncthreads()
+#endif
+
+namespace mjk {
+
+#define KMINVAL 1e-15f
+#define KMINIMP 0.0001f
+#define KMAXIMP 0.9999f
+
+// ------------------------------------------------------------------ wave helpers
+KDEV float wsum(float v) {
+#if WAVE == 64
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+#endif
+  return v;
+}
+KDEV double wsumd(double v) {
+#if WAVE == 64
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+#endif
+  return v;
+}
+KDEV int wsumi(int v) {
+#if WAVE == 64
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+#endif
+  return v;
+}
+KDEV int wori(int v) {
+#if WAVE == 64
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+#endif
+  return v;
+}
+KDEV int wmini(int v) {
+#if WAVE == 64
+  for (int o = 32; o > 0; o >>= 1) { int t = __shfl_xor(v, o, 64); v = t < v ? t : v; }
+#endif
+  return v;
+}
+// exclusive prefix sum over the wave
+KDEV int wscan_excl(int v) {
+#if WAVE == 64
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(x, o, 64);
+    if (LANE >= o) x += t;
+  }
+  return x - v;
+#else
+  (void)v;
+  return 0;
+#endif
+}
+
+KDEV int* iptr(float* w, int off) { return reinterpret_cast<int*>(w + off); }
+KDEV double* dptr(float* w, int off) { return reinterpret_cast<double*>(w + off); }
+
+// (row, col) with col <= row of lower-triangular entry t (row-major packing)
+KDEV void tri_rc(int t, int* r, int* c) {
+  int row = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+  while ((row + 1) * (row + 2) / 2 <= t) row++;
+  while (row * (row + 1) / 2 > t) row--;
+  *r = row;
+  *c = t - row * (row + 1) / 2;
+}
+
+// ------------------------------------------------------------------ small math
+KDEV void quat2mat(const float* q, float* R) {
+  float w = q[0], x = q[1], y = q[2], z = q[3];
+  R[0] = w * w + x * x - y * y - z * z; R[1] = 2.f * (x * y - w * z); R[2] = 2.f * (x * z + w * y);
+  R[3] = 2.f * (x * y + w * z); R[4] = w * w - x * x + y * y - z * z; R[5] = 2.f * (y * z - w * x);
+  R[6] = 2.f * (x * z - w * y); R[7] = 2.f * (y * z + w * x); R[8] = w * w - x * x - y * y + z * z;
+}
+KDEV void mulquat(const float* a, const float* b, float* o) {
+  float r0 = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+  float r1 = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+  float r2 = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+  float r3 = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+  o[0] = r0; o[1] = r1; o[2] = r2; o[3] = r3;
+}
+KDEV void normq(float* q) {
+  float n = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  if (n < KMINVAL) { q[0] = 1.f; q[1] = q[2] = q[3] = 0.f; return; }
+  float in = 1.0f / n;
+  q[0] *= in; q[1] *= in; q[2] *= in; q[3] *= in;
+}
+KDEV void mv3(const float* R, const float* v, float* o) {
+  float a = R[0] * v[0] + R[1] * v[1] + R[2] * v[2];
+  float b = R[3] * v[0] + R[4] * v[1] + R[5] * v[2];
+  float c = R[6] * v[0] + R[7] * v[1] + R[8] * v[2];
+  o[0] = a; o[1] = b; o[2] = c;
+}
+KDEV void mtv3(const float* R, const float* v, float* o) {
+  float a = R[0] * v[0] + R[3] * v[1] + R[6] * v[2];
+  float b = R[1] * v[0] + R[4] * v[1] + R[7] * v[2];
+  float c = R[2] * v[0] + R[5] * v[1] + R[8] * v[2];
+  o[0] = a; o[1] = b; o[2] = c;
+}
+KDEV void cross3(const float* a, const float* b, float* o) {
+  float x = a[1] * b[2] - a[2] * b[1], y = a[2] * b[0] - a[0] * b[2], z = a[0] * b[1] - a[1] * b[0];
+  o[0] = x; o[1] = y; o[2] = z;
+}
+KDEV float dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+KDEV float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+KDEV void mul_inert(const float* i, const float* v, float* r) {
+  r[0] = i[0] * v[0] + i[3] * v[1] + i[4] * v[2] - i[8] * v[4] + i[7] * v[5];
+  r[1] = i[3] * v[0] + i[1] * v[1] + i[5] * v[2] + i[8] * v[3] - i[6] * v[5];
+  r[2] = i[4] * v[0] + i[5] * v[1] + i[2] * v[2] - i[7] * v[3] + i[6] * v[4];
+  r[3] = i[8] * v[1] - i[7] * v[2] + i[9] * v[3];
+  r[4] = i[6] * v[2] - i[8] * v[0] + i[9] * v[4];
+  r[5] = i[7] * v[0] - i[6] * v[1] + i[9] * v[5];
+}
+KDEV void cross_motion(const float* v, const float* u, float* r) {
+  float a[3], b[3], c[3];
+  cross3(v, u, a); cross3(v, u + 3, b); cross3(v + 3, u, c);
+  r[0] = a[0]; r[1] = a[1]; r[2] = a[2];
+  r[3] = b[0] + c[0]; r[4] = b[1] + c[1]; r[5] = b[2] + c[2];
+}
+KDEV void cross_force(const float* v, const float* f, float* r) {
+  float a[3], b[3], c[3];
+  cross3(v, f, a); cross3(v + 3, f + 3, b); cross3(v, f + 3, c);
+  r[0] = a[0] + b[0]; r[1] = a[1] + b[1]; r[2] = a[2] + b[2];
+  r[3] = c[0]; r[4] = c[1]; r[5] = c[2];
+}
+
+// ------------------------------------------------------------------ kinematics
+// mj_kinematics, level by level (bodies of one tree depth in parallel).
+// Reference src/compat/mkx_kernels/kinematics.hpp:79-144.
+KDEV void st_kinematics(const DModel& m, const Layout& L, float* w) {
+  float *xpos = w + L.xpos, *xquat = w + L.xquat, *xmat = w + L.xmat, *xipos = w + L.xipos;
+  float *xanc = w + L.xanchor, *xax = w + L.xaxis;
+  const float* qpos = w + L.qpos;
+  if (LANE == 0) {
+    xpos[0] = xpos[1] = xpos[2] = 0.f;
+    xquat[0] = 1.f; xquat[1] = xquat[2] = xquat[3] = 0.f;
+    for (int i = 0; i < 9; i++) xmat[i] = (i % 4 == 0) ? 1.f : 0.f;
+    xipos[0] = xipos[1] = xipos[2] = 0.f;
+  }
+  KSYNC();
+  for (int lv = 0; lv < m.nlevel; lv++) {
+    int a = m.level_adr[lv], n = m.level_adr[lv + 1] - a;
+    for (int i = LANE; i < n; i += WAVE) {
+      int b = m.level_body[a + i];
+      int par = m.body_parentid[b];
+      int ja = m.body_jntadr[b], jn = m.body_jntnum[b];
+      float pos[3], q[4];
+      if (jn > 0 && m.jnt_type[ja] == JFREE) {
+        int qa = m.jnt_qposadr[ja];
+        pos[0] = qpos[qa]; pos[1] = qpos[qa + 1]; pos[2] = qpos[qa + 2];
+        q[0] = qpos[qa + 3]; q[1] = qpos[qa + 4]; q[2] = qpos[qa + 5]; q[3] = qpos[qa + 6];
+        normq(q);
+        for (int k = 0; k < 3; k++) { xanc[3 * ja + k] = pos[k]; xax[3 * ja + k] = m.jnt_axis[3 * ja + k]; }
+      } else {
+        float t[3];
+        mv3(xmat + 9 * par, m.body_pos + 3 * b, t);
+        pos[0] = xpos[3 * par] + t[0]; pos[1] = xpos[3 * par + 1] + t[1]; pos[2] = xpos[3 * par + 2] + t[2];
+        mulquat(xquat + 4 * par, m.body_quat + 4 * b, q);
+        for (int j = ja; j < ja + jn; j++) {
+          int qa = m.jnt_qposadr[j];
+          float R[9], anc[3], ax[3];
+          quat2mat(q, R);
+          mv3(R, m.jnt_pos + 3 * j, anc);
+          anc[0] += pos[0]; anc[1] += pos[1]; anc[2] += pos[2];
+          mv3(R, m.jnt_axis + 3 * j, ax);
+          for (int k = 0; k < 3; k++) { xanc[3 * j + k] = anc[k]; xax[3 * j + k] = ax[k]; }
+          int jt = m.jnt_type[j];
+          if (jt == JSLIDE) {
+            float s = qpos[qa] - m.qpos0[qa];
+            for (int k = 0; k < 3; k++) pos[k] += ax[k] * s;
+          } else {
+            float ql[4];
+            if (jt == JHINGE) {
+              float ang = 0.5f * (qpos[qa] - m.qpos0[qa]);
+              float s = sinf(ang), c = cosf(ang);
+              ql[0] = c; ql[1] = m.jnt_axis[3 * j] * s; ql[2] = m.jnt_axis[3 * j + 1] * s; ql[3] = m.jnt_axis[3 * j + 2] * s;
+            } else {
+              ql[0] = qpos[qa]; ql[1] = qpos[qa + 1]; ql[2] = qpos[qa + 2]; ql[3] = qpos[qa + 3];
+              normq(ql);
+            }
+            mulquat(q, ql, q);
+            float R2[9], r[3];
+            quat2mat(q, R2);
+            mv3(R2, m.jnt_pos + 3 * j, r);
+            for (int k = 0; k < 3; k++) pos[k] = anc[k] - r[k];
+          }
+        }
+        normq(q);
+      }
+      for (int k = 0; k < 3; k++) xpos[3 * b + k] = pos[k];
+      for (int k = 0; k < 4; k++) xquat[4 * b + k] = q[k];
+      float R[9];
+      quat2mat(q, R);
+      for (int k = 0; k < 9; k++) xmat[9 * b + k] = R[k];
+      float t[3];
+      mv3(R, m.body_ipos + 3 * b, t);
+      for (int k = 0; k < 3; k++) xipos[3 * b + k] = pos[k] + t[k];
+    }
+    KSYNC();
+  }
+  float *gx = w + L.gxpos, *gm = w + L.gxmat;
+  for (int g = LANE; g < m.ngeom; g += WAVE) {
+    int b = m.geom_bodyid[g];
+    float t[3], qg[4];
+    mv3(xmat + 9 * b, m.geom_pos + 3 * g, t);
+    for (int k = 0; k < 3; k++) gx[3 * g + k] = xpos[3 * b + k] + t[k];
+    mulquat(xquat + 4 * b, m.geom_quat + 4 * g, qg);
+    quat2mat(qg, gm + 9 * g);
+  }
+  KSYNC();
+}
+
+// mj_comPos: subtree COM (bodies are numbered in depth-first order, so a
+// subtree is the id range [b, body_subtree_end[b])), cinert and cdof.
+// Reference forward.hpp:146-233.
+KDEV void st_com_pos(const DModel& m, const Layout& L, float* w) {
+  const float *xipos = w + L.xipos, *xquat = w + L.xquat, *xmat = w + L.xmat;
+  float *scom = w + L.scom, *cinert = w + L.cinert, *cdof = w + L.cdof;
+  for (int b = LANE; b < m.nbody; b += WAVE) {
+    float s[3] = {0.f, 0.f, 0.f};
+    for (int k = b; k < m.body_subtree_end[b]; k++) {
+      float mk = m.body_mass[k];
+      s[0] += mk * xipos[3 * k]; s[1] += mk * xipos[3 * k + 1]; s[2] += mk * xipos[3 * k + 2];
+    }
+    float sm = m.body_subtreemass[b];
+    for (int k = 0; k < 3; k++) scom[3 * b + k] = sm < KMINVAL ? xipos[3 * b + k] : s[k] / sm;
+  }
+  KSYNC();
+  for (int b = LANE; b < m.nbody; b += WAVE) {
+    float* o = cinert + 10 * b;
+    if (b == 0) { for (int k = 0; k < 10; k++) o[k] = 0.f; continue; }
+    float qi[4], R[9];
+    mulquat(xquat + 4 * b, m.body_iquat + 4 * b, qi);
+    quat2mat(qi, R);
+    const float* I = m.body_inertia + 3 * b;
+    float mass = m.body_mass[b];
+    const float* c = scom + 3 * m.body_rootid[b];
+    float d[3] = {xipos[3 * b] - c[0], xipos[3 * b + 1] - c[1], xipos[3 * b + 2] - c[2]};
+    float A[9];
+    for (int r = 0; r < 3; r++)
+      for (int k = 0; k < 3; k++) A[3 * r + k] = R[3 * r] * I[0] * R[3 * k] + R[3 * r + 1] * I[1] * R[3 * k + 1] + R[3 * r + 2] * I[2] * R[3 * k + 2];
+    float sq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    o[0] = A[0] + mass * (sq - d[0] * d[0]);
+    o[1] = A[4] + mass * (sq - d[1] * d[1]);
+    o[2] = A[8] + mass * (sq - d[2] * d[2]);
+    o[3] = A[1] - mass * d[0] * d[1];
+    o[4] = A[2] - mass * d[0] * d[2];
+    o[5] = A[5] - mass * d[1] * d[2];
+    o[6] = mass * d[0]; o[7] = mass * d[1]; o[8] = mass * d[2]; o[9] = mass;
+  }
+  const float *xanc = w + L.xanchor, *xax = w + L.xaxis;
+  for (int k = LANE; k < m.nv; k += WAVE) {
+    int j = m.dof_jntid[k], b = m.dof_bodyid[k];
+    int jt = m.jnt_type[j], i = k - m.jnt_dofadr[j];
+    float* cd = cdof + 6 * k;
+    const float* c = scom + 3 * m.body_rootid[b];
+    const float* anc = xanc + 3 * j;
+    float off[3] = {c[0] - anc[0], c[1] - anc[1], c[2] - anc[2]};
+    float ax[3];
+    bool rot = true;
+    if (jt == JFREE && i < 3) {
+      for (int q = 0; q < 6; q++) cd[q] = 0.f;
+      cd[3 + i] = 1.f;
+      rot = false;
+    } else if (jt == JFREE || jt == JBALL) {
+      int col = jt == JFREE ? i - 3 : i;
+      ax[0] = xmat[9 * b + col]; ax[1] = xmat[9 * b + 3 + col]; ax[2] = xmat[9 * b + 6 + col];
+    } else if (jt == JSLIDE) {
+      cd[0] = cd[1] = cd[2] = 0.f;
+      cd[3] = xax[3 * j]; cd[4] = xax[3 * j + 1]; cd[5] = xax[3 * j + 2];
+      rot = false;
+    } else {
+      ax[0] = xax[3 * j]; ax[1] = xax[3 * j + 1]; ax[2] = xax[3 * j + 2];
+    }
+    if (rot) {
+      cd[0] = ax[0]; cd[1] = ax[1]; cd[2] = ax[2];
+      cross3(ax, off, cd + 3);
+    }
+  }
+  KSYNC();
+}
+
+// mj_crb: composite inertia by subtree ranges, then qM entries along each dof's
+// ancestor chain (reference forward.hpp:240-279).
+KDEV void st_crb(const DModel& m, const Layout& L, float* w) {
+  const float *cinert = w + L.cinert, *cdof = w + L.cdof;
+  float *crb = w + L.crb, *buf = w + L.buf6, *M = w + L.qM;
+  int nv = m.nv;
+  for (int b = LANE; b < m.nbody; b += WAVE) {
+    float s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (b > 0)
+      for (int k = b; k < m.body_subtree_end[b]; k++)
+        for (int q = 0; q < 10; q++) s[q] += cinert[10 * k + q];
+    for (int q = 0; q < 10; q++) crb[10 * b + q] = s[q];
+  }
+  for (int t = LANE; t < nv * nv; t += WAVE) M[t] = 0.f;
+  KSYNC();
+  for (int i = LANE; i < nv; i += WAVE) mul_inert(crb + 10 * m.dof_bodyid[i], cdof + 6 * i, buf + 6 * i);
+  KSYNC();
+  for (int i = LANE; i < nv; i += WAVE) {
+    const float* bi = buf + 6 * i;
+    for (int j = i; j >= 0; j = m.dof_parentid[j]) {
+      const float* cj = cdof + 6 * j;
+      float s = cj[0] * bi[0] + cj[1] * bi[1] + cj[2] * bi[2] + cj[3] * bi[3] + cj[4] * bi[4] + cj[5] * bi[5];
+      M[i * nv + j] = s;
+      M[j * nv + i] = s;
+    }
+    M[i * nv + i] += m.dof_armature[i];
+  }
+  KSYNC();
+}
+
+// Tree-sparse L^T D L factorisation (MuJoCo mj_factorM) of A (dense storage)
+// into LD. Step k updates the entries (i, j), i and j strict ancestors of k,
+// j an ancestor of i, in parallel. Returns ST_M_NOT_PD on a non-positive pivot.
+KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
+  int nv = m.nv;
+  for (int t = LANE; t < nv * nv; t += WAVE) LD[t] = A[t];
+  KSYNC();
+  int bad = 0;
+  for (int k = nv - 1; k >= 0; k--) {
+    float dk = LD[k * nv + k];
+    if (!(dk > KMINVAL)) { bad = 1; dk = KMINVAL; }
+    const int* ch = m.dof_chain + k * MAX_DOF_DEPTH;
+    int na = m.dof_nchain[k] - 1;  // strict ancestors
+    int np = na * (na + 1) / 2;
+    float inv = 1.0f / dk;
+    for (int t = LANE; t < np; t += WAVE) {
+      int bb, aa;
+      tri_rc(t, &bb, &aa);
+      int i = ch[1 + aa], j = ch[1 + bb];  // j is an ancestor of (or equal to) i
+      LD[i * nv + j] -= LD[k * nv + i] * inv * LD[k * nv + j];
+    }
+    KSYNC();
+    for (int a = 1 + LANE; a <= na; a += WAVE) LD[k * nv + ch[a]] *= inv;
+    KSYNC();
+  }
+  return bad ? ST_M_NOT_PD : 0;
+}
+
+// x <- (L^T D L)^-1 x, level-parallel over the dof tree.
+KDEV void solve_ld(const DModel& m, const float* LD, float* x) {
+  int nv = m.nv;
+  for (int lv = m.ndoflevel - 1; lv >= 0; lv--) {
+    int a = m.doflevel_adr[lv], n = m.doflevel_adr[lv + 1] - a;
+    for (int t = LANE; t < n; t += WAVE) {
+      int i = m.doflevel_dof[a + t];
+      float s = x[i];
+      for (int k = i + 1; k < m.dof_subtree_end[i]; k++) s -= LD[k * nv + i] * x[k];
+      x[i] = s;
+    }
+    KSYNC();
+  }
+  for (int i = LANE; i < nv; i += WAVE) x[i] /= LD[i * nv + i];
+  KSYNC();
+  for (int lv = 1; lv < m.ndoflevel; lv++) {
+    int a = m.doflevel_adr[lv], n = m.doflevel_adr[lv + 1] - a;
+    for (int t = LANE; t < n; t += WAVE) {
+      int k = m.doflevel_dof[a + t];
+      const int* ch = m.dof_chain + k * MAX_DOF_DEPTH;
+      float s = x[k];
+      for (int q = 1; q < m.dof_nchain[k]; q++) s -= LD[k * nv + ch[q]] * x[ch[q]];
+      x[k] = s;
+    }
+    KSYNC();
+  }
+}
+
+// mj_comVel with MuJoCo's grouping of free / ball dofs (reference forward.hpp:287-332)
+KDEV void st_com_vel(const DModel& m, const Layout& L, float* w) {
+  float *cvel = w + L.cvel, *cdd = w + L.cdofdot;
+  const float *cdof = w + L.cdof, *qvel = w + L.qvel;
+  if (LANE == 0) for (int k = 0; k < 6; k++) cvel[k] = 0.f;
+  KSYNC();
+  for (int lv = 0; lv < m.nlevel; lv++) {
+    int a = m.level_adr[lv], n = m.level_adr[lv + 1] - a;
+    for (int i = LANE; i < n; i += WAVE) {
+      int b = m.level_body[a + i];
+      float v[6];
+      for (int k = 0; k < 6; k++) v[k] = cvel[6 * m.body_parentid[b] + k];
+      int ja = m.body_jntadr[b], jn = m.body_jntnum[b];
+      for (int j = ja; j < ja + jn; j++) {
+        int da = m.jnt_dofadr[j], jt = m.jnt_type[j];
+        if (jt == JFREE) {
+          for (int q = 0; q < 3; q++) for (int c = 0; c < 6; c++) cdd[6 * (da + q) + c] = 0.f;
+          for (int q = 0; q < 3; q++) for (int c = 0; c < 6; c++) v[c] += cdof[6 * (da + q) + c] * qvel[da + q];
+          da += 3;
+        }
+        if (jt == JFREE || jt == JBALL) {
+          for (int q = 0; q < 3; q++) cross_motion(v, cdof + 6 * (da + q), cdd + 6 * (da + q));
+          for (int q = 0; q < 3; q++) for (int c = 0; c < 6; c++) v[c] += cdof[6 * (da + q) + c] * qvel[da + q];
+        } else {
+          cross_motion(v, cdof + 6 * da, cdd + 6 * da);
+          for (int c = 0; c < 6; c++) v[c] += cdof[6 * da + c] * qvel[da];
+        }
+      }
+      for (int k = 0; k < 6; k++) cvel[6 * b + k] = v[k];
+    }
+    KSYNC();
+  }
+}
+
+// mj_rne (zero acceleration) -> qfrc_bias, then passive and actuator forces,
+// qfrc_smooth and qacc_smooth = M^-1 qfrc_smooth.
+// Reference forward.hpp:338-469.
+KDEV int st_smooth(const DModel& m, const Layout& L, float* w) {
+  float *cacc = w + L.cacc, *cfrc = w + L.cfrc;
+  const float *cinert = w + L.cinert, *cvel = w + L.cvel, *cdd = w + L.cdofdot, *cdof = w + L.cdof;
+  const float *qvel = w + L.qvel, *qpos = w + L.qpos, *ctrl = w + L.ctrl;
+  if (LANE == 0) {
+    for (int k = 0; k < 6; k++) cacc[k] = 0.f;
+    if (!(m.disableflags & DS_GRAVITY)) { cacc[3] = -m.gravity[0]; cacc[4] = -m.gravity[1]; cacc[5] = -m.gravity[2]; }
+    for (int k = 0; k < 6; k++) cfrc[k] = 0.f;
+  }
+  KSYNC();
+  for (int lv = 0; lv < m.nlevel; lv++) {
+    int a = m.level_adr[lv], n = m.level_adr[lv + 1] - a;
+    for (int i = LANE; i < n; i += WAVE) {
+      int b = m.level_body[a + i];
+      float ac[6];
+      for (int k = 0; k < 6; k++) ac[k] = cacc[6 * m.body_parentid[b] + k];
+      int da = m.body_dofadr[b], dn = m.body_dofnum[b];
+      for (int q = da; q < da + dn; q++) for (int c = 0; c < 6; c++) ac[c] += cdd[6 * q + c] * qvel[q];
+      for (int k = 0; k < 6; k++) cacc[6 * b + k] = ac[k];
+      float f1[6], mv[6], f2[6];
+      mul_inert(cinert + 10 * b, ac, f1);
+      mul_inert(cinert + 10 * b, cvel + 6 * b, mv);
+      cross_force(cvel + 6 * b, mv, f2);
+      for (int k = 0; k < 6; k++) cfrc[6 * b + k] = f1[k] + f2[k];
+    }
+    KSYNC();
+  }
+  // subtree sums of body forces into cacc (no longer needed)
+  for (int b = LANE; b < m.nbody; b += WAVE) {
+    float s[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = b; k < m.body_subtree_end[b]; k++) for (int c = 0; c < 6; c++) s[c] += cfrc[6 * k + c];
+    for (int c = 0; c < 6; c++) cacc[6 * b + c] = s[c];
+  }
+  KSYNC();
+  float *qfs = w + L.qfrc_smooth, *qas = w + L.qacc_smooth, *af = w + L.act_force;
+  for (int k = LANE; k < m.nv; k += WAVE) {
+    const float* cd = cdof + 6 * k;
+    const float* f = cacc + 6 * m.dof_bodyid[k];
+    float bias = cd[0] * f[0] + cd[1] * f[1] + cd[2] * f[2] + cd[3] * f[3] + cd[4] * f[4] + cd[5] * f[5];
+    float pas = 0.f;
+    if (!(m.disableflags & DS_PASSIVE)) {
+      int j = m.dof_jntid[k], jt = m.jnt_type[j];
+      if ((jt == JHINGE || jt == JSLIDE) && m.jnt_stiffness[j] != 0.f) {
+        int qa = m.jnt_qposadr[j];
+        pas -= m.jnt_stiffness[j] * (qpos[qa] - m.qpos_spring[qa]);
+      }
+      pas -= m.dof_damping[k] * qvel[k];
+    }
+    float act = 0.f;
+    for (int t = m.dof_act_adr[k]; t < m.dof_act_adr[k] + m.dof_act_num[k]; t++) {
+      int u = m.dof_act_list[t];
+      float frc = 0.f;
+      if (!(m.disableflags & DS_ACTUATION)) {
+        float gear = m.act_gear[u];
+        float c = ctrl[u];
+        if (m.act_ctrllimited[u] && !(m.disableflags & DS_CLAMPCTRL)) c = clampf(c, m.act_ctrlrange[2 * u], m.act_ctrlrange[2 * u + 1]);
+        float len = gear * qpos[m.act_qadr[u]], vel = gear * qvel[k];
+        const float *g = m.act_gain + 3 * u, *bi = m.act_bias + 3 * u;
+        frc = g[0] * c + bi[0] + bi[1] * len + bi[2] * vel;
+        if (m.act_forcelimited[u]) frc = clampf(frc, m.act_forcerange[2 * u], m.act_forcerange[2 * u + 1]);
+        act += gear * frc;
+      }
+      af[u] = frc;
+    }
+    float s = pas - bias + act;
+    qfs[k] = s;
+    qas[k] = s;
+  }
+  KSYNC();
+  solve_ld(m, w + L.qLD, qas);
+  return 0;
+}
+
+// ------------------------------------------------------------------ collision
+struct RawC { float dist, pos[3], n[3]; };
+
+KDEV int k_sphere_sphere(const float* c1, float r1, const float* c2, float r2, float margin, RawC* o) {
+  float dv[3] = {c2[0] - c1[0], c2[1] - c1[1], c2[2] - c1[2]};
+  float Ls = sqrtf(dot3(dv, dv));
+  float dist = Ls - r1 - r2;
+  if (dist > margin) return 0;
+  float n[3];
+  if (Ls < KMINVAL) { n[0] = 1.f; n[1] = 0.f; n[2] = 0.f; }
+  else { float in = 1.0f / Ls; n[0] = dv[0] * in; n[1] = dv[1] * in; n[2] = dv[2] * in; }
+  o->dist = dist;
+  for (int i = 0; i < 3; i++) { o->n[i] = n[i]; o->pos[i] = c1[i] + n[i] * (r1 + 0.5f * dist); }
+  return 1;
+}
+KDEV int k_plane_point(const float* pp, const float* pn, const float* c, float r, float margin, RawC* o) {
+  float dv[3] = {c[0] - pp[0], c[1] - pp[1], c[2] - pp[2]};
+  float dist = dot3(pn, dv) - r;
+  if (dist > margin) return 0;
+  o->dist = dist;
+  for (int i = 0; i < 3; i++) { o->n[i] = pn[i]; o->pos[i] = c[i] - pn[i] * (r + 0.5f * dist); }
+  return 1;
+}
+KDEV void k_seg_seg(const float* p1, const float* d1, const float* p2, const float* d2, float* so, float* to) {
+  float r[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
+  float a = dot3(d1, d1), e = dot3(d2, d2), f = dot3(d2, r);
+  float s, t;
+  if (a <= KMINVAL && e <= KMINVAL) { s = t = 0.f; }
+  else if (a <= KMINVAL) { s = 0.f; t = clampf(f / e, 0.f, 1.f); }
+  else {
+    float c = dot3(d1, r);
+    if (e <= KMINVAL) { t = 0.f; s = clampf(-c / a, 0.f, 1.f); }
+    else {
+      float b = dot3(d1, d2);
+      float denom = a * e - b * b;
+      s = (denom > 1e-10f * a * e) ? clampf((b * f - c * e) / denom, 0.f, 1.f) : 0.f;
+      t = (b * s + f) / e;
+      if (t < 0.f) { t = 0.f; s = clampf(-c / a, 0.f, 1.f); }
+      else if (t > 1.f) { t = 1.f; s = clampf((b - c) / a, 0.f, 1.f); }
+    }
+  }
+  *so = s; *to = t;
+}
+KDEV void k_capsule_ends(const float* x, const float* R, float h, float* p0, float* d) {
+  float ax[3] = {R[2], R[5], R[8]};
+  for (int i = 0; i < 3; i++) { p0[i] = x[i] - ax[i] * h; d[i] = 2.f * h * ax[i]; }
+}
+KDEV int k_sphere_box(const float* c, float r, const float* bx, const float* R, const float* s, float margin, RawC* o) {
+  float rel[3] = {c[0] - bx[0], c[1] - bx[1], c[2] - bx[2]}, pl[3];
+  mtv3(R, rel, pl);
+  float q[3];
+  bool inside = true;
+  for (int i = 0; i < 3; i++) {
+    q[i] = clampf(pl[i], -s[i], s[i]);
+    if (pl[i] < -s[i] || pl[i] > s[i]) inside = false;
+  }
+  float nl[3], dist;
+  if (!inside) {
+    float dv[3] = {pl[0] - q[0], pl[1] - q[1], pl[2] - q[2]};
+    float Ls = sqrtf(dot3(dv, dv));
+    dist = Ls - r;
+    if (dist > margin) return 0;
+    for (int i = 0; i < 3; i++) nl[i] = -dv[i] / Ls;
+  } else {
+    int k = 0;
+    float best = s[0] - fabsf(pl[0]);
+    for (int i = 1; i < 3; i++) { float dp = s[i] - fabsf(pl[i]); if (dp < best) { best = dp; k = i; } }
+    dist = -best - r;
+    if (dist > margin) return 0;
+    nl[0] = nl[1] = nl[2] = 0.f;
+    nl[k] = pl[k] >= 0.f ? -1.f : 1.f;
+  }
+  float n[3];
+  mv3(R, nl, n);
+  o->dist = dist;
+  for (int i = 0; i < 3; i++) { o->n[i] = n[i]; o->pos[i] = c[i] + n[i] * (r + 0.5f * dist); }
+  return 1;
+}
+
+// narrow phase for one pair (geom1 has the lower type); -1 for unsupported
+KDEV int k_narrow(int t1, int t2, const float* s1, const float* s2, const float* x1, const float* R1,
+                  const float* x2, const float* R2, float margin, RawC* out) {
+  if (t1 == GPLANE) {
+    float pn[3] = {R1[2], R1[5], R1[8]};
+    if (t2 == GSPHERE) return k_plane_point(x1, pn, x2, s2[0], margin, out);
+    if (t2 == GCAPSULE) {
+      float p0[3], dd[3];
+      k_capsule_ends(x2, R2, s2[1], p0, dd);
+      float pA[3] = {p0[0] + dd[0], p0[1] + dd[1], p0[2] + dd[2]};
+      int n = k_plane_point(x1, pn, pA, s2[0], margin, out);
+      n += k_plane_point(x1, pn, p0, s2[0], margin, out + n);
+      return n;
+    }
+    if (t2 == GBOX) {
+      int n = 0;
+      for (int k = 0; k < 8 && n < 4; k++) {
+        float cl[3] = {(k & 1) ? s2[0] : -s2[0], (k & 2) ? s2[1] : -s2[1], (k & 4) ? s2[2] : -s2[2]}, cw[3];
+        mv3(R2, cl, cw);
+        cw[0] += x2[0]; cw[1] += x2[1]; cw[2] += x2[2];
+        n += k_plane_point(x1, pn, cw, 0.f, margin, out + n);
+      }
+      return n;
+    }
+    return -1;
+  }
+  if (t1 == GSPHERE) {
+    if (t2 == GSPHERE) return k_sphere_sphere(x1, s1[0], x2, s2[0], margin, out);
+    if (t2 == GCAPSULE) {
+      float p0[3], dd[3];
+      k_capsule_ends(x2, R2, s2[1], p0, dd);
+      float rel[3] = {x1[0] - p0[0], x1[1] - p0[1], x1[2] - p0[2]};
+      float a = dot3(dd, dd);
+      float t = a > KMINVAL ? clampf(dot3(rel, dd) / a, 0.f, 1.f) : 0.f;
+      float c2[3] = {p0[0] + t * dd[0], p0[1] + t * dd[1], p0[2] + t * dd[2]};
+      return k_sphere_sphere(x1, s1[0], c2, s2[0], margin, out);
+    }
+    if (t2 == GBOX) return k_sphere_box(x1, s1[0], x2, R2, s2, margin, out);
+    return -1;
+  }
+  if (t1 == GCAPSULE && t2 == GCAPSULE) {
+    float a0[3], ad[3], b0[3], bd[3], s, t;
+    k_capsule_ends(x1, R1, s1[1], a0, ad);
+    k_capsule_ends(x2, R2, s2[1], b0, bd);
+    k_seg_seg(a0, ad, b0, bd, &s, &t);
+    float c1[3] = {a0[0] + s * ad[0], a0[1] + s * ad[1], a0[2] + s * ad[2]};
+    float c2[3] = {b0[0] + t * bd[0], b0[1] + t * bd[1], b0[2] + t * bd[2]};
+    return k_sphere_sphere(c1, s1[0], c2, s2[0], margin, out);
+  }
+  return -1;
+}
+
+// contact frame from the normal (MuJoCo mju_makeFrame)
+KDEV void k_make_frame(const float* n_in, float* f) {
+  float n[3] = {n_in[0], n_in[1], n_in[2]};
+  float nn = sqrtf(dot3(n, n));
+  if (nn < KMINVAL) { n[0] = 1.f; n[1] = n[2] = 0.f; }
+  else { float in = 1.0f / nn; n[0] *= in; n[1] *= in; n[2] *= in; }
+  float y[3];
+  if (fabsf(n[1]) < 0.5f) { y[0] = 0.f; y[1] = 1.f; y[2] = 0.f; }
+  else { y[0] = 0.f; y[1] = 0.f; y[2] = 1.f; }
+  float s = dot3(n, y);
+  for (int i = 0; i < 3; i++) y[i] -= s * n[i];
+  float yn = sqrtf(dot3(y, y));
+  float iy = 1.0f / yn;
+  y[0] *= iy; y[1] *= iy; y[2] *= iy;
+  float z[3];
+  cross3(n, y, z);
+  f[0] = n[0]; f[1] = n[1]; f[2] = n[2];
+  f[3] = y[0]; f[4] = y[1]; f[5] = y[2];
+  f[6] = z[0]; f[7] = z[1]; f[8] = z[2];
+}
+
+// mj_collision over the static pair table: bounding-sphere cull
+// (reference collision.hpp:301-305), narrow phase per lane, wave prefix scan
+// to compact contacts in pair order.
+KDEV int st_collision(const DModel& m, const Layout& L, float* w) {
+  int* misc = iptr(w, L.misc);
+  int st = 0;
+  if (m.disableflags & (DS_CONTACT | DS_CONSTRAINT)) {
+    if (LANE == 0) misc[MISC_NCON] = 0;
+    KSYNC();
+    return 0;
+  }
+  const float *gx = w + L.gxpos, *gm = w + L.gxmat;
+  float* con = w + L.con;
+  int base = 0;
+  for (int p0 = 0; p0 < m.npair; p0 += WAVE) {
+    int p = p0 + LANE;
+    RawC rc[MAXC_PAIR];
+    int cnt = 0;
+    if (p < m.npair) {
+      int g1 = m.pair_g1[p], g2 = m.pair_g2[p];
+      const float *x1 = gx + 3 * g1, *x2 = gx + 3 * g2;
+      float rb1 = m.geom_rbound[g1], rb2 = m.geom_rbound[g2];
+      float margin = m.pair_margin[p];
+      bool cull = false;
+      if (rb1 > 0.f && rb2 > 0.f) {
+        float dv[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
+        cull = sqrtf(dot3(dv, dv)) > rb1 + rb2 + margin;
+      }
+      if (!cull) cnt = k_narrow(m.geom_type[g1], m.geom_type[g2], m.geom_size + 3 * g1, m.geom_size + 3 * g2,
+                                x1, gm + 9 * g1, x2, gm + 9 * g2, margin, rc);
+    }
+    int badp = cnt < 0 ? 1 : 0;
+    if (cnt < 0) cnt = 0;
+    if (wori(badp)) st |= ST_BAD_PAIR;
+    int off = wscan_excl(cnt);
+    int tot = wsumi(cnt);
+    for (int k = 0; k < cnt; k++) {
+      int idx = base + off + k;
+      if (idx >= L.maxcon) break;
+      float* c = con + CON_STRIDE * idx;
+      c[CON_POS] = rc[k].pos[0]; c[CON_POS + 1] = rc[k].pos[1]; c[CON_POS + 2] = rc[k].pos[2];
+      k_make_frame(rc[k].n, c + CON_FRAME);
+      c[CON_DIST] = rc[k].dist;
+      iptr(c, CON_PAIR)[0] = p;
+    }
+    base += tot;
+  }
+  if (base > L.maxcon) { st |= ST_CON_OVERFLOW; base = L.maxcon; }
+  if (LANE == 0) misc[MISC_NCON] = base;
+  KSYNC();
+  return st;
+}
+
+// ------------------------------------------------------------------ constraints
+KDEV float k_impedance(const float* si, float x) {
+  float dmin = clampf(si[0], KMINIMP, KMAXIMP), dmax = clampf(si[1], KMINIMP, KMAXIMP);
+  float width = si[2], mid = clampf(si[3], KMINIMP, KMAXIMP), power = si[4] < 1.f ? 1.f : si[4];
+  if (dmin == dmax || width <= KMINVAL) return dmax;
+  float xx = fabsf(x) / width;
+  if (xx >= 1.f) return dmax;
+  float y;
+  if (power == 1.f) y = xx;
+  else if (xx <= mid) y = powf(xx, power) / powf(mid, power - 1.f);
+  else y = 1.f - powf(1.f - xx, power) / powf(1.f - mid, power - 1.f);
+  return dmin + y * (dmax - dmin);
+}
+
+// mj_makeConstraint + impedance + reference acceleration: dof friction rows,
+// joint-limit rows (one per violated side), contact rows (frictionless or
+// pyramidal). Row layout and formulas match the FP64 oracle; the
+// per-row quantities follow reference solver.hpp:120-139,299-371,398-587.
+KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
+  int* misc = iptr(w, L.misc);
+  int nv = m.nv;
+  int st = 0;
+  float *J = w + L.efc_J, *D = w + L.efc_D, *Rr = w + L.efc_R, *aref = w + L.efc_aref, *floss = w + L.efc_floss;
+  float *epos = w + L.efc_pos, *emarg = w + L.efc_margin;
+  int *etype = iptr(w, L.efc_type), *eid = iptr(w, L.efc_id);
+  const float *qpos = w + L.qpos, *qvel = w + L.qvel;
+  float* con = w + L.con;
+  int nrow = 0;
+  if (m.disableflags & DS_CONSTRAINT) {
+    if (LANE == 0) misc[MISC_NEFC] = 0;
+    KSYNC();
+    return 0;
+  }
+  // dof friction rows
+  int nf = (m.disableflags & DS_FRICTIONLOSS) ? 0 : m.nfloss;
+  if (nf > L.maxefc) { nf = L.maxefc; st |= ST_EFC_OVERFLOW; }
+  for (int t = LANE; t < nf; t += WAVE) {
+    int k = m.floss_dof[t];
+    etype[t] = EFC_FRICTION_DOF; eid[t] = k; epos[t] = 0.f; emarg[t] = 0.f; floss[t] = m.dof_frictionloss[k];
+  }
+  nrow = nf;
+  // joint limits: 0..2 rows per limited joint
+  if (!(m.disableflags & DS_LIMIT)) {
+    for (int t0 = 0; t0 < m.nlim; t0 += WAVE) {
+      int t = t0 + LANE;
+      int cnt = 0, j = -1;
+      float d0 = 0.f, d1 = 0.f;
+      bool s0 = false, s1 = false;
+      if (t < m.nlim) {
+        j = m.lim_jnt[t];
+        float v = qpos[m.jnt_qposadr[j]];
+        d0 = v - m.jnt_range[2 * j];
+        d1 = m.jnt_range[2 * j + 1] - v;
+        s0 = d0 < m.jnt_margin[j];
+        s1 = d1 < m.jnt_margin[j];
+        cnt = (int)s0 + (int)s1;
+      }
+      int off = wscan_excl(cnt), tot = wsumi(cnt);
+      int r = nrow + off;
+      if (s0 && r < L.maxefc) { etype[r] = EFC_LIMIT_JOINT; eid[r] = 2 * j; epos[r] = d0; emarg[r] = m.jnt_margin[j]; floss[r] = 0.f; }
+      if (s0) r++;
+      if (s1 && r < L.maxefc) { etype[r] = EFC_LIMIT_JOINT; eid[r] = 2 * j + 1; epos[r] = d1; emarg[r] = m.jnt_margin[j]; floss[r] = 0.f; }
+      nrow += tot;
+    }
+    if (nrow > L.maxefc) { nrow = L.maxefc; st |= ST_EFC_OVERFLOW; }
+  }
+  // contact rows: count per contact, compact
+  int ncon = misc[MISC_NCON];
+  if (!(m.disableflags & DS_CONTACT)) {
+    int cut = 0x7fffffff;
+    for (int c0 = 0; c0 < ncon; c0 += WAVE) {
+      int c = c0 + LANE;
+      int cnt = 0;
+      if (c < ncon) {
+        float* cr = con + CON_STRIDE * c;
+        int p = iptr(cr, CON_PAIR)[0];
+        float incl = m.pair_margin[p] - m.pair_gap[p];
+        int dim = m.pair_condim[p];
+        if (cr[CON_DIST] < incl) cnt = dim == 1 ? 1 : (m.cone == 0 ? 2 * (dim - 1) : dim);
+      }
+      int off = wscan_excl(cnt), tot = wsumi(cnt);
+      int dropr0 = 0x7fffffff;
+      if (c < ncon) {
+        float* cr = con + CON_STRIDE * c;
+        int r0 = nrow + off;
+        if (r0 + cnt > L.maxefc) { if (cnt > 0) dropr0 = r0; cnt = 0; }
+        iptr(cr, CON_NROW)[0] = cnt;
+        iptr(cr, CON_ROW0)[0] = r0;
+        int p = iptr(cr, CON_PAIR)[0];
+        int dim = m.pair_condim[p];
+        for (int k = 0; k < cnt; k++) {
+          etype[r0 + k] = dim == 1 ? EFC_CONTACT_FRICTIONLESS : (m.cone == 0 ? EFC_CONTACT_PYRAMIDAL : EFC_CONTACT_ELLIPTIC);
+          eid[r0 + k] = c; epos[r0 + k] = cr[CON_DIST]; emarg[r0 + k] = m.pair_margin[p] - m.pair_gap[p]; floss[r0 + k] = 0.f;
+        }
+      }
+      nrow += tot;
+      int dmin = wmini(dropr0);
+      if (dmin < cut) cut = dmin;
+    }
+    if (cut < nrow) { nrow = cut; st |= ST_EFC_OVERFLOW; }
+    if (nrow > L.maxefc) { nrow = L.maxefc; st |= ST_EFC_OVERFLOW; }
+  }
+  if (m.cone != 0) st |= ST_BAD_PAIR;  // elliptic cones are not supported by this kernel version
+  KSYNC();
+  // Jacobian rows
+  for (int t = LANE; t < nrow * nv; t += WAVE) J[t] = 0.f;
+  KSYNC();
+  int nlr = nrow;
+  for (int r = LANE; r < nlr; r += WAVE) {
+    int ty = etype[r];
+    if (ty == EFC_FRICTION_DOF) J[r * nv + eid[r]] = 1.f;
+    else if (ty == EFC_LIMIT_JOINT) {
+      int j = eid[r] >> 1;
+      J[r * nv + m.jnt_dofadr[j]] = (eid[r] & 1) ? -1.f : 1.f;
+    }
+  }
+  const float *cdof = w + L.cdof, *scom = w + L.scom;
+  for (int t = LANE; t < ncon * nv; t += WAVE) {
+    int c = t / nv, k = t - c * nv;
+    float* cr = con + CON_STRIDE * c;
+    int cnt = iptr(cr, CON_NROW)[0];
+    if (cnt == 0) continue;
+    int r0 = iptr(cr, CON_ROW0)[0];
+    int p = iptr(cr, CON_PAIR)[0];
+    int b1 = m.pair_b1[p], b2 = m.pair_b2[p];
+    float jp[3] = {0.f, 0.f, 0.f}, jr[3] = {0.f, 0.f, 0.f};
+    const float* cd = cdof + 6 * k;
+    uint64_t bit = (uint64_t)1 << k;
+    if (m.body_dofmask[b2] & bit) {
+      const float* cc = scom + 3 * m.body_rootid[b2];
+      float off[3] = {cr[0] - cc[0], cr[1] - cc[1], cr[2] - cc[2]}, tt[3];
+      cross3(cd, off, tt);
+      for (int i = 0; i < 3; i++) { jp[i] += cd[3 + i] + tt[i]; jr[i] += cd[i]; }
+    }
+    if (m.body_dofmask[b1] & bit) {
+      const float* cc = scom + 3 * m.body_rootid[b1];
+      float off[3] = {cr[0] - cc[0], cr[1] - cc[1], cr[2] - cc[2]}, tt[3];
+      cross3(cd, off, tt);
+      for (int i = 0; i < 3; i++) { jp[i] -= cd[3 + i] + tt[i]; jr[i] -= cd[i]; }
+    }
+    const float* fr = cr + CON_FRAME;
+    float Jn = dot3(fr, jp);
+    if (cnt == 1) { J[r0 * nv + k] = Jn; continue; }
+    float Jd[5] = {dot3(fr + 3, jp), dot3(fr + 6, jp), dot3(fr, jr), dot3(fr + 3, jr), dot3(fr + 6, jr)};
+    const float* mu = m.pair_friction + 5 * p;
+    for (int q = 0; q < cnt / 2; q++) {
+      J[(r0 + 2 * q) * nv + k] = Jn + mu[q] * Jd[q];
+      J[(r0 + 2 * q + 1) * nv + k] = Jn - mu[q] * Jd[q];
+    }
+  }
+  KSYNC();
+  // impedance, aref, regulariser
+  for (int r = LANE; r < nrow; r += WAVE) {
+    int ty = etype[r];
+    const float *sr, *si;
+    float dA;
+    float mu0 = 0.f;
+    if (ty == EFC_FRICTION_DOF) {
+      int k = eid[r];
+      sr = m.dof_solref + 2 * k; si = m.dof_solimp + 5 * k; dA = m.dof_invweight0[k];
+    } else if (ty == EFC_LIMIT_JOINT) {
+      int j = eid[r] >> 1;
+      sr = m.jnt_solref + 2 * j; si = m.jnt_solimp + 5 * j; dA = m.dof_invweight0[m.jnt_dofadr[j]];
+    } else {
+      int p = iptr(con + CON_STRIDE * eid[r], CON_PAIR)[0];
+      sr = m.pair_solref + 2 * p; si = m.pair_solimp + 5 * p;
+      float tran = m.body_invweight0[2 * m.pair_b1[p]] + m.body_invweight0[2 * m.pair_b2[p]];
+      mu0 = m.pair_friction[5 * p];
+      dA = ty == EFC_CONTACT_PYRAMIDAL ? tran * (1.f + mu0 * mu0) : tran;
+    }
+    float v = 0.f;
+    for (int k = 0; k < nv; k++) v += J[r * nv + k] * qvel[k];
+    float tc = sr[0], dr = sr[1];
+    float dmax = clampf(si[1], KMINIMP, KMAXIMP);
+    float kk, bb;
+    if (tc > 0.f) {
+      if (!(m.disableflags & DS_REFSAFE)) tc = fmaxf(tc, 2.f * m.timestep);
+      kk = 1.f / (dmax * dmax * tc * tc * dr * dr);
+      bb = 2.f / (dmax * tc);
+    } else {
+      kk = -tc / (dmax * dmax);
+      bb = -dr / dmax;
+    }
+    float x = epos[r] - emarg[r];
+    float imp = k_impedance(si, x);
+    aref[r] = -bb * v - kk * imp * x;
+    float R = fmaxf(dA * (1.f - imp) / imp, KMINVAL);
+    if (ty == EFC_CONTACT_PYRAMIDAL) R = fmaxf(2.f * mu0 * mu0 / m.impratio * R, KMINVAL);
+    Rr[r] = R;
+    D[r] = 1.f / R;
+  }
+  if (LANE == 0) misc[MISC_NEFC] = nrow;
+  KSYNC();
+  return st;
+}
+
+// ------------------------------------------------------------------ solver
+// y = M x (dense qM), lanes over rows
+KDEV void mulM(const DModel& m, const float* M, const float* x, float* y) {
+  int nv = m.nv;
+  for (int i = LANE; i < nv; i += WAVE) {
+    float s = 0.f;
+    for (int j = 0; j < nv; j++) s += M[i * nv + j] * x[j];
+    y[i] = s;
+  }
+}
+
+// qacc <- x; Ma = M x; Jaref = J x - aref
+KDEV void set_qacc(const DModel& m, const Layout& L, float* w, const float* x, int nefc) {
+  int nv = m.nv;
+  float *qacc = w + L.qacc, *Ma = w + L.Ma, *Jaref = w + L.efc_Jaref;
+  const float *J = w + L.efc_J, *aref = w + L.efc_aref, *M = w + L.qM;
+  for (int i = LANE; i < nv; i += WAVE) {
+    float s = 0.f;
+    for (int j = 0; j < nv; j++) s += M[i * nv + j] * x[j];
+    Ma[i] = s;
+  }
+  for (int r = LANE; r < nefc; r += WAVE) {
+    float s = 0.f;
+    for (int k = 0; k < nv; k++) s += J[r * nv + k] * x[k];
+    Jaref[r] = s - aref[r];
+  }
+  KSYNC();
+  for (int i = LANE; i < nv; i += WAVE) qacc[i] = x[i];
+  KSYNC();
+}
+
+// constraint forces / active flags / cost at the current Jaref, gradient;
+// returns the total cost (wave-uniform)
+KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc) {
+  int nv = m.nv;
+  const float *Jaref = w + L.efc_Jaref, *D = w + L.efc_D, *Rr = w + L.efc_R, *floss = w + L.efc_floss;
+  float* force = w + L.efc_force;
+  int *act = iptr(w, L.efc_act), *etype = iptr(w, L.efc_type);
+  float part = 0.f;
+  for (int r = LANE; r < nefc; r += WAVE) {
+    float x = Jaref[r], d = D[r];
+    int ty = etype[r];
+    if (ty == EFC_EQUALITY) { act[r] = 1; force[r] = -d * x; part += 0.5f * d * x * x; }
+    else if (ty == EFC_FRICTION_DOF) {
+      float f = floss[r], R = Rr[r];
+      if (x <= -R * f) { act[r] = 0; force[r] = f; part += -f * x - 0.5f * R * f * f; }
+      else if (x >= R * f) { act[r] = 0; force[r] = -f; part += f * x - 0.5f * R * f * f; }
+      else { act[r] = 1; force[r] = -d * x; part += 0.5f * d * x * x; }
+    } else {
+      if (x < 0.f) { act[r] = 1; force[r] = -d * x; part += 0.5f * d * x * x; }
+      else { act[r] = 0; force[r] = 0.f; }
+    }
+  }
+  const float *Ma = w + L.Ma, *qfs = w + L.qfrc_smooth, *qacc = w + L.qacc, *qas = w + L.qacc_smooth;
+  for (int i = LANE; i < nv; i += WAVE) part += 0.5f * (Ma[i] - qfs[i]) * (qacc[i] - qas[i]);
+  float cost = wsum(part);
+  KSYNC();
+  const float* J = w + L.efc_J;
+  float* grad = w + L.grad;
+  for (int i = LANE; i < nv; i += WAVE) {
+    float g = Ma[i] - qfs[i];
+    for (int r = 0; r < nefc; r++) g -= J[r * nv + i] * force[r];
+    grad[i] = g;
+  }
+  KSYNC();
+  return cost;
+}
+
+// H = M + J^T diag(D act) J (FP32 accumulate), dense Cholesky in FP64 (in
+// place, lower), then Mgrad = H^-1 grad solved in FP64 by one lane.
+KDEV int newton_direction(const DModel& m, const Layout& L, float* w, int nefc) {
+  int nv = m.nv;
+  double* H = dptr(w, L.H);
+  const float *M = w + L.qM, *J = w + L.efc_J, *D = w + L.efc_D;
+  const int* act = iptr(w, L.efc_act);
+  int ntri = nv * (nv + 1) / 2;
+  for (int t = LANE; t < ntri; t += WAVE) {
+    int i, j;
+    tri_rc(t, &i, &j);
+    float s = M[i * nv + j];
+    for (int r = 0; r < nefc; r++)
+      if (act[r]) s += D[r] * J[r * nv + i] * J[r * nv + j];
+    H[i * nv + j] = (double)s;
+  }
+  KSYNC();
+  int bad = 0;
+  for (int k = 0; k < nv; k++) {
+    double dkk = H[k * nv + k];
+    KSYNC();
+    if (!(dkk > 0.0)) { bad = 1; dkk = 1e-30; }
+    double lkk = sqrt(dkk), il = 1.0 / lkk;
+    for (int i = k + 1 + LANE; i < nv; i += WAVE) H[i * nv + k] *= il;
+    if (LANE == 0) H[k * nv + k] = lkk;
+    KSYNC();
+    int nt = (nv - k - 1) * (nv - k) / 2;
+    for (int t = LANE; t < nt; t += WAVE) {
+      int a, b;
+      tri_rc(t, &a, &b);
+      int i = k + 1 + a, j = k + 1 + b;
+      H[i * nv + j] -= H[i * nv + k] * H[j * nv + k];
+    }
+    KSYNC();
+  }
+  const float* grad = w + L.grad;
+  float* Mg = w + L.Mgrad;
+  if (LANE == 0) {
+    double* y = dptr(w, L.dvec);
+    for (int i = 0; i < nv; i++) {
+      double s = grad[i];
+      for (int q = 0; q < i; q++) s -= H[i * nv + q] * y[q];
+      y[i] = s / H[i * nv + i];
+    }
+    for (int i = nv - 1; i >= 0; i--) {
+      double s = y[i];
+      for (int q = i + 1; q < nv; q++) s -= H[q * nv + i] * y[q];
+      y[i] = s / H[i * nv + i];
+    }
+    for (int i = 0; i < nv; i++) Mg[i] = (float)y[i];
+  }
+  KSYNC();
+  return bad ? ST_NOT_PD : 0;
+}
+
+struct LsPt { float alpha, f, d1, d2; int nchange; };
+
+// cost along search at alpha; nchange counts rows whose zone differs from
+// their zone at ref_alpha (0 means one quadratic piece spans both points)
+KDEV LsPt ls_eval(const Layout& L, float* w, int nefc, float alpha, float ref_alpha, const float* gq) {
+  const float *Jaref = w + L.efc_Jaref, *Jv = w + L.efc_Jv, *D = w + L.efc_D, *Rr = w + L.efc_R, *floss = w + L.efc_floss;
+  const int* etype = iptr(w, L.efc_type);
+  float f = 0.f, d1 = 0.f, d2 = 0.f;
+  int nch = 0;
+  for (int r = LANE; r < nefc; r += WAVE) {
+    float jv = Jv[r];
+    float x = Jaref[r] + alpha * jv, xr = Jaref[r] + ref_alpha * jv, d = D[r];
+    int ty = etype[r];
+    if (ty == EFC_FRICTION_DOF) {
+      float fl = floss[r], R = Rr[r];
+      int z = x <= -R * fl ? 0 : (x >= R * fl ? 2 : 1);
+      int zr = xr <= -R * fl ? 0 : (xr >= R * fl ? 2 : 1);
+      nch += z != zr;
+      if (z == 0) { f += -fl * x - 0.5f * R * fl * fl; d1 += -fl * jv; }
+      else if (z == 2) { f += fl * x - 0.5f * R * fl * fl; d1 += fl * jv; }
+      else { f += 0.5f * d * x * x; d1 += d * x * jv; d2 += d * jv * jv; }
+    } else if (ty == EFC_EQUALITY) {
+      f += 0.5f * d * x * x; d1 += d * x * jv; d2 += d * jv * jv;
+    } else {
+      nch += (x < 0.f) != (xr < 0.f);
+      if (x < 0.f) { f += 0.5f * d * x * x; d1 += d * x * jv; d2 += d * jv * jv; }
+    }
+  }
+  LsPt p;
+  p.alpha = alpha;
+  p.f = wsum(f) + gq[0] + alpha * gq[1] + 0.5f * alpha * alpha * gq[2];
+  p.d1 = wsum(d1) + gq[1] + alpha * gq[2];
+  p.d2 = wsum(d2) + gq[2];
+  p.nchange = wsumi(nch);
+  return p;
+}
+
+// Exact line search on the convex piecewise-quadratic cost (same algorithm as
+// the oracle): Newton steps until the derivative changes sign, then Newton
+// from the left end / secant on the bracket. A Newton step whose end point is
+// in the same quadratic piece as its start is the exact minimiser.
+KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int* exact) {
+  int nv = m.nv;
+  const float *s = w + L.search, *M = w + L.qM, *J = w + L.efc_J;
+  float *Mv = w + L.Mv, *Jv = w + L.efc_Jv;
+  const float *Ma = w + L.Ma, *qfs = w + L.qfrc_smooth, *qacc = w + L.qacc, *qas = w + L.qacc_smooth;
+  float p0s = 0.f;
+  for (int i = LANE; i < nv; i += WAVE) p0s += s[i] * s[i];
+  float snorm = sqrtf(wsum(p0s));
+  if (!(snorm >= KMINVAL)) return 0.f;
+  float scale = 1.f / (m.meaninertia * (float)(nv > 1 ? nv : 1));
+  float gtol = m.tolerance * m.ls_tolerance * snorm / scale;
+  mulM(m, M, s, Mv);
+  for (int r = LANE; r < nefc; r += WAVE) {
+    float a = 0.f;
+    for (int k = 0; k < nv; k++) a += J[r * nv + k] * s[k];
+    Jv[r] = a;
+  }
+  KSYNC();
+  float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+  for (int i = LANE; i < nv; i += WAVE) {
+    g0 += 0.5f * (Ma[i] - qfs[i]) * (qacc[i] - qas[i]);
+    g1 += (Ma[i] - qfs[i]) * s[i];
+    g2 += s[i] * Mv[i];
+  }
+  float gq[3] = {wsum(g0), wsum(g1), wsum(g2)};
+  *exact = 0;
+  LsPt p0 = ls_eval(L, w, nefc, 0.f, 0.f, gq);
+  if (!(p0.d1 < 0.f)) return 0.f;
+  int it = 1, maxit = m.ls_iterations;
+  LsPt lo = p0;
+  LsPt pt = ls_eval(L, w, nefc, -p0.d1 / p0.d2, 0.f, gq);
+  if (pt.nchange == 0) { *exact = 1; return pt.alpha; }
+  if (fabsf(pt.d1) <= gtol) return pt.alpha;
+  while (pt.d1 < 0.f && it < maxit) {
+    lo = pt;
+    pt = ls_eval(L, w, nefc, pt.alpha - pt.d1 / pt.d2, lo.alpha, gq);
+    it++;
+    if (pt.nchange == 0 || fabsf(pt.d1) <= gtol) return pt.alpha;
+  }
+  if (pt.d1 < 0.f) return pt.alpha;
+  LsPt hi = pt;
+  while (it < maxit) {
+    float a = lo.alpha - lo.d1 / lo.d2;
+    bool newton = a > lo.alpha && a < hi.alpha;
+    if (!newton) a = lo.alpha - lo.d1 * (hi.alpha - lo.alpha) / (hi.d1 - lo.d1);
+    if (!(a > lo.alpha && a < hi.alpha)) a = 0.5f * (lo.alpha + hi.alpha);
+    pt = ls_eval(L, w, nefc, a, lo.alpha, gq);
+    it++;
+    if ((newton && pt.nchange == 0) || fabsf(pt.d1) <= gtol) return pt.alpha;
+    if (pt.d1 < 0.f) lo = pt; else hi = pt;
+    if (!(hi.alpha > lo.alpha)) break;
+  }
+  return lo.f <= hi.f ? lo.alpha : hi.alpha;
+}
+
+// mj_fwdConstraint: warm start choice, Newton or CG iterations with MuJoCo's
+// termination test, qfrc_constraint = J^T force.
+KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out) {
+  int* misc = iptr(w, L.misc);
+  int nv = m.nv;
+  int nefc = misc[MISC_NEFC];
+  float *qacc = w + L.qacc, *qfc = w + L.qfrc_constraint;
+  int st = 0;
+  *niter_out = 0;
+  if (nefc == 0) {
+    const float* qas = w + L.qacc_smooth;
+    for (int i = LANE; i < nv; i += WAVE) { qacc[i] = qas[i]; qfc[i] = 0.f; }
+    KSYNC();
+    return 0;
+  }
+  set_qacc(m, L, w, w + L.qacc_smooth, nefc);
+  float cost = cost_grad(m, L, w, nefc);
+  if (!(m.disableflags & DS_WARMSTART)) {
+    float cs = cost;
+    set_qacc(m, L, w, w + L.qacc_ws, nefc);
+    cost = cost_grad(m, L, w, nefc);
+    if (cost > cs) {
+      set_qacc(m, L, w, w + L.qacc_smooth, nefc);
+      cost = cost_grad(m, L, w, nefc);
+    }
+  }
+  float scale = 1.f / (m.meaninertia * (float)(nv > 1 ? nv : 1));
+  bool newton = m.solver == 2;
+  float *grad = w + L.grad, *Mg = w + L.Mgrad, *srch = w + L.search, *Ma = w + L.Ma, *Mv = w + L.Mv;
+  float *go = w + L.grad_old, *mgo = w + L.Mgrad_old, *Jaref = w + L.efc_Jaref;
+  const float* Jv = w + L.efc_Jv;
+  if (newton) st |= newton_direction(m, L, w, nefc);
+  else {
+    for (int i = LANE; i < nv; i += WAVE) Mg[i] = grad[i];
+    KSYNC();
+    solve_ld(m, w + L.qLD, Mg);
+  }
+  for (int i = LANE; i < nv; i += WAVE) srch[i] = -Mg[i];
+  KSYNC();
+  int it;
+  for (it = 0; it < m.iterations; it++) {
+    int exact = 0;
+    float alpha = linesearch(m, L, w, nefc, &exact);
+    *niter_out = it + 1;
+    if (alpha == 0.f) break;
+    for (int i = LANE; i < nv; i += WAVE) {
+      qacc[i] += alpha * srch[i];
+      Ma[i] += alpha * Mv[i];
+      go[i] = grad[i];
+      mgo[i] = Mg[i];
+    }
+    for (int r = LANE; r < nefc; r += WAVE) Jaref[r] += alpha * Jv[r];
+    KSYNC();
+    float oldcost = cost;
+    cost = cost_grad(m, L, w, nefc);
+    float gp = 0.f;
+    for (int i = LANE; i < nv; i += WAVE) gp += grad[i] * grad[i];
+    float improvement = scale * (oldcost - cost), gradient = scale * sqrtf(wsum(gp));
+    if (improvement < m.tolerance || gradient < m.tolerance) break;
+    // a Newton step that stayed inside one quadratic piece (no row changed
+    // zone) landed on that piece's stationary point: the exact minimiser
+    if (newton && exact) break;
+    if (newton) st |= newton_direction(m, L, w, nefc);
+    else {
+      for (int i = LANE; i < nv; i += WAVE) Mg[i] = grad[i];
+      KSYNC();
+      solve_ld(m, w + L.qLD, Mg);
+    }
+    if (newton) {
+      for (int i = LANE; i < nv; i += WAVE) srch[i] = -Mg[i];
+    } else {
+      float nump = 0.f, denp = 0.f;
+      for (int i = LANE; i < nv; i += WAVE) { nump += grad[i] * (Mg[i] - mgo[i]); denp += go[i] * mgo[i]; }
+      float num = wsum(nump), den = wsum(denp);
+      float beta = num / fmaxf(KMINVAL, den);
+      if (beta < 0.f) beta = 0.f;
+      for (int i = LANE; i < nv; i += WAVE) srch[i] = -Mg[i] + beta * srch[i];
+    }
+    KSYNC();
+  }
+  const float *J = w + L.efc_J, *force = w + L.efc_force;
+  for (int i = LANE; i < nv; i += WAVE) {
+    float s = 0.f;
+    for (int r = 0; r < nefc; r++) s += J[r * nv + i] * force[r];
+    qfc[i] = s;
+  }
+  KSYNC();
+  return st;
+}
+
+// ------------------------------------------------------------------ integration
+// Euler (implicit dof damping when present) / implicitfast, then the position
+// update; free and ball quaternions are integrated in FP64
+// (reference euler.hpp:26-110 restated with MuJoCo's mj_integratePos).
+KDEV int st_integrate(const DModel& m, const Layout& L, float* w) {
+  int nv = m.nv;
+  float *qvel = w + L.qvel, *qpos = w + L.qpos, *qacc = w + L.qacc, *ws = w + L.qacc_ws;
+  float* acc = w + L.tmpv;
+  float h = m.timestep;
+  int st = 0;
+  bool implicit = m.integrator == 2 || m.integrator == 3;
+  bool damp = m.has_damping && !(m.disableflags & DS_EULERDAMP);
+  if (implicit || damp) {
+    float* A = w + L.qLD;  // scratch: qLD of M is dead after the solver
+    float* Bm = w + L.qM;   // M + h*D built in place of qM (dead after the solver)
+    for (int k = LANE; k < nv; k += WAVE) Bm[k * nv + k] += h * m.dof_damping[k];
+    KSYNC();
+    if (implicit) {
+      const float* af = w + L.act_force;
+      for (int k = LANE; k < nv; k += WAVE) {
+        float add = 0.f;
+        for (int t = m.dof_act_adr[k]; t < m.dof_act_adr[k] + m.dof_act_num[k]; t++) {
+          int u = m.dof_act_list[t];
+          if (m.disableflags & DS_ACTUATION) break;
+          float f = af[u];
+          if (m.act_forcelimited[u] && (f <= m.act_forcerange[2 * u] || f >= m.act_forcerange[2 * u + 1])) continue;
+          add -= h * m.act_gear[u] * m.act_gear[u] * m.act_bias[3 * u + 2];
+        }
+        Bm[k * nv + k] += add;
+      }
+      KSYNC();
+    }
+    st |= factor_ld(m, Bm, A);
+    const float *qfs = w + L.qfrc_smooth, *qfc = w + L.qfrc_constraint;
+    for (int k = LANE; k < nv; k += WAVE) acc[k] = qfs[k] + qfc[k];
+    KSYNC();
+    solve_ld(m, A, acc);
+  } else {
+    for (int k = LANE; k < nv; k += WAVE) acc[k] = qacc[k];
+    KSYNC();
+  }
+  for (int k = LANE; k < nv; k += WAVE) qvel[k] += h * acc[k];
+  KSYNC();
+  for (int j = LANE; j < m.njnt; j += WAVE) {
+    int qa = m.jnt_qposadr[j], da = m.jnt_dofadr[j], jt = m.jnt_type[j];
+    if (jt == JFREE) {
+      for (int i = 0; i < 3; i++) qpos[qa + i] += h * qvel[da + i];
+      qa += 3; da += 3;
+    }
+    if (jt == JFREE || jt == JBALL) {
+      double wv[3] = {qvel[da], qvel[da + 1], qvel[da + 2]};
+      double wn = sqrt(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]);
+      double q[4] = {qpos[qa], qpos[qa + 1], qpos[qa + 2], qpos[qa + 3]};
+      double qn = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+      if (qn < 1e-15) { q[0] = 1; q[1] = q[2] = q[3] = 0; }
+      else for (int i = 0; i < 4; i++) q[i] /= qn;
+      double qr[4] = {1, 0, 0, 0};
+      if (wn >= 1e-15) {
+        double ang = 0.5 * (double)h * wn, sa = sin(ang) / wn;
+        qr[0] = cos(ang); qr[1] = wv[0] * sa; qr[2] = wv[1] * sa; qr[3] = wv[2] * sa;
+      }
+      double o0 = q[0] * qr[0] - q[1] * qr[1] - q[2] * qr[2] - q[3] * qr[3];
+      double o1 = q[0] * qr[1] + q[1] * qr[0] + q[2] * qr[3] - q[3] * qr[2];
+      double o2 = q[0] * qr[2] - q[1] * qr[3] + q[2] * qr[0] + q[3] * qr[1];
+      double o3 = q[0] * qr[3] + q[1] * qr[2] - q[2] * qr[1] + q[3] * qr[0];
+      double on = sqrt(o0 * o0 + o1 * o1 + o2 * o2 + o3 * o3);
+      qpos[qa] = (float)(o0 / on); qpos[qa + 1] = (float)(o1 / on);
+      qpos[qa + 2] = (float)(o2 / on); qpos[qa + 3] = (float)(o3 / on);
+    } else {
+      qpos[qa] += h * qvel[da];
+    }
+  }
+  for (int k = LANE; k < nv; k += WAVE) ws[k] = qacc[k];
+  KSYNC();
+  return st;
+}
+
+// ------------------------------------------------------------------ one env
+// aux: optional per-env outputs of the last substep's pre-integration state
+// (xpos, xquat, subtree_com, cinert, cvel, qfrc_actuator).
+struct AuxOut {
+  float *xpos, *xquat, *subtree_com, *cinert, *cvel, *qfrc_actuator;
+};
+
+// Global arrays of one launch. State arrays are env-major (env * dim + i), as
+// the C API exchanges them (reference include/mjmlx/mjmlx.h:137-138,206-216).
+struct StepIO {
+  float *qpos, *qvel, *qacc_ws;
+  const float* ctrl;
+  int *niter, *status, *ncon, *nefc;
+  float *dump1, *dump2;  // debug: workspace after constraint build / after solve (last substep)
+  AuxOut aux;
+  int has_aux;
+};
+
+KDEV void dump_ws(const Layout& L, const float* w, float* dst, int env) {
+  if (!dst) return;
+  for (int t = LANE; t < L.total; t += WAVE) dst[(size_t)env * L.total + t] = w[t];
+  KSYNC();
+}
+
+KDEV void write_aux(const DModel& m, const Layout& L, const float* w, const AuxOut& aux, int env) {
+  int nb = m.nbody;
+  for (int t = LANE; t < 3 * nb; t += WAVE) {
+    aux.xpos[(size_t)env * 3 * nb + t] = w[L.xpos + t];
+    aux.subtree_com[(size_t)env * 3 * nb + t] = w[L.scom + t];
+  }
+  for (int t = LANE; t < 4 * nb; t += WAVE) aux.xquat[(size_t)env * 4 * nb + t] = w[L.xquat + t];
+  for (int t = LANE; t < 10 * nb; t += WAVE) aux.cinert[(size_t)env * 10 * nb + t] = w[L.cinert + t];
+  for (int t = LANE; t < 6 * nb; t += WAVE) aux.cvel[(size_t)env * 6 * nb + t] = w[L.cvel + t];
+  for (int k = LANE; k < m.nv; k += WAVE) {
+    float s = 0.f;
+    for (int t = m.dof_act_adr[k]; t < m.dof_act_adr[k] + m.dof_act_num[k]; t++) {
+      int u = m.dof_act_list[t];
+      s += m.act_gear[u] * w[L.act_force + u];
+    }
+    aux.qfrc_actuator[(size_t)env * m.nv + k] = s;
+  }
+}
+
+// nsub physics substeps of one env with the same ctrl; state stays in the
+// workspace between substeps and goes back to HBM once at the end.
+KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, int env, int nsub) {
+  int nq = m.nq, nv = m.nv, nu = m.nu;
+  for (int i = LANE; i < nq; i += WAVE) w[L.qpos + i] = io.qpos[(size_t)env * nq + i];
+  for (int i = LANE; i < nv; i += WAVE) {
+    w[L.qvel + i] = io.qvel[(size_t)env * nv + i];
+    w[L.qacc_ws + i] = io.qacc_ws[(size_t)env * nv + i];
+  }
+  for (int i = LANE; i < nu; i += WAVE) w[L.ctrl + i] = io.ctrl ? io.ctrl[(size_t)env * nu + i] : 0.f;
+  KSYNC();
+  if (nsub == 0) {
+    // derived quantities of the current state only (no solve, no integration)
+    st_kinematics(m, L, w);
+    st_com_pos(m, L, w);
+    st_crb(m, L, w);
+    if (nv > 0) factor_ld(m, w + L.qM, w + L.qLD);
+    st_com_vel(m, L, w);
+    st_smooth(m, L, w);
+    if (io.has_aux) write_aux(m, L, w, io.aux, env);
+    return;
+  }
+  int st = 0, niter = 0;
+  for (int s = 0; s < nsub; s++) {
+    bool last = s == nsub - 1;
+    st_kinematics(m, L, w);
+    st_com_pos(m, L, w);
+    st_crb(m, L, w);
+    if (nv > 0) st |= factor_ld(m, w + L.qM, w + L.qLD);
+    st_com_vel(m, L, w);
+    st |= st_smooth(m, L, w);
+    if (last && io.has_aux) write_aux(m, L, w, io.aux, env);
+    st |= st_collision(m, L, w);
+    st |= st_constraints(m, L, w);
+    if (last) dump_ws(L, w, io.dump1, env);
+    int it = 0;
+    st |= st_solve(m, L, w, &it);
+    niter += it;
+    if (last) dump_ws(L, w, io.dump2, env);
+    st |= st_integrate(m, L, w);
+  }
+  int bad = 0;
+  for (int i = LANE; i < nq; i += WAVE) {
+    float v = w[L.qpos + i];
+    if (!(v == v) || fabsf(v) > 3.0e38f) bad = 1;
+    io.qpos[(size_t)env * nq + i] = v;
+  }
+  for (int i = LANE; i < nv; i += WAVE) {
+    float v = w[L.qvel + i];
+    if (!(v == v) || fabsf(v) > 3.0e38f) bad = 1;
+    io.qvel[(size_t)env * nv + i] = v;
+    io.qacc_ws[(size_t)env * nv + i] = w[L.qacc_ws + i];
+  }
+  if (wori(bad)) st |= ST_NONFINITE;
+  if (LANE == 0) {
+    const int* misc = reinterpret_cast<const int*>(w + L.misc);
+    if (io.niter) io.niter[env] = niter;
+    if (io.status) io.status[env] = st;
+    if (io.ncon) io.ncon[env] = misc[MISC_NCON];
+    if (io.nefc) io.nefc[env] = misc[MISC_NEFC];
+  }
+}
+
+}  // namespace mjk

@@ -1,0 +1,261 @@
+"""ctypes binding of the engine's C ABI (lib/libmjmlx.so).
+
+Mirrors the reference's Python caller surface (reference python/rl_bindings.cpp
+NB_MODULE at :223-332: load_model, create_batched, step(ctrl, n), state,
+set_state, sync) on top of the same C entry points a C caller uses
+(include/mjmlx/mjmlx.h). Loading fails loudly when the library is missing:
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBDIR = os.path.join(PKG, "lib")
+LIBPATH = os.path.join(LIBDIR, "libmjmlx.so")
+
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int)
+_D = C.POINTER(C.c_double)
+
+
+class ModelInfo(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nq", "nv", "nu", "na", "nbody", "njnt", "ngeom", "nsite", "ncon")]
+
+
+class ModelOptions(C.Structure):
+    _fields_ = [("self_collision", C.c_int), ("max_condim", C.c_int)]
+
+
+class BatchedConfig(C.Structure):
+    _fields_ = [("num_envs", C.c_int), ("integrator", C.c_int), ("use_gpu", C.c_int),
+                ("solver_iterations", C.c_int), ("domain_rand", C.c_int)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"HIP engine library not built: {LIBPATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIBPATH)
+        sig = {
+            "mjmlx_version": (C.c_char_p, []),
+            "mjmlx_hip_device_count": (C.c_int, []),
+            "mjmlx_load_model": (C.c_void_p, [C.c_char_p]),
+            "mjmlx_load_model_from_string": (C.c_void_p, [C.c_char_p]),
+            "mjmlx_load_model_from_string_ex": (C.c_void_p, [C.c_char_p, C.POINTER(ModelOptions)]),
+            "mjmlx_default_model_options": (ModelOptions, []),
+            "mjmlx_free_model": (None, [C.c_void_p]),
+            "mjmlx_model_info": (ModelInfo, [C.c_void_p]),
+            "mjmlx_model_opt_timestep": (C.c_float, [C.c_void_p]),
+            "mjmlx_model_body_mass": (C.c_float, [C.c_void_p, C.c_int]),
+            "mjmlx_name2id": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
+            "mjmlx_model_field": (C.c_int, [C.c_void_p, C.c_char_p, _D, C.c_int]),
+            "mjmlx_batched_create": (C.c_void_p, [C.c_void_p, C.POINTER(BatchedConfig)]),
+            "mjmlx_batched_free": (None, [C.c_void_p]),
+            "mjmlx_batched_step": (None, [C.c_void_p, _F]),
+            "mjmlx_batched_step_n": (None, [C.c_void_p, _F, C.c_int]),
+            "mjmlx_batched_reset": (None, [C.c_void_p, _I]),
+            "mjmlx_batched_get_state": (None, [C.c_void_p, _F, _F, _I, _I]),
+            "mjmlx_batched_set_state": (None, [C.c_void_p, _F, _F]),
+            "mjmlx_batched_get_qpos": (_F, [C.c_void_p, _I]),
+            "mjmlx_batched_get_qvel": (_F, [C.c_void_p, _I]),
+            "mjmlx_batched_get_xpos": (_F, [C.c_void_p, _I]),
+            "mjmlx_batched_get_xquat": (_F, [C.c_void_p, _I]),
+            "mjmlx_batched_get_subtree_com": (_F, [C.c_void_p, _I]),
+            "mjmlx_batched_get_cinert": (_F, [C.c_void_p, _I]),
+            "mjmlx_batched_get_cvel": (_F, [C.c_void_p, _I]),
+            "mjmlx_batched_get_qfrc_actuator": (_F, [C.c_void_p, _I]),
+            "mjmlx_batched_get_cfrc_ext": (_F, [C.c_void_p, _I]),
+            "mjmlx_batched_set_env_qpos": (None, [C.c_void_p, C.c_int, _F, C.c_int]),
+            "mjmlx_batched_set_env_qvel": (None, [C.c_void_p, C.c_int, _F, C.c_int]),
+            "mjmlx_batched_set_pack_spec": (C.c_int, [C.c_void_p, _I, C.c_int, C.c_int, C.c_int]),
+            "mjmlx_batched_get_pack": (C.c_int, [C.c_void_p, _F]),
+            "mjmlx_batched_eval_state": (None, [C.c_void_p]),
+            "mjmlx_batched_set_warmstart_mode": (C.c_int, [C.c_void_p, C.c_int]),
+            "mjmlx_batched_get_qacc_warmstart": (C.c_int, [C.c_void_p, _F]),
+            "mjmlx_batched_set_qacc_warmstart": (C.c_int, [C.c_void_p, _F]),
+            "mjmlx_batched_solver_stats": (C.c_int, [C.c_void_p, _I, _I, _I, _I]),
+            "mjmlx_batched_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+            "mjmlx_batched_last_step_ms": (C.c_float, [C.c_void_p]),
+            "mjmlx_batched_capacity": (C.c_int, [C.c_void_p, _I, _I, _I]),
+            "mjmlx_batched_debug_step": (C.c_int, [C.c_void_p, _F, _F, _F]),
+            "mjmlx_batched_layout": (C.c_int, [C.c_void_p, C.c_char_p, _I, _I, _I]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(_F)
+
+
+class Model:
+    """Compiled MJCF model (mjmlx_load_model_from_string_ex)."""
+
+    def __init__(self, xml: str, self_collision: bool = True, max_condim: int = 0):
+        o = ModelOptions(int(self_collision), int(max_condim))
+        self.h = lib().mjmlx_load_model_from_string_ex(xml.encode(), C.byref(o))
+        if not self.h:
+            raise RuntimeError("mjmlx_load_model_from_string_ex failed (see stderr)")
+        self.info = lib().mjmlx_model_info(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mjmlx_free_model(self.h)
+            self.h = None
+
+    @property
+    def nq(self):
+        return self.info.nq
+
+    @property
+    def nv(self):
+        return self.info.nv
+
+    @property
+    def nu(self):
+        return self.info.nu
+
+    def field(self, name: str) -> np.ndarray:
+        n = lib().mjmlx_model_field(self.h, name.encode(), None, 0)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(max(n, 1))
+        lib().mjmlx_model_field(self.h, name.encode(), out.ctypes.data_as(_D), n)
+        return out[:n]
+
+
+INTEGRATORS = {"Euler": 0, "RK4": 1, "implicit": 2, "implicitfast": 3}
+
+
+class BatchedSim:
+    """num_envs copies of a model stepped on the GPU (mjmlx_batched_*)."""
+
+    def __init__(self, model: Model, num_envs: int, integrator: int | None = None, solver_iterations: int = 0):
+        if integrator is None:
+            integrator = int(model.field("integrator")[0])
+        cfg = BatchedConfig(num_envs, integrator, 1, solver_iterations, 0)
+        self.model = model
+        self.h = lib().mjmlx_batched_create(model.h, C.byref(cfg))
+        if not self.h:
+            raise RuntimeError("mjmlx_batched_create failed (see stderr)")
+        self.B = num_envs
+        self.nq, self.nv, self.nu = model.nq, model.nv, model.nu
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mjmlx_batched_free(self.h)
+            self.h = None
+
+    def step(self, ctrl=None, n: int = 1):
+        c = None if ctrl is None else np.ascontiguousarray(ctrl, dtype=np.float32)
+        if c is not None:
+            assert c.size == self.B * self.nu, (c.shape, self.B, self.nu)
+        lib().mjmlx_batched_step_n(self.h, _fp(c), n)
+
+    def sync(self):
+        lib().mjmlx_batched_eval_state(self.h)
+
+    def state(self):
+        qp = np.zeros((self.B, self.nq), np.float32)
+        qv = np.zeros((self.B, self.nv), np.float32)
+        lib().mjmlx_batched_get_state(self.h, _fp(qp), _fp(qv), None, None)
+        return qp, qv
+
+    def set_state(self, qpos=None, qvel=None):
+        qp = None if qpos is None else np.ascontiguousarray(qpos, dtype=np.float32)
+        qv = None if qvel is None else np.ascontiguousarray(qvel, dtype=np.float32)
+        lib().mjmlx_batched_set_state(self.h, _fp(qp), _fp(qv))
+
+    def qacc_warmstart(self):
+        out = np.zeros((self.B, self.nv), np.float32)
+        assert lib().mjmlx_batched_get_qacc_warmstart(self.h, _fp(out)) == 0
+        return out
+
+    def set_qacc_warmstart(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert lib().mjmlx_batched_set_qacc_warmstart(self.h, _fp(a)) == 0
+
+    def reset(self, mask):
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        lib().mjmlx_batched_reset(self.h, m.ctypes.data_as(_I))
+
+    def stats(self):
+        out = [np.zeros(self.B, np.int32) for _ in range(4)]
+        assert lib().mjmlx_batched_solver_stats(self.h, *(o.ctypes.data_as(_I) for o in out)) == 0
+        return dict(niter=out[0], status=out[1], ncon=out[2], nefc=out[3])
+
+    def set_launch(self, persistent: bool, grid: int = 0):
+        assert lib().mjmlx_batched_set_launch(self.h, int(persistent), grid) == 0
+
+    def set_warmstart_mode(self, mode: int):
+        return lib().mjmlx_batched_set_warmstart_mode(self.h, mode)
+
+    def last_step_ms(self) -> float:
+        return lib().mjmlx_batched_last_step_ms(self.h)
+
+    def capacity(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        lib().mjmlx_batched_capacity(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def layout(self, name: str):
+        off, ln, kind = C.c_int(), C.c_int(), C.c_int()
+        if lib().mjmlx_batched_layout(self.h, name.encode(), C.byref(off), C.byref(ln), C.byref(kind)) != 0:
+            raise KeyError(name)
+        return off.value, ln.value, kind.value
+
+    def debug_step(self, ctrl=None):
+        total = self.layout("total")[1]
+        d1 = np.zeros(self.B * total, np.float32)
+        d2 = np.zeros(self.B * total, np.float32)
+        c = None if ctrl is None else np.ascontiguousarray(ctrl, dtype=np.float32)
+        assert lib().mjmlx_batched_debug_step(self.h, _fp(c), _fp(d1), _fp(d2)) == 0
+        return DebugDump(self, d1, d2, total)
+
+    def _getter(self, fn: str, shape):
+        n = C.c_int()
+        p = getattr(lib(), fn)(self.h, C.byref(n))
+        if not p:
+            raise RuntimeError(f"{fn} failed")
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy().reshape(shape)
+
+    def xpos(self):
+        return self._getter("mjmlx_batched_get_xpos", (self.B, -1, 3))
+
+    def xquat(self):
+        return self._getter("mjmlx_batched_get_xquat", (self.B, -1, 4))
+
+    def subtree_com(self):
+        return self._getter("mjmlx_batched_get_subtree_com", (self.B, -1, 3))
+
+    def cinert(self):
+        return self._getter("mjmlx_batched_get_cinert", (self.B, -1, 10))
+
+    def cvel(self):
+        return self._getter("mjmlx_batched_get_cvel", (self.B, -1, 6))
+
+    def qfrc_actuator(self):
+        return self._getter("mjmlx_batched_get_qfrc_actuator", (self.B, -1))
+
+
+class DebugDump:
+    def __init__(self, sim: BatchedSim, d1, d2, total):
+        self.sim, self.d = sim, {1: d1, 2: d2}
+        self.total = total
+
+    def get(self, which: int, env: int, name: str) -> np.ndarray:
+        off, ln, kind = self.sim.layout(name)
+        seg = self.d[which][env * self.total + off: env * self.total + off + ln].copy()
+        return seg.view(np.int32) if kind == 1 else seg

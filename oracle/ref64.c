@@ -1,0 +1,1509 @@
+/* ref64.c -- FP64 CPU oracle for the batched MuJoCo step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker; see ref64.h). Restates the MuJoCo C
+ * 3.5.0 step that the reference treats as ground truth (reference
+ * src/mjb.cpp:807-839, tests/test_gpu_vs_cpu.cpp:177-267) in plain C double
+ * precision. Where the reference's own kernels restate a MuJoCo formula the
+ * citation points there; where the reference departs from MuJoCo (single
+ * contact per pair, no warm start, its own solver tolerances: reference
+ * src/compat/mkx_kernels/collision.hpp:493-500, solver.hpp:41-45,592) this file
+ * follows MuJoCo instead.
+ *
+ * Parity status of each part (how it is pinned) is listed in DESIGN.md of the
+ * repository and checked by tests/test_oracle.py.
+ */
+#include "ref64.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MINVAL 1e-15
+#define MINIMP 0.0001
+#define MAXIMP 0.9999
+
+enum { JFREE = 0, JBALL = 1, JSLIDE = 2, JHINGE = 3 };
+enum { GPLANE = 0, GSPHERE = 2, GCAPSULE = 3, GELLIPSOID = 4, GCYLINDER = 5, GBOX = 6 };
+enum { EFC_EQUALITY = 0, EFC_FRICTION_DOF = 1, EFC_LIMIT_JOINT = 3, EFC_CONTACT_FRICTIONLESS = 5,
+       EFC_CONTACT_PYRAMIDAL = 6, EFC_CONTACT_ELLIPTIC = 7 };
+#define DS_CONSTRAINT (1 << 0)
+#define DS_FRICTIONLOSS (1 << 2)
+#define DS_LIMIT (1 << 3)
+#define DS_CONTACT (1 << 4)
+#define DS_PASSIVE (1 << 5)
+#define DS_GRAVITY (1 << 6)
+#define DS_CLAMPCTRL (1 << 7)
+#define DS_WARMSTART (1 << 8)
+#define DS_FILTERPARENT (1 << 9)
+#define DS_ACTUATION (1 << 10)
+#define DS_REFSAFE (1 << 11)
+#define DS_EULERDAMP (1 << 15)
+
+struct orc_prep {
+  orc_model_t m;
+  int npair;
+  orc_pair_t* pairs;
+  double* body_invweight0;
+  double* dof_invweight0;
+  double* body_subtreemass;
+  double meaninertia;
+};
+
+struct orc_data {
+  int nq, nv, nu, nbody, njnt, ngeom;
+  double *qpos, *qvel, *ctrl, *qacc_ws, *qacc, *qacc_smooth;
+  double *xpos, *xquat, *xmat, *xipos, *ximat, *xanchor, *xaxis, *geom_xpos, *geom_xmat;
+  double *subtree_com, *cinert, *cdof, *cvel, *cdof_dot, *crb, *cacc, *cfrc;
+  double *qM, *qLD;
+  double *qfrc_bias, *qfrc_passive, *qfrc_actuator, *qfrc_smooth, *qfrc_constraint, *actuator_force;
+  int ncon;
+  orc_contact_t con[ORC_MAXCON];
+  int nefc;
+  int efc_type[ORC_MAXEFC], efc_id[ORC_MAXEFC];
+  double *efc_J;
+  double efc_pos[ORC_MAXEFC], efc_margin[ORC_MAXEFC], efc_D[ORC_MAXEFC], efc_R[ORC_MAXEFC],
+      efc_aref[ORC_MAXEFC], efc_force[ORC_MAXEFC], efc_floss[ORC_MAXEFC], efc_vel[ORC_MAXEFC],
+      efc_diagApprox[ORC_MAXEFC], efc_Jaref[ORC_MAXEFC], efc_Jv[ORC_MAXEFC], efc_imp[ORC_MAXEFC];
+  int efc_active[ORC_MAXEFC];
+  int niter;
+  int status;
+  /* scratch */
+  double *Ma, *grad, *Mgrad, *search, *Mv, *H, *tmp1, *tmp2;
+  double cost;
+};
+
+/* ------------------------------------------------------------------ math */
+static void quat2mat(const double q[4], double R[9]) {
+  double w = q[0], x = q[1], y = q[2], z = q[3];
+  R[0] = w * w + x * x - y * y - z * z; R[1] = 2 * (x * y - w * z); R[2] = 2 * (x * z + w * y);
+  R[3] = 2 * (x * y + w * z); R[4] = w * w - x * x + y * y - z * z; R[5] = 2 * (y * z - w * x);
+  R[6] = 2 * (x * z - w * y); R[7] = 2 * (y * z + w * x); R[8] = w * w - x * x - y * y + z * z;
+}
+static void mulquat(const double a[4], const double b[4], double o[4]) {
+  double r0 = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+  double r1 = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+  double r2 = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+  double r3 = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+  o[0] = r0; o[1] = r1; o[2] = r2; o[3] = r3;
+}
+static double normalize4(double q[4]) {
+  double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  if (n < MINVAL) { q[0] = 1; q[1] = q[2] = q[3] = 0; return 0; }
+  q[0] /= n; q[1] /= n; q[2] /= n; q[3] /= n;
+  return n;
+}
+static double normalize3(double v[3]) {
+  double n = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (n < MINVAL) { v[0] = 1; v[1] = v[2] = 0; return 0; }
+  v[0] /= n; v[1] /= n; v[2] /= n;
+  return n;
+}
+static void mulmatvec3(const double R[9], const double v[3], double o[3]) {
+  double a = R[0] * v[0] + R[1] * v[1] + R[2] * v[2];
+  double b = R[3] * v[0] + R[4] * v[1] + R[5] * v[2];
+  double c = R[6] * v[0] + R[7] * v[1] + R[8] * v[2];
+  o[0] = a; o[1] = b; o[2] = c;
+}
+static void mulmatTvec3(const double R[9], const double v[3], double o[3]) {
+  double a = R[0] * v[0] + R[3] * v[1] + R[6] * v[2];
+  double b = R[1] * v[0] + R[4] * v[1] + R[7] * v[2];
+  double c = R[2] * v[0] + R[5] * v[1] + R[8] * v[2];
+  o[0] = a; o[1] = b; o[2] = c;
+}
+static void cross3(const double a[3], const double b[3], double o[3]) {
+  double x = a[1] * b[2] - a[2] * b[1], y = a[2] * b[0] - a[0] * b[2], z = a[0] * b[1] - a[1] * b[0];
+  o[0] = x; o[1] = y; o[2] = z;
+}
+static double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static void axisangle2quat(const double ax[3], double a, double q[4]) {
+  double s = sin(0.5 * a);
+  q[0] = cos(0.5 * a); q[1] = ax[0] * s; q[2] = ax[1] * s; q[3] = ax[2] * s;
+}
+static double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+/* spatial inertia (10-vector about a point) times motion vector [ang; lin] */
+static void mul_inert_vec(const double* i, const double* v, double* r) {
+  r[0] = i[0] * v[0] + i[3] * v[1] + i[4] * v[2] - i[8] * v[4] + i[7] * v[5];
+  r[1] = i[3] * v[0] + i[1] * v[1] + i[5] * v[2] + i[8] * v[3] - i[6] * v[5];
+  r[2] = i[4] * v[0] + i[5] * v[1] + i[2] * v[2] - i[7] * v[3] + i[6] * v[4];
+  r[3] = i[8] * v[1] - i[7] * v[2] + i[9] * v[3];
+  r[4] = i[6] * v[2] - i[8] * v[0] + i[9] * v[4];
+  r[5] = i[7] * v[0] - i[6] * v[1] + i[9] * v[5];
+}
+/* motion cross motion: v x u */
+static void cross_motion(const double* v, const double* u, double* r) {
+  double a[3], b[3], c[3];
+  cross3(v, u, a);
+  cross3(v, u + 3, b);
+  cross3(v + 3, u, c);
+  r[0] = a[0]; r[1] = a[1]; r[2] = a[2];
+  r[3] = b[0] + c[0]; r[4] = b[1] + c[1]; r[5] = b[2] + c[2];
+}
+/* motion cross force: v x* f */
+static void cross_force(const double* v, const double* f, double* r) {
+  double a[3], b[3], c[3];
+  cross3(v, f, a);
+  cross3(v + 3, f + 3, b);
+  cross3(v, f + 3, c);
+  r[0] = a[0] + b[0]; r[1] = a[1] + b[1]; r[2] = a[2] + b[2];
+  r[3] = c[0]; r[4] = c[1]; r[5] = c[2];
+}
+
+/* ------------------------------------------------------------------ prepare */
+static int excluded(const orc_model_t* m, int b1, int b2) {
+  int lo = b1 < b2 ? b1 : b2, hi = b1 < b2 ? b2 : b1;
+  int sig = (lo << 16) | hi;
+  for (int i = 0; i < m->nexclude; i++) if (m->exclude_signature[i] == sig) return 1;
+  return 0;
+}
+
+/* Static pair table: contype/conaffinity mask, weld, filterparent, exclude,
+ * then MuJoCo's parameter mixing (priority, solmix, max friction/condim).
+ * Follows reference src/io.cpp:922-959 (mixing) and :961-1070 (filter). */
+static void build_pairs(orc_prep_t* p) {
+  const orc_model_t* m = &p->m;
+  int cap = m->ngeom * (m->ngeom - 1) / 2 + m->npair + 1;
+  p->pairs = (orc_pair_t*)calloc(cap, sizeof(orc_pair_t));
+  p->npair = 0;
+  if (m->disableflags & (DS_CONTACT | DS_CONSTRAINT)) return;
+  for (int i = 0; i < m->ngeom; i++) {
+    for (int j = i + 1; j < m->ngeom; j++) {
+      int g1 = i, g2 = j;
+      if (m->geom_type[g1] > m->geom_type[g2]) { g1 = j; g2 = i; }
+      int mask = (m->geom_contype[g1] & m->geom_conaffinity[g2]) | (m->geom_contype[g2] & m->geom_conaffinity[g1]);
+      if (!mask) continue;
+      int b1 = m->geom_bodyid[g1], b2 = m->geom_bodyid[g2];
+      int w1 = m->body_weldid[b1], w2 = m->body_weldid[b2];
+      if (w1 == w2) continue;
+      if (!(m->disableflags & DS_FILTERPARENT)) {
+        int w1p = m->body_weldid[m->body_parentid[w1]], w2p = m->body_weldid[m->body_parentid[w2]];
+        if (w1 != 0 && w2 != 0 && (w1 == w2p || w2 == w1p)) continue;
+      }
+      if (excluded(m, b1, b2)) continue;
+      if (m->geom_type[g1] == GPLANE && m->geom_type[g2] == GPLANE) continue;
+      orc_pair_t* q = &p->pairs[p->npair++];
+      q->g1 = g1; q->g2 = g2; q->b1 = b1; q->b2 = b2;
+      q->margin = fmax(m->geom_margin[g1], m->geom_margin[g2]);
+      q->gap = fmax(m->geom_gap[g1], m->geom_gap[g2]);
+      double fr[3];
+      int p1 = m->geom_priority[g1], p2 = m->geom_priority[g2];
+      if (p1 != p2) {
+        int gp = p1 > p2 ? g1 : g2;
+        q->condim = m->geom_condim[gp];
+        for (int k = 0; k < 3; k++) fr[k] = m->geom_friction[3 * gp + k];
+        for (int k = 0; k < 2; k++) q->solref[k] = m->geom_solref[2 * gp + k];
+        for (int k = 0; k < 5; k++) q->solimp[k] = m->geom_solimp[5 * gp + k];
+      } else {
+        q->condim = m->geom_condim[g1] > m->geom_condim[g2] ? m->geom_condim[g1] : m->geom_condim[g2];
+        double s1 = m->geom_solmix[g1], s2 = m->geom_solmix[g2], mix;
+        if (s1 >= MINVAL && s2 >= MINVAL) mix = s1 / (s1 + s2);
+        else if (s1 < MINVAL && s2 < MINVAL) mix = 0.5;
+        else mix = s1 < MINVAL ? 0.0 : 1.0;
+        const double *r1 = m->geom_solref + 2 * g1, *r2 = m->geom_solref + 2 * g2;
+        if (r1[0] > 0 && r2[0] > 0) for (int k = 0; k < 2; k++) q->solref[k] = mix * r1[k] + (1 - mix) * r2[k];
+        else for (int k = 0; k < 2; k++) q->solref[k] = fmin(r1[k], r2[k]);
+        for (int k = 0; k < 5; k++) q->solimp[k] = mix * m->geom_solimp[5 * g1 + k] + (1 - mix) * m->geom_solimp[5 * g2 + k];
+        for (int k = 0; k < 3; k++) fr[k] = fmax(m->geom_friction[3 * g1 + k], m->geom_friction[3 * g2 + k]);
+      }
+      q->friction[0] = fr[0]; q->friction[1] = fr[0]; q->friction[2] = fr[1]; q->friction[3] = fr[2]; q->friction[4] = fr[2];
+    }
+  }
+  for (int k = 0; k < m->npair; k++) {
+    int g1 = m->pair_geom1[k], g2 = m->pair_geom2[k];
+    if (m->geom_type[g1] > m->geom_type[g2]) { int t = g1; g1 = g2; g2 = t; }
+    orc_pair_t q;
+    memset(&q, 0, sizeof(q));
+    q.g1 = g1; q.g2 = g2; q.b1 = m->geom_bodyid[g1]; q.b2 = m->geom_bodyid[g2];
+    q.condim = m->pair_dim[k]; q.margin = m->pair_margin[k]; q.gap = m->pair_gap[k];
+    for (int i = 0; i < 5; i++) { q.friction[i] = m->pair_friction[5 * k + i]; q.solimp[i] = m->pair_solimp[5 * k + i]; }
+    for (int i = 0; i < 2; i++) q.solref[i] = m->pair_solref[2 * k + i];
+    int found = -1;
+    for (int e = 0; e < p->npair; e++) if (p->pairs[e].g1 == g1 && p->pairs[e].g2 == g2) found = e;
+    if (found >= 0) p->pairs[found] = q;
+    else p->pairs[p->npair++] = q;
+  }
+}
+
+static void data_alloc(orc_data_t* d, const orc_model_t* m) {
+  d->nq = m->nq; d->nv = m->nv; d->nu = m->nu; d->nbody = m->nbody; d->njnt = m->njnt; d->ngeom = m->ngeom;
+  int nv = m->nv > 0 ? m->nv : 1, nq = m->nq > 0 ? m->nq : 1, nu = m->nu > 0 ? m->nu : 1;
+  int nb = m->nbody, nj = m->njnt > 0 ? m->njnt : 1, ng = m->ngeom > 0 ? m->ngeom : 1;
+#define AL(f, n) d->f = (double*)calloc((size_t)(n), sizeof(double))
+  AL(qpos, nq); AL(qvel, nv); AL(ctrl, nu); AL(qacc_ws, nv); AL(qacc, nv); AL(qacc_smooth, nv);
+  AL(xpos, 3 * nb); AL(xquat, 4 * nb); AL(xmat, 9 * nb); AL(xipos, 3 * nb); AL(ximat, 9 * nb);
+  AL(xanchor, 3 * nj); AL(xaxis, 3 * nj); AL(geom_xpos, 3 * ng); AL(geom_xmat, 9 * ng);
+  AL(subtree_com, 3 * nb); AL(cinert, 10 * nb); AL(cdof, 6 * nv); AL(cvel, 6 * nb); AL(cdof_dot, 6 * nv);
+  AL(crb, 10 * nb); AL(cacc, 6 * nb); AL(cfrc, 6 * nb);
+  AL(qM, nv * nv); AL(qLD, nv * nv);
+  AL(qfrc_bias, nv); AL(qfrc_passive, nv); AL(qfrc_actuator, nv); AL(qfrc_smooth, nv); AL(qfrc_constraint, nv);
+  AL(actuator_force, nu);
+  AL(efc_J, (size_t)ORC_MAXEFC * nv);
+  AL(Ma, nv); AL(grad, nv); AL(Mgrad, nv); AL(search, nv); AL(Mv, nv); AL(H, nv * nv); AL(tmp1, nv); AL(tmp2, nv);
+#undef AL
+}
+
+static void free_data_arrays(orc_data_t* d) {
+  double* ptrs[] = {d->qpos, d->qvel, d->ctrl, d->qacc_ws, d->qacc, d->qacc_smooth, d->xpos, d->xquat, d->xmat,
+                    d->xipos, d->ximat, d->xanchor, d->xaxis, d->geom_xpos, d->geom_xmat, d->subtree_com,
+                    d->cinert, d->cdof, d->cvel, d->cdof_dot, d->crb, d->cacc, d->cfrc, d->qM, d->qLD,
+                    d->qfrc_bias, d->qfrc_passive, d->qfrc_actuator, d->qfrc_smooth, d->qfrc_constraint,
+                    d->actuator_force, d->efc_J, d->Ma, d->grad, d->Mgrad, d->search, d->Mv, d->H, d->tmp1, d->tmp2};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+}
+
+/* forward declarations of the pipeline pieces used by orc_prepare */
+static void kinematics(const orc_prep_t* p, orc_data_t* d);
+static void com_pos(const orc_prep_t* p, orc_data_t* d);
+static void crb_mass(const orc_prep_t* p, orc_data_t* d);
+static int factor_m(const orc_prep_t* p, orc_data_t* d);
+static void solve_m(const orc_prep_t* p, const orc_data_t* d, double* x);
+
+orc_prep_t* orc_prepare(const orc_model_t* m) {
+  orc_prep_t* p = (orc_prep_t*)calloc(1, sizeof(orc_prep_t));
+  p->m = *m;
+  int nb = m->nbody, nv = m->nv;
+  p->body_subtreemass = (double*)calloc(nb, sizeof(double));
+  for (int b = 0; b < nb; b++) p->body_subtreemass[b] = m->body_mass[b];
+  for (int b = nb - 1; b > 0; b--) p->body_subtreemass[m->body_parentid[b]] += p->body_subtreemass[b];
+  build_pairs(p);
+  p->body_invweight0 = (double*)calloc(2 * nb, sizeof(double));
+  p->dof_invweight0 = (double*)calloc(nv > 0 ? nv : 1, sizeof(double));
+  p->meaninertia = 1.0;
+  if (nv == 0) return p;
+  /* mj_setConst subset: kinematics and M at qpos0 */
+  orc_data_t* d = orc_data_new(p);
+  memcpy(d->qpos, m->qpos0, sizeof(double) * m->nq);
+  kinematics(p, d);
+  com_pos(p, d);
+  crb_mass(p, d);
+  if (factor_m(p, d) != 0) { orc_data_free(d); orc_free(p); return NULL; }
+  double tr = 0;
+  for (int i = 0; i < nv; i++) tr += d->qM[i * nv + i];
+  p->meaninertia = tr / nv;
+  if (p->meaninertia < MINVAL) p->meaninertia = 1.0;
+  /* columns of M^-1 */
+  double* Minv = (double*)calloc((size_t)nv * nv, sizeof(double));
+  double* e = (double*)calloc(nv, sizeof(double));
+  for (int c = 0; c < nv; c++) {
+    memset(e, 0, sizeof(double) * nv);
+    e[c] = 1;
+    solve_m(p, d, e);
+    for (int r = 0; r < nv; r++) Minv[r * nv + c] = e[r];
+  }
+  /* body invweight0: mean translational / rotational diagonal of J M^-1 J^T at the body COM */
+  double* J = (double*)calloc((size_t)6 * nv, sizeof(double));
+  for (int b = 1; b < nb; b++) {
+    if (m->body_weldid[b] == 0) continue;
+    memset(J, 0, sizeof(double) * 6 * nv);
+    int bb = b;
+    while (bb > 0 && m->body_dofnum[bb] == 0) bb = m->body_parentid[bb];
+    if (bb == 0) continue;
+    const double* c = d->subtree_com + 3 * m->body_rootid[b];
+    double off[3] = {d->xipos[3 * b] - c[0], d->xipos[3 * b + 1] - c[1], d->xipos[3 * b + 2] - c[2]};
+    for (int dd = m->body_dofadr[bb] + m->body_dofnum[bb] - 1; dd >= 0; dd = m->dof_parentid[dd]) {
+      const double* cd = d->cdof + 6 * dd;
+      double t[3];
+      cross3(cd, off, t);
+      J[0 * nv + dd] = cd[3] + t[0]; J[1 * nv + dd] = cd[4] + t[1]; J[2 * nv + dd] = cd[5] + t[2];
+      J[3 * nv + dd] = cd[0]; J[4 * nv + dd] = cd[1]; J[5 * nv + dd] = cd[2];
+    }
+    double diag[6];
+    for (int r = 0; r < 6; r++) {
+      double s = 0;
+      for (int i = 0; i < nv; i++) for (int k = 0; k < nv; k++) s += J[r * nv + i] * Minv[i * nv + k] * J[r * nv + k];
+      diag[r] = s;
+    }
+    p->body_invweight0[2 * b] = (diag[0] + diag[1] + diag[2]) / 3.0;
+    p->body_invweight0[2 * b + 1] = (diag[3] + diag[4] + diag[5]) / 3.0;
+  }
+  for (int j = 0; j < m->njnt; j++) {
+    int a = m->jnt_dofadr[j];
+    if (m->jnt_type[j] == JFREE) {
+      double t = (Minv[a * nv + a] + Minv[(a + 1) * nv + a + 1] + Minv[(a + 2) * nv + a + 2]) / 3.0;
+      double r = (Minv[(a + 3) * nv + a + 3] + Minv[(a + 4) * nv + a + 4] + Minv[(a + 5) * nv + a + 5]) / 3.0;
+      for (int i = 0; i < 3; i++) { p->dof_invweight0[a + i] = t; p->dof_invweight0[a + 3 + i] = r; }
+    } else if (m->jnt_type[j] == JBALL) {
+      double r = (Minv[a * nv + a] + Minv[(a + 1) * nv + a + 1] + Minv[(a + 2) * nv + a + 2]) / 3.0;
+      for (int i = 0; i < 3; i++) p->dof_invweight0[a + i] = r;
+    } else {
+      p->dof_invweight0[a] = Minv[a * nv + a];
+    }
+  }
+  free(J); free(e); free(Minv);
+  orc_data_free(d);
+  return p;
+}
+
+void orc_free(orc_prep_t* p) {
+  if (!p) return;
+  free(p->pairs); free(p->body_invweight0); free(p->dof_invweight0); free(p->body_subtreemass);
+  free(p);
+}
+int orc_npair(const orc_prep_t* p) { return p->npair; }
+const orc_pair_t* orc_pairs(const orc_prep_t* p) { return p->pairs; }
+const double* orc_body_invweight0(const orc_prep_t* p) { return p->body_invweight0; }
+const double* orc_dof_invweight0(const orc_prep_t* p) { return p->dof_invweight0; }
+double orc_meaninertia(const orc_prep_t* p) { return p->meaninertia; }
+
+orc_data_t* orc_data_new(const orc_prep_t* p) {
+  orc_data_t* d = (orc_data_t*)calloc(1, sizeof(orc_data_t));
+  data_alloc(d, &p->m);
+  memcpy(d->qpos, p->m.qpos0, sizeof(double) * p->m.nq);
+  return d;
+}
+void orc_data_free(orc_data_t* d) {
+  if (!d) return;
+  free_data_arrays(d);
+  free(d);
+}
+void orc_set_state(orc_data_t* d, const double* qpos, const double* qvel, const double* qacc_ws, const double* ctrl) {
+  if (qpos) memcpy(d->qpos, qpos, sizeof(double) * d->nq);
+  if (qvel) memcpy(d->qvel, qvel, sizeof(double) * d->nv);
+  if (qacc_ws) memcpy(d->qacc_ws, qacc_ws, sizeof(double) * d->nv);
+  if (ctrl) memcpy(d->ctrl, ctrl, sizeof(double) * d->nu);
+}
+void orc_get_state(const orc_data_t* d, double* qpos, double* qvel, double* qacc_ws) {
+  if (qpos) memcpy(qpos, d->qpos, sizeof(double) * d->nq);
+  if (qvel) memcpy(qvel, d->qvel, sizeof(double) * d->nv);
+  if (qacc_ws) memcpy(qacc_ws, d->qacc_ws, sizeof(double) * d->nv);
+}
+
+/* ------------------------------------------------------------------ smooth dynamics */
+/* mj_kinematics: body chain, joint anchors/axes, inertial and geom frames.
+ * Same composition as reference src/compat/mkx_kernels/kinematics.hpp:79-144. */
+static void kinematics(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  d->xpos[0] = d->xpos[1] = d->xpos[2] = 0;
+  d->xquat[0] = 1; d->xquat[1] = d->xquat[2] = d->xquat[3] = 0;
+  quat2mat(d->xquat, d->xmat);
+  memcpy(d->xipos, d->xpos, 3 * sizeof(double));
+  memcpy(d->ximat, d->xmat, 9 * sizeof(double));
+  for (int b = 1; b < m->nbody; b++) {
+    int par = m->body_parentid[b];
+    int ja = m->body_jntadr[b], jn = m->body_jntnum[b];
+    double pos[3], q[4];
+    if (jn > 0 && m->jnt_type[ja] == JFREE) {
+      int a = m->jnt_qposadr[ja];
+      for (int i = 0; i < 3; i++) pos[i] = d->qpos[a + i];
+      for (int i = 0; i < 4; i++) q[i] = d->qpos[a + 3 + i];
+      normalize4(q);
+      for (int i = 0; i < 3; i++) { d->xanchor[3 * ja + i] = pos[i]; d->xaxis[3 * ja + i] = m->jnt_axis[3 * ja + i]; }
+    } else {
+      double t[3];
+      mulmatvec3(d->xmat + 9 * par, m->body_pos + 3 * b, t);
+      for (int i = 0; i < 3; i++) pos[i] = d->xpos[3 * par + i] + t[i];
+      mulquat(d->xquat + 4 * par, m->body_quat + 4 * b, q);
+      for (int j = ja; j < ja + jn; j++) {
+        int a = m->jnt_qposadr[j];
+        double R[9], anc[3], ax[3];
+        quat2mat(q, R);
+        mulmatvec3(R, m->jnt_pos + 3 * j, anc);
+        for (int i = 0; i < 3; i++) anc[i] += pos[i];
+        mulmatvec3(R, m->jnt_axis + 3 * j, ax);
+        memcpy(d->xanchor + 3 * j, anc, 3 * sizeof(double));
+        memcpy(d->xaxis + 3 * j, ax, 3 * sizeof(double));
+        if (m->jnt_type[j] == JSLIDE) {
+          double s = d->qpos[a] - m->qpos0[a];
+          for (int i = 0; i < 3; i++) pos[i] += ax[i] * s;
+        } else {
+          double ql[4];
+          if (m->jnt_type[j] == JHINGE) axisangle2quat(m->jnt_axis + 3 * j, d->qpos[a] - m->qpos0[a], ql);
+          else { for (int i = 0; i < 4; i++) ql[i] = d->qpos[a + i]; normalize4(ql); }
+          mulquat(q, ql, q);
+          double R2[9], r[3];
+          quat2mat(q, R2);
+          mulmatvec3(R2, m->jnt_pos + 3 * j, r);
+          for (int i = 0; i < 3; i++) pos[i] = anc[i] - r[i];
+        }
+      }
+      normalize4(q);
+    }
+    memcpy(d->xpos + 3 * b, pos, 3 * sizeof(double));
+    memcpy(d->xquat + 4 * b, q, 4 * sizeof(double));
+    quat2mat(q, d->xmat + 9 * b);
+    double t[3], qi[4];
+    mulmatvec3(d->xmat + 9 * b, m->body_ipos + 3 * b, t);
+    for (int i = 0; i < 3; i++) d->xipos[3 * b + i] = pos[i] + t[i];
+    mulquat(q, m->body_iquat + 4 * b, qi);
+    quat2mat(qi, d->ximat + 9 * b);
+  }
+  for (int g = 0; g < m->ngeom; g++) {
+    int b = m->geom_bodyid[g];
+    double t[3], qg[4];
+    mulmatvec3(d->xmat + 9 * b, m->geom_pos + 3 * g, t);
+    for (int i = 0; i < 3; i++) d->geom_xpos[3 * g + i] = d->xpos[3 * b + i] + t[i];
+    mulquat(d->xquat + 4 * b, m->geom_quat + 4 * g, qg);
+    quat2mat(qg, d->geom_xmat + 9 * g);
+  }
+}
+
+/* mj_comPos: subtree COM, body inertia about the root's subtree COM (cinert)
+ * and dof motion subspaces (cdof). Reference forward.hpp:146-233. */
+static void com_pos(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  int nb = m->nbody;
+  for (int b = 0; b < nb; b++) for (int i = 0; i < 3; i++) d->subtree_com[3 * b + i] = m->body_mass[b] * d->xipos[3 * b + i];
+  for (int b = nb - 1; b > 0; b--) {
+    int par = m->body_parentid[b];
+    for (int i = 0; i < 3; i++) d->subtree_com[3 * par + i] += d->subtree_com[3 * b + i];
+  }
+  for (int b = 0; b < nb; b++) {
+    double sm = p->body_subtreemass[b];
+    for (int i = 0; i < 3; i++) d->subtree_com[3 * b + i] = sm < MINVAL ? d->xipos[3 * b + i] : d->subtree_com[3 * b + i] / sm;
+  }
+  memset(d->cinert, 0, 10 * sizeof(double));
+  for (int b = 1; b < nb; b++) {
+    const double* R = d->ximat + 9 * b;
+    const double* I = m->body_inertia + 3 * b;
+    double mass = m->body_mass[b];
+    const double* c = d->subtree_com + 3 * m->body_rootid[b];
+    double dd[3] = {d->xipos[3 * b] - c[0], d->xipos[3 * b + 1] - c[1], d->xipos[3 * b + 2] - c[2]};
+    double A[9];
+    for (int r = 0; r < 3; r++)
+      for (int k = 0; k < 3; k++) A[3 * r + k] = R[3 * r] * I[0] * R[3 * k] + R[3 * r + 1] * I[1] * R[3 * k + 1] + R[3 * r + 2] * I[2] * R[3 * k + 2];
+    double* o = d->cinert + 10 * b;
+    double sq = dot3(dd, dd);
+    o[0] = A[0] + mass * (sq - dd[0] * dd[0]);
+    o[1] = A[4] + mass * (sq - dd[1] * dd[1]);
+    o[2] = A[8] + mass * (sq - dd[2] * dd[2]);
+    o[3] = A[1] - mass * dd[0] * dd[1];
+    o[4] = A[2] - mass * dd[0] * dd[2];
+    o[5] = A[5] - mass * dd[1] * dd[2];
+    o[6] = mass * dd[0]; o[7] = mass * dd[1]; o[8] = mass * dd[2]; o[9] = mass;
+  }
+  for (int j = 0; j < m->njnt; j++) {
+    int b = m->jnt_bodyid[j], da = m->jnt_dofadr[j];
+    const double* c = d->subtree_com + 3 * m->body_rootid[b];
+    const double* anc = d->xanchor + 3 * j;
+    double off[3] = {c[0] - anc[0], c[1] - anc[1], c[2] - anc[2]};
+    const double* R = d->xmat + 9 * b;
+    int t = m->jnt_type[j];
+    if (t == JFREE || t == JBALL) {
+      int r0 = da;
+      if (t == JFREE) {
+        for (int i = 0; i < 3; i++) {
+          double* cd = d->cdof + 6 * (da + i);
+          memset(cd, 0, 6 * sizeof(double));
+          cd[3 + i] = 1;
+        }
+        r0 = da + 3;
+      }
+      for (int i = 0; i < 3; i++) {
+        double ax[3] = {R[i], R[3 + i], R[6 + i]};
+        double* cd = d->cdof + 6 * (r0 + i);
+        cd[0] = ax[0]; cd[1] = ax[1]; cd[2] = ax[2];
+        cross3(ax, off, cd + 3);
+      }
+    } else if (t == JSLIDE) {
+      double* cd = d->cdof + 6 * da;
+      cd[0] = cd[1] = cd[2] = 0;
+      memcpy(cd + 3, d->xaxis + 3 * j, 3 * sizeof(double));
+    } else {
+      double* cd = d->cdof + 6 * da;
+      memcpy(cd, d->xaxis + 3 * j, 3 * sizeof(double));
+      cross3(d->xaxis + 3 * j, off, cd + 3);
+    }
+  }
+}
+
+/* mj_crb: composite inertia and the joint-space inertia matrix (dense storage
+ * of the tree-sparse qM). Reference forward.hpp:240-279. */
+static void crb_mass(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  int nv = m->nv;
+  memcpy(d->crb, d->cinert, sizeof(double) * 10 * m->nbody);
+  for (int b = m->nbody - 1; b > 0; b--) {
+    int par = m->body_parentid[b];
+    if (par > 0) for (int k = 0; k < 10; k++) d->crb[10 * par + k] += d->crb[10 * b + k];
+  }
+  memset(d->qM, 0, sizeof(double) * nv * nv);
+  for (int i = 0; i < nv; i++) {
+    double buf[6];
+    mul_inert_vec(d->crb + 10 * m->dof_bodyid[i], d->cdof + 6 * i, buf);
+    for (int j = i; j >= 0; j = m->dof_parentid[j]) {
+      double s = 0;
+      for (int k = 0; k < 6; k++) s += d->cdof[6 * j + k] * buf[k];
+      d->qM[i * nv + j] = s;
+      d->qM[j * nv + i] = s;
+    }
+    d->qM[i * nv + i] += m->dof_armature[i];
+  }
+}
+
+/* mj_factorM: tree-sparse L^T D L factorisation of qM (dense storage; entries
+ * outside the dof-ancestor pattern stay zero). L is unit lower triangular, D is
+ * kept on the diagonal of qLD. */
+static int factor_m_into(const orc_model_t* m, const double* M, double* LD) {
+  int nv = m->nv;
+  memcpy(LD, M, sizeof(double) * nv * nv);
+  for (int k = nv - 1; k >= 0; k--) {
+    double dk = LD[k * nv + k];
+    if (!(dk > MINVAL)) return -1;
+    for (int i = m->dof_parentid[k]; i >= 0; i = m->dof_parentid[i]) {
+      double a = LD[k * nv + i] / dk;
+      for (int j = i; j >= 0; j = m->dof_parentid[j]) LD[i * nv + j] -= a * LD[k * nv + j];
+      LD[k * nv + i] = a;
+    }
+  }
+  return 0;
+}
+static void solve_ld(const orc_model_t* m, const double* LD, double* x) {
+  int nv = m->nv;
+  for (int k = nv - 1; k >= 0; k--)
+    for (int i = m->dof_parentid[k]; i >= 0; i = m->dof_parentid[i]) x[i] -= LD[k * nv + i] * x[k];
+  for (int k = 0; k < nv; k++) x[k] /= LD[k * nv + k];
+  for (int k = 0; k < nv; k++)
+    for (int i = m->dof_parentid[k]; i >= 0; i = m->dof_parentid[i]) x[k] -= LD[k * nv + i] * x[i];
+}
+static int factor_m(const orc_prep_t* p, orc_data_t* d) { return factor_m_into(&p->m, d->qM, d->qLD); }
+static void solve_m(const orc_prep_t* p, const orc_data_t* d, double* x) { solve_ld(&p->m, d->qLD, x); }
+
+/* mj_comVel: body velocities and cdof_dot with MuJoCo's grouping of free and
+ * ball dofs (reference forward.hpp:287-332). */
+static void com_vel(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  memset(d->cvel, 0, 6 * sizeof(double));
+  for (int b = 1; b < m->nbody; b++) {
+    double cvel[6];
+    memcpy(cvel, d->cvel + 6 * m->body_parentid[b], 6 * sizeof(double));
+    int ja = m->body_jntadr[b], jn = m->body_jntnum[b];
+    for (int j = ja; j < ja + jn; j++) {
+      int da = m->jnt_dofadr[j], t = m->jnt_type[j];
+      if (t == JFREE) {
+        for (int k = 0; k < 3; k++) memset(d->cdof_dot + 6 * (da + k), 0, 6 * sizeof(double));
+        for (int k = 0; k < 3; k++) for (int c = 0; c < 6; c++) cvel[c] += d->cdof[6 * (da + k) + c] * d->qvel[da + k];
+        da += 3;
+      }
+      if (t == JFREE || t == JBALL) {
+        for (int k = 0; k < 3; k++) cross_motion(cvel, d->cdof + 6 * (da + k), d->cdof_dot + 6 * (da + k));
+        for (int k = 0; k < 3; k++) for (int c = 0; c < 6; c++) cvel[c] += d->cdof[6 * (da + k) + c] * d->qvel[da + k];
+      } else {
+        cross_motion(cvel, d->cdof + 6 * da, d->cdof_dot + 6 * da);
+        for (int c = 0; c < 6; c++) cvel[c] += d->cdof[6 * da + c] * d->qvel[da];
+      }
+    }
+    memcpy(d->cvel + 6 * b, cvel, 6 * sizeof(double));
+  }
+}
+
+/* mj_rne with zero acceleration: qfrc_bias = C(q, qdot) qdot + g(q).
+ * Reference forward.hpp:338-397. */
+static void rne_bias(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  memset(d->cacc, 0, 6 * sizeof(double));
+  if (!(m->disableflags & DS_GRAVITY)) for (int i = 0; i < 3; i++) d->cacc[3 + i] = -m->gravity[i];
+  for (int b = 1; b < m->nbody; b++) {
+    double* a = d->cacc + 6 * b;
+    memcpy(a, d->cacc + 6 * m->body_parentid[b], 6 * sizeof(double));
+    int da = m->body_dofadr[b], dn = m->body_dofnum[b];
+    for (int k = da; k < da + dn; k++) for (int c = 0; c < 6; c++) a[c] += d->cdof_dot[6 * k + c] * d->qvel[k];
+    double f1[6], mv[6], f2[6];
+    mul_inert_vec(d->cinert + 10 * b, a, f1);
+    mul_inert_vec(d->cinert + 10 * b, d->cvel + 6 * b, mv);
+    cross_force(d->cvel + 6 * b, mv, f2);
+    for (int c = 0; c < 6; c++) d->cfrc[6 * b + c] = f1[c] + f2[c];
+  }
+  memset(d->cfrc, 0, 6 * sizeof(double));
+  for (int b = m->nbody - 1; b > 0; b--) {
+    int par = m->body_parentid[b];
+    for (int c = 0; c < 6; c++) d->cfrc[6 * par + c] += d->cfrc[6 * b + c];
+  }
+  for (int k = 0; k < m->nv; k++) {
+    double s = 0;
+    for (int c = 0; c < 6; c++) s += d->cdof[6 * k + c] * d->cfrc[6 * m->dof_bodyid[k] + c];
+    d->qfrc_bias[k] = s;
+  }
+}
+
+/* mj_passive subset: joint springs (hinge/slide) and dof damping. */
+static void passive(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  memset(d->qfrc_passive, 0, sizeof(double) * m->nv);
+  if (m->disableflags & DS_PASSIVE) return;
+  for (int j = 0; j < m->njnt; j++) {
+    int t = m->jnt_type[j];
+    if ((t == JHINGE || t == JSLIDE) && m->jnt_stiffness[j] != 0) {
+      int a = m->jnt_qposadr[j];
+      d->qfrc_passive[m->jnt_dofadr[j]] -= m->jnt_stiffness[j] * (d->qpos[a] - m->qpos_spring[a]);
+    }
+  }
+  for (int k = 0; k < m->nv; k++) d->qfrc_passive[k] -= m->dof_damping[k] * d->qvel[k];
+}
+
+/* mj_fwdActuation for joint transmissions: force = gain*ctrl + b0 + b1*length
+ * + b2*velocity, ctrl and force clamping, qfrc_actuator = moment * force.
+ * Reference forward.hpp:411-435. */
+static void actuation(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  memset(d->qfrc_actuator, 0, sizeof(double) * m->nv);
+  for (int u = 0; u < m->nu; u++) {
+    d->actuator_force[u] = 0;
+    if (m->disableflags & DS_ACTUATION) continue;
+    int j = m->actuator_trnid[u];
+    int da = m->jnt_dofadr[j], qa = m->jnt_qposadr[j];
+    double gear = m->actuator_gear[u];
+    double c = d->ctrl[u];
+    if (m->actuator_ctrllimited[u] && !(m->disableflags & DS_CLAMPCTRL)) c = clampd(c, m->actuator_ctrlrange[2 * u], m->actuator_ctrlrange[2 * u + 1]);
+    double len = gear * d->qpos[qa], vel = gear * d->qvel[da];
+    const double *g = m->actuator_gainprm + 3 * u, *bi = m->actuator_biasprm + 3 * u;
+    double f = g[0] * c + bi[0] + bi[1] * len + bi[2] * vel;
+    if (m->actuator_forcelimited[u]) f = clampd(f, m->actuator_forcerange[2 * u], m->actuator_forcerange[2 * u + 1]);
+    d->actuator_force[u] = f;
+    d->qfrc_actuator[da] += gear * f;
+  }
+}
+
+/* ------------------------------------------------------------------ collision */
+/* Contact frame from the normal: tangent 1 is the projection of (0,1,0) --
+ * or (0,0,1) when the normal is close to y -- orthogonal to the normal, and
+ * tangent 2 = normal x tangent 1 (MuJoCo mju_makeFrame). */
+static void make_frame(const double n_in[3], double f[9]) {
+  double n[3] = {n_in[0], n_in[1], n_in[2]};
+  normalize3(n);
+  double y[3];
+  if (fabs(n[1]) < 0.5) { y[0] = 0; y[1] = 1; y[2] = 0; }
+  else { y[0] = 0; y[1] = 0; y[2] = 1; }
+  double s = dot3(n, y);
+  for (int i = 0; i < 3; i++) y[i] -= s * n[i];
+  normalize3(y);
+  double z[3];
+  cross3(n, y, z);
+  f[0] = n[0]; f[1] = n[1]; f[2] = n[2];
+  f[3] = y[0]; f[4] = y[1]; f[5] = y[2];
+  f[6] = z[0]; f[7] = z[1]; f[8] = z[2];
+}
+
+typedef struct { double dist, pos[3], n[3]; } rawc_t;
+
+static int sphere_sphere(const double c1[3], double r1, const double c2[3], double r2, double margin, rawc_t* out) {
+  double dv[3] = {c2[0] - c1[0], c2[1] - c1[1], c2[2] - c1[2]};
+  double L = sqrt(dot3(dv, dv));
+  double dist = L - r1 - r2;
+  if (dist > margin) return 0;
+  double n[3];
+  if (L < MINVAL) { n[0] = 1; n[1] = 0; n[2] = 0; }
+  else { n[0] = dv[0] / L; n[1] = dv[1] / L; n[2] = dv[2] / L; }
+  out->dist = dist;
+  for (int i = 0; i < 3; i++) { out->n[i] = n[i]; out->pos[i] = c1[i] + n[i] * (r1 + 0.5 * dist); }
+  return 1;
+}
+
+/* plane (z axis of its frame) against a sphere centre */
+static int plane_point(const double pp[3], const double pn[3], const double c[3], double r, double margin, rawc_t* out) {
+  double dv[3] = {c[0] - pp[0], c[1] - pp[1], c[2] - pp[2]};
+  double dist = dot3(pn, dv) - r;
+  if (dist > margin) return 0;
+  out->dist = dist;
+  for (int i = 0; i < 3; i++) { out->n[i] = pn[i]; out->pos[i] = c[i] - pn[i] * (r + 0.5 * dist); }
+  return 1;
+}
+
+/* closest points of segments p1 + s*d1 and p2 + t*d2, s,t in [0,1]
+ * (the clamped-parameter method of Ericson, Real-Time Collision Detection 5.1.9) */
+static void seg_seg(const double p1[3], const double d1[3], const double p2[3], const double d2[3], double* s_out, double* t_out) {
+  double r[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
+  double a = dot3(d1, d1), e = dot3(d2, d2), f = dot3(d2, r);
+  double s, t;
+  if (a <= MINVAL && e <= MINVAL) { s = t = 0; }
+  else if (a <= MINVAL) { s = 0; t = clampd(f / e, 0, 1); }
+  else {
+    double c = dot3(d1, r);
+    if (e <= MINVAL) { t = 0; s = clampd(-c / a, 0, 1); }
+    else {
+      double b = dot3(d1, d2);
+      double denom = a * e - b * b;
+      s = (denom > 1e-10 * a * e) ? clampd((b * f - c * e) / denom, 0, 1) : 0;
+      t = (b * s + f) / e;
+      if (t < 0) { t = 0; s = clampd(-c / a, 0, 1); }
+      else if (t > 1) { t = 1; s = clampd((b - c) / a, 0, 1); }
+    }
+  }
+  *s_out = s; *t_out = t;
+}
+
+static void capsule_ends(const double* x, const double* R, double h, double p0[3], double d[3]) {
+  double ax[3] = {R[2], R[5], R[8]};
+  for (int i = 0; i < 3; i++) { p0[i] = x[i] - ax[i] * h; d[i] = 2 * h * ax[i]; }
+}
+
+/* sphere centre vs box (box frame x/R, half sizes s): one contact; normal points
+ * from the sphere towards the box */
+static int sphere_box(const double c[3], double r, const double bx[3], const double R[9], const double s[3], double margin, rawc_t* out) {
+  double rel[3] = {c[0] - bx[0], c[1] - bx[1], c[2] - bx[2]}, pl[3];
+  mulmatTvec3(R, rel, pl);
+  double q[3];
+  int inside = 1;
+  for (int i = 0; i < 3; i++) {
+    q[i] = clampd(pl[i], -s[i], s[i]);
+    if (pl[i] < -s[i] || pl[i] > s[i]) inside = 0;
+  }
+  double nl[3], dist;
+  if (!inside) {
+    double dv[3] = {pl[0] - q[0], pl[1] - q[1], pl[2] - q[2]};
+    double L = sqrt(dot3(dv, dv));
+    dist = L - r;
+    if (dist > margin) return 0;
+    for (int i = 0; i < 3; i++) nl[i] = -dv[i] / L;  /* towards the box */
+  } else {
+    int k = 0;
+    double best = s[0] - fabs(pl[0]);
+    for (int i = 1; i < 3; i++) { double dp = s[i] - fabs(pl[i]); if (dp < best) { best = dp; k = i; } }
+    dist = -best - r;
+    if (dist > margin) return 0;
+    nl[0] = nl[1] = nl[2] = 0;
+    nl[k] = pl[k] >= 0 ? -1.0 : 1.0;
+  }
+  double n[3];
+  mulmatvec3(R, nl, n);
+  out->dist = dist;
+  for (int i = 0; i < 3; i++) { out->n[i] = n[i]; out->pos[i] = c[i] + n[i] * (r + 0.5 * dist); }
+  return 1;
+}
+
+/* narrow phase for one pair; geom1 has the lower type. Returns the number of
+ * raw contacts written (normal from geom1 to geom2). */
+static int narrowphase(int t1, int t2, const double* s1, const double* s2, const double* x1, const double* R1,
+                       const double* x2, const double* R2, double margin, rawc_t* out, int maxc) {
+  (void)maxc;
+  if (t1 == GPLANE) {
+    double pn[3] = {R1[2], R1[5], R1[8]};
+    if (t2 == GSPHERE) return plane_point(x1, pn, x2, s2[0], margin, out);
+    if (t2 == GCAPSULE) {
+      double p0[3], dd[3];
+      capsule_ends(x2, R2, s2[1], p0, dd);
+      double pA[3] = {p0[0] + dd[0], p0[1] + dd[1], p0[2] + dd[2]};
+      int n = 0;
+      n += plane_point(x1, pn, pA, s2[0], margin, out + n);
+      n += plane_point(x1, pn, p0, s2[0], margin, out + n);
+      return n;
+    }
+    if (t2 == GBOX) {
+      int n = 0;
+      for (int k = 0; k < 8 && n < 4; k++) {
+        double cl[3] = {(k & 1) ? s2[0] : -s2[0], (k & 2) ? s2[1] : -s2[1], (k & 4) ? s2[2] : -s2[2]}, cw[3];
+        mulmatvec3(R2, cl, cw);
+        for (int i = 0; i < 3; i++) cw[i] += x2[i];
+        n += plane_point(x1, pn, cw, 0.0, margin, out + n);
+      }
+      return n;
+    }
+    return -1;
+  }
+  if (t1 == GSPHERE) {
+    if (t2 == GSPHERE) return sphere_sphere(x1, s1[0], x2, s2[0], margin, out);
+    if (t2 == GCAPSULE) {
+      double p0[3], dd[3];
+      capsule_ends(x2, R2, s2[1], p0, dd);
+      double rel[3] = {x1[0] - p0[0], x1[1] - p0[1], x1[2] - p0[2]};
+      double a = dot3(dd, dd);
+      double t = a > MINVAL ? clampd(dot3(rel, dd) / a, 0, 1) : 0;
+      double c2[3] = {p0[0] + t * dd[0], p0[1] + t * dd[1], p0[2] + t * dd[2]};
+      return sphere_sphere(x1, s1[0], c2, s2[0], margin, out);
+    }
+    if (t2 == GBOX) return sphere_box(x1, s1[0], x2, R2, s2, margin, out);
+    return -1;
+  }
+  if (t1 == GCAPSULE && t2 == GCAPSULE) {
+    double a0[3], ad[3], b0[3], bd[3], s, t;
+    capsule_ends(x1, R1, s1[1], a0, ad);
+    capsule_ends(x2, R2, s2[1], b0, bd);
+    seg_seg(a0, ad, b0, bd, &s, &t);
+    double c1[3] = {a0[0] + s * ad[0], a0[1] + s * ad[1], a0[2] + s * ad[2]};
+    double c2[3] = {b0[0] + t * bd[0], b0[1] + t * bd[1], b0[2] + t * bd[2]};
+    return sphere_sphere(c1, s1[0], c2, s2[0], margin, out);
+  }
+  return -1;
+}
+
+int orc_collide_pair(const orc_prep_t* p, int pi, const double* xpos1, const double* xmat1, const double* xpos2,
+                     const double* xmat2, orc_contact_t* out, int maxc) {
+  const orc_model_t* m = &p->m;
+  const orc_pair_t* pr = &p->pairs[pi];
+  rawc_t raw[8];
+  int n = narrowphase(m->geom_type[pr->g1], m->geom_type[pr->g2], m->geom_size + 3 * pr->g1, m->geom_size + 3 * pr->g2,
+                      xpos1, xmat1, xpos2, xmat2, pr->margin, raw, 8);
+  if (n < 0) return n;
+  if (n > maxc) n = maxc;
+  for (int k = 0; k < n; k++) {
+    orc_contact_t* c = &out[k];
+    memset(c, 0, sizeof(*c));
+    c->dist = raw[k].dist;
+    memcpy(c->pos, raw[k].pos, 3 * sizeof(double));
+    make_frame(raw[k].n, c->frame);
+    c->geom1 = pr->g1; c->geom2 = pr->g2; c->pair = pi; c->dim = pr->condim;
+    memcpy(c->friction, pr->friction, 5 * sizeof(double));
+    memcpy(c->solref, pr->solref, 2 * sizeof(double));
+    memcpy(c->solimp, pr->solimp, 5 * sizeof(double));
+    c->includemargin = pr->margin - pr->gap;
+  }
+  return n;
+}
+
+/* mj_collision over the static pair table with the bounding-sphere cull of
+ * reference collision.hpp:301-305 (planes are never culled). Contacts are
+ * ordered by pair index, then by the narrow phase's own order. */
+static int collision(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  d->ncon = 0;
+  int status = 0;
+  if (m->disableflags & (DS_CONTACT | DS_CONSTRAINT)) return 0;
+  for (int pi = 0; pi < p->npair; pi++) {
+    const orc_pair_t* pr = &p->pairs[pi];
+    const double *x1 = d->geom_xpos + 3 * pr->g1, *x2 = d->geom_xpos + 3 * pr->g2;
+    double rb1 = m->geom_rbound[pr->g1], rb2 = m->geom_rbound[pr->g2];
+    if (rb1 > 0 && rb2 > 0) {
+      double dv[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
+      if (sqrt(dot3(dv, dv)) > rb1 + rb2 + pr->margin) continue;
+    }
+    orc_contact_t tmp[8];
+    int n = orc_collide_pair(p, pi, x1, d->geom_xmat + 9 * pr->g1, x2, d->geom_xmat + 9 * pr->g2, tmp, 8);
+    if (n < 0) { status |= 4; continue; }
+    for (int k = 0; k < n; k++) {
+      if (d->ncon >= ORC_MAXCON) { status |= 1; break; }
+      d->con[d->ncon++] = tmp[k];
+    }
+  }
+  return status;
+}
+
+/* ------------------------------------------------------------------ constraints */
+/* translational / rotational Jacobian of a world point moving with body b
+ * (mj_jac from cdof about the root subtree COM; reference solver.hpp:474-508) */
+static void jac_point(const orc_prep_t* p, const orc_data_t* d, int b, const double pt[3], double* jp, double* jr, double sgn) {
+  const orc_model_t* m = &p->m;
+  int nv = m->nv;
+  int bb = b;
+  while (bb > 0 && m->body_dofnum[bb] == 0) bb = m->body_parentid[bb];
+  if (bb == 0) return;
+  const double* c = d->subtree_com + 3 * m->body_rootid[b];
+  double off[3] = {pt[0] - c[0], pt[1] - c[1], pt[2] - c[2]};
+  for (int k = m->body_dofadr[bb] + m->body_dofnum[bb] - 1; k >= 0; k = m->dof_parentid[k]) {
+    const double* cd = d->cdof + 6 * k;
+    double t[3];
+    cross3(cd, off, t);
+    for (int i = 0; i < 3; i++) {
+      jp[i * nv + k] += sgn * (cd[3 + i] + t[i]);
+      jr[i * nv + k] += sgn * cd[i];
+    }
+  }
+}
+
+/* MuJoCo impedance: solimp sigmoid of |pos - margin| / width, clamped to
+ * [dmin, dmax] (reference solver.hpp:120-139) */
+static double impedance(const double si[5], double x) {
+  double dmin = clampd(si[0], MINIMP, MAXIMP), dmax = clampd(si[1], MINIMP, MAXIMP);
+  double width = si[2], mid = clampd(si[3], MINIMP, MAXIMP), power = si[4] < 1 ? 1 : si[4];
+  if (dmin == dmax || width <= MINVAL) return dmax;
+  double xx = fabs(x) / width;
+  if (xx >= 1) return dmax;
+  double y;
+  if (power == 1) y = xx;
+  else if (xx <= mid) y = pow(xx, power) / pow(mid, power - 1);
+  else y = 1 - pow(1 - xx, power) / pow(1 - mid, power - 1);
+  return dmin + y * (dmax - dmin);
+}
+
+static int add_row(orc_data_t* d, int nv, int type, int id, double pos, double margin, double dA, double floss) {
+  if (d->nefc >= ORC_MAXEFC) return -1;
+  int r = d->nefc++;
+  memset(d->efc_J + (size_t)r * nv, 0, sizeof(double) * nv);
+  d->efc_type[r] = type; d->efc_id[r] = id; d->efc_pos[r] = pos; d->efc_margin[r] = margin;
+  d->efc_diagApprox[r] = dA; d->efc_floss[r] = floss;
+  return r;
+}
+
+/* mj_makeConstraint + mj_makeImpedance + mj_referenceConstraint for dof
+ * friction, joint limits and contacts (pyramidal / frictionless / elliptic
+ * rows). Row order: friction, limits, contacts. Per-row k, b, imp, R and aref
+ * follow reference solver.hpp:120-139,346-371,510-587; pyramidal rows share
+ * R = 2 mu^2 / impratio * R_first with R_first from diagApprox = tran*(1+mu^2)
+ * (reference solver.hpp:519-525). */
+static int make_constraint(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  int nv = m->nv;
+  int status = 0;
+  d->nefc = 0;
+  if (m->disableflags & DS_CONSTRAINT) return 0;
+  /* solref / solimp per row, kept in local arrays */
+  static __thread double sref[ORC_MAXEFC][2], simp[ORC_MAXEFC][5];
+  if (!(m->disableflags & DS_FRICTIONLOSS)) {
+    for (int k = 0; k < nv; k++) {
+      if (m->dof_frictionloss[k] <= 0) continue;
+      int r = add_row(d, nv, EFC_FRICTION_DOF, k, 0, 0, p->dof_invweight0[k], m->dof_frictionloss[k]);
+      if (r < 0) { status |= 2; break; }
+      d->efc_J[(size_t)r * nv + k] = 1;
+      memcpy(sref[r], m->dof_solref + 2 * k, 2 * sizeof(double));
+      memcpy(simp[r], m->dof_solimp + 5 * k, 5 * sizeof(double));
+    }
+  }
+  if (!(m->disableflags & DS_LIMIT)) {
+    for (int j = 0; j < m->njnt; j++) {
+      int t = m->jnt_type[j];
+      if (!m->jnt_limited[j] || (t != JHINGE && t != JSLIDE)) continue;
+      int qa = m->jnt_qposadr[j], da = m->jnt_dofadr[j];
+      double v = d->qpos[qa];
+      for (int side = 0; side < 2; side++) {
+        double dist = side == 0 ? v - m->jnt_range[2 * j] : m->jnt_range[2 * j + 1] - v;
+        if (dist < m->jnt_margin[j]) {
+          int r = add_row(d, nv, EFC_LIMIT_JOINT, j, dist, m->jnt_margin[j], p->dof_invweight0[da], 0);
+          if (r < 0) { status |= 2; break; }
+          d->efc_J[(size_t)r * nv + da] = side == 0 ? 1.0 : -1.0;
+          memcpy(sref[r], m->jnt_solref + 2 * j, 2 * sizeof(double));
+          memcpy(simp[r], m->jnt_solimp + 5 * j, 5 * sizeof(double));
+        }
+      }
+    }
+  }
+  double* jac = (double*)calloc((size_t)6 * nv, sizeof(double));
+  if (!(m->disableflags & DS_CONTACT)) {
+    for (int ci = 0; ci < d->ncon; ci++) {
+      orc_contact_t* c = &d->con[ci];
+      if (!(c->dist < c->includemargin)) continue;
+      const orc_pair_t* pr = &p->pairs[c->pair];
+      memset(jac, 0, sizeof(double) * 6 * nv);
+      jac_point(p, d, pr->b2, c->pos, jac, jac + 3 * nv, 1.0);
+      jac_point(p, d, pr->b1, c->pos, jac, jac + 3 * nv, -1.0);
+      /* rows of the contact frame: translational (n, t1, t2) and rotational (n, t1, t2) */
+      double Jf[6][64 * 4];
+      double* Jrow[6];
+      double* big = NULL;
+      if (nv > 64 * 4) { big = (double*)calloc((size_t)6 * nv, sizeof(double)); for (int i = 0; i < 6; i++) Jrow[i] = big + (size_t)i * nv; }
+      else for (int i = 0; i < 6; i++) Jrow[i] = Jf[i];
+      for (int a = 0; a < 3; a++) {
+        const double* ax = c->frame + 3 * a;
+        for (int k = 0; k < nv; k++) {
+          Jrow[a][k] = ax[0] * jac[0 * nv + k] + ax[1] * jac[1 * nv + k] + ax[2] * jac[2 * nv + k];
+          Jrow[3 + a][k] = ax[0] * jac[3 * nv + k] + ax[1] * jac[4 * nv + k] + ax[2] * jac[5 * nv + k];
+        }
+      }
+      double tran = p->body_invweight0[2 * pr->b1] + p->body_invweight0[2 * pr->b2];
+      int dim = c->dim;
+      if (dim == 1) {
+        int r = add_row(d, nv, EFC_CONTACT_FRICTIONLESS, ci, c->dist, c->includemargin, tran, 0);
+        if (r < 0) { status |= 2; free(big); break; }
+        memcpy(d->efc_J + (size_t)r * nv, Jrow[0], sizeof(double) * nv);
+        memcpy(sref[r], c->solref, 2 * sizeof(double)); memcpy(simp[r], c->solimp, 5 * sizeof(double));
+      } else if (m->cone == 0) {
+        int nrow = 2 * (dim - 1);
+        if (d->nefc + nrow > ORC_MAXEFC) { status |= 2; free(big); break; }
+        double dA = tran * (1 + c->friction[0] * c->friction[0]);
+        for (int k = 0; k < dim - 1; k++) {
+          /* direction k: 0,1 tangents (translational); 2 spin (rotational normal); 3,4 rolling */
+          const double* Jk = k < 2 ? Jrow[1 + k] : (k == 2 ? Jrow[3] : Jrow[4 + (k - 3)]);
+          for (int sg = 0; sg < 2; sg++) {
+            int r = add_row(d, nv, EFC_CONTACT_PYRAMIDAL, ci, c->dist, c->includemargin, dA, 0);
+            double mu = c->friction[k] * (sg == 0 ? 1.0 : -1.0);
+            for (int q = 0; q < nv; q++) d->efc_J[(size_t)r * nv + q] = Jrow[0][q] + mu * Jk[q];
+            memcpy(sref[r], c->solref, 2 * sizeof(double)); memcpy(simp[r], c->solimp, 5 * sizeof(double));
+          }
+        }
+      } else {
+        /* elliptic cones are outside the supported set of this oracle version */
+        status |= 8;
+      }
+      free(big);
+    }
+  }
+  free(jac);
+  /* impedance, reference acceleration, regulariser */
+  for (int r = 0; r < d->nefc; r++) {
+    double v = 0;
+    for (int k = 0; k < nv; k++) v += d->efc_J[(size_t)r * nv + k] * d->qvel[k];
+    d->efc_vel[r] = v;
+    double tc = sref[r][0], dr = sref[r][1];
+    double dmax = clampd(simp[r][1], MINIMP, MAXIMP);
+    double kk, bb;
+    if (tc > 0) {
+      if (!(m->disableflags & DS_REFSAFE)) tc = fmax(tc, 2 * m->timestep);
+      kk = 1.0 / (dmax * dmax * tc * tc * dr * dr);
+      bb = 2.0 / (dmax * tc);
+    } else {
+      kk = -tc / (dmax * dmax);
+      bb = -dr / dmax;
+    }
+    double x = d->efc_pos[r] - d->efc_margin[r];
+    double imp = impedance(simp[r], x);
+    d->efc_imp[r] = imp;
+    d->efc_aref[r] = -bb * v - kk * imp * x;
+    d->efc_R[r] = fmax(d->efc_diagApprox[r] * (1 - imp) / imp, MINVAL);
+  }
+  /* pyramidal contacts: one R for all edges of a contact */
+  for (int r = 0; r < d->nefc;) {
+    if (d->efc_type[r] != EFC_CONTACT_PYRAMIDAL) { r++; continue; }
+    const orc_contact_t* c = &d->con[d->efc_id[r]];
+    int nrow = 2 * (c->dim - 1);
+    double mu0 = c->friction[0];
+    double Rpy = fmax(2 * mu0 * mu0 / m->impratio * d->efc_R[r], MINVAL);
+    for (int k = 0; k < nrow; k++) d->efc_R[r + k] = Rpy;
+    r += nrow;
+  }
+  for (int r = 0; r < d->nefc; r++) d->efc_D[r] = 1.0 / d->efc_R[r];
+  return status;
+}
+
+/* ------------------------------------------------------------------ solver */
+/* Constraint state at Jaref = J*qacc - aref: force, active flag and cost.
+ * One-sided rows are active for x < 0; friction rows are quadratic inside
+ * |D x| < floss and linear outside. */
+static double update_constraint(orc_data_t* d) {
+  double cost = 0;
+  for (int r = 0; r < d->nefc; r++) {
+    double x = d->efc_Jaref[r], D = d->efc_D[r];
+    int t = d->efc_type[r];
+    if (t == EFC_EQUALITY) {
+      d->efc_active[r] = 1; d->efc_force[r] = -D * x; cost += 0.5 * D * x * x;
+    } else if (t == EFC_FRICTION_DOF) {
+      double f = d->efc_floss[r], R = d->efc_R[r];
+      if (x <= -R * f) { d->efc_active[r] = 0; d->efc_force[r] = f; cost += -f * x - 0.5 * R * f * f; }
+      else if (x >= R * f) { d->efc_active[r] = 0; d->efc_force[r] = -f; cost += f * x - 0.5 * R * f * f; }
+      else { d->efc_active[r] = 1; d->efc_force[r] = -D * x; cost += 0.5 * D * x * x; }
+    } else {
+      if (x < 0) { d->efc_active[r] = 1; d->efc_force[r] = -D * x; cost += 0.5 * D * x * x; }
+      else { d->efc_active[r] = 0; d->efc_force[r] = 0; }
+    }
+  }
+  return cost;
+}
+
+/* total cost and gradient at the current qacc (Ma, Jaref already set) */
+static void cost_grad(const orc_prep_t* p, orc_data_t* d) {
+  int nv = p->m.nv;
+  double cc = update_constraint(d);
+  double g = 0;
+  for (int k = 0; k < nv; k++) g += 0.5 * (d->Ma[k] - d->qfrc_smooth[k]) * (d->qacc[k] - d->qacc_smooth[k]);
+  d->cost = g + cc;
+  for (int k = 0; k < nv; k++) d->grad[k] = d->Ma[k] - d->qfrc_smooth[k];
+  for (int r = 0; r < d->nefc; r++) {
+    double f = d->efc_force[r];
+    if (f == 0) continue;
+    const double* J = d->efc_J + (size_t)r * nv;
+    for (int k = 0; k < nv; k++) d->grad[k] -= J[k] * f;
+  }
+}
+
+static void mul_m(const orc_data_t* d, int nv, const double* x, double* y) {
+  for (int i = 0; i < nv; i++) {
+    double s = 0;
+    for (int j = 0; j < nv; j++) s += d->qM[i * nv + j] * x[j];
+    y[i] = s;
+  }
+}
+
+static void set_qacc(const orc_prep_t* p, orc_data_t* d, const double* qacc) {
+  int nv = p->m.nv;
+  memcpy(d->qacc, qacc, sizeof(double) * nv);
+  mul_m(d, nv, d->qacc, d->Ma);
+  for (int r = 0; r < d->nefc; r++) {
+    const double* J = d->efc_J + (size_t)r * nv;
+    double s = 0;
+    for (int k = 0; k < nv; k++) s += J[k] * qacc[k];
+    d->efc_Jaref[r] = s - d->efc_aref[r];
+  }
+}
+
+/* 1-D cost along the search direction: value and first/second derivative at
+ * step alpha (piecewise quadratic; rows switch with the sign / zone of x).
+ * nchange counts rows whose zone at alpha differs from their zone at ref. */
+typedef struct { double alpha, f, d1, d2; int nchange; } lspt_t;
+static lspt_t ls_eval(const orc_data_t* d, double alpha, double ref, const double gq[3]) {
+  lspt_t pt;
+  pt.alpha = alpha;
+  pt.f = gq[0] + alpha * gq[1] + 0.5 * alpha * alpha * gq[2];
+  pt.d1 = gq[1] + alpha * gq[2];
+  pt.d2 = gq[2];
+  pt.nchange = 0;
+  for (int r = 0; r < d->nefc; r++) {
+    double jv = d->efc_Jv[r];
+    double x = d->efc_Jaref[r] + alpha * jv, xr = d->efc_Jaref[r] + ref * jv, D = d->efc_D[r];
+    int t = d->efc_type[r];
+    if (t == EFC_FRICTION_DOF) {
+      double f = d->efc_floss[r], R = d->efc_R[r];
+      int z = x <= -R * f ? 0 : (x >= R * f ? 2 : 1), zr = xr <= -R * f ? 0 : (xr >= R * f ? 2 : 1);
+      pt.nchange += z != zr;
+      if (z == 0) { pt.f += -f * x - 0.5 * R * f * f; pt.d1 += -f * jv; }
+      else if (z == 2) { pt.f += f * x - 0.5 * R * f * f; pt.d1 += f * jv; }
+      else { pt.f += 0.5 * D * x * x; pt.d1 += D * x * jv; pt.d2 += D * jv * jv; }
+    } else if (t == EFC_EQUALITY) {
+      pt.f += 0.5 * D * x * x; pt.d1 += D * x * jv; pt.d2 += D * jv * jv;
+    } else {
+      pt.nchange += (x < 0) != (xr < 0);
+      if (x < 0) { pt.f += 0.5 * D * x * x; pt.d1 += D * x * jv; pt.d2 += D * jv * jv; }
+    }
+  }
+  return pt;
+}
+
+/* Exact line search on the convex piecewise-quadratic cost: Newton steps
+ * until the derivative changes sign, then Newton-from-the-left / secant on
+ * the bracket. Stops when |f'| <= gtol (MuJoCo's scaled ls_tolerance) or when
+ * a Newton step ends in the same quadratic piece it started from (then it is
+ * the exact minimiser). Returns 0 when search is not a descent direction. */
+static double linesearch(const orc_prep_t* p, orc_data_t* d, int* exact) {
+  const orc_model_t* m = &p->m;
+  int nv = m->nv;
+  double snorm = 0;
+  for (int k = 0; k < nv; k++) snorm += d->search[k] * d->search[k];
+  snorm = sqrt(snorm);
+  if (snorm < MINVAL) return 0;
+  double scale = 1.0 / (p->meaninertia * (nv > 1 ? nv : 1));
+  double gtol = m->tolerance * m->ls_tolerance * snorm / scale;
+  mul_m(d, nv, d->search, d->Mv);
+  for (int r = 0; r < d->nefc; r++) {
+    const double* J = d->efc_J + (size_t)r * nv;
+    double s = 0;
+    for (int k = 0; k < nv; k++) s += J[k] * d->search[k];
+    d->efc_Jv[r] = s;
+  }
+  double gq[3] = {0, 0, 0};
+  for (int k = 0; k < nv; k++) {
+    gq[0] += 0.5 * (d->Ma[k] - d->qfrc_smooth[k]) * (d->qacc[k] - d->qacc_smooth[k]);
+    gq[1] += (d->Ma[k] - d->qfrc_smooth[k]) * d->search[k];
+    gq[2] += d->search[k] * d->Mv[k];
+  }
+  *exact = 0;
+  lspt_t p0 = ls_eval(d, 0, 0, gq);
+  if (!(p0.d1 < 0)) return 0;
+  int it = 1, maxit = m->ls_iterations;
+  lspt_t lo = p0;
+  lspt_t pt = ls_eval(d, -p0.d1 / p0.d2, 0, gq);
+  if (pt.nchange == 0) { *exact = 1; return pt.alpha; }
+  if (fabs(pt.d1) <= gtol) return pt.alpha;
+  while (pt.d1 < 0 && it < maxit) {
+    lo = pt;
+    pt = ls_eval(d, pt.alpha - pt.d1 / pt.d2, lo.alpha, gq);
+    it++;
+    if (pt.nchange == 0 || fabs(pt.d1) <= gtol) return pt.alpha;
+  }
+  if (pt.d1 < 0) return pt.alpha;
+  lspt_t hi = pt;
+  while (it < maxit) {
+    double a = lo.alpha - lo.d1 / lo.d2;
+    int newton = a > lo.alpha && a < hi.alpha;
+    if (!newton) a = lo.alpha - lo.d1 * (hi.alpha - lo.alpha) / (hi.d1 - lo.d1);
+    if (!(a > lo.alpha && a < hi.alpha)) a = 0.5 * (lo.alpha + hi.alpha);
+    pt = ls_eval(d, a, lo.alpha, gq);
+    it++;
+    if ((newton && pt.nchange == 0) || fabs(pt.d1) <= gtol) return pt.alpha;
+    if (pt.d1 < 0) lo = pt; else hi = pt;
+    if (!(hi.alpha > lo.alpha)) break;
+  }
+  return lo.f <= hi.f ? lo.alpha : hi.alpha;
+}
+
+/* H = M + J^T diag(D * active) J, dense Cholesky in place; returns 0 on success */
+static int hessian_factor(const orc_prep_t* p, orc_data_t* d) {
+  int nv = p->m.nv;
+  double* H = d->H;
+  memcpy(H, d->qM, sizeof(double) * nv * nv);
+  for (int r = 0; r < d->nefc; r++) {
+    if (!d->efc_active[r]) continue;
+    const double* J = d->efc_J + (size_t)r * nv;
+    double D = d->efc_D[r];
+    for (int i = 0; i < nv; i++) {
+      if (J[i] == 0) continue;
+      double a = D * J[i];
+      for (int j = 0; j <= i; j++) H[i * nv + j] += a * J[j];
+    }
+  }
+  for (int k = 0; k < nv; k++) {
+    double s = H[k * nv + k];
+    for (int q = 0; q < k; q++) s -= H[k * nv + q] * H[k * nv + q];
+    if (!(s > 0)) return -1;
+    double l = sqrt(s);
+    H[k * nv + k] = l;
+    for (int i = k + 1; i < nv; i++) {
+      double t = H[i * nv + k];
+      for (int q = 0; q < k; q++) t -= H[i * nv + q] * H[k * nv + q];
+      H[i * nv + k] = t / l;
+    }
+  }
+  return 0;
+}
+static void hessian_solve(const orc_prep_t* p, const orc_data_t* d, const double* b, double* x) {
+  int nv = p->m.nv;
+  const double* L = d->H;
+  for (int i = 0; i < nv; i++) {
+    double s = b[i];
+    for (int q = 0; q < i; q++) s -= L[i * nv + q] * x[q];
+    x[i] = s / L[i * nv + i];
+  }
+  for (int i = nv - 1; i >= 0; i--) {
+    double s = x[i];
+    for (int q = i + 1; q < nv; q++) s -= L[q * nv + i] * x[q];
+    x[i] = s / L[i * nv + i];
+  }
+}
+
+/* mj_fwdConstraint: warm start (previous qacc vs qacc_smooth by total cost),
+ * then MuJoCo's primal Newton (H^-1 direction) or nonlinear CG (M^-1
+ * preconditioned Polak-Ribiere) with exact line search; stop when
+ * scale*(cost decrease) < tol or scale*|grad| < tol, scale = 1/(meaninertia*nv). */
+static int solve(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  int nv = m->nv;
+  d->niter = 0;
+  if (d->nefc == 0) {
+    memcpy(d->qacc, d->qacc_smooth, sizeof(double) * nv);
+    memset(d->qfrc_constraint, 0, sizeof(double) * nv);
+    return 0;
+  }
+  set_qacc(p, d, d->qacc_smooth);
+  cost_grad(p, d);
+  if (!(m->disableflags & DS_WARMSTART)) {
+    double c_smooth = d->cost;
+    double* keep = d->tmp2;
+    memcpy(keep, d->qacc_smooth, sizeof(double) * nv);
+    set_qacc(p, d, d->qacc_ws);
+    cost_grad(p, d);
+    if (d->cost > c_smooth) { set_qacc(p, d, keep); cost_grad(p, d); }
+  }
+  double scale = 1.0 / (p->meaninertia * (nv > 1 ? nv : 1));
+  int newton = m->solver == 2;
+  int status = 0;
+  if (newton) {
+    if (hessian_factor(p, d) != 0) return 16;
+    hessian_solve(p, d, d->grad, d->Mgrad);
+  } else {
+    memcpy(d->Mgrad, d->grad, sizeof(double) * nv);
+    solve_m(p, d, d->Mgrad);
+  }
+  for (int k = 0; k < nv; k++) d->search[k] = -d->Mgrad[k];
+  double* grad_old = (double*)calloc(nv, sizeof(double));
+  double* mgrad_old = (double*)calloc(nv, sizeof(double));
+  for (int it = 0; it < m->iterations; it++) {
+    int exact = 0;
+    double alpha = linesearch(p, d, &exact);
+    d->niter = it + 1;
+    if (alpha == 0) break;
+    for (int k = 0; k < nv; k++) {
+      d->qacc[k] += alpha * d->search[k];
+      d->Ma[k] += alpha * d->Mv[k];
+    }
+    for (int r = 0; r < d->nefc; r++) d->efc_Jaref[r] += alpha * d->efc_Jv[r];
+    double oldcost = d->cost;
+    memcpy(grad_old, d->grad, sizeof(double) * nv);
+    memcpy(mgrad_old, d->Mgrad, sizeof(double) * nv);
+    cost_grad(p, d);
+    double gn = 0;
+    for (int k = 0; k < nv; k++) gn += d->grad[k] * d->grad[k];
+    double improvement = scale * (oldcost - d->cost), gradient = scale * sqrt(gn);
+    if (improvement < m->tolerance || gradient < m->tolerance) break;
+    /* a Newton step that stayed inside one quadratic piece landed on that
+     * piece's stationary point, which is the exact minimiser */
+    if (newton && exact) break;
+    if (newton) {
+      if (hessian_factor(p, d) != 0) { status = 16; break; }
+      hessian_solve(p, d, d->grad, d->Mgrad);
+    } else {
+      memcpy(d->Mgrad, d->grad, sizeof(double) * nv);
+      solve_m(p, d, d->Mgrad);
+    }
+    if (newton) {
+      for (int k = 0; k < nv; k++) d->search[k] = -d->Mgrad[k];
+    } else {
+      double num = 0, den = 0;
+      for (int k = 0; k < nv; k++) { num += d->grad[k] * (d->Mgrad[k] - mgrad_old[k]); den += grad_old[k] * mgrad_old[k]; }
+      double beta = num / fmax(MINVAL, den);
+      if (beta < 0) beta = 0;
+      for (int k = 0; k < nv; k++) d->search[k] = -d->Mgrad[k] + beta * d->search[k];
+    }
+  }
+  free(grad_old); free(mgrad_old);
+  for (int k = 0; k < nv; k++) {
+    double s = 0;
+    for (int r = 0; r < d->nefc; r++) s += d->efc_J[(size_t)r * nv + k] * d->efc_force[r];
+    d->qfrc_constraint[k] = s;
+  }
+  return status;
+}
+
+/* ------------------------------------------------------------------ pipeline */
+int orc_forward(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  int nv = m->nv;
+  int status = 0;
+  kinematics(p, d);
+  com_pos(p, d);
+  crb_mass(p, d);
+  if (nv > 0 && factor_m(p, d) != 0) return 32;
+  status |= collision(p, d);
+  status |= make_constraint(p, d);
+  com_vel(p, d);
+  passive(p, d);
+  rne_bias(p, d);
+  actuation(p, d);
+  for (int k = 0; k < nv; k++) d->qfrc_smooth[k] = d->qfrc_passive[k] - d->qfrc_bias[k] + d->qfrc_actuator[k];
+  memcpy(d->qacc_smooth, d->qfrc_smooth, sizeof(double) * nv);
+  if (nv > 0) solve_m(p, d, d->qacc_smooth);
+  /* aref needs efc_vel which needs qvel only: already computed in make_constraint */
+  status |= solve(p, d);
+  return status;
+}
+
+void orc_kinematics(const orc_prep_t* p, orc_data_t* d) {
+  kinematics(p, d);
+  com_pos(p, d);
+  crb_mass(p, d);
+  if (p->m.nv > 0) factor_m(p, d);
+  com_vel(p, d);
+}
+
+/* mj_integratePos: hinge/slide add h*qvel; ball/free rotate the quaternion by
+ * exp(h*omega/2) with omega in the local frame (reference euler.hpp:55-104). */
+static void integrate_pos(const orc_model_t* m, double* qpos, const double* qvel, double h) {
+  for (int j = 0; j < m->njnt; j++) {
+    int qa = m->jnt_qposadr[j], da = m->jnt_dofadr[j];
+    int t = m->jnt_type[j];
+    if (t == JFREE) {
+      for (int i = 0; i < 3; i++) qpos[qa + i] += h * qvel[da + i];
+      qa += 3; da += 3;
+    }
+    if (t == JFREE || t == JBALL) {
+      double w[3] = {qvel[da], qvel[da + 1], qvel[da + 2]};
+      double ang = h * normalize3(w);
+      double qr[4], *q = qpos + qa;
+      if (ang == 0) { qr[0] = 1; qr[1] = qr[2] = qr[3] = 0; }
+      else axisangle2quat(w, ang, qr);
+      normalize4(q);
+      mulquat(q, qr, q);
+      normalize4(q);
+    } else {
+      qpos[qa] += h * qvel[da];
+    }
+  }
+}
+
+/* Euler (implicit joint damping when present) and implicitfast (actuator and
+ * passive velocity derivatives, Coriolis dropped); then qvel += h*qacc,
+ * qpos integrated with the new qvel, qacc_warmstart = solver qacc. */
+static int integrate(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  int nv = m->nv;
+  double h = m->timestep;
+  double* acc = d->tmp1;
+  memcpy(acc, d->qacc, sizeof(double) * nv);
+  int damp = 0;
+  for (int k = 0; k < nv; k++) if (m->dof_damping[k] > 0) damp = 1;
+  int implicit = m->integrator == 3 || m->integrator == 2;
+  if (implicit || (damp && !(m->disableflags & DS_EULERDAMP))) {
+    double* A = (double*)calloc((size_t)nv * nv, sizeof(double));
+    memcpy(A, d->qM, sizeof(double) * nv * nv);
+    for (int k = 0; k < nv; k++) A[k * nv + k] += h * m->dof_damping[k];
+    if (implicit) {
+      for (int u = 0; u < m->nu; u++) {
+        if (m->disableflags & DS_ACTUATION) break;
+        int j = m->actuator_trnid[u], da = m->jnt_dofadr[j];
+        double gear = m->actuator_gear[u];
+        double f = d->actuator_force[u];
+        if (m->actuator_forcelimited[u] && (f <= m->actuator_forcerange[2 * u] || f >= m->actuator_forcerange[2 * u + 1])) continue;
+        A[da * nv + da] -= h * gear * gear * m->actuator_biasprm[3 * u + 2];
+      }
+    }
+    double* rhs = d->tmp2;
+    for (int k = 0; k < nv; k++) rhs[k] = d->qfrc_smooth[k] + d->qfrc_constraint[k];
+    if (factor_m_into(m, A, A) != 0) { free(A); return 32; }
+    memcpy(acc, rhs, sizeof(double) * nv);
+    solve_ld(m, A, acc);
+    free(A);
+  }
+  for (int k = 0; k < nv; k++) d->qvel[k] += h * acc[k];
+  integrate_pos(m, d->qpos, d->qvel, h);
+  memcpy(d->qacc_ws, d->qacc, sizeof(double) * nv);
+  return 0;
+}
+
+int orc_step(const orc_prep_t* p, orc_data_t* d) {
+  int st = orc_forward(p, d);
+  st |= integrate(p, d);
+  d->status = st;
+  return st;
+}
+
+int orc_ncon(const orc_data_t* d) { return d->ncon; }
+const orc_contact_t* orc_contacts(const orc_data_t* d) { return d->con; }
+int orc_nefc(const orc_data_t* d) { return d->nefc; }
+int orc_solver_niter(const orc_data_t* d) { return d->niter; }
+
+int orc_get(const orc_data_t* d, const char* name, double* out, int cap) {
+  struct { const char* n; const double* p; int len; } f[] = {
+      {"qpos", d->qpos, d->nq}, {"qvel", d->qvel, d->nv}, {"qacc", d->qacc, d->nv},
+      {"qacc_smooth", d->qacc_smooth, d->nv}, {"qacc_warmstart", d->qacc_ws, d->nv},
+      {"xpos", d->xpos, 3 * d->nbody}, {"xquat", d->xquat, 4 * d->nbody}, {"xmat", d->xmat, 9 * d->nbody},
+      {"xipos", d->xipos, 3 * d->nbody}, {"ximat", d->ximat, 9 * d->nbody},
+      {"xanchor", d->xanchor, 3 * d->njnt}, {"xaxis", d->xaxis, 3 * d->njnt},
+      {"geom_xpos", d->geom_xpos, 3 * d->ngeom}, {"geom_xmat", d->geom_xmat, 9 * d->ngeom},
+      {"subtree_com", d->subtree_com, 3 * d->nbody}, {"cinert", d->cinert, 10 * d->nbody},
+      {"cdof", d->cdof, 6 * d->nv}, {"cvel", d->cvel, 6 * d->nbody}, {"cdof_dot", d->cdof_dot, 6 * d->nv},
+      {"crb", d->crb, 10 * d->nbody}, {"qM", d->qM, d->nv * d->nv}, {"qLD", d->qLD, d->nv * d->nv},
+      {"qfrc_bias", d->qfrc_bias, d->nv}, {"qfrc_passive", d->qfrc_passive, d->nv},
+      {"qfrc_actuator", d->qfrc_actuator, d->nv}, {"qfrc_smooth", d->qfrc_smooth, d->nv},
+      {"qfrc_constraint", d->qfrc_constraint, d->nv}, {"actuator_force", d->actuator_force, d->nu},
+      {"efc_J", d->efc_J, d->nefc * d->nv}, {"efc_D", d->efc_D, d->nefc}, {"efc_R", d->efc_R, d->nefc},
+      {"efc_aref", d->efc_aref, d->nefc}, {"efc_force", d->efc_force, d->nefc}, {"efc_pos", d->efc_pos, d->nefc},
+      {"efc_vel", d->efc_vel, d->nefc}, {"efc_imp", d->efc_imp, d->nefc}, {"efc_diagApprox", d->efc_diagApprox, d->nefc},
+  };
+  for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); i++) {
+    if (strcmp(f[i].n, name) == 0) {
+      int n = f[i].len < cap ? f[i].len : cap;
+      if (out && n > 0) memcpy(out, f[i].p, sizeof(double) * n);
+      return f[i].len;
+    }
+  }
+  if (strcmp(name, "efc_type") == 0 || strcmp(name, "efc_id") == 0) {
+    const int* src = name[4] == 't' ? d->efc_type : d->efc_id;
+    int n = d->nefc < cap ? d->nefc : cap;
+    for (int i = 0; i < n; i++) out[i] = src[i];
+    return d->nefc;
+  }
+  return -1;
+}
+
+/* ------------------------------------------------------------------ batch */
+typedef struct {
+  const orc_prep_t* p;
+  int nenv, nsub, tid, nthreads;
+  double *qpos, *qvel, *qacc_ws;
+  const double* ctrl;
+  int* niter;
+  int bad;
+} job_t;
+
+static void* batch_worker(void* arg) {
+  job_t* j = (job_t*)arg;
+  const orc_model_t* m = &j->p->m;
+  orc_data_t* d = orc_data_new(j->p);
+  for (int e = j->tid; e < j->nenv; e += j->nthreads) {
+    orc_set_state(d, j->qpos + (size_t)e * m->nq, j->qvel + (size_t)e * m->nv, j->qacc_ws + (size_t)e * m->nv,
+                  j->ctrl ? j->ctrl + (size_t)e * m->nu : NULL);
+    int st = 0, it = 0;
+    for (int s = 0; s < j->nsub; s++) { st |= orc_step(j->p, d); it += d->niter; }
+    if (st) j->bad++;
+    if (j->niter) j->niter[e] = it;
+    orc_get_state(d, j->qpos + (size_t)e * m->nq, j->qvel + (size_t)e * m->nv, j->qacc_ws + (size_t)e * m->nv);
+  }
+  orc_data_free(d);
+  return NULL;
+}
+
+int orc_batch_step(const orc_prep_t* p, int nenv, double* qpos, double* qvel, double* qacc_ws, const double* ctrl,
+                   int nsub, int nthreads, int* niter_out) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > nenv) nthreads = nenv > 0 ? nenv : 1;
+  pthread_t th[256];
+  job_t jobs[256];
+  if (nthreads > 256) nthreads = 256;
+  for (int t = 0; t < nthreads; t++) {
+    jobs[t].p = p; jobs[t].nenv = nenv; jobs[t].nsub = nsub; jobs[t].tid = t; jobs[t].nthreads = nthreads;
+    jobs[t].qpos = qpos; jobs[t].qvel = qvel; jobs[t].qacc_ws = qacc_ws; jobs[t].ctrl = ctrl;
+    jobs[t].niter = niter_out; jobs[t].bad = 0;
+  }
+  if (nthreads == 1) batch_worker(&jobs[0]);
+  else {
+    for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, batch_worker, &jobs[t]);
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  }
+  int bad = 0;
+  for (int t = 0; t < nthreads; t++) bad += jobs[t].bad;
+  return bad;
+}

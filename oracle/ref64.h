@@ -1,0 +1,111 @@
+/* ref64: FP64 CPU oracle for the batched MuJoCo step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * The engine never links, loads or calls it.
+ *
+ * It restates, in double precision, the MuJoCo C 3.5.0 pipeline that the
+ * reference uses as its oracle (reference src/mjb.cpp:807-839 calls mj_step per
+ * env; tests/test_gpu_vs_cpu.cpp compares against it), because MuJoCo C itself
+ * is absent from this environment. Each function cites the reference file:line
+ * whose semantics it follows, or states the MuJoCo behaviour it restates.
+ */
+#ifndef REF64_H
+#define REF64_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Raw compiled model (mjModel field names). Pointers are borrowed. */
+typedef struct {
+  int nq, nv, nu, nbody, njnt, ngeom;
+  double timestep, gravity[3], tolerance, ls_tolerance, impratio;
+  int iterations, ls_iterations, solver, cone, integrator, disableflags;
+  const int *body_parentid, *body_rootid, *body_weldid, *body_jntnum, *body_jntadr, *body_dofnum, *body_dofadr;
+  const double *body_pos, *body_quat, *body_ipos, *body_iquat, *body_mass, *body_inertia;
+  const int *jnt_type, *jnt_qposadr, *jnt_dofadr, *jnt_bodyid, *jnt_limited;
+  const double *jnt_pos, *jnt_axis, *jnt_range, *jnt_margin, *jnt_solref, *jnt_solimp, *jnt_stiffness;
+  const int *dof_bodyid, *dof_jntid, *dof_parentid;
+  const double *dof_armature, *dof_damping, *dof_frictionloss, *dof_solref, *dof_solimp;
+  const int *geom_type, *geom_contype, *geom_conaffinity, *geom_condim, *geom_bodyid, *geom_priority;
+  const double *geom_size, *geom_pos, *geom_quat, *geom_friction, *geom_solref, *geom_solimp,
+      *geom_margin, *geom_gap, *geom_solmix, *geom_rbound;
+  const int *actuator_trnid, *actuator_ctrllimited, *actuator_forcelimited;
+  const double *actuator_gear, *actuator_gainprm, *actuator_biasprm, *actuator_ctrlrange, *actuator_forcerange;
+  int nexclude;
+  const int* exclude_signature;
+  int npair;
+  const int *pair_geom1, *pair_geom2, *pair_dim;
+  const double *pair_friction, *pair_solref, *pair_solimp, *pair_margin, *pair_gap;
+  const double *qpos0, *qpos_spring;
+} orc_model_t;
+
+/* Limits of the oracle's per-env work arrays. */
+#define ORC_MAXCON 512
+#define ORC_MAXEFC 2048
+
+typedef struct {
+  int g1, g2, b1, b2, condim;
+  double friction[5], solref[2], solimp[5], margin, gap;
+} orc_pair_t;
+
+typedef struct {
+  double dist, pos[3], frame[9];
+  int geom1, geom2, pair, dim;
+  double friction[5], solref[2], solimp[5], includemargin;
+} orc_contact_t;
+
+/* Prepared model: raw fields + derived tables (pair table, invweight0,
+ * meaninertia) computed by the oracle itself from the raw fields. */
+typedef struct orc_prep orc_prep_t;
+
+orc_prep_t* orc_prepare(const orc_model_t* m);
+void orc_free(orc_prep_t* p);
+int orc_npair(const orc_prep_t* p);
+const orc_pair_t* orc_pairs(const orc_prep_t* p);
+const double* orc_body_invweight0(const orc_prep_t* p);
+const double* orc_dof_invweight0(const orc_prep_t* p);
+double orc_meaninertia(const orc_prep_t* p);
+
+/* Per-env working data of one step (everything the pipeline computes). */
+typedef struct orc_data orc_data_t;
+orc_data_t* orc_data_new(const orc_prep_t* p);
+void orc_data_free(orc_data_t* d);
+
+/* state in / out */
+void orc_set_state(orc_data_t* d, const double* qpos, const double* qvel, const double* qacc_ws, const double* ctrl);
+void orc_get_state(const orc_data_t* d, double* qpos, double* qvel, double* qacc_ws);
+
+/* One physics step (mj_step): forward pipeline, solver, integrator.
+ * Returns 0, or a negative status (>0 capacity overflow bits). */
+int orc_step(const orc_prep_t* p, orc_data_t* d);
+/* Forward pipeline only (no integration), leaving every field readable. */
+int orc_forward(const orc_prep_t* p, orc_data_t* d);
+/* Kinematics-only pieces (for unit checks). */
+void orc_kinematics(const orc_prep_t* p, orc_data_t* d);
+
+/* Field access: copies a named field into out (capacity cap doubles);
+ * returns the number of doubles, or -1 for an unknown name. */
+int orc_get(const orc_data_t* d, const char* name, double* out, int cap);
+int orc_ncon(const orc_data_t* d);
+const orc_contact_t* orc_contacts(const orc_data_t* d);
+int orc_nefc(const orc_data_t* d);
+int orc_solver_niter(const orc_data_t* d);
+
+/* Narrow-phase entry for unit tests: collide one prepared pair given the two
+ * geoms' world poses; writes up to maxc contacts, returns the count. */
+int orc_collide_pair(const orc_prep_t* p, int pair, const double* xpos1, const double* xmat1,
+                     const double* xpos2, const double* xmat2, orc_contact_t* out, int maxc);
+
+/* Batched stepping: nsub substeps per env with the same ctrl, over nthreads
+ * POSIX threads (env i goes to thread i % nthreads, as reference
+ * src/mjb.cpp:826-838 assigns envs). Arrays are env-major. Returns the number
+ * of envs whose step reported an overflow. */
+int orc_batch_step(const orc_prep_t* p, int nenv, double* qpos, double* qvel, double* qacc_ws,
+                   const double* ctrl, int nsub, int nthreads, int* niter_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
