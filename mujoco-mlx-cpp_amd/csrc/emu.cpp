@@ -123,3 +123,19 @@ __attribute__((visibility("default"))) void mjemu_stats(void* p, int* niter, int
 extern "C" __attribute__((visibility("default"))) int mjemu_model_field(void* p, const char* name, double* out, int cap) {
   return mjh::model_field(static_cast<EmuSim*>(p)->model, name, out, cap);
 }
+
+// compile only (no kernel tables): model fields of any MJCF the compiler accepts
+extern "C" __attribute__((visibility("default"))) void* mjemu_compile(const char* xml, int self_collision, int max_condim) {
+  try {
+    auto* s = new EmuSim();
+    mjh::LoadOptions o;
+    o.self_collision = self_collision;
+    o.max_condim = max_condim;
+    s->model = mjh::compile_mjcf(xml, o);
+    s->B = 0;
+    return s;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "mjemu_compile: %s\n", e.what());
+    return nullptr;
+  }
+}

@@ -1191,6 +1191,37 @@ static double linesearch(const orc_prep_t* p, orc_data_t* d, int* exact) {
   return lo.f <= hi.f ? lo.alpha : hi.alpha;
 }
 
+/* dense Cholesky A = L L^T in place (lower triangle), n x n row-major */
+static int chol_dense(double* A, int n) {
+  for (int k = 0; k < n; k++) {
+    double s = A[k * n + k];
+    for (int q = 0; q < k; q++) s -= A[k * n + q] * A[k * n + q];
+    if (!(s > 0)) return -1;
+    double l = sqrt(s);
+    A[k * n + k] = l;
+    for (int i = k + 1; i < n; i++) {
+      double t = A[i * n + k];
+      for (int q = 0; q < k; q++) t -= A[i * n + q] * A[k * n + q];
+      A[i * n + k] = t / l;
+    }
+    for (int j = k + 1; j < n; j++) A[k * n + j] = 0;
+  }
+  return 0;
+}
+/* x = (L L^T)^-1 b with L from chol_dense */
+static void chol_solve(const double* L, int n, const double* b, double* x) {
+  for (int i = 0; i < n; i++) {
+    double s = b[i];
+    for (int q = 0; q < i; q++) s -= L[i * n + q] * x[q];
+    x[i] = s / L[i * n + i];
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double s = x[i];
+    for (int q = i + 1; q < n; q++) s -= L[q * n + i] * x[q];
+    x[i] = s / L[i * n + i];
+  }
+}
+
 /* H = M + J^T diag(D * active) J, dense Cholesky in place; returns 0 on success */
 static int hessian_factor(const orc_prep_t* p, orc_data_t* d) {
   int nv = p->m.nv;
@@ -1206,33 +1237,10 @@ static int hessian_factor(const orc_prep_t* p, orc_data_t* d) {
       for (int j = 0; j <= i; j++) H[i * nv + j] += a * J[j];
     }
   }
-  for (int k = 0; k < nv; k++) {
-    double s = H[k * nv + k];
-    for (int q = 0; q < k; q++) s -= H[k * nv + q] * H[k * nv + q];
-    if (!(s > 0)) return -1;
-    double l = sqrt(s);
-    H[k * nv + k] = l;
-    for (int i = k + 1; i < nv; i++) {
-      double t = H[i * nv + k];
-      for (int q = 0; q < k; q++) t -= H[i * nv + q] * H[k * nv + q];
-      H[i * nv + k] = t / l;
-    }
-  }
-  return 0;
+  return chol_dense(H, nv);
 }
 static void hessian_solve(const orc_prep_t* p, const orc_data_t* d, const double* b, double* x) {
-  int nv = p->m.nv;
-  const double* L = d->H;
-  for (int i = 0; i < nv; i++) {
-    double s = b[i];
-    for (int q = 0; q < i; q++) s -= L[i * nv + q] * x[q];
-    x[i] = s / L[i * nv + i];
-  }
-  for (int i = nv - 1; i >= 0; i--) {
-    double s = x[i];
-    for (int q = i + 1; q < nv; q++) s -= L[q * nv + i] * x[q];
-    x[i] = s / L[i * nv + i];
-  }
+  chol_solve(d->H, p->m.nv, b, x);
 }
 
 /* mj_fwdConstraint: warm start (previous qacc vs qacc_smooth by total cost),
@@ -1349,6 +1357,18 @@ void orc_kinematics(const orc_prep_t* p, orc_data_t* d) {
   com_vel(p, d);
 }
 
+/* q <- q * exp(h*omega/2), omega in the local frame, renormalised */
+static void quat_integrate(double* q, const double* omega, double h) {
+  double w[3] = {omega[0], omega[1], omega[2]};
+  double ang = h * normalize3(w);
+  double qr[4];
+  if (ang == 0) { qr[0] = 1; qr[1] = qr[2] = qr[3] = 0; }
+  else axisangle2quat(w, ang, qr);
+  normalize4(q);
+  mulquat(q, qr, q);
+  normalize4(q);
+}
+
 /* mj_integratePos: hinge/slide add h*qvel; ball/free rotate the quaternion by
  * exp(h*omega/2) with omega in the local frame (reference euler.hpp:55-104). */
 static void integrate_pos(const orc_model_t* m, double* qpos, const double* qvel, double h) {
@@ -1360,14 +1380,7 @@ static void integrate_pos(const orc_model_t* m, double* qpos, const double* qvel
       qa += 3; da += 3;
     }
     if (t == JFREE || t == JBALL) {
-      double w[3] = {qvel[da], qvel[da + 1], qvel[da + 2]};
-      double ang = h * normalize3(w);
-      double qr[4], *q = qpos + qa;
-      if (ang == 0) { qr[0] = 1; qr[1] = qr[2] = qr[3] = 0; }
-      else axisangle2quat(w, ang, qr);
-      normalize4(q);
-      mulquat(q, qr, q);
-      normalize4(q);
+      quat_integrate(qpos + qa, qvel + da, h);
     } else {
       qpos[qa] += h * qvel[da];
     }
@@ -1506,4 +1519,36 @@ int orc_batch_step(const orc_prep_t* p, int nenv, double* qpos, double* qvel, do
   int bad = 0;
   for (int t = 0; t < nthreads; t++) bad += jobs[t].bad;
   return bad;
+}
+
+/* ------------------------------------------------------------------ math KAT entry */
+/* Exposes the oracle's own math routines for known-answer tests. */
+int orc_math(const char* op, const double* in, double* out) {
+  if (strcmp(op, "quat_mul") == 0) { mulquat(in, in + 4, out); return 4; }
+  if (strcmp(op, "quat2mat") == 0) { quat2mat(in, out); return 9; }
+  if (strcmp(op, "quat_rotate") == 0) { double R[9]; quat2mat(in, R); mulmatvec3(R, in + 4, out); return 3; }
+  if (strcmp(op, "axisangle") == 0) { axisangle2quat(in, in[3], out); return 4; }
+  if (strcmp(op, "quat_integrate") == 0) { memcpy(out, in, 4 * sizeof(double)); quat_integrate(out, in + 4, in[7]); return 4; }
+  if (strcmp(op, "cross") == 0) { cross3(in, in + 3, out); return 3; }
+  if (strcmp(op, "mul_inert") == 0) { mul_inert_vec(in, in + 10, out); return 6; }
+  if (strcmp(op, "cross_motion") == 0) { cross_motion(in, in + 6, out); return 6; }
+  if (strcmp(op, "cross_force") == 0) { cross_force(in, in + 6, out); return 6; }
+  if (strcmp(op, "seg_seg") == 0) { seg_seg(in, in + 3, in + 6, in + 9, out, out + 1); return 2; }
+  if (strcmp(op, "make_frame") == 0) { make_frame(in, out); return 9; }
+  if (strcmp(op, "impedance") == 0) { out[0] = impedance(in, in[5]); return 1; }
+  if (strcmp(op, "cholesky") == 0) {
+    int n = (int)in[0];
+    memcpy(out, in + 1, sizeof(double) * n * n);
+    return chol_dense(out, n) == 0 ? n * n : -1;
+  }
+  if (strcmp(op, "chol_solve") == 0) {
+    int n = (int)in[0];
+    double* L = (double*)malloc(sizeof(double) * n * n);
+    memcpy(L, in + 1, sizeof(double) * n * n);
+    int ok = chol_dense(L, n);
+    if (ok == 0) chol_solve(L, n, in + 1 + n * n, out);
+    free(L);
+    return ok == 0 ? n : -1;
+  }
+  return -1;
 }

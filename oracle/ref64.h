@@ -105,6 +105,12 @@ int orc_collide_pair(const orc_prep_t* p, int pair, const double* xpos1, const d
 int orc_batch_step(const orc_prep_t* p, int nenv, double* qpos, double* qvel, double* qacc_ws,
                    const double* ctrl, int nsub, int nthreads, int* niter_out);
 
+/* The oracle's own math routines, for known-answer tests: quat_mul,
+ * quat2mat, quat_rotate, axisangle, quat_integrate, cross, mul_inert,
+ * cross_motion, cross_force, seg_seg, make_frame, impedance, cholesky,
+ * chol_solve. Returns the output length, -1 on failure. */
+int orc_math(const char* op, const double* in, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -255,3 +255,18 @@ def fields_from(getter) -> dict:
         else:
             out[name] = v
     return out
+
+
+def math(op: str, inp, nout: int = 64) -> np.ndarray:
+    """Call one of the oracle's math routines (orc_math) on a flat input."""
+    L = lib()
+    if not hasattr(L, "_math_set"):
+        L.orc_math.argtypes = [C.c_char_p, _D, _D]
+        L.orc_math.restype = C.c_int
+        L._math_set = True
+    a = np.ascontiguousarray(np.asarray(inp, dtype=np.float64).ravel())
+    out = np.zeros(max(nout, 64) + a.size)
+    n = L.orc_math(op.encode(), _dp(a), _dp(out))
+    if n < 0:
+        raise ValueError(f"orc_math({op}) failed")
+    return out[:n].copy()

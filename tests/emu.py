@@ -37,6 +37,8 @@ def lib():
         L.mjemu_dump.argtypes = [C.c_void_p, C.c_int]
         L.mjemu_stats.argtypes = [C.c_void_p, _I, _I, _I, _I]
         L.mjemu_model_field.argtypes = [C.c_void_p, C.c_char_p, _D, C.c_int]
+        L.mjemu_compile.restype = C.c_void_p
+        L.mjemu_compile.argtypes = [C.c_char_p, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -103,3 +105,19 @@ class EmuSim:
         if kind == 1:
             return seg.view(np.int32)
         return seg
+
+
+class CompiledModel:
+    """Model fields of an MJCF compiled by the engine's compiler, without
+    building kernel tables (accepts pair types the kernels do not support)."""
+
+    def __init__(self, xml: str, self_collision: int = 1, max_condim: int = 0):
+        self.h = lib().mjemu_compile(xml.encode(), self_collision, max_condim)
+        if not self.h:
+            raise RuntimeError("mjemu_compile failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mjemu_free(self.h)
+
+    field = EmuSim.field
