@@ -44,6 +44,14 @@ def build_engine():
     return out
 
 
+def build_prof():
+    """Profiling build of the engine (per-stage cycle counters compiled in)."""
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libmjmlx_prof.so")
+    _run([HIPCC, *HIP_FLAGS, "-DMJK_PROFILE", "-shared", f"-I{INC}", f"-I{CSRC}", "-o", out, "engine.hip", *HOST_SRC])
+    return out
+
+
 def build_mjb():
     out = os.path.join(LIB, "libmjb.so")
     _run([HIPCC, "-std=c++17", "-O2", "-fPIC", "-shared", f"-I{INC}", f"-I{CSRC}", "-o", out, "mjb.cpp",
@@ -68,4 +76,4 @@ def build_all():
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "engine": build_engine, "mjb": build_mjb, "emu": build_emu}[w]()
+        {"all": build_all, "engine": build_engine, "mjb": build_mjb, "emu": build_emu, "prof": build_prof}[w]()

@@ -71,7 +71,7 @@ struct Layout {
   int xpos, xquat, xmat, xipos, xanchor, xaxis, gxpos, gxmat, scom, cinert, cdof, cvel, cdofdot, crb, cacc, cfrc, buf6;
   int H;
   int con, efc_J, efc_D, efc_R, efc_aref, efc_Jaref, efc_Jv, efc_force, efc_floss, efc_pos, efc_margin,
-      efc_type, efc_act, efc_id;
+      efc_type, efc_act, efc_id, efc_Dact;
   int Ma, grad, Mgrad, search, Mv, grad_old, Mgrad_old, tmpv, dvec;
 };
 
@@ -145,6 +145,7 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.efc_type = o; o += maxefc;
   L.efc_act = o; o += maxefc;
   L.efc_id = o; o += maxefc;
+  L.efc_Dact = o; o += maxefc;
   L.Ma = o; o += nvp;
   L.grad = o; o += nvp;
   L.Mgrad = o; o += nvp;

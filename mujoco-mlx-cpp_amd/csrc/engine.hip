@@ -716,6 +716,32 @@ MJMLX_API int mjmlx_batched_debug_step(MjmlxBatchedSim* s, const float* ctrl, fl
   } catch (const std::exception& e) { report("mjmlx_batched_debug_step", e); return -1; }
 }
 
+// Per-stage cycle counts of one call (profiling builds only: compiled with
+// MJK_PROFILE); out is unsigned long long[num_envs * 16]. Returns 0, or -1
+// in builds without the instrumentation.
+MJMLX_API int mjmlx_batched_profile_step(MjmlxBatchedSim* s, const float* ctrl, int nsub, unsigned long long* out) {
+#if defined(MJK_PROFILE)
+  if (!s || !out || nsub <= 0) return -1;
+  try {
+    unsigned long long* d = nullptr;
+    size_t n = (size_t)s->B * mjk::KP_N;
+    HIPOK(hipMalloc(&d, n * sizeof(unsigned long long)));
+    upload_ctrl(s, ctrl);
+    mjk::StepIO io = make_io(s, nullptr, nullptr);
+    io.prof = d;
+    hipLaunchKernelGGL(k_step, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub);
+    HIPOK(hipGetLastError());
+    HIPOK(hipMemcpyAsync(out, d, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
+    HIPOK(hipStreamSynchronize(s->stream));
+    (void)hipFree(d);
+    return 0;
+  } catch (const std::exception& e) { report("mjmlx_batched_profile_step", e); return -1; }
+#else
+  (void)s; (void)ctrl; (void)nsub; (void)out;
+  return -1;
+#endif
+}
+
 MJMLX_API int mjmlx_batched_layout(const MjmlxBatchedSim* s, const char* name, int* off, int* len, int* kind) {
   if (!s || !name) return -1;
   if (std::strcmp(name, "total") == 0) { *off = 0; *len = s->L.total; *kind = 0; return 0; }

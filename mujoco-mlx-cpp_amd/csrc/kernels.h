@@ -30,40 +30,106 @@
 
 namespace mjk {
 
+// Per-stage cycle accounting, compiled in only for profiling builds
+// (MJK_PROFILE); the product build has empty marks.
+struct KProfAcc {
+  unsigned long long t[16];
+  unsigned long long last;
+};
+enum { KP_KIN = 0, KP_COMPOS, KP_CRB, KP_FACTOR, KP_COMVEL, KP_SMOOTH, KP_COLL, KP_CONSTR, KP_SOLVE_INIT, KP_LS,
+       KP_COSTGRAD, KP_DIR, KP_SOLVE_REST, KP_INTEGRATE, KP_IO, KP_SPARE, KP_N };
+#if defined(MJK_PROFILE) && !defined(MJ_EMU)
+#define KPROF_START(pa) ((pa)->last = clock64())
+#define KPROF_MARK(pa, i)                          \
+  do {                                             \
+    unsigned long long t_ = clock64();             \
+    (pa)->t[i] += t_ - (pa)->last;                 \
+    (pa)->last = t_;                               \
+  } while (0)
+#else
+#define KPROF_START(pa) ((void)0)
+#define KPROF_MARK(pa, i) ((void)(pa))
+#endif
+
 #define KMINVAL 1e-15f
 #define KMINIMP 0.0001f
 #define KMAXIMP 0.9999f
 
 // ------------------------------------------------------------------ wave helpers
+// Wave-wide reductions: DPP exchanges inside each 16-lane row (quad
+// permutations, half-row and row mirrors), then the four row totals are read
+// into scalar registers and added in a fixed order, so every lane gets the
+// same bit pattern and the result can drive uniform control flow.
+#if WAVE == 64
+KDEV int dpp_i(int v, int ctrl_sel) {
+  switch (ctrl_sel) {
+    case 0: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    case 1: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    case 2: return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    default: return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false); // row_mirror
+  }
+}
+KDEV float dpp_f(float v, int c) { return __int_as_float(dpp_i(__float_as_int(v), c)); }
+KDEV float rl_f(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
+KDEV double rl_d(double v, int lane) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+#endif
 KDEV float wsum(float v) {
 #if WAVE == 64
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-#endif
+  v += dpp_f(v, 0);
+  v += dpp_f(v, 1);
+  v += dpp_f(v, 2);
+  v += dpp_f(v, 3);
+  return (rl_f(v, 0) + rl_f(v, 16)) + (rl_f(v, 32) + rl_f(v, 48));
+#else
   return v;
+#endif
 }
 KDEV double wsumd(double v) {
 #if WAVE == 64
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-#endif
+  for (int c = 0; c < 4; c++) {
+    long long b = __double_as_longlong(v);
+    int lo = dpp_i((int)(b & 0xffffffffll), c), hi = dpp_i((int)(b >> 32), c);
+    v += __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+  return (rl_d(v, 0) + rl_d(v, 16)) + (rl_d(v, 32) + rl_d(v, 48));
+#else
   return v;
+#endif
 }
 KDEV int wsumi(int v) {
 #if WAVE == 64
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-#endif
+  for (int c = 0; c < 4; c++) v += dpp_i(v, c);
+  return (__builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16)) +
+         (__builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48));
+#else
   return v;
+#endif
 }
 KDEV int wori(int v) {
 #if WAVE == 64
-  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
-#endif
+  for (int c = 0; c < 4; c++) v |= dpp_i(v, c);
+  return __builtin_amdgcn_readlane(v, 0) | __builtin_amdgcn_readlane(v, 16) | __builtin_amdgcn_readlane(v, 32) |
+         __builtin_amdgcn_readlane(v, 48);
+#else
   return v;
+#endif
 }
 KDEV int wmini(int v) {
 #if WAVE == 64
-  for (int o = 32; o > 0; o >>= 1) { int t = __shfl_xor(v, o, 64); v = t < v ? t : v; }
-#endif
+  for (int c = 0; c < 4; c++) { int t = dpp_i(v, c); v = t < v ? t : v; }
+  int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+  int c2 = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+  a = a < b ? a : b;
+  c2 = c2 < d ? c2 : d;
+  return a < c2 ? a : c2;
+#else
   return v;
+#endif
 }
 // exclusive prefix sum over the wave
 KDEV int wscan_excl(int v) {
@@ -130,6 +196,34 @@ KDEV void cross3(const float* a, const float* b, float* o) {
 }
 KDEV float dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 KDEV float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// sum_k a[k*sa] * b[k*sb]: four independent partial sums so that the LDS
+// loads of consecutive terms overlap instead of serialising on one chain
+KDEV float kdot(const float* a, int sa, const float* b, int sb, int n) {
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  int k = 0;
+  for (; k + 3 < n; k += 4) {
+    s0 += a[k * sa] * b[k * sb];
+    s1 += a[(k + 1) * sa] * b[(k + 1) * sb];
+    s2 += a[(k + 2) * sa] * b[(k + 2) * sb];
+    s3 += a[(k + 3) * sa] * b[(k + 3) * sb];
+  }
+  for (; k < n; k++) s0 += a[k * sa] * b[k * sb];
+  return (s0 + s1) + (s2 + s3);
+}
+// sum_k c[k] * a[k*sa] * b[k*sb]
+KDEV float kdot3(const float* c, const float* a, int sa, const float* b, int sb, int n) {
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  int k = 0;
+  for (; k + 3 < n; k += 4) {
+    s0 += c[k] * a[k * sa] * b[k * sb];
+    s1 += c[k + 1] * a[(k + 1) * sa] * b[(k + 1) * sb];
+    s2 += c[k + 2] * a[(k + 2) * sa] * b[(k + 2) * sb];
+    s3 += c[k + 3] * a[(k + 3) * sa] * b[(k + 3) * sb];
+  }
+  for (; k < n; k++) s0 += c[k] * a[k * sa] * b[k * sb];
+  return (s0 + s1) + (s2 + s3);
+}
 
 KDEV void mul_inert(const float* i, const float* v, float* r) {
   r[0] = i[0] * v[0] + i[3] * v[1] + i[4] * v[2] - i[8] * v[4] + i[7] * v[5];
@@ -340,7 +434,10 @@ KDEV void st_crb(const DModel& m, const Layout& L, float* w) {
 
 // Tree-sparse L^T D L factorisation (MuJoCo mj_factorM) of A (dense storage)
 // into LD. Step k updates the entries (i, j), i and j strict ancestors of k,
-// j an ancestor of i, in parallel. Returns ST_M_NOT_PD on a non-positive pivot.
+// j an ancestor of i, in parallel, from the still unscaled row k (row k is
+// final once step k starts: its entries are only touched by steps of its
+// descendants, which have larger indices). Rows are scaled by 1/D in one
+// final pass. Returns ST_M_NOT_PD on a non-positive pivot.
 KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
   int nv = m.nv;
   for (int t = LANE; t < nv * nv; t += WAVE) LD[t] = A[t];
@@ -360,38 +457,44 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
       LD[i * nv + j] -= LD[k * nv + i] * inv * LD[k * nv + j];
     }
     KSYNC();
-    for (int a = 1 + LANE; a <= na; a += WAVE) LD[k * nv + ch[a]] *= inv;
-    KSYNC();
   }
+  for (int t = LANE; t < nv * nv; t += WAVE) {
+    int k = t / nv, i = t - k * nv;
+    if (i < k) {
+      float dk = LD[k * nv + k];
+      LD[t] = dk > KMINVAL ? LD[t] / dk : LD[t] / KMINVAL;
+    }
+  }
+  KSYNC();
   return bad ? ST_M_NOT_PD : 0;
 }
 
-// x <- (L^T D L)^-1 x, level-parallel over the dof tree.
+// x <- (L^T D L)^-1 x with L unit lower (entries outside the dof tree are 0).
+// GPU: lane i holds x_i; each column step broadcasts one solved entry with
+// readlane and updates the other lanes, with no LDS round trip per step.
 KDEV void solve_ld(const DModel& m, const float* LD, float* x) {
   int nv = m.nv;
-  for (int lv = m.ndoflevel - 1; lv >= 0; lv--) {
-    int a = m.doflevel_adr[lv], n = m.doflevel_adr[lv + 1] - a;
-    for (int t = LANE; t < n; t += WAVE) {
-      int i = m.doflevel_dof[a + t];
-      float s = x[i];
-      for (int k = i + 1; k < m.dof_subtree_end[i]; k++) s -= LD[k * nv + i] * x[k];
-      x[i] = s;
-    }
-    KSYNC();
+#if WAVE == 64
+  float v = LANE < nv ? x[LANE] : 0.f;
+  for (int k = nv - 1; k >= 1; k--) {  // L^T y = x: column k updates rows i < k
+    float xk = rl_f(v, k);
+    if (LANE < k) v -= LD[k * nv + LANE] * xk;
   }
-  for (int i = LANE; i < nv; i += WAVE) x[i] /= LD[i * nv + i];
+  if (LANE < nv) v /= LD[LANE * nv + LANE];
+  for (int i = 0; i < nv - 1; i++) {   // L z = y: solved entry i updates rows k > i
+    float xi = rl_f(v, i);
+    if (LANE > i && LANE < nv) v -= LD[LANE * nv + i] * xi;
+  }
   KSYNC();
-  for (int lv = 1; lv < m.ndoflevel; lv++) {
-    int a = m.doflevel_adr[lv], n = m.doflevel_adr[lv + 1] - a;
-    for (int t = LANE; t < n; t += WAVE) {
-      int k = m.doflevel_dof[a + t];
-      const int* ch = m.dof_chain + k * MAX_DOF_DEPTH;
-      float s = x[k];
-      for (int q = 1; q < m.dof_nchain[k]; q++) s -= LD[k * nv + ch[q]] * x[ch[q]];
-      x[k] = s;
-    }
-    KSYNC();
-  }
+  if (LANE < nv) x[LANE] = v;
+  KSYNC();
+#else
+  for (int k = nv - 1; k >= 1; k--)
+    for (int i = 0; i < k; i++) x[i] -= LD[k * nv + i] * x[k];
+  for (int i = 0; i < nv; i++) x[i] /= LD[i * nv + i];
+  for (int i = 0; i < nv - 1; i++)
+    for (int k = i + 1; k < nv; k++) x[k] -= LD[k * nv + i] * x[i];
+#endif
 }
 
 // mj_comVel with MuJoCo's grouping of free / ball dofs (reference forward.hpp:287-332)
@@ -879,8 +982,7 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
       mu0 = m.pair_friction[5 * p];
       dA = ty == EFC_CONTACT_PYRAMIDAL ? tran * (1.f + mu0 * mu0) : tran;
     }
-    float v = 0.f;
-    for (int k = 0; k < nv; k++) v += J[r * nv + k] * qvel[k];
+    float v = kdot(J + r * nv, 1, qvel, 1, nv);
     float tc = sr[0], dr = sr[1];
     float dmax = clampf(si[1], KMINIMP, KMAXIMP);
     float kk, bb;
@@ -909,11 +1011,7 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
 // y = M x (dense qM), lanes over rows
 KDEV void mulM(const DModel& m, const float* M, const float* x, float* y) {
   int nv = m.nv;
-  for (int i = LANE; i < nv; i += WAVE) {
-    float s = 0.f;
-    for (int j = 0; j < nv; j++) s += M[i * nv + j] * x[j];
-    y[i] = s;
-  }
+  for (int i = LANE; i < nv; i += WAVE) y[i] = kdot(M + i * nv, 1, x, 1, nv);
 }
 
 // qacc <- x; Ma = M x; Jaref = J x - aref
@@ -921,16 +1019,8 @@ KDEV void set_qacc(const DModel& m, const Layout& L, float* w, const float* x, i
   int nv = m.nv;
   float *qacc = w + L.qacc, *Ma = w + L.Ma, *Jaref = w + L.efc_Jaref;
   const float *J = w + L.efc_J, *aref = w + L.efc_aref, *M = w + L.qM;
-  for (int i = LANE; i < nv; i += WAVE) {
-    float s = 0.f;
-    for (int j = 0; j < nv; j++) s += M[i * nv + j] * x[j];
-    Ma[i] = s;
-  }
-  for (int r = LANE; r < nefc; r += WAVE) {
-    float s = 0.f;
-    for (int k = 0; k < nv; k++) s += J[r * nv + k] * x[k];
-    Jaref[r] = s - aref[r];
-  }
+  for (int i = LANE; i < nv; i += WAVE) Ma[i] = kdot(M + i * nv, 1, x, 1, nv);
+  for (int r = LANE; r < nefc; r += WAVE) Jaref[r] = kdot(J + r * nv, 1, x, 1, nv) - aref[r];
   KSYNC();
   for (int i = LANE; i < nv; i += WAVE) qacc[i] = x[i];
   KSYNC();
@@ -938,15 +1028,17 @@ KDEV void set_qacc(const DModel& m, const Layout& L, float* w, const float* x, i
 
 // constraint forces / active flags / cost at the current Jaref, gradient;
 // returns the total cost (wave-uniform)
-KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc) {
+KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* nchanged) {
   int nv = m.nv;
   const float *Jaref = w + L.efc_Jaref, *D = w + L.efc_D, *Rr = w + L.efc_R, *floss = w + L.efc_floss;
-  float* force = w + L.efc_force;
+  float *force = w + L.efc_force, *Dact = w + L.efc_Dact;
   int *act = iptr(w, L.efc_act), *etype = iptr(w, L.efc_type);
   float part = 0.f;
+  int nch = 0;
   for (int r = LANE; r < nefc; r += WAVE) {
     float x = Jaref[r], d = D[r];
     int ty = etype[r];
+    int old = act[r];
     if (ty == EFC_EQUALITY) { act[r] = 1; force[r] = -d * x; part += 0.5f * d * x * x; }
     else if (ty == EFC_FRICTION_DOF) {
       float f = floss[r], R = Rr[r];
@@ -957,75 +1049,93 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc) {
       if (x < 0.f) { act[r] = 1; force[r] = -d * x; part += 0.5f * d * x * x; }
       else { act[r] = 0; force[r] = 0.f; }
     }
+    nch += act[r] != old;
+    Dact[r] = act[r] ? d : 0.f;
   }
   const float *Ma = w + L.Ma, *qfs = w + L.qfrc_smooth, *qacc = w + L.qacc, *qas = w + L.qacc_smooth;
   for (int i = LANE; i < nv; i += WAVE) part += 0.5f * (Ma[i] - qfs[i]) * (qacc[i] - qas[i]);
   float cost = wsum(part);
+  *nchanged = wsumi(nch);
   KSYNC();
   const float* J = w + L.efc_J;
   float* grad = w + L.grad;
-  for (int i = LANE; i < nv; i += WAVE) {
-    float g = Ma[i] - qfs[i];
-    for (int r = 0; r < nefc; r++) g -= J[r * nv + i] * force[r];
-    grad[i] = g;
-  }
+  for (int i = LANE; i < nv; i += WAVE) grad[i] = Ma[i] - qfs[i] - kdot(J + i, nv, force, 1, nefc);
   KSYNC();
   return cost;
 }
 
-// H = M + J^T diag(D act) J (FP32 accumulate), dense Cholesky in FP64 (in
-// place, lower), then Mgrad = H^-1 grad solved in FP64 by one lane.
-KDEV int newton_direction(const DModel& m, const Layout& L, float* w, int nefc) {
+// H = M + J^T diag(D act) J (FP32 accumulate), factored in FP64 as
+// H = L D L^T (right-looking, one LDS phase per column, the column left
+// unscaled); the reciprocal pivots go to dvec.
+KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc) {
   int nv = m.nv;
   double* H = dptr(w, L.H);
-  const float *M = w + L.qM, *J = w + L.efc_J, *D = w + L.efc_D;
-  const int* act = iptr(w, L.efc_act);
+  double* idg = dptr(w, L.dvec);
+  const float *M = w + L.qM, *J = w + L.efc_J, *Dact = w + L.efc_Dact;
   int ntri = nv * (nv + 1) / 2;
   for (int t = LANE; t < ntri; t += WAVE) {
     int i, j;
     tri_rc(t, &i, &j);
-    float s = M[i * nv + j];
-    for (int r = 0; r < nefc; r++)
-      if (act[r]) s += D[r] * J[r * nv + i] * J[r * nv + j];
-    H[i * nv + j] = (double)s;
+    H[i * nv + j] = (double)(M[i * nv + j] + kdot3(Dact, J + i, nv, J + j, nv, nefc));
   }
   KSYNC();
   int bad = 0;
-  for (int k = 0; k < nv; k++) {
+  for (int k = 0; k < nv - 1; k++) {
     double dkk = H[k * nv + k];
-    KSYNC();
     if (!(dkk > 0.0)) { bad = 1; dkk = 1e-30; }
-    double lkk = sqrt(dkk), il = 1.0 / lkk;
-    for (int i = k + 1 + LANE; i < nv; i += WAVE) H[i * nv + k] *= il;
-    if (LANE == 0) H[k * nv + k] = lkk;
-    KSYNC();
+    double ik = 1.0 / dkk;
     int nt = (nv - k - 1) * (nv - k) / 2;
     for (int t = LANE; t < nt; t += WAVE) {
       int a, b;
       tri_rc(t, &a, &b);
       int i = k + 1 + a, j = k + 1 + b;
-      H[i * nv + j] -= H[i * nv + k] * H[j * nv + k];
+      H[i * nv + j] -= H[i * nv + k] * ik * H[j * nv + k];
     }
     KSYNC();
   }
+  for (int i = LANE; i < nv; i += WAVE) {
+    double d = H[i * nv + i];
+    if (!(d > 0.0)) { d = 1e-30; }
+    idg[i] = 1.0 / d;
+  }
+  int badl = !(H[(nv - 1) * nv + nv - 1] > 0.0);
+  KSYNC();
+  return (bad || badl) ? ST_NOT_PD : 0;
+}
+
+// Mgrad = H^-1 grad with the factor of newton_factor, in FP64; on the GPU the
+// vector is held across lanes and each column step is a readlane broadcast.
+KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
+  int nv = m.nv;
+  const double* H = dptr(w, L.H);
+  const double* idg = dptr(w, L.dvec);
   const float* grad = w + L.grad;
   float* Mg = w + L.Mgrad;
-  if (LANE == 0) {
-    double* y = dptr(w, L.dvec);
-    for (int i = 0; i < nv; i++) {
-      double s = grad[i];
-      for (int q = 0; q < i; q++) s -= H[i * nv + q] * y[q];
-      y[i] = s / H[i * nv + i];
-    }
-    for (int i = nv - 1; i >= 0; i--) {
-      double s = y[i];
-      for (int q = i + 1; q < nv; q++) s -= H[q * nv + i] * y[q];
-      y[i] = s / H[i * nv + i];
-    }
-    for (int i = 0; i < nv; i++) Mg[i] = (float)y[i];
+#if WAVE == 64
+  double v = LANE < nv ? (double)grad[LANE] : 0.0;
+  double id = LANE < nv ? idg[LANE] : 1.0;
+  for (int k = 0; k < nv - 1; k++) {          // L y = g, L_ik = H_ik / H_kk
+    double yk = rl_d(v, k) * rl_d(id, k);
+    if (LANE > k && LANE < nv) v -= H[LANE * nv + k] * yk;
+  }
+  v *= id;                                      // D z = y
+  for (int k = nv - 1; k >= 1; k--) {          // L^T x = z
+    double xk = rl_d(v, k);
+    if (LANE < k) v -= H[k * nv + LANE] * id * xk;
   }
   KSYNC();
-  return bad ? ST_NOT_PD : 0;
+  if (LANE < nv) Mg[LANE] = (float)v;
+#else
+  double y[64];
+  for (int i = 0; i < nv; i++) y[i] = grad[i];
+  for (int k = 0; k < nv - 1; k++)
+    for (int i = k + 1; i < nv; i++) y[i] -= H[i * nv + k] * idg[k] * y[k];
+  for (int i = 0; i < nv; i++) y[i] *= idg[i];
+  for (int k = nv - 1; k >= 1; k--)
+    for (int i = 0; i < k; i++) y[i] -= H[k * nv + i] * idg[i] * y[k];
+  for (int i = 0; i < nv; i++) Mg[i] = (float)y[i];
+#endif
+  KSYNC();
 }
 
 struct LsPt { float alpha, f, d1, d2; int nchange; };
@@ -1081,11 +1191,7 @@ KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int*
   float scale = 1.f / (m.meaninertia * (float)(nv > 1 ? nv : 1));
   float gtol = m.tolerance * m.ls_tolerance * snorm / scale;
   mulM(m, M, s, Mv);
-  for (int r = LANE; r < nefc; r += WAVE) {
-    float a = 0.f;
-    for (int k = 0; k < nv; k++) a += J[r * nv + k] * s[k];
-    Jv[r] = a;
-  }
+  for (int r = LANE; r < nefc; r += WAVE) Jv[r] = kdot(J + r * nv, 1, s, 1, nv);
   KSYNC();
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   for (int i = LANE; i < nv; i += WAVE) {
@@ -1126,7 +1232,7 @@ KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int*
 
 // mj_fwdConstraint: warm start choice, Newton or CG iterations with MuJoCo's
 // termination test, qfrc_constraint = J^T force.
-KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out) {
+KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KProfAcc* pa) {
   int* misc = iptr(w, L.misc);
   int nv = m.nv;
   int nefc = misc[MISC_NEFC];
@@ -1139,15 +1245,16 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out) {
     KSYNC();
     return 0;
   }
+  int nch = 0;
   set_qacc(m, L, w, w + L.qacc_smooth, nefc);
-  float cost = cost_grad(m, L, w, nefc);
+  float cost = cost_grad(m, L, w, nefc, &nch);
   if (!(m.disableflags & DS_WARMSTART)) {
     float cs = cost;
     set_qacc(m, L, w, w + L.qacc_ws, nefc);
-    cost = cost_grad(m, L, w, nefc);
+    cost = cost_grad(m, L, w, nefc, &nch);
     if (cost > cs) {
       set_qacc(m, L, w, w + L.qacc_smooth, nefc);
-      cost = cost_grad(m, L, w, nefc);
+      cost = cost_grad(m, L, w, nefc, &nch);
     }
   }
   float scale = 1.f / (m.meaninertia * (float)(nv > 1 ? nv : 1));
@@ -1155,8 +1262,11 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out) {
   float *grad = w + L.grad, *Mg = w + L.Mgrad, *srch = w + L.search, *Ma = w + L.Ma, *Mv = w + L.Mv;
   float *go = w + L.grad_old, *mgo = w + L.Mgrad_old, *Jaref = w + L.efc_Jaref;
   const float* Jv = w + L.efc_Jv;
-  if (newton) st |= newton_direction(m, L, w, nefc);
-  else {
+  KPROF_MARK(pa, KP_SOLVE_INIT);
+  if (newton) {
+    st |= newton_factor(m, L, w, nefc);
+    newton_solve(m, L, w);
+  } else {
     for (int i = LANE; i < nv; i += WAVE) Mg[i] = grad[i];
     KSYNC();
     solve_ld(m, w + L.qLD, Mg);
@@ -1166,7 +1276,9 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out) {
   int it;
   for (it = 0; it < m.iterations; it++) {
     int exact = 0;
+    KPROF_MARK(pa, KP_DIR);
     float alpha = linesearch(m, L, w, nefc, &exact);
+    KPROF_MARK(pa, KP_LS);
     *niter_out = it + 1;
     if (alpha == 0.f) break;
     for (int i = LANE; i < nv; i += WAVE) {
@@ -1178,7 +1290,9 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out) {
     for (int r = LANE; r < nefc; r += WAVE) Jaref[r] += alpha * Jv[r];
     KSYNC();
     float oldcost = cost;
-    cost = cost_grad(m, L, w, nefc);
+    KPROF_MARK(pa, KP_SOLVE_REST);
+    cost = cost_grad(m, L, w, nefc, &nch);
+    KPROF_MARK(pa, KP_COSTGRAD);
     float gp = 0.f;
     for (int i = LANE; i < nv; i += WAVE) gp += grad[i] * grad[i];
     float improvement = scale * (oldcost - cost), gradient = scale * sqrtf(wsum(gp));
@@ -1186,8 +1300,11 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out) {
     // a Newton step that stayed inside one quadratic piece (no row changed
     // zone) landed on that piece's stationary point: the exact minimiser
     if (newton && exact) break;
-    if (newton) st |= newton_direction(m, L, w, nefc);
-    else {
+    if (newton) {
+      // H depends on qacc only through the active set: refactor only when it changed
+      if (nch) st |= newton_factor(m, L, w, nefc);
+      newton_solve(m, L, w);
+    } else {
       for (int i = LANE; i < nv; i += WAVE) Mg[i] = grad[i];
       KSYNC();
       solve_ld(m, w + L.qLD, Mg);
@@ -1204,6 +1321,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out) {
     }
     KSYNC();
   }
+  KPROF_MARK(pa, KP_DIR);
   const float *J = w + L.efc_J, *force = w + L.efc_force;
   for (int i = LANE; i < nv; i += WAVE) {
     float s = 0.f;
@@ -1211,6 +1329,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out) {
     qfc[i] = s;
   }
   KSYNC();
+  KPROF_MARK(pa, KP_SOLVE_REST);
   return st;
 }
 
@@ -1307,6 +1426,7 @@ struct StepIO {
   float *dump1, *dump2;  // debug: workspace after constraint build / after solve (last substep)
   AuxOut aux;
   int has_aux;
+  unsigned long long* prof;  // profiling builds: per-env stage cycles [env * KP_N + stage]
 };
 
 KDEV void dump_ws(const Layout& L, const float* w, float* dst, int env) {
@@ -1357,23 +1477,35 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     return;
   }
   int st = 0, niter = 0;
+  KProfAcc pa;
+  for (int i = 0; i < KP_N; i++) pa.t[i] = 0;
+  KPROF_START(&pa);
   for (int s = 0; s < nsub; s++) {
     bool last = s == nsub - 1;
     st_kinematics(m, L, w);
+    KPROF_MARK(&pa, KP_KIN);
     st_com_pos(m, L, w);
+    KPROF_MARK(&pa, KP_COMPOS);
     st_crb(m, L, w);
+    KPROF_MARK(&pa, KP_CRB);
     if (nv > 0) st |= factor_ld(m, w + L.qM, w + L.qLD);
+    KPROF_MARK(&pa, KP_FACTOR);
     st_com_vel(m, L, w);
+    KPROF_MARK(&pa, KP_COMVEL);
     st |= st_smooth(m, L, w);
+    KPROF_MARK(&pa, KP_SMOOTH);
     if (last && io.has_aux) write_aux(m, L, w, io.aux, env);
     st |= st_collision(m, L, w);
+    KPROF_MARK(&pa, KP_COLL);
     st |= st_constraints(m, L, w);
+    KPROF_MARK(&pa, KP_CONSTR);
     if (last) dump_ws(L, w, io.dump1, env);
     int it = 0;
-    st |= st_solve(m, L, w, &it);
+    st |= st_solve(m, L, w, &it, &pa);
     niter += it;
     if (last) dump_ws(L, w, io.dump2, env);
     st |= st_integrate(m, L, w);
+    KPROF_MARK(&pa, KP_INTEGRATE);
   }
   int bad = 0;
   for (int i = LANE; i < nq; i += WAVE) {
@@ -1388,8 +1520,11 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     io.qacc_ws[(size_t)env * nv + i] = w[L.qacc_ws + i];
   }
   if (wori(bad)) st |= ST_NONFINITE;
+  KPROF_MARK(&pa, KP_IO);
   if (LANE == 0) {
     const int* misc = reinterpret_cast<const int*>(w + L.misc);
+    if (io.prof)
+      for (int i = 0; i < KP_N; i++) io.prof[(size_t)env * KP_N + i] = pa.t[i];
     if (io.niter) io.niter[env] = niter;
     if (io.status) io.status[env] = st;
     if (io.ncon) io.ncon[env] = misc[MISC_NCON];
