@@ -69,6 +69,59 @@ MJMLX_API int mjmlx_batched_profile_step(MjmlxBatchedSim* sim, const float* ctrl
 /* offset / length (floats) / kind (0 float, 1 int) of a workspace field */
 MJMLX_API int mjmlx_batched_layout(const MjmlxBatchedSim* sim, const char* name, int* off, int* len, int* kind);
 
+/* ---- learned warm start (no counterpart in the reference) ----------------
+ * A small MLP nin -> nh -> nh -> nout (SiLU) with FP32 master weights and BF16
+ * copies, run and trained on the GPU by the engine's own MFMA kernels. For the
+ * warm start, nin = nq + nv + nu + 16 * 8 (mjmlx_batched_feature_dim) and
+ * nout = nv. Inputs: qpos, qvel, ctrl and the first 16 contacts as (frame
+ * normal, penetration depth, one-hot limb class of the two geoms); output:
+ * the solver's converged qacc. One MLP must not be trained and used by a sim
+ * at the same time. */
+typedef struct MjmlxMLP MjmlxMLP;
+
+/* nh must be a multiple of 32; returns NULL without a HIP device */
+MJMLX_API MjmlxMLP* mjmlx_mlp_create(int nin, int nh, int nout, int seed);
+MJMLX_API void mjmlx_mlp_free(MjmlxMLP* mlp);
+/* nparam counts the packed parameters: W1 [nin x nh], b1, W2 [nh x nh], b2, W3 [nh x nout], b3 */
+MJMLX_API int mjmlx_mlp_dims(const MjmlxMLP* mlp, int* nin, int* nh, int* nout, int* nparam);
+/* x_n = (x - in_mean) / in_std and y = y_n * out_std + out_mean (NULL: 0 / 1) */
+MJMLX_API int mjmlx_mlp_set_normalization(MjmlxMLP* mlp, const float* in_mean, const float* in_std,
+                                          const float* out_mean, const float* out_std);
+MJMLX_API int mjmlx_mlp_get_normalization(const MjmlxMLP* mlp, float* in_mean, float* in_std, float* out_mean,
+                                          float* out_std);
+/* packed FP32 parameters in the order W1 b1 W2 b2 W3 b3 (row-major, y = x W + b); return the count */
+MJMLX_API int mjmlx_mlp_get_params(const MjmlxMLP* mlp, float* out);
+MJMLX_API int mjmlx_mlp_set_params(MjmlxMLP* mlp, const float* in);
+/* network on host rows in normalised units: y [rows x nout] from x [rows x nin] */
+MJMLX_API int mjmlx_mlp_forward(MjmlxMLP* mlp, const float* x, int rows, float* y);
+/* gradient of mean((MLP(x) - y)^2) (normalised units) w.r.t. the packed parameters; returns the loss */
+MJMLX_API float mjmlx_mlp_grad(MjmlxMLP* mlp, const float* x, const float* y, int rows, float* grad_out);
+/* one AdamW step (beta 0.9 / 0.999, eps 1e-8) on a host batch; returns the loss before the step */
+MJMLX_API float mjmlx_mlp_train_step(MjmlxMLP* mlp, const float* x, const float* y, int rows, float lr,
+                                     float weight_decay);
+/* device-resident training set of raw (unnormalised) rows; reserve discards previous rows */
+MJMLX_API int mjmlx_mlp_dataset_reserve(MjmlxMLP* mlp, long rows);
+MJMLX_API long mjmlx_mlp_dataset_size(const MjmlxMLP* mlp);
+MJMLX_API long mjmlx_mlp_dataset_append(MjmlxMLP* mlp, const float* x, const float* y, long rows);
+MJMLX_API int mjmlx_mlp_dataset_get(const MjmlxMLP* mlp, long first, long rows, float* x, float* y);
+/* normalisation set to the per-column mean / standard deviation of the training set */
+MJMLX_API int mjmlx_mlp_fit_normalization(MjmlxMLP* mlp);
+/* AdamW over the training set in shuffled batches (batch a multiple of 32);
+ * returns the mean loss (normalised units) of the last epoch */
+MJMLX_API float mjmlx_mlp_fit(MjmlxMLP* mlp, int epochs, int batch, float lr, float weight_decay, int seed);
+
+/* width of the warm-start features of this sim's model */
+MJMLX_API int mjmlx_batched_feature_dim(const MjmlxBatchedSim* sim);
+/* installs the network used by warm-start mode 2 (NULL removes it and leaves mode 2);
+ * returns -1 when its widths do not match the model */
+MJMLX_API int mjmlx_batched_set_warmstart_mlp(MjmlxBatchedSim* sim, MjmlxMLP* mlp);
+/* steps like step_n and appends one (features, converged qacc) row per env and
+ * substep to mlp's training set (the first substep after a state change has no
+ * features and is skipped); returns the rows appended or -1 */
+MJMLX_API long mjmlx_batched_record(MjmlxBatchedSim* sim, MjmlxMLP* mlp, const float* ctrl, int nsub);
+/* raw features after the last substep [num_envs x feature_dim] (ctrl columns zero); returns the width or -1 */
+MJMLX_API int mjmlx_batched_get_features(const MjmlxBatchedSim* sim, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,7 +40,7 @@ def _run(cmd, cwd=CSRC):
 def build_engine():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libmjmlx.so")
-    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INC}", f"-I{CSRC}", "-o", out, "engine.hip", *HOST_SRC])
+    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INC}", f"-I{CSRC}", "-o", out, "engine.hip", "mlp.hip", *HOST_SRC])
     return out
 
 
@@ -48,7 +48,8 @@ def build_prof():
     """Profiling build of the engine (per-stage cycle counters compiled in)."""
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libmjmlx_prof.so")
-    _run([HIPCC, *HIP_FLAGS, "-DMJK_PROFILE", "-shared", f"-I{INC}", f"-I{CSRC}", "-o", out, "engine.hip", *HOST_SRC])
+    _run([HIPCC, *HIP_FLAGS, "-DMJK_PROFILE", "-shared", f"-I{INC}", f"-I{CSRC}", "-o", out, "engine.hip", "mlp.hip",
+          *HOST_SRC])
     return out
 
 

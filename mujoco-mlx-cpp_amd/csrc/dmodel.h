@@ -25,6 +25,9 @@ enum { ST_CON_OVERFLOW = 1, ST_EFC_OVERFLOW = 2, ST_BAD_PAIR = 4, ST_NOT_PD = 16
 
 // contact record in the workspace (floats)
 enum { CON_POS = 0, CON_FRAME = 3, CON_DIST = 12, CON_PAIR = 13, CON_NROW = 14, CON_ROW0 = 15, CON_STRIDE = 16 };
+// learned warm-start features: the first FEAT_NCON contacts, each as
+// (frame normal 3, penetration depth, limb one-hot 4)
+enum { FEAT_NCON = 16, FEAT_PER_CON = 8, FEAT_NCLASS = 4 };
 // max contacts one narrow-phase call emits
 enum { MAXC_PAIR = 4 };
 enum { MAX_DOF_DEPTH = 64 };
@@ -35,6 +38,7 @@ struct DModel {
   int iterations, ls_iterations, solver, cone, integrator, disableflags;
   int has_damping;
   float timestep, gravity[3], tolerance, ls_tolerance, impratio, meaninertia;
+  int nfeat;  // learned warm-start input width: nq + nv + nu + FEAT_NCON * FEAT_PER_CON
   // bodies
   const int *body_parentid, *body_rootid, *body_jntnum, *body_jntadr, *body_dofnum, *body_dofadr,
       *body_subtree_end, *level_adr, *level_body, *body_lastdof;
@@ -50,7 +54,7 @@ struct DModel {
       *doflevel_dof, *dof_chain, *dof_nchain, *dof_act_adr, *dof_act_num, *dof_act_list;
   const float *dof_armature, *dof_damping, *dof_frictionloss, *dof_invweight0, *dof_solref, *dof_solimp;
   // geoms
-  const int *geom_type, *geom_bodyid;
+  const int *geom_type, *geom_bodyid, *geom_fclass;
   const float *geom_size, *geom_pos, *geom_quat, *geom_rbound;
   // static pair table (geom1 has the lower type)
   const int *pair_g1, *pair_g2, *pair_b1, *pair_b2, *pair_condim;
@@ -67,7 +71,7 @@ struct DModel {
 // Newton Hessian overlays it.
 struct Layout {
   int maxcon, maxefc, nv, total;
-  int qpos, qvel, ctrl, qacc_ws, qacc, qM, qLD, qfrc_smooth, qacc_smooth, qfrc_constraint, act_force, red, misc;
+  int qpos, qvel, ctrl, qacc_ws, qacc_pred, qacc, qM, qLD, qfrc_smooth, qacc_smooth, qfrc_constraint, act_force, red, misc;
   int xpos, xquat, xmat, xipos, xanchor, xaxis, gxpos, gxmat, scom, cinert, cdof, cvel, cdofdot, crb, cacc, cfrc, buf6;
   int H;
   int con, efc_J, efc_D, efc_R, efc_aref, efc_Jaref, efc_Jv, efc_force, efc_floss, efc_pos, efc_margin,
@@ -96,6 +100,7 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.qvel = o; o += al2(nvp);
   L.ctrl = o; o += al2(nup);
   L.qacc_ws = o; o += al2(nvp);
+  L.qacc_pred = o; o += al2(nvp);
   L.qacc = o; o += al2(nvp);
   L.qM = o; o += al2(nvp * nvp);
   L.qLD = o; o += al2(nvp * nvp);

@@ -23,7 +23,8 @@ struct EmuSim {
   mjk::Layout L;
   std::vector<mjh::LayoutField> fields;
   int B;
-  std::vector<float> qpos, qvel, qacc_ws, ws, dump1, dump2;
+  std::vector<float> qpos, qvel, qacc_ws, ws, dump1, dump2, feat, pred;
+  bool use_pred = false;
   std::vector<int> niter, status, ncon, nefc;
 };
 }  // namespace
@@ -83,6 +84,9 @@ __attribute__((visibility("default"))) int mjemu_step(void* p, const float* ctrl
   mjk::StepIO io{};
   io.qpos = s->qpos.data(); io.qvel = s->qvel.data(); io.qacc_ws = s->qacc_ws.data(); io.ctrl = ctrl;
   io.niter = s->niter.data(); io.status = s->status.data(); io.ncon = s->ncon.data(); io.nefc = s->nefc.data();
+  s->feat.assign((size_t)s->B * s->dm.nfeat, 0.f);
+  io.feat = s->feat.data();
+  if (s->use_pred) io.qacc_pred = s->pred.data();
   if (dump) {
     s->dump1.assign((size_t)s->B * s->L.total, 0.f);
     s->dump2.assign((size_t)s->B * s->L.total, 0.f);
@@ -95,6 +99,20 @@ __attribute__((visibility("default"))) int mjemu_step(void* p, const float* ctrl
     if (s->status[e]) bad++;
   }
   return bad;
+}
+
+// learned warm-start features written by the last step, float[B * nfeat]; returns nfeat
+__attribute__((visibility("default"))) int mjemu_features(void* p, float* out) {
+  auto* s = static_cast<EmuSim*>(p);
+  if (out && !s->feat.empty()) std::memcpy(out, s->feat.data(), s->feat.size() * sizeof(float));
+  return s->dm.nfeat;
+}
+
+// predicted qacc offered to the next steps' solver as a warm-start candidate (null: none)
+__attribute__((visibility("default"))) void mjemu_set_pred(void* p, const float* pred) {
+  auto* s = static_cast<EmuSim*>(p);
+  s->use_pred = pred != nullptr;
+  if (pred) s->pred.assign(pred, pred + (size_t)s->B * s->model.nv);
 }
 
 __attribute__((visibility("default"))) int mjemu_layout(void* p, const char* name, int* off, int* len, int* kind) {

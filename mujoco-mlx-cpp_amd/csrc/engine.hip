@@ -25,6 +25,7 @@
 #include "hostmodel.h"
 #include "kernels.h"
 #include "mjmlx/mjmlx_hip.h"
+#include "mlp.h"
 
 namespace {
 
@@ -129,6 +130,12 @@ struct MjmlxBatchedSim {
   int* d_pack_bodies = nullptr;
   float* d_pack = nullptr;
   int pack_nsel = 0, pack_lo = 0, pack_ndof = 0, pack_stride = -1;
+  // learned warm start: raw features after each substep (double-buffered) and predictions
+  MjmlxMLP* mlp = nullptr;
+  float* d_feat[2] = {};
+  float* d_pred = nullptr;
+  int feat_cur = 0;
+  bool feat_valid = false;
   // scratch host buffers for dumps
   float *d_dump1 = nullptr, *d_dump2 = nullptr;
   size_t dump_cap = 0;
@@ -145,7 +152,8 @@ void free_sim(MjmlxBatchedSim* s) {
   if (!s) return;
   if (s->stream) hipStreamSynchronize(s->stream);
   void* dev[] = {s->d_blob, s->d_qpos, s->d_qvel, s->d_qws, s->d_ctrl, s->d_qpos0, s->d_niter, s->d_status,
-                 s->d_ncon, s->d_nefc, s->d_queue, s->d_mask, s->d_pack_bodies, s->d_pack, s->d_dump1, s->d_dump2};
+                 s->d_ncon, s->d_nefc, s->d_queue, s->d_mask, s->d_pack_bodies, s->d_pack, s->d_dump1, s->d_dump2,
+                 s->d_feat[0], s->d_feat[1], s->d_pred};
   for (void* p : dev) if (p) hipFree(p);
   for (int i = 0; i < 7; i++) { if (s->d_aux[i]) hipFree(s->d_aux[i]); if (s->h_aux[i]) hipHostFree(s->h_aux[i]); }
   if (s->h_qpos) hipHostFree(s->h_qpos);
@@ -198,6 +206,47 @@ void launch(MjmlxBatchedSim* s, int nsub, float* dump1, float* dump2) {
   HIPOK(hipGetLastError());
   HIPOK(hipEventRecord(s->ev1, s->stream));
   s->timed = true;
+}
+
+void ensure_learned(MjmlxBatchedSim* s) {
+  if (s->d_pred) return;
+  size_t nf = (size_t)s->B * s->dm.nfeat;
+  for (int i = 0; i < 2; i++) {
+    HIPOK(hipMalloc(&s->d_feat[i], nf * sizeof(float)));
+    HIPOK(hipMemsetAsync(s->d_feat[i], 0, nf * sizeof(float), s->stream));
+  }
+  HIPOK(hipMalloc(&s->d_pred, (size_t)s->B * std::max(1, s->nv) * sizeof(float)));
+  HIPOK(hipMemsetAsync(s->d_pred, 0, (size_t)s->B * std::max(1, s->nv) * sizeof(float), s->stream));
+  s->feat_valid = false;
+}
+
+// One launch per substep. With warm-start mode 2 the network predicts each
+// substep's qacc from the previous substep's features and this call's ctrl
+// before the launch; with `rec` every substep that has input features appends
+// (features, converged qacc) rows to rec's training set.
+long step_substeps(MjmlxBatchedSim* s, int n, MjmlxMLP* rec) {
+  ensure_learned(s);
+  long rows = 0;
+  int coff = s->nq + s->nv;
+  HIPOK(hipEventRecord(s->ev0, s->stream));
+  for (int k = 0; k < n; k++) {
+    mjk::StepIO io = make_io(s, nullptr, nullptr);
+    io.accumulate = k > 0;
+    if (s->ws_mode == 2 && s->mlp && s->feat_valid) {
+      mjmlp::pack_features(s->mlp, s->d_feat[s->feat_cur], s->d_ctrl, coff, s->nu, s->B, s->stream);
+      mjmlp::predict(s->mlp, s->B, s->d_pred, s->stream);
+      io.qacc_pred = s->d_pred;
+    }
+    io.feat = s->d_feat[s->feat_cur ^ 1];
+    hipLaunchKernelGGL(k_step, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, 1);
+    HIPOK(hipGetLastError());
+    if (rec && s->feat_valid) rows += mjmlp::record(rec, s->d_feat[s->feat_cur], s->d_ctrl, coff, s->nu, s->d_qws, s->B, s->stream);
+    s->feat_cur ^= 1;
+    s->feat_valid = true;
+  }
+  HIPOK(hipEventRecord(s->ev1, s->stream));
+  s->timed = true;
+  return rows;
 }
 
 void ensure_aux(MjmlxBatchedSim* s) {
@@ -463,7 +512,8 @@ MJMLX_API void mjmlx_batched_step_n(MjmlxBatchedSim* sim, const float* ctrl, int
   if (!sim || n <= 0) return;
   try {
     upload_ctrl(sim, ctrl);
-    launch(sim, n, nullptr, nullptr);
+    if (sim->ws_mode == 2) step_substeps(sim, n, nullptr);
+    else launch(sim, n, nullptr, nullptr);
   } catch (const std::exception& e) { report("mjmlx_batched_step_n", e); }
 }
 
@@ -472,6 +522,7 @@ MJMLX_API void mjmlx_batched_step(MjmlxBatchedSim* sim, const float* ctrl) { mjm
 MJMLX_API void mjmlx_batched_reset(MjmlxBatchedSim* sim, const int* mask) {
   if (!sim || !mask) return;
   try {
+    sim->feat_valid = false;
     std::memcpy(sim->h_mask, mask, sim->B * sizeof(int));
     HIPOK(hipMemcpyAsync(sim->d_mask, sim->h_mask, sim->B * sizeof(int), hipMemcpyHostToDevice, sim->stream));
     hipLaunchKernelGGL(k_reset, dim3((sim->B + 255) / 256), dim3(256), 0, sim->stream, sim->d_mask, sim->d_qpos,
@@ -498,6 +549,7 @@ MJMLX_API void mjmlx_batched_get_state(const MjmlxBatchedSim* cs, float* qpos_ou
 MJMLX_API void mjmlx_batched_set_state(MjmlxBatchedSim* s, const float* qpos, const float* qvel) {
   if (!s) return;
   try {
+    s->feat_valid = false;
     HIPOK(hipStreamSynchronize(s->stream));
     if (qpos) {
       std::memcpy(s->h_qpos, qpos, (size_t)s->B * s->nq * sizeof(float));
@@ -548,6 +600,7 @@ MJMLX_API const float* mjmlx_batched_get_sensordata(const MjmlxBatchedSim*, int*
 
 MJMLX_API void mjmlx_batched_set_env_qpos(MjmlxBatchedSim* s, int env, const float* qpos, int nq) {
   if (!s || !qpos || env < 0 || env >= s->B || nq != s->nq) return;
+  s->feat_valid = false;
   try {
     HIPOK(hipStreamSynchronize(s->stream));
     HIPOK(hipMemcpy(s->d_qpos + (size_t)env * s->nq, qpos, s->nq * sizeof(float), hipMemcpyHostToDevice));
@@ -555,6 +608,7 @@ MJMLX_API void mjmlx_batched_set_env_qpos(MjmlxBatchedSim* s, int env, const flo
 }
 MJMLX_API void mjmlx_batched_set_env_qvel(MjmlxBatchedSim* s, int env, const float* qvel, int nv) {
   if (!s || !qvel || env < 0 || env >= s->B || nv != s->nv) return;
+  s->feat_valid = false;
   try {
     HIPOK(hipStreamSynchronize(s->stream));
     HIPOK(hipMemcpy(s->d_qvel + (size_t)env * s->nv, qvel, s->nv * sizeof(float), hipMemcpyHostToDevice));
@@ -626,7 +680,11 @@ MJMLX_API int mjmlx_camera_gpu_export_fd(MjmlxCameraRenderer*, int, uint64_t*) {
 
 // ---- additions (mjmlx_hip.h)
 MJMLX_API int mjmlx_batched_set_warmstart_mode(MjmlxBatchedSim* s, int mode) {
-  if (!s || mode < 0 || mode > 1) return -1;
+  if (!s || mode < 0 || mode > 2) return -1;
+  if (mode == 2 && !s->mlp) {
+    std::fprintf(stderr, "mjmlx_batched_set_warmstart_mode error: mode 2 needs mjmlx_batched_set_warmstart_mlp first\n");
+    return -1;
+  }
   s->ws_mode = mode;
   if (mode == 1) s->dm.disableflags |= mjk::DS_WARMSTART;
   else s->dm.disableflags &= ~mjk::DS_WARMSTART;
@@ -740,6 +798,51 @@ MJMLX_API int mjmlx_batched_profile_step(MjmlxBatchedSim* s, const float* ctrl, 
   (void)s; (void)ctrl; (void)nsub; (void)out;
   return -1;
 #endif
+}
+
+MJMLX_API int mjmlx_batched_feature_dim(const MjmlxBatchedSim* s) { return s ? s->dm.nfeat : -1; }
+
+MJMLX_API int mjmlx_batched_set_warmstart_mlp(MjmlxBatchedSim* s, MjmlxMLP* mlp) {
+  if (!s) return -1;
+  if (!mlp) {
+    s->mlp = nullptr;
+    if (s->ws_mode == 2) s->ws_mode = 0;
+    return 0;
+  }
+  int nin = 0, nout = 0;
+  mjmlx_mlp_dims(mlp, &nin, nullptr, &nout, nullptr);
+  if (nin != s->dm.nfeat || nout != s->nv) {
+    std::fprintf(stderr, "mjmlx_batched_set_warmstart_mlp error: network is %d -> %d, the model needs %d -> %d\n", nin,
+                 nout, s->dm.nfeat, s->nv);
+    return -1;
+  }
+  try {
+    ensure_learned(s);
+    s->mlp = mlp;
+    return 0;
+  } catch (const std::exception& e) { report("mjmlx_batched_set_warmstart_mlp", e); return -1; }
+}
+
+MJMLX_API long mjmlx_batched_record(MjmlxBatchedSim* s, MjmlxMLP* mlp, const float* ctrl, int nsub) {
+  if (!s || !mlp || nsub <= 0) return -1;
+  int nin = 0, nout = 0;
+  mjmlx_mlp_dims(mlp, &nin, nullptr, &nout, nullptr);
+  if (nin != s->dm.nfeat || nout != s->nv) return -1;
+  try {
+    upload_ctrl(s, ctrl);
+    return step_substeps(s, nsub, mlp);
+  } catch (const std::exception& e) { report("mjmlx_batched_record", e); return -1; }
+}
+
+MJMLX_API int mjmlx_batched_get_features(const MjmlxBatchedSim* cs, float* out) {
+  auto* s = const_cast<MjmlxBatchedSim*>(cs);
+  if (!s || !out || !s->d_pred || !s->feat_valid) return -1;
+  try {
+    HIPOK(hipMemcpyAsync(out, s->d_feat[s->feat_cur], (size_t)s->B * s->dm.nfeat * sizeof(float), hipMemcpyDeviceToHost,
+                         s->stream));
+    HIPOK(hipStreamSynchronize(s->stream));
+    return s->dm.nfeat;
+  } catch (const std::exception& e) { report("mjmlx_batched_get_features", e); return -1; }
 }
 
 MJMLX_API int mjmlx_batched_layout(const MjmlxBatchedSim* s, const char* name, int* off, int* len, int* kind) {

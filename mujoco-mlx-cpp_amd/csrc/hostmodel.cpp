@@ -167,6 +167,25 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
     for (int k = 0; k < 2; k++) psr[2 * i + k] = (float)p.solref[k];
     pmg[i] = (float)p.margin; pgp[i] = (float)p.gap;
   }
+  // limb class of each geom for the learned warm-start features: the index
+  // (mod FEAT_NCLASS) of the root body's child whose subtree holds the geom;
+  // -1 on the world and on root bodies
+  std::vector<int> fclass(m.ngeom, -1);
+  {
+    std::vector<int> limb(nb, -1);
+    for (int b = 1; b < nb; b++) {
+      int par = m.body_parentid[b];
+      if (par == 0) continue;
+      if (m.body_parentid[par] == 0) {
+        int k = 0;
+        for (int c = 1; c < b; c++) if (m.body_parentid[c] == par) k++;
+        limb[b] = k % FEAT_NCLASS;
+      } else {
+        limb[b] = limb[par];
+      }
+    }
+    for (int g = 0; g < m.ngeom; g++) fclass[g] = limb[m.geom_bodyid[g]];
+  }
   std::vector<float> gain(3 * m.nu), bias(3 * m.nu);
   for (int i = 0; i < 3 * m.nu; i++) { gain[i] = (float)m.actuator_gainprm[i]; bias[i] = (float)m.actuator_biasprm[i]; }
 
@@ -184,6 +203,7 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   for (int i = 0; i < 3; i++) d.gravity[i] = (float)m.gravity[i];
   d.tolerance = (float)m.tolerance; d.ls_tolerance = (float)m.ls_tolerance; d.impratio = (float)m.impratio;
   d.meaninertia = (float)m.meaninertia;
+  d.nfeat = m.nq + nv + m.nu + FEAT_NCON * FEAT_PER_CON;
   std::vector<float> subtreemass = tof(m.body_subtreemass);
 #define ADDP(field, vec) d.field = reinterpret_cast<decltype(d.field)>(B.add(vec))
   ADDP(body_parentid, m.body_parentid); ADDP(body_rootid, m.body_rootid); ADDP(body_jntnum, m.body_jntnum);
@@ -203,7 +223,7 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   ADDP(dof_act_list, dof_act_list);
   ADDP(dof_armature, tof(m.dof_armature)); ADDP(dof_damping, tof(m.dof_damping)); ADDP(dof_frictionloss, tof(m.dof_frictionloss));
   ADDP(dof_invweight0, tof(m.dof_invweight0)); ADDP(dof_solref, tof(m.dof_solref)); ADDP(dof_solimp, tof(m.dof_solimp));
-  ADDP(geom_type, m.geom_type); ADDP(geom_bodyid, m.geom_bodyid);
+  ADDP(geom_type, m.geom_type); ADDP(geom_bodyid, m.geom_bodyid); ADDP(geom_fclass, fclass);
   ADDP(geom_size, tof(m.geom_size)); ADDP(geom_pos, tof(m.geom_pos)); ADDP(geom_quat, tof(m.geom_quat));
   ADDP(geom_rbound, tof(m.geom_rbound));
   ADDP(pair_g1, pg1); ADDP(pair_g2, pg2); ADDP(pair_b1, pb1); ADDP(pair_b2, pb2); ADDP(pair_condim, pcd);
@@ -223,7 +243,7 @@ std::vector<LayoutField> layout_fields(const Model& m, const mjk::Layout& L) {
   int nb = m.nbody, nv = m.nv, nj = m.njnt, ng = m.ngeom;
   std::vector<LayoutField> f = {
       {"qpos", L.qpos, m.nq, 0}, {"qvel", L.qvel, nv, 0}, {"ctrl", L.ctrl, m.nu, 0},
-      {"qacc_warmstart", L.qacc_ws, nv, 0}, {"qacc", L.qacc, nv, 0}, {"qM", L.qM, nv * nv, 0},
+      {"qacc_warmstart", L.qacc_ws, nv, 0}, {"qacc_pred", L.qacc_pred, nv, 0}, {"qacc", L.qacc, nv, 0}, {"qM", L.qM, nv * nv, 0},
       {"qLD", L.qLD, nv * nv, 0}, {"qfrc_smooth", L.qfrc_smooth, nv, 0}, {"qacc_smooth", L.qacc_smooth, nv, 0},
       {"qfrc_constraint", L.qfrc_constraint, nv, 0}, {"actuator_force", L.act_force, m.nu, 0},
       {"misc", L.misc, mjk::MISC_SIZE, 1},

@@ -1232,7 +1232,7 @@ KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int*
 
 // mj_fwdConstraint: warm start choice, Newton or CG iterations with MuJoCo's
 // termination test, qfrc_constraint = J^T force.
-KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KProfAcc* pa) {
+KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KProfAcc* pa, bool use_pred) {
   int* misc = iptr(w, L.misc);
   int nv = m.nv;
   int nefc = misc[MISC_NEFC];
@@ -1252,7 +1252,19 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     float cs = cost;
     set_qacc(m, L, w, w + L.qacc_ws, nefc);
     cost = cost_grad(m, L, w, nefc, &nch);
-    if (cost > cs) {
+    const float* best = cost > cs ? w + L.qacc_smooth : w + L.qacc_ws;
+    float cbest = cost > cs ? cs : cost;
+    if (use_pred) {
+      // learned candidate: kept only when its cost is strictly the lowest
+      set_qacc(m, L, w, w + L.qacc_pred, nefc);
+      float cp = cost_grad(m, L, w, nefc, &nch);
+      if (cp < cbest) { best = w + L.qacc_pred; cbest = cp; }
+      cost = cp;
+      if (best != w + L.qacc_pred) {
+        set_qacc(m, L, w, best, nefc);
+        cost = cost_grad(m, L, w, nefc, &nch);
+      }
+    } else if (cost > cs) {
       set_qacc(m, L, w, w + L.qacc_smooth, nefc);
       cost = cost_grad(m, L, w, nefc, &nch);
     }
@@ -1423,6 +1435,9 @@ struct StepIO {
   float *qpos, *qvel, *qacc_ws;
   const float* ctrl;
   int *niter, *status, *ncon, *nefc;
+  int accumulate;            // add niter / OR status into the outputs instead of overwriting them
+  const float* qacc_pred;    // learned warm start: predicted qacc [env * nv], or null
+  float* feat;               // learned warm start: raw features after the last substep [env * nfeat], or null
   float *dump1, *dump2;  // debug: workspace after constraint build / after solve (last substep)
   AuxOut aux;
   int has_aux;
@@ -1456,12 +1471,41 @@ KDEV void write_aux(const DModel& m, const Layout& L, const float* w, const AuxO
 
 // nsub physics substeps of one env with the same ctrl; state stays in the
 // workspace between substeps and goes back to HBM once at the end.
+// Learned warm-start input of one env after a substep: qpos, qvel, ctrl
+// (written as zeros here: the caller inserts the ctrl of the substep the
+// prediction is for), then the first FEAT_NCON contacts as (normal, depth =
+// -dist, one-hot of the limb class of either geom), zero-padded.
+KDEV void write_features(const DModel& m, const Layout& L, const float* w, float* feat, int env) {
+  int nq = m.nq, nv = m.nv, nu = m.nu, n0 = nq + nv + nu, nf = m.nfeat;
+  int ncon = reinterpret_cast<const int*>(w + L.misc)[MISC_NCON];
+  for (int i = LANE; i < nf; i += WAVE) {
+    float x = 0.f;
+    if (i < nq) x = w[L.qpos + i];
+    else if (i < nq + nv) x = w[L.qvel + i - nq];
+    else if (i >= n0) {
+      int j = i - n0, c = j / FEAT_PER_CON, k = j - c * FEAT_PER_CON;
+      if (c < ncon) {
+        const float* cr = w + L.con + c * CON_STRIDE;
+        if (k < 3) x = cr[CON_FRAME + k];
+        else if (k == 3) x = -cr[CON_DIST];
+        else {
+          int p = reinterpret_cast<const int*>(cr + CON_PAIR)[0];
+          int c1 = m.geom_fclass[m.pair_g1[p]], c2 = m.geom_fclass[m.pair_g2[p]];
+          x = (c1 == k - 4 || c2 == k - 4) ? 1.f : 0.f;
+        }
+      }
+    }
+    feat[(size_t)env * nf + i] = x;
+  }
+}
+
 KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, int env, int nsub) {
   int nq = m.nq, nv = m.nv, nu = m.nu;
   for (int i = LANE; i < nq; i += WAVE) w[L.qpos + i] = io.qpos[(size_t)env * nq + i];
   for (int i = LANE; i < nv; i += WAVE) {
     w[L.qvel + i] = io.qvel[(size_t)env * nv + i];
     w[L.qacc_ws + i] = io.qacc_ws[(size_t)env * nv + i];
+    if (io.qacc_pred) w[L.qacc_pred + i] = io.qacc_pred[(size_t)env * nv + i];
   }
   for (int i = LANE; i < nu; i += WAVE) w[L.ctrl + i] = io.ctrl ? io.ctrl[(size_t)env * nu + i] : 0.f;
   KSYNC();
@@ -1501,7 +1545,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     KPROF_MARK(&pa, KP_CONSTR);
     if (last) dump_ws(L, w, io.dump1, env);
     int it = 0;
-    st |= st_solve(m, L, w, &it, &pa);
+    st |= st_solve(m, L, w, &it, &pa, io.qacc_pred != nullptr && s == 0);
     niter += it;
     if (last) dump_ws(L, w, io.dump2, env);
     st |= st_integrate(m, L, w);
@@ -1520,13 +1564,14 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     io.qacc_ws[(size_t)env * nv + i] = w[L.qacc_ws + i];
   }
   if (wori(bad)) st |= ST_NONFINITE;
+  if (io.feat) write_features(m, L, w, io.feat, env);
   KPROF_MARK(&pa, KP_IO);
   if (LANE == 0) {
     const int* misc = reinterpret_cast<const int*>(w + L.misc);
     if (io.prof)
       for (int i = 0; i < KP_N; i++) io.prof[(size_t)env * KP_N + i] = pa.t[i];
-    if (io.niter) io.niter[env] = niter;
-    if (io.status) io.status[env] = st;
+    if (io.niter) io.niter[env] = io.accumulate ? io.niter[env] + niter : niter;
+    if (io.status) io.status[env] = io.accumulate ? (io.status[env] | st) : st;
     if (io.ncon) io.ncon[env] = misc[MISC_NCON];
     if (io.nefc) io.nefc[env] = misc[MISC_NEFC];
   }

@@ -87,6 +87,26 @@ def lib() -> C.CDLL:
             "mjmlx_batched_capacity": (C.c_int, [C.c_void_p, _I, _I, _I]),
             "mjmlx_batched_debug_step": (C.c_int, [C.c_void_p, _F, _F, _F]),
             "mjmlx_batched_layout": (C.c_int, [C.c_void_p, C.c_char_p, _I, _I, _I]),
+            "mjmlx_mlp_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
+            "mjmlx_mlp_free": (None, [C.c_void_p]),
+            "mjmlx_mlp_dims": (C.c_int, [C.c_void_p, _I, _I, _I, _I]),
+            "mjmlx_mlp_set_normalization": (C.c_int, [C.c_void_p, _F, _F, _F, _F]),
+            "mjmlx_mlp_get_normalization": (C.c_int, [C.c_void_p, _F, _F, _F, _F]),
+            "mjmlx_mlp_get_params": (C.c_int, [C.c_void_p, _F]),
+            "mjmlx_mlp_set_params": (C.c_int, [C.c_void_p, _F]),
+            "mjmlx_mlp_forward": (C.c_int, [C.c_void_p, _F, C.c_int, _F]),
+            "mjmlx_mlp_grad": (C.c_float, [C.c_void_p, _F, _F, C.c_int, _F]),
+            "mjmlx_mlp_train_step": (C.c_float, [C.c_void_p, _F, _F, C.c_int, C.c_float, C.c_float]),
+            "mjmlx_mlp_dataset_reserve": (C.c_int, [C.c_void_p, C.c_long]),
+            "mjmlx_mlp_dataset_size": (C.c_long, [C.c_void_p]),
+            "mjmlx_mlp_dataset_append": (C.c_long, [C.c_void_p, _F, _F, C.c_long]),
+            "mjmlx_mlp_dataset_get": (C.c_int, [C.c_void_p, C.c_long, C.c_long, _F, _F]),
+            "mjmlx_mlp_fit_normalization": (C.c_int, [C.c_void_p]),
+            "mjmlx_mlp_fit": (C.c_float, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int]),
+            "mjmlx_batched_feature_dim": (C.c_int, [C.c_void_p]),
+            "mjmlx_batched_set_warmstart_mlp": (C.c_int, [C.c_void_p, C.c_void_p]),
+            "mjmlx_batched_record": (C.c_long, [C.c_void_p, C.c_void_p, _F, C.c_int]),
+            "mjmlx_batched_get_features": (C.c_int, [C.c_void_p, _F]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -224,6 +244,24 @@ class BatchedSim:
         assert lib().mjmlx_batched_debug_step(self.h, _fp(c), _fp(d1), _fp(d2)) == 0
         return DebugDump(self, d1, d2, total)
 
+    def feature_dim(self) -> int:
+        return lib().mjmlx_batched_feature_dim(self.h)
+
+    def set_warmstart_mlp(self, mlp):
+        self._mlp = mlp  # the sim borrows the network: keep it alive
+        assert lib().mjmlx_batched_set_warmstart_mlp(self.h, None if mlp is None else mlp.h) == 0
+
+    def record(self, mlp, ctrl=None, n: int = 1) -> int:
+        c = None if ctrl is None else np.ascontiguousarray(ctrl, dtype=np.float32)
+        r = lib().mjmlx_batched_record(self.h, mlp.h, _fp(c), n)
+        assert r >= 0
+        return r
+
+    def features(self):
+        out = np.zeros((self.B, self.feature_dim()), np.float32)
+        assert lib().mjmlx_batched_get_features(self.h, _fp(out)) > 0
+        return out
+
     def _getter(self, fn: str, shape):
         n = C.c_int()
         p = getattr(lib(), fn)(self.h, C.byref(n))
@@ -259,3 +297,87 @@ class DebugDump:
         off, ln, kind = self.sim.layout(name)
         seg = self.d[which][env * self.total + off: env * self.total + off + ln].copy()
         return seg.view(np.int32) if kind == 1 else seg
+
+
+class MLP:
+    """The learned warm-start network (mjmlx_mlp_*), FP32 master weights on the GPU."""
+
+    def __init__(self, nin: int, nh: int, nout: int, seed: int = 0):
+        self.h = lib().mjmlx_mlp_create(nin, nh, nout, seed)
+        if not self.h:
+            raise RuntimeError("mjmlx_mlp_create failed (see stderr)")
+        self.nin, self.nh, self.nout = nin, nh, nout
+        n = C.c_int()
+        lib().mjmlx_mlp_dims(self.h, None, None, None, C.byref(n))
+        self.nparam = n.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mjmlx_mlp_free(self.h)
+            self.h = None
+
+    def params(self):
+        out = np.zeros(self.nparam, np.float32)
+        assert lib().mjmlx_mlp_get_params(self.h, _fp(out)) == self.nparam
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        assert lib().mjmlx_mlp_set_params(self.h, _fp(p)) == self.nparam
+
+    def set_normalization(self, in_mean=None, in_std=None, out_mean=None, out_std=None):
+        a = [None if v is None else np.ascontiguousarray(v, dtype=np.float32) for v in (in_mean, in_std, out_mean, out_std)]
+        assert lib().mjmlx_mlp_set_normalization(self.h, *(_fp(v) for v in a)) == 0
+
+    def normalization(self):
+        a = [np.zeros(self.nin, np.float32), np.zeros(self.nin, np.float32),
+             np.zeros(self.nout, np.float32), np.zeros(self.nout, np.float32)]
+        assert lib().mjmlx_mlp_get_normalization(self.h, *(_fp(v) for v in a)) == 0
+        return a
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.nout), np.float32)
+        assert lib().mjmlx_mlp_forward(self.h, _fp(x), x.shape[0], _fp(y)) == 0
+        return y
+
+    def grad(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(self.nparam, np.float32)
+        loss = lib().mjmlx_mlp_grad(self.h, _fp(x), _fp(y), x.shape[0], _fp(g))
+        assert loss >= 0
+        return loss, g
+
+    def train_step(self, x, y, lr: float, wd: float = 0.0) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = lib().mjmlx_mlp_train_step(self.h, _fp(x), _fp(y), x.shape[0], lr, wd)
+        assert loss >= 0
+        return loss
+
+    def reserve(self, rows: int):
+        assert lib().mjmlx_mlp_dataset_reserve(self.h, rows) == 0
+
+    def dataset_size(self) -> int:
+        return lib().mjmlx_mlp_dataset_size(self.h)
+
+    def append(self, x, y) -> int:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        return lib().mjmlx_mlp_dataset_append(self.h, _fp(x), _fp(y), x.shape[0])
+
+    def dataset(self, first: int, rows: int):
+        x = np.zeros((rows, self.nin), np.float32)
+        y = np.zeros((rows, self.nout), np.float32)
+        assert lib().mjmlx_mlp_dataset_get(self.h, first, rows, _fp(x), _fp(y)) == 0
+        return x, y
+
+    def fit_normalization(self):
+        assert lib().mjmlx_mlp_fit_normalization(self.h) == 0
+
+    def fit(self, epochs: int, batch: int, lr: float = 1e-3, wd: float = 0.0, seed: int = 0) -> float:
+        loss = lib().mjmlx_mlp_fit(self.h, epochs, batch, lr, wd, seed)
+        assert loss >= 0, "mjmlx_mlp_fit failed (see stderr)"
+        return loss

@@ -37,6 +37,8 @@ def lib():
         L.mjemu_dump.argtypes = [C.c_void_p, C.c_int]
         L.mjemu_stats.argtypes = [C.c_void_p, _I, _I, _I, _I]
         L.mjemu_model_field.argtypes = [C.c_void_p, C.c_char_p, _D, C.c_int]
+        L.mjemu_features.argtypes = [C.c_void_p, _F]
+        L.mjemu_set_pred.argtypes = [C.c_void_p, _F]
         L.mjemu_compile.restype = C.c_void_p
         L.mjemu_compile.argtypes = [C.c_char_p, C.c_int, C.c_int]
         _lib = L
@@ -96,6 +98,18 @@ class EmuSim:
         out = [np.zeros(self.B, np.int32) for _ in range(4)]
         lib().mjemu_stats(self.h, *(o.ctypes.data_as(_I) for o in out))
         return dict(niter=out[0], status=out[1], ncon=out[2], nefc=out[3])
+
+    def features(self) -> np.ndarray:
+        """Learned warm-start features written by the last step, [B, nfeat]."""
+        nf = lib().mjemu_features(self.h, None)
+        out = np.zeros((self.B, nf), np.float32)
+        lib().mjemu_features(self.h, _fp(out))
+        return out
+
+    def set_pred(self, pred=None):
+        """Predicted qacc [B, nv] offered to the solver as a warm-start candidate (None: none)."""
+        self._pred = None if pred is None else np.ascontiguousarray(pred, dtype=np.float32)
+        lib().mjemu_set_pred(self.h, _fp(self._pred))
 
     def dump(self, which: int, env: int, name: str) -> np.ndarray:
         off, ln, kind = self.layout(name)
