@@ -43,11 +43,12 @@ void report(const char* fn, const std::exception& e) { std::fprintf(stderr, "%s 
 }  // namespace
 
 // ------------------------------------------------------------------ kernels
+template <int NPSET>
 __global__ void __launch_bounds__(64) k_step(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub) {
   extern __shared__ __align__(16) float lds[];
   int env = blockIdx.x;
   if (env >= B) return;
-  mjk::run_env(m, L, lds, io, env, nsub);
+  mjk::run_env<NPSET>(m, L, lds, io, env, nsub);
 }
 
 __global__ void __launch_bounds__(64) k_step_queue(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub,
@@ -59,7 +60,7 @@ __global__ void __launch_bounds__(64) k_step_queue(mjk::DModel m, mjk::Layout L,
     if (threadIdx.x == 0) t = atomicAdd(queue, 1);
     int env = __shfl(t, 0, 64);
     if (env >= B) break;
-    mjk::run_env(m, L, lds, io, env, nsub);
+    mjk::run_env<0>(m, L, lds, io, env, nsub);
   }
 }
 
@@ -125,6 +126,7 @@ struct MjmlxBatchedSim {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_ctrl = nullptr;
   bool timed = false, ctrl_pending = false;
   int persistent = 0, grid = 0, ncu = 0;
+  int npset = 0;  // narrow-phase set compiled into the step kernel
   int ws_mode = 0;
   // packed readback
   int* d_pack_bodies = nullptr;
@@ -167,6 +169,14 @@ void free_sim(MjmlxBatchedSim* s) {
   delete s;
 }
 
+// one workgroup (one wave) per env, with the narrow-phase set the model needs
+void launch_step(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
+  if (s->npset == 1)
+    hipLaunchKernelGGL(k_step<1>, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub);
+  else
+    hipLaunchKernelGGL(k_step<0>, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub);
+}
+
 mjk::StepIO make_io(MjmlxBatchedSim* s, float* dump1, float* dump2) {
   mjk::StepIO io{};
   io.qpos = s->d_qpos; io.qvel = s->d_qvel; io.qacc_ws = s->d_qws; io.ctrl = s->d_ctrl;
@@ -175,6 +185,7 @@ mjk::StepIO make_io(MjmlxBatchedSim* s, float* dump1, float* dump2) {
   io.has_aux = s->aux_on ? 1 : 0;
   io.aux.xpos = s->d_aux[0]; io.aux.xquat = s->d_aux[1]; io.aux.subtree_com = s->d_aux[2];
   io.aux.cinert = s->d_aux[3]; io.aux.cvel = s->d_aux[4]; io.aux.qfrc_actuator = s->d_aux[5];
+  io.aux.cfrc_ext = s->d_aux[6];
   return io;
 }
 
@@ -201,7 +212,7 @@ void launch(MjmlxBatchedSim* s, int nsub, float* dump1, float* dump2) {
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s->stream, s->d_queue, 1);
     hipLaunchKernelGGL(k_step_queue, dim3(grid), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub, s->d_queue);
   } else {
-    hipLaunchKernelGGL(k_step, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub);
+    launch_step(s, io, nsub);
   }
   HIPOK(hipGetLastError());
   HIPOK(hipEventRecord(s->ev1, s->stream));
@@ -238,7 +249,7 @@ long step_substeps(MjmlxBatchedSim* s, int n, MjmlxMLP* rec) {
       io.qacc_pred = s->d_pred;
     }
     io.feat = s->d_feat[s->feat_cur ^ 1];
-    hipLaunchKernelGGL(k_step, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, 1);
+    launch_step(s, io, 1);
     HIPOK(hipGetLastError());
     if (rec && s->feat_valid) rows += mjmlp::record(rec, s->d_feat[s->feat_cur], s->d_ctrl, coff, s->nu, s->d_qws, s->B, s->stream);
     s->feat_cur ^= 1;
@@ -268,13 +279,8 @@ const float* get_aux(const MjmlxBatchedSim* cs, int i, int* n_out) {
   try {
     ensure_aux(s);
     size_t n = s->aux_len(i);
-    if (i == 6) {
-      // per-body contact forces are not computed by this engine version: zeros
-      std::memset(s->h_aux[6], 0, n * sizeof(float));
-    } else {
-      HIPOK(hipMemcpyAsync(s->h_aux[i], s->d_aux[i], n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-      HIPOK(hipStreamSynchronize(s->stream));
-    }
+    HIPOK(hipMemcpyAsync(s->h_aux[i], s->d_aux[i], n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPOK(hipStreamSynchronize(s->stream));
     if (n_out) *n_out = (int)n;
     return s->h_aux[i];
   } catch (const std::exception& e) {
@@ -445,6 +451,10 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     }
     std::vector<mjh::CollisionPair> pairs;
     s->blob = mjh::build_blob(m, &pairs);
+    for (auto& p : pairs) {
+      int t1 = m.geom_type[p.g1], t2 = m.geom_type[p.g2];
+      if ((t1 == mjk::GCAPSULE && t2 == mjk::GBOX) || (t1 == mjk::GBOX && t2 == mjk::GBOX)) s->npset = 1;
+    }
     s->B = config->num_envs; s->nq = m.nq; s->nv = m.nv; s->nu = m.nu; s->nb = m.nbody;
     int maxcon = std::max(1, std::min(mjh::max_contacts(m, pairs), 32));
     int maxefc = std::max(1, std::min(mjh::max_rows(m, pairs), 96));
@@ -461,7 +471,8 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     if (s->lds_bytes > (size_t)prop.sharedMemPerBlock)
       throw std::runtime_error("per-env workspace (" + std::to_string(s->lds_bytes) + " B) exceeds the LDS per workgroup");
     if (s->lds_bytes > 65536) {
-      HIPOK(hipFuncSetAttribute((const void*)k_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
+      HIPOK(hipFuncSetAttribute((const void*)k_step<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
+      HIPOK(hipFuncSetAttribute((const void*)k_step<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
       HIPOK(hipFuncSetAttribute((const void*)k_step_queue, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
     }
     HIPOK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -787,7 +798,7 @@ MJMLX_API int mjmlx_batched_profile_step(MjmlxBatchedSim* s, const float* ctrl, 
     upload_ctrl(s, ctrl);
     mjk::StepIO io = make_io(s, nullptr, nullptr);
     io.prof = d;
-    hipLaunchKernelGGL(k_step, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub);
+    launch_step(s, io, nsub);
     HIPOK(hipGetLastError());
     HIPOK(hipMemcpyAsync(out, d, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
     HIPOK(hipStreamSynchronize(s->stream));

@@ -30,7 +30,8 @@ bool pair_supported(int t1, int t2) {
   using namespace mjk;
   if (t1 == GPLANE) return t2 == GSPHERE || t2 == GCAPSULE || t2 == GBOX;
   if (t1 == GSPHERE) return t2 == GSPHERE || t2 == GCAPSULE || t2 == GBOX;
-  if (t1 == GCAPSULE) return t2 == GCAPSULE;
+  if (t1 == GCAPSULE) return t2 == GCAPSULE || t2 == GBOX;
+  if (t1 == GBOX) return t2 == GBOX;
   return false;
 }
 
@@ -38,6 +39,8 @@ int pair_maxc(int t1, int t2) {
   using namespace mjk;
   if (t1 == GPLANE && t2 == GCAPSULE) return 2;
   if (t1 == GPLANE && t2 == GBOX) return 4;
+  if (t1 == GCAPSULE && t2 == GBOX) return 2;
+  if (t1 == GBOX && t2 == GBOX) return 4;
   return 1;
 }
 

@@ -686,7 +686,250 @@ KDEV int k_sphere_box(const float* c, float r, const float* bx, const float* R, 
   return 1;
 }
 
+// signed distance of a point (box frame) to a box of half-sizes h (negative inside)
+KDEV float k_box_sdf(const float* p, const float* h) {
+  float q[3] = {fabsf(p[0]) - h[0], fabsf(p[1]) - h[1], fabsf(p[2]) - h[2]};
+  float o[3] = {fmaxf(q[0], 0.f), fmaxf(q[1], 0.f), fmaxf(q[2], 0.f)};
+  return sqrtf(dot3(o, o)) + fminf(fmaxf(q[0], fmaxf(q[1], q[2])), 0.f);
+}
+
+// Capsule (geom1) against box (geom2), up to 2 contacts. The box signed
+// distance along the capsule axis is convex, so a golden-section search finds
+// its minimum t*; the capsule is then a sphere at t* (and, when they are
+// within the margin and apart from t*, at the segment ends). When both ends
+// touch and the interior is no deeper, the two ends are the contacts (a
+// capsule lying on a face).
+KDEV int k_capsule_box(const float* xc, const float* Rc, const float* sc, const float* xb, const float* Rb,
+                       const float* sb, float margin, RawC* out) {
+  float p0[3], d[3];
+  k_capsule_ends(xc, Rc, sc[1], p0, d);
+  float r = sc[0];
+  float l0[3], ld[3], rel[3] = {p0[0] - xb[0], p0[1] - xb[1], p0[2] - xb[2]};
+  mtv3(Rb, rel, l0);
+  mtv3(Rb, d, ld);
+  const float gr = 0.381966011f;  // 2 - golden ratio
+  float a = 0.f, b = 1.f;
+  float t1 = a + gr * (b - a), t2 = b - gr * (b - a);
+  float q1[3] = {l0[0] + t1 * ld[0], l0[1] + t1 * ld[1], l0[2] + t1 * ld[2]};
+  float q2[3] = {l0[0] + t2 * ld[0], l0[1] + t2 * ld[1], l0[2] + t2 * ld[2]};
+  float f1 = k_box_sdf(q1, sb), f2 = k_box_sdf(q2, sb);
+  for (int it = 0; it < 32; it++) {
+    if (f1 <= f2) {
+      b = t2; t2 = t1; f2 = f1;
+      t1 = a + gr * (b - a);
+      float q[3] = {l0[0] + t1 * ld[0], l0[1] + t1 * ld[1], l0[2] + t1 * ld[2]};
+      f1 = k_box_sdf(q, sb);
+    } else {
+      a = t1; t1 = t2; f1 = f2;
+      t2 = b - gr * (b - a);
+      float q[3] = {l0[0] + t2 * ld[0], l0[1] + t2 * ld[1], l0[2] + t2 * ld[2]};
+      f2 = k_box_sdf(q, sb);
+    }
+  }
+  float ts = 0.5f * (a + b);
+  float qs[3] = {l0[0] + ts * ld[0], l0[1] + ts * ld[1], l0[2] + ts * ld[2]};
+  float qa[3] = {l0[0], l0[1], l0[2]}, qb[3] = {l0[0] + ld[0], l0[1] + ld[1], l0[2] + ld[2]};
+  float fs = k_box_sdf(qs, sb), fa = k_box_sdf(qa, sb), fb = k_box_sdf(qb, sb);
+  if (fs - r > margin) return 0;
+  float tc[2];
+  int nc = 0;
+  if (fa - r <= margin && fb - r <= margin && fs >= fminf(fa, fb) - 1e-6f) {
+    tc[0] = 0.f; tc[1] = 1.f; nc = 2;
+  } else {
+    tc[nc++] = ts;
+    float te = fa <= fb ? 0.f : 1.f, fe = fminf(fa, fb);
+    if (fe - r <= margin && fabsf(te - ts) > 0.1f) tc[nc++] = te;
+  }
+  int n = 0;
+  for (int k = 0; k < nc; k++) {
+    float c[3] = {p0[0] + tc[k] * d[0], p0[1] + tc[k] * d[1], p0[2] + tc[k] * d[2]};
+    n += k_sphere_box(c, r, xb, Rb, sb, margin, out + n);
+  }
+  return n;
+}
+
+// Box (geom1) against box (geom2) by the separating-axis test over the 15
+// axes, up to 4 contacts. Face axis: the incident face of the other box is
+// clipped against the side planes of the reference face and the points
+// below it (within the margin) are kept, reduced to the deepest point plus
+// the three that span the largest area. Edge axis: one contact at the
+// closest points of the two supporting edges. Face axes are preferred unless
+// an edge axis separates clearly more.
+KDEV int k_box_box(const float* x1, const float* R1, const float* h1, const float* x2, const float* R2,
+                   const float* h2, float margin, RawC* out) {
+  float C[9], t[3], dw[3] = {x2[0] - x1[0], x2[1] - x1[1], x2[2] - x1[2]};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) C[3 * i + j] = R1[i] * R2[j] + R1[3 + i] * R2[3 + j] + R1[6 + i] * R2[6 + j];
+  mtv3(R1, dw, t);  // box 2 centre in box 1's frame; C[i][j] = a_i . b_j
+  float best_face = -3.0e38f, best_edge = -3.0e38f;
+  int face = -1, edge = -1;
+  float edge_ax[3] = {0.f, 0.f, 0.f};
+  for (int i = 0; i < 3; i++) {  // axes of box 1
+    float rb = h2[0] * fabsf(C[3 * i]) + h2[1] * fabsf(C[3 * i + 1]) + h2[2] * fabsf(C[3 * i + 2]);
+    float sep = fabsf(t[i]) - h1[i] - rb;
+    if (sep > margin) return 0;
+    if (sep > best_face) { best_face = sep; face = i; }
+  }
+  for (int j = 0; j < 3; j++) {  // axes of box 2
+    float ra = h1[0] * fabsf(C[j]) + h1[1] * fabsf(C[3 + j]) + h1[2] * fabsf(C[6 + j]);
+    float tb = t[0] * C[j] + t[1] * C[3 + j] + t[2] * C[6 + j];
+    float sep = fabsf(tb) - ra - h2[j];
+    if (sep > margin) return 0;
+    if (sep > best_face) { best_face = sep; face = 3 + j; }
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {  // a_i x b_j in box 1's frame
+      float bj[3] = {C[j], C[3 + j], C[6 + j]}, ai[3] = {0.f, 0.f, 0.f}, L[3];
+      ai[i] = 1.f;
+      cross3(ai, bj, L);
+      float ln = sqrtf(dot3(L, L));
+      if (ln < 1e-5f) continue;  // parallel edges: covered by the face axes
+      for (int k = 0; k < 3; k++) L[k] /= ln;
+      float ra = h1[0] * fabsf(L[0]) + h1[1] * fabsf(L[1]) + h1[2] * fabsf(L[2]);
+      float lb[3] = {L[0] * C[0] + L[1] * C[3] + L[2] * C[6], L[0] * C[1] + L[1] * C[4] + L[2] * C[7],
+                     L[0] * C[2] + L[1] * C[5] + L[2] * C[8]};
+      float rb = h2[0] * fabsf(lb[0]) + h2[1] * fabsf(lb[1]) + h2[2] * fabsf(lb[2]);
+      float sep = fabsf(dot3(t, L)) - ra - rb;
+      if (sep > margin) return 0;
+      if (sep > best_edge) {
+        best_edge = sep; edge = 3 * i + j;
+        float sg = dot3(t, L) < 0.f ? -1.f : 1.f;
+        edge_ax[0] = sg * L[0]; edge_ax[1] = sg * L[1]; edge_ax[2] = sg * L[2];
+      }
+    }
+  float hmin = fminf(fminf(fminf(h1[0], h1[1]), fminf(h1[2], h2[0])), fminf(h2[1], h2[2]));
+  if (edge >= 0 && best_edge > 0.95f * best_face + 0.01f * hmin) {
+    // edge-edge: supporting edge of box 1 along +axis, of box 2 along -axis (box 1 frame)
+    int i = edge / 3, j = edge % 3;
+    float p1[3], p2[3], e1[3] = {0.f, 0.f, 0.f}, e2[3], bj[3] = {C[j], C[3 + j], C[6 + j]};
+    for (int k = 0; k < 3; k++) p1[k] = k == i ? 0.f : (edge_ax[k] >= 0.f ? h1[k] : -h1[k]);
+    p1[i] = -h1[i];
+    e1[i] = 2.f * h1[i];
+    float c2[3] = {t[0], t[1], t[2]};
+    for (int k = 0; k < 3; k++) {
+      if (k == j) continue;
+      float bk[3] = {C[k], C[3 + k], C[6 + k]};
+      float sg = dot3(bk, edge_ax) > 0.f ? -1.f : 1.f;
+      for (int q = 0; q < 3; q++) c2[q] += sg * h2[k] * bk[q];
+    }
+    for (int q = 0; q < 3; q++) { p2[q] = c2[q] - h2[j] * bj[q]; e2[q] = 2.f * h2[j] * bj[q]; }
+    float s, u;
+    k_seg_seg(p1, e1, p2, e2, &s, &u);
+    float m1[3], mid[3], nw[3];
+    for (int q = 0; q < 3; q++) m1[q] = 0.5f * (p1[q] + s * e1[q] + p2[q] + u * e2[q]);
+    mv3(R1, m1, mid);
+    mv3(R1, edge_ax, nw);
+    out->dist = best_edge;
+    for (int q = 0; q < 3; q++) { out->pos[q] = mid[q] + x1[q]; out->n[q] = nw[q]; }
+    return 1;
+  }
+  // face contact: reference box owns the axis; normal nr (world) points from it to the incident box
+  bool ref1 = face < 3;
+  int ax = ref1 ? face : face - 3;
+  const float *xr = ref1 ? x1 : x2, *Rr = ref1 ? R1 : R2, *hr = ref1 ? h1 : h2;
+  const float *xi = ref1 ? x2 : x1, *Ri = ref1 ? R2 : R1, *hi = ref1 ? h2 : h1;
+  float dri[3] = {xi[0] - xr[0], xi[1] - xr[1], xi[2] - xr[2]};
+  float nr[3] = {Rr[ax], Rr[3 + ax], Rr[6 + ax]};
+  if (dot3(nr, dri) < 0.f) { nr[0] = -nr[0]; nr[1] = -nr[1]; nr[2] = -nr[2]; }
+  // incident face: the face of the other box most anti-parallel to nr
+  int fi = 0;
+  float fmin = 3.0e38f, fsg = 1.f;
+  for (int k = 0; k < 3; k++) {
+    float dk = Ri[k] * nr[0] + Ri[3 + k] * nr[1] + Ri[6 + k] * nr[2];
+    if (-fabsf(dk) < fmin) { fmin = -fabsf(dk); fi = k; fsg = dk > 0.f ? -1.f : 1.f; }
+  }
+  // incident face corners in the reference box frame
+  float poly[8][3], tmp[8][3];
+  int np = 4;
+  {
+    int u = (fi + 1) % 3, v = (fi + 2) % 3;
+    const float su[4] = {-1.f, 1.f, 1.f, -1.f}, sv[4] = {-1.f, -1.f, 1.f, 1.f};
+    for (int c = 0; c < 4; c++) {
+      float pl[3] = {0.f, 0.f, 0.f}, pw[3], rl[3];
+      pl[fi] = fsg * hi[fi]; pl[u] = su[c] * hi[u]; pl[v] = sv[c] * hi[v];
+      mv3(Ri, pl, pw);
+      for (int q = 0; q < 3; q++) pw[q] += xi[q] - xr[q];
+      mtv3(Rr, pw, rl);
+      for (int q = 0; q < 3; q++) poly[c][q] = rl[q];
+    }
+  }
+  // clip against the four side planes |p[k]| <= hr[k], k != ax
+  for (int k = 0; k < 3; k++) {
+    if (k == ax) continue;
+    for (int sg = -1; sg <= 1; sg += 2) {
+      int nn = 0;
+      for (int c = 0; c < np; c++) {
+        const float* A = poly[c];
+        const float* Bp = poly[(c + 1) % np];
+        float da = sg * A[k] - hr[k], db = sg * Bp[k] - hr[k];
+        if (da <= 0.f && nn < 8) { for (int q = 0; q < 3; q++) tmp[nn][q] = A[q]; nn++; }
+        if ((da < 0.f) != (db < 0.f) && nn < 8) {
+          float tt = da / (da - db);
+          for (int q = 0; q < 3; q++) tmp[nn][q] = A[q] + tt * (Bp[q] - A[q]);
+          nn++;
+        }
+      }
+      np = nn;
+      for (int c = 0; c < np; c++) for (int q = 0; q < 3; q++) poly[c][q] = tmp[c][q];
+    }
+  }
+  // depth of each clipped point below the reference face (normal along +/- axis ax)
+  float sgn = 0.f;
+  {
+    float nl[3];
+    mtv3(Rr, nr, nl);
+    sgn = nl[ax] >= 0.f ? 1.f : -1.f;
+  }
+  float dep[8];
+  int keep[8], nk = 0;
+  for (int c = 0; c < np; c++) {
+    dep[c] = sgn * poly[c][ax] - hr[ax];
+    if (dep[c] <= margin) keep[nk++] = c;
+  }
+  if (nk == 0) return 0;
+  int sel[4], ns = 0;
+  if (nk <= 4) {
+    for (int c = 0; c < nk; c++) sel[ns++] = keep[c];
+  } else {
+    // deepest, farthest from it, then the largest triangle on each side of that chord
+    int i0 = keep[0];
+    for (int c = 1; c < nk; c++) if (dep[keep[c]] < dep[i0]) i0 = keep[c];
+    int i1 = -1;
+    float dbest = -1.f;
+    for (int c = 0; c < nk; c++) {
+      float dd[3] = {poly[keep[c]][0] - poly[i0][0], poly[keep[c]][1] - poly[i0][1], poly[keep[c]][2] - poly[i0][2]};
+      float l2 = dot3(dd, dd);
+      if (l2 > dbest) { dbest = l2; i1 = keep[c]; }
+    }
+    float e[3] = {poly[i1][0] - poly[i0][0], poly[i1][1] - poly[i0][1], poly[i1][2] - poly[i0][2]};
+    int i2 = -1, i3 = -1;
+    float amax = 0.f, amin = 0.f;
+    for (int c = 0; c < nk; c++) {
+      float f[3] = {poly[keep[c]][0] - poly[i0][0], poly[keep[c]][1] - poly[i0][1], poly[keep[c]][2] - poly[i0][2]}, cr[3];
+      cross3(e, f, cr);
+      float area = cr[ax] * sgn;
+      if (area > amax) { amax = area; i2 = keep[c]; }
+      if (area < amin) { amin = area; i3 = keep[c]; }
+    }
+    sel[ns++] = i0; sel[ns++] = i1;
+    if (i2 >= 0) sel[ns++] = i2;
+    if (i3 >= 0) sel[ns++] = i3;
+  }
+  float nw[3] = {ref1 ? nr[0] : -nr[0], ref1 ? nr[1] : -nr[1], ref1 ? nr[2] : -nr[2]};
+  for (int c = 0; c < ns; c++) {
+    const float* pl = poly[sel[c]];
+    float d = dep[sel[c]];
+    float mid[3] = {pl[0], pl[1], pl[2]}, mw[3];
+    mid[ax] -= 0.5f * sgn * d;  // halfway between the point and the reference face
+    mv3(Rr, mid, mw);
+    out[c].dist = d;
+    for (int q = 0; q < 3; q++) { out[c].pos[q] = mw[q] + xr[q]; out[c].n[q] = nw[q]; }
+  }
+  return ns;
+}
+
 // narrow phase for one pair (geom1 has the lower type); -1 for unsupported
+template <int NPSET>
 KDEV int k_narrow(int t1, int t2, const float* s1, const float* s2, const float* x1, const float* R1,
                   const float* x2, const float* R2, float margin, RawC* out) {
   if (t1 == GPLANE) {
@@ -726,6 +969,8 @@ KDEV int k_narrow(int t1, int t2, const float* s1, const float* s2, const float*
     if (t2 == GBOX) return k_sphere_box(x1, s1[0], x2, R2, s2, margin, out);
     return -1;
   }
+  if (NPSET >= 1 && t1 == GCAPSULE && t2 == GBOX) return k_capsule_box(x1, R1, s1, x2, R2, s2, margin, out);
+  if (NPSET >= 1 && t1 == GBOX && t2 == GBOX) return k_box_box(x1, R1, s1, x2, R2, s2, margin, out);
   if (t1 == GCAPSULE && t2 == GCAPSULE) {
     float a0[3], ad[3], b0[3], bd[3], s, t;
     k_capsule_ends(x1, R1, s1[1], a0, ad);
@@ -762,6 +1007,7 @@ KDEV void k_make_frame(const float* n_in, float* f) {
 // mj_collision over the static pair table: bounding-sphere cull
 // (reference collision.hpp:301-305), narrow phase per lane, wave prefix scan
 // to compact contacts in pair order.
+template <int NPSET>
 KDEV int st_collision(const DModel& m, const Layout& L, float* w) {
   int* misc = iptr(w, L.misc);
   int st = 0;
@@ -787,7 +1033,7 @@ KDEV int st_collision(const DModel& m, const Layout& L, float* w) {
         float dv[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
         cull = sqrtf(dot3(dv, dv)) > rb1 + rb2 + margin;
       }
-      if (!cull) cnt = k_narrow(m.geom_type[g1], m.geom_type[g2], m.geom_size + 3 * g1, m.geom_size + 3 * g2,
+      if (!cull) cnt = k_narrow<NPSET>(m.geom_type[g1], m.geom_type[g2], m.geom_size + 3 * g1, m.geom_size + 3 * g2,
                                 x1, gm + 9 * g1, x2, gm + 9 * g2, margin, rc);
     }
     int badp = cnt < 0 ? 1 : 0;
@@ -1426,7 +1672,7 @@ KDEV int st_integrate(const DModel& m, const Layout& L, float* w) {
 // aux: optional per-env outputs of the last substep's pre-integration state
 // (xpos, xquat, subtree_com, cinert, cvel, qfrc_actuator).
 struct AuxOut {
-  float *xpos, *xquat, *subtree_com, *cinert, *cvel, *qfrc_actuator;
+  float *xpos, *xquat, *subtree_com, *cinert, *cvel, *qfrc_actuator, *cfrc_ext;
 };
 
 // Global arrays of one launch. State arrays are env-major (env * dim + i), as
@@ -1448,6 +1694,33 @@ KDEV void dump_ws(const Layout& L, const float* w, float* dst, int env) {
   if (!dst) return;
   for (int t = LANE; t < L.total; t += WAVE) dst[(size_t)env * L.total + t] = w[t];
   KSYNC();
+}
+
+// Per-body contact force after the solve, [torque 3, force 3] per body: each
+// contact's normal force (the sum of its rows' forces) along the contact
+// normal, added to body2 and subtracted from body1; torque zero (reference
+// src/compat/mkx_kernels/solver.hpp:906-929).
+KDEV void write_cfrc_ext(const DModel& m, const Layout& L, const float* w, const AuxOut& aux, int env, bool solved) {
+  int nb = m.nbody;
+  const int* misc = reinterpret_cast<const int*>(w + L.misc);
+  int ncon = solved ? misc[MISC_NCON] : 0, nefc = misc[MISC_NEFC];
+  const float* force = w + L.efc_force;
+  for (int b = LANE; b < nb; b += WAVE) {
+    float f[3] = {0.f, 0.f, 0.f};
+    for (int c = 0; c < ncon; c++) {
+      const float* cr = w + L.con + c * CON_STRIDE;
+      int p = reinterpret_cast<const int*>(cr + CON_PAIR)[0];
+      float sgn = m.pair_b2[p] == b ? 1.f : (m.pair_b1[p] == b ? -1.f : 0.f);
+      if (sgn == 0.f) continue;
+      int nrow = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
+      float fn = 0.f;
+      for (int r = r0; r < r0 + nrow && r < nefc; r++) fn += force[r];
+      for (int k = 0; k < 3; k++) f[k] += sgn * fn * cr[CON_FRAME + k];
+    }
+    float* o = aux.cfrc_ext + ((size_t)env * nb + b) * 6;
+    o[0] = o[1] = o[2] = 0.f;
+    o[3] = f[0]; o[4] = f[1]; o[5] = f[2];
+  }
 }
 
 KDEV void write_aux(const DModel& m, const Layout& L, const float* w, const AuxOut& aux, int env) {
@@ -1499,6 +1772,10 @@ KDEV void write_features(const DModel& m, const Layout& L, const float* w, float
   }
 }
 
+// NPSET selects the narrow phases compiled in (0: plane / sphere / capsule
+// pairs and box against plane and sphere; 1: also capsule-box and box-box),
+// so models without the heavier pairs do not pay their registers.
+template <int NPSET>
 KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, int env, int nsub) {
   int nq = m.nq, nv = m.nv, nu = m.nu;
   for (int i = LANE; i < nq; i += WAVE) w[L.qpos + i] = io.qpos[(size_t)env * nq + i];
@@ -1517,7 +1794,10 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     if (nv > 0) factor_ld(m, w + L.qM, w + L.qLD);
     st_com_vel(m, L, w);
     st_smooth(m, L, w);
-    if (io.has_aux) write_aux(m, L, w, io.aux, env);
+    if (io.has_aux) {
+      write_aux(m, L, w, io.aux, env);
+      write_cfrc_ext(m, L, w, io.aux, env, false);
+    }
     return;
   }
   int st = 0, niter = 0;
@@ -1539,7 +1819,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     st |= st_smooth(m, L, w);
     KPROF_MARK(&pa, KP_SMOOTH);
     if (last && io.has_aux) write_aux(m, L, w, io.aux, env);
-    st |= st_collision(m, L, w);
+    st |= st_collision<NPSET>(m, L, w);
     KPROF_MARK(&pa, KP_COLL);
     st |= st_constraints(m, L, w);
     KPROF_MARK(&pa, KP_CONSTR);
@@ -1547,6 +1827,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     int it = 0;
     st |= st_solve(m, L, w, &it, &pa, io.qacc_pred != nullptr && s == 0);
     niter += it;
+    if (last && io.has_aux) write_cfrc_ext(m, L, w, io.aux, env, true);
     if (last) dump_ws(L, w, io.dump2, env);
     st |= st_integrate(m, L, w);
     KPROF_MARK(&pa, KP_INTEGRATE);

@@ -761,6 +761,216 @@ static int sphere_box(const double c[3], double r, const double bx[3], const dou
   return 1;
 }
 
+/* signed distance of a point in the box frame (negative inside) */
+static double box_sdf(const double p[3], const double h[3]) {
+  double q[3], o[3];
+  for (int i = 0; i < 3; i++) { q[i] = fabs(p[i]) - h[i]; o[i] = q[i] > 0 ? q[i] : 0; }
+  double in = q[0];
+  if (q[1] > in) in = q[1];
+  if (q[2] > in) in = q[2];
+  return sqrt(dot3(o, o)) + (in < 0 ? in : 0);
+}
+
+/* capsule (geom1) vs box (geom2), up to 2 contacts: spheres on the capsule
+ * axis at the minimiser of the (convex) box signed distance along the axis,
+ * and at the ends when they are within the margin and apart from it; both
+ * ends when both touch and the interior is no deeper. The minimiser is found
+ * by golden-section search. */
+static int capsule_box(const double* xc, const double* Rc, const double* sc, const double* xb, const double* Rb,
+                       const double* sb, double margin, rawc_t* out) {
+  double p0[3], d[3], l0[3], ld[3], rel[3];
+  capsule_ends(xc, Rc, sc[1], p0, d);
+  for (int i = 0; i < 3; i++) rel[i] = p0[i] - xb[i];
+  mulmatTvec3(Rb, rel, l0);
+  mulmatTvec3(Rb, d, ld);
+  double r = sc[0];
+#define ALONG(tt, q) do { for (int i_ = 0; i_ < 3; i_++) (q)[i_] = l0[i_] + (tt) * ld[i_]; } while (0)
+  const double gr = 0.5 * (3.0 - sqrt(5.0));
+  double a = 0, b = 1, t1 = a + gr * (b - a), t2 = b - gr * (b - a), q[3];
+  ALONG(t1, q); double f1 = box_sdf(q, sb);
+  ALONG(t2, q); double f2 = box_sdf(q, sb);
+  for (int it = 0; it < 80; it++) {
+    if (f1 <= f2) { b = t2; t2 = t1; f2 = f1; t1 = a + gr * (b - a); ALONG(t1, q); f1 = box_sdf(q, sb); }
+    else { a = t1; t1 = t2; f1 = f2; t2 = b - gr * (b - a); ALONG(t2, q); f2 = box_sdf(q, sb); }
+  }
+  double ts = 0.5 * (a + b), fs, fa, fb;
+  ALONG(ts, q); fs = box_sdf(q, sb);
+  ALONG(0.0, q); fa = box_sdf(q, sb);
+  ALONG(1.0, q); fb = box_sdf(q, sb);
+#undef ALONG
+  if (fs - r > margin) return 0;
+  double tc[2];
+  int nc = 0;
+  double fe = fa <= fb ? fa : fb;
+  if (fa - r <= margin && fb - r <= margin && fs >= fe - 1e-6) { tc[0] = 0; tc[1] = 1; nc = 2; }
+  else {
+    tc[nc++] = ts;
+    double te = fa <= fb ? 0.0 : 1.0;
+    if (fe - r <= margin && fabs(te - ts) > 0.1) tc[nc++] = te;
+  }
+  int n = 0;
+  for (int k = 0; k < nc; k++) {
+    double c[3] = {p0[0] + tc[k] * d[0], p0[1] + tc[k] * d[1], p0[2] + tc[k] * d[2]};
+    n += sphere_box(c, r, xb, Rb, sb, margin, out + n);
+  }
+  return n;
+}
+
+/* box (geom1) vs box (geom2) by separating axes, up to 4 contacts.
+ * Face axis (preferred unless an edge axis separates more by the tolerance
+ * sep_e > 0.95 sep_f + 0.01 h_min): clip the incident face against the
+ * reference face's side planes, keep points within the margin below it;
+ * more than 4 are reduced to the deepest, the farthest from it, and the
+ * points of largest positive and negative area across that chord. Edge
+ * axis: one contact at the closest points of the supporting edges. */
+static int box_box(const double* x1, const double* R1, const double* h1, const double* x2, const double* R2,
+                   const double* h2, double margin, rawc_t* out) {
+  double C[3][3], t[3], dw[3];
+  for (int i = 0; i < 3; i++) dw[i] = x2[i] - x1[i];
+  mulmatTvec3(R1, dw, t);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) C[i][j] = R1[i] * R2[j] + R1[3 + i] * R2[3 + j] + R1[6 + i] * R2[6 + j];
+  double bf = -1e300, be = -1e300, eax[3] = {0, 0, 0};
+  int face = -1, edge = -1;
+  for (int i = 0; i < 3; i++) {
+    double sep = fabs(t[i]) - h1[i] - (h2[0] * fabs(C[i][0]) + h2[1] * fabs(C[i][1]) + h2[2] * fabs(C[i][2]));
+    if (sep > margin) return 0;
+    if (sep > bf) { bf = sep; face = i; }
+  }
+  for (int j = 0; j < 3; j++) {
+    double tb = t[0] * C[0][j] + t[1] * C[1][j] + t[2] * C[2][j];
+    double sep = fabs(tb) - h2[j] - (h1[0] * fabs(C[0][j]) + h1[1] * fabs(C[1][j]) + h1[2] * fabs(C[2][j]));
+    if (sep > margin) return 0;
+    if (sep > bf) { bf = sep; face = 3 + j; }
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double ai[3] = {0, 0, 0}, bj[3] = {C[0][j], C[1][j], C[2][j]}, L[3];
+      ai[i] = 1;
+      cross3(ai, bj, L);
+      double ln = sqrt(dot3(L, L));
+      if (ln < 1e-5) continue;
+      for (int k = 0; k < 3; k++) L[k] /= ln;
+      double lb[3];
+      for (int k = 0; k < 3; k++) lb[k] = L[0] * C[0][k] + L[1] * C[1][k] + L[2] * C[2][k];
+      double sep = fabs(dot3(t, L)) - (h1[0] * fabs(L[0]) + h1[1] * fabs(L[1]) + h1[2] * fabs(L[2]))
+                   - (h2[0] * fabs(lb[0]) + h2[1] * fabs(lb[1]) + h2[2] * fabs(lb[2]));
+      if (sep > margin) return 0;
+      if (sep > be) {
+        be = sep; edge = 3 * i + j;
+        double sg = dot3(t, L) < 0 ? -1 : 1;
+        for (int k = 0; k < 3; k++) eax[k] = sg * L[k];
+      }
+    }
+  double hmin = h1[0];
+  for (int k = 0; k < 3; k++) { if (h1[k] < hmin) hmin = h1[k]; if (h2[k] < hmin) hmin = h2[k]; }
+  if (edge >= 0 && be > 0.95 * bf + 0.01 * hmin) {
+    int i = edge / 3, j = edge % 3;
+    double p1[3], e1[3] = {0, 0, 0}, p2[3], e2[3], c2[3] = {t[0], t[1], t[2]};
+    for (int k = 0; k < 3; k++) p1[k] = eax[k] >= 0 ? h1[k] : -h1[k];
+    p1[i] = -h1[i];
+    e1[i] = 2 * h1[i];
+    for (int k = 0; k < 3; k++) {
+      if (k == j) continue;
+      double bk[3] = {C[0][k], C[1][k], C[2][k]};
+      double sg = dot3(bk, eax) > 0 ? -1 : 1;
+      for (int q = 0; q < 3; q++) c2[q] += sg * h2[k] * bk[q];
+    }
+    for (int q = 0; q < 3; q++) { p2[q] = c2[q] - h2[j] * C[q][j]; e2[q] = 2 * h2[j] * C[q][j]; }
+    double s, u, m1[3], mw[3], nw[3];
+    seg_seg(p1, e1, p2, e2, &s, &u);
+    for (int q = 0; q < 3; q++) m1[q] = 0.5 * (p1[q] + s * e1[q] + p2[q] + u * e2[q]);
+    mulmatvec3(R1, m1, mw);
+    mulmatvec3(R1, eax, nw);
+    out->dist = be;
+    for (int q = 0; q < 3; q++) { out->pos[q] = mw[q] + x1[q]; out->n[q] = nw[q]; }
+    return 1;
+  }
+  int ref1 = face < 3, ax = ref1 ? face : face - 3;
+  const double *xr = ref1 ? x1 : x2, *Rr = ref1 ? R1 : R2, *hr = ref1 ? h1 : h2;
+  const double *xi = ref1 ? x2 : x1, *Ri = ref1 ? R2 : R1, *hi = ref1 ? h2 : h1;
+  double nr[3] = {Rr[ax], Rr[3 + ax], Rr[6 + ax]}, dri[3];
+  for (int q = 0; q < 3; q++) dri[q] = xi[q] - xr[q];
+  if (dot3(nr, dri) < 0) for (int q = 0; q < 3; q++) nr[q] = -nr[q];
+  int fi = 0;
+  double best = -1, fsg = 1;
+  for (int k = 0; k < 3; k++) {
+    double dk = Ri[k] * nr[0] + Ri[3 + k] * nr[1] + Ri[6 + k] * nr[2];
+    if (fabs(dk) > best) { best = fabs(dk); fi = k; fsg = dk > 0 ? -1 : 1; }
+  }
+  double poly[8][3], tmp[8][3];
+  int np = 4;
+  int u = (fi + 1) % 3, v = (fi + 2) % 3;
+  const double su[4] = {-1, 1, 1, -1}, sv[4] = {-1, -1, 1, 1};
+  for (int c = 0; c < 4; c++) {
+    double pl[3] = {0, 0, 0}, pw[3];
+    pl[fi] = fsg * hi[fi]; pl[u] = su[c] * hi[u]; pl[v] = sv[c] * hi[v];
+    mulmatvec3(Ri, pl, pw);
+    for (int q = 0; q < 3; q++) pw[q] += xi[q] - xr[q];
+    mulmatTvec3(Rr, pw, poly[c]);
+  }
+  for (int k = 0; k < 3; k++) {
+    if (k == ax) continue;
+    for (int sg = -1; sg <= 1; sg += 2) {
+      int nn = 0;
+      for (int c = 0; c < np; c++) {
+        const double *A = poly[c], *Bq = poly[(c + 1) % np];
+        double da = sg * A[k] - hr[k], db = sg * Bq[k] - hr[k];
+        if (da <= 0 && nn < 8) { memcpy(tmp[nn], A, sizeof(tmp[nn])); nn++; }
+        if (((da < 0) != (db < 0)) && nn < 8) {
+          double tt = da / (da - db);
+          for (int q = 0; q < 3; q++) tmp[nn][q] = A[q] + tt * (Bq[q] - A[q]);
+          nn++;
+        }
+      }
+      np = nn;
+      memcpy(poly, tmp, sizeof(double) * 3 * (size_t)np);
+    }
+  }
+  double nl[3];
+  mulmatTvec3(Rr, nr, nl);
+  double sgn = nl[ax] >= 0 ? 1 : -1;
+  double dep[8];
+  int keep[8], nk = 0;
+  for (int c = 0; c < np; c++) {
+    dep[c] = sgn * poly[c][ax] - hr[ax];
+    if (dep[c] <= margin) keep[nk++] = c;
+  }
+  if (!nk) return 0;
+  int sel[4], ns = 0;
+  if (nk <= 4) for (int c = 0; c < nk; c++) sel[ns++] = keep[c];
+  else {
+    int i0 = keep[0], i1 = -1, i2 = -1, i3 = -1;
+    for (int c = 1; c < nk; c++) if (dep[keep[c]] < dep[i0]) i0 = keep[c];
+    double dbest = -1, amax = 0, amin = 0, e[3];
+    for (int c = 0; c < nk; c++) {
+      double dd[3];
+      for (int q = 0; q < 3; q++) dd[q] = poly[keep[c]][q] - poly[i0][q];
+      if (dot3(dd, dd) > dbest) { dbest = dot3(dd, dd); i1 = keep[c]; }
+    }
+    for (int q = 0; q < 3; q++) e[q] = poly[i1][q] - poly[i0][q];
+    for (int c = 0; c < nk; c++) {
+      double f[3], cr[3];
+      for (int q = 0; q < 3; q++) f[q] = poly[keep[c]][q] - poly[i0][q];
+      cross3(e, f, cr);
+      double area = cr[ax] * sgn;
+      if (area > amax) { amax = area; i2 = keep[c]; }
+      if (area < amin) { amin = area; i3 = keep[c]; }
+    }
+    sel[ns++] = i0; sel[ns++] = i1;
+    if (i2 >= 0) sel[ns++] = i2;
+    if (i3 >= 0) sel[ns++] = i3;
+  }
+  for (int c = 0; c < ns; c++) {
+    double mid[3] = {poly[sel[c]][0], poly[sel[c]][1], poly[sel[c]][2]}, mw[3];
+    mid[ax] -= 0.5 * sgn * dep[sel[c]];
+    mulmatvec3(Rr, mid, mw);
+    out[c].dist = dep[sel[c]];
+    for (int q = 0; q < 3; q++) { out[c].pos[q] = mw[q] + xr[q]; out[c].n[q] = ref1 ? nr[q] : -nr[q]; }
+  }
+  return ns;
+}
+
 /* narrow phase for one pair; geom1 has the lower type. Returns the number of
  * raw contacts written (normal from geom1 to geom2). */
 static int narrowphase(int t1, int t2, const double* s1, const double* s2, const double* x1, const double* R1,
@@ -804,6 +1014,8 @@ static int narrowphase(int t1, int t2, const double* s1, const double* s2, const
     if (t2 == GBOX) return sphere_box(x1, s1[0], x2, R2, s2, margin, out);
     return -1;
   }
+  if (t1 == GCAPSULE && t2 == GBOX) return capsule_box(x1, R1, s1, x2, R2, s2, margin, out);
+  if (t1 == GBOX && t2 == GBOX) return box_box(x1, R1, s1, x2, R2, s2, margin, out);
   if (t1 == GCAPSULE && t2 == GCAPSULE) {
     double a0[3], ad[3], b0[3], bd[3], s, t;
     capsule_ends(x1, R1, s1[1], a0, ad);

@@ -1,0 +1,197 @@
+"""Capsule-box and box-box narrow phases.
+
+The FP64 oracle is pinned by closed-form configurations (stacked faces,
+rotated faces, crossed edges, a capsule lying on a face), by brute-force
+sampling of the capsule axis, and by the surface invariant of every contact
+(the two points dist/2 either side of the contact position lie on the two
+geoms' surfaces). The kernel source, run by its serial interpreter, is then
+compared with the oracle on box scenes.
+
+Parity with MuJoCo's own box-box and capsule-box routines (their exact
+contact points and counts) is unpinned: MuJoCo is not available here.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import emu
+import ref64
+import states
+from mjmlx_hip import scenes
+
+SCENE = """<mujoco><compiler angle="radian"/><option gravity="0 0 0"/><worldbody>
+<body><freejoint/><geom type="box" size="0.2 0.2 0.1"/></body>
+<body><freejoint/><geom type="box" size="0.1 0.1 0.1"/></body>
+<body><freejoint/><geom type="capsule" size="0.05 0.1"/></body>
+</worldbody></mujoco>"""
+
+
+@pytest.fixture(scope="module")
+def orc():
+    e = emu.CompiledModel(SCENE)
+    o = ref64.Oracle(ref64.fields_from(e.field))
+    o._cm = e
+    return o
+
+
+def _pair(orc, g1, g2):
+    for i, p in enumerate(orc.pairs()):
+        if (p[0], p[1]) == (g1, g2):
+            return i
+    raise KeyError((g1, g2))
+
+
+def collide(orc, g1, g2, x1, R1, x2, R2, maxc=8):
+    out = (ref64.OrcContact * maxc)()
+    arr = lambda a: ref64._dp(np.ascontiguousarray(np.asarray(a, np.float64).ravel()))
+    n = ref64.lib().orc_collide_pair(orc.p, _pair(orc, g1, g2), arr(x1), arr(R1), arr(x2), arr(R2), out, maxc)
+    assert n >= 0
+    return [dict(dist=out[k].dist, pos=np.array(out[k].pos), n=np.array(out[k].frame[:3])) for k in range(n)]
+
+
+def rot(axis, ang):
+    axis = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+
+
+def box_sdf(p, x, R, h):
+    q = np.abs(R.T @ (p - x)) - h
+    return np.linalg.norm(np.maximum(q, 0)) + min(q.max(), 0.0)
+
+
+def test_stacked_faces_closed_form(orc):
+    I = np.eye(3)
+    c = collide(orc, 0, 1, [0, 0, 0], I, [0.02, 0.03, 0.19], I)
+    assert len(c) == 4
+    for k in c:
+        assert k["dist"] == pytest.approx(-0.01, abs=1e-12)
+        assert np.allclose(k["n"], [0, 0, 1]) and k["pos"][2] == pytest.approx(0.095)
+    xy = sorted((round(k["pos"][0], 9), round(k["pos"][1], 9)) for k in c)
+    assert xy == sorted((round(0.02 + sx, 9), round(0.03 + sy, 9)) for sx in (-0.1, 0.1) for sy in (-0.1, 0.1))
+    # the bigger box on top: the contacts are the smaller box's top corners, normal still geom1 -> geom2
+    c = collide(orc, 0, 1, [0.0, 0.0, 0.19], I, [0.01, 0.0, 0.0], I)
+    assert len(c) == 4 and all(np.allclose(k["n"], [0, 0, -1]) for k in c)
+    assert all(k["dist"] == pytest.approx(-0.01) for k in c)
+    assert sorted(round(k["pos"][0], 9) for k in c) == [-0.09, -0.09, 0.11, 0.11]
+    # rotated 45 degrees about the normal: the corners of the rotated face
+    Rz = rot([0, 0, 1], np.pi / 4)
+    c = collide(orc, 0, 1, [0, 0, 0], I, [0, 0, 0.195], Rz)
+    assert len(c) == 4 and all(k["dist"] == pytest.approx(-0.005) for k in c)
+    assert sorted(np.round(np.hypot(k["pos"][0], k["pos"][1]), 9) for k in c) == [round(0.1 * np.sqrt(2), 9)] * 4
+    # separated
+    assert collide(orc, 0, 1, [0, 0, 0], I, [0, 0, 0.21], I) == []
+
+
+def test_crossed_edges_closed_form(orc):
+    # box 1 stands on an edge along y, box 2 on an edge along x above it
+    RA = rot([0, 1, 0], np.pi / 4)
+    RB = rot([1, 0, 0], np.pi / 4)
+    hA = np.array([0.2, 0.2, 0.1])
+    # top edge of A: the vertex direction (x, z) = (+-0.2, 0.1) rotated; pick the highest
+    corners = [RA @ np.array([sx * 0.2, 0, sz * 0.1]) for sx in (-1, 1) for sz in (-1, 1)]
+    top = max(p[2] for p in corners)
+    zB = top + 0.1 * np.sqrt(2) - 0.003
+    c = collide(orc, 0, 1, [0, 0, 0], RA, [0, 0, zB], RB)
+    assert len(c) == 1
+    assert c[0]["dist"] == pytest.approx(-0.003, abs=1e-9)
+    assert np.allclose(c[0]["n"], [0, 0, 1], atol=1e-9)
+    assert c[0]["pos"][2] == pytest.approx(top - 0.0015, abs=1e-9)
+    assert box_sdf(c[0]["pos"], np.zeros(3), RA, hA) < 0
+
+
+def test_capsule_box_closed_form(orc):
+    I = np.eye(3)
+    Ry = rot([0, 1, 0], np.pi / 2)  # capsule axis along x
+    # lying on the top face: one contact at each end
+    c = collide(orc, 2, 0, [0.05, 0, 0.1 + 0.05 - 0.004], Ry, [0, 0, 0], I)
+    assert len(c) == 2 and all(k["dist"] == pytest.approx(-0.004) for k in c)
+    assert sorted(round(k["pos"][0], 9) for k in c) == [-0.05, 0.15]
+    assert all(np.allclose(k["n"], [0, 0, -1]) for k in c)  # capsule (geom1) -> box
+    # standing on the face: the lower end only
+    c = collide(orc, 2, 0, [0.0, 0.0, 0.1 + 0.1 + 0.05 - 0.002], I, [0, 0, 0], I)
+    assert len(c) == 1 and c[0]["dist"] == pytest.approx(-0.002)
+
+
+def test_capsule_box_depth_by_sampling(orc):
+    rng = np.random.default_rng(7)
+    hA = np.array([0.2, 0.2, 0.1])
+    ts = np.linspace(0, 1, 4001)
+    checked = 0
+    for _ in range(200):
+        x = rng.uniform(-0.3, 0.3, 3)
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        w, a, b, cz = q
+        R = np.array([[1 - 2 * (b * b + cz * cz), 2 * (a * b - cz * w), 2 * (a * cz + b * w)],
+                      [2 * (a * b + cz * w), 1 - 2 * (a * a + cz * cz), 2 * (b * cz - a * w)],
+                      [2 * (a * cz - b * w), 2 * (b * cz + a * w), 1 - 2 * (a * a + b * b)]])
+        p0 = x - 0.1 * R[:, 2]
+        d = 0.2 * R[:, 2]
+        sd = np.array([box_sdf(p0 + t * d, np.zeros(3), np.eye(3), hA) for t in ts]) - 0.05
+        c = collide(orc, 2, 0, x, R, [0, 0, 0], np.eye(3))
+        if sd.min() < -1e-3:
+            assert 1 <= len(c) <= 2
+            assert min(k["dist"] for k in c) == pytest.approx(sd.min(), abs=2e-5)
+            checked += 1
+        elif sd.min() > 1e-4:
+            assert c == []
+    assert checked > 20
+
+
+def test_contact_points_lie_on_both_surfaces(orc):
+    rng = np.random.default_rng(11)
+    h1, h2 = np.array([0.2, 0.2, 0.1]), np.array([0.1, 0.1, 0.1])
+    nface = nedge = 0
+    for _ in range(400):
+        R1, R2 = rot(rng.normal(size=3), rng.uniform(0, np.pi)), rot(rng.normal(size=3), rng.uniform(0, np.pi))
+        x2 = rng.uniform(-0.25, 0.25, 3)
+        c = collide(orc, 0, 1, [0, 0, 0], R1, x2, R2)
+        for k in c:
+            assert abs(np.linalg.norm(k["n"]) - 1) < 1e-12
+            p1 = k["pos"] - 0.5 * k["dist"] * k["n"]
+            p2 = k["pos"] + 0.5 * k["dist"] * k["n"]
+            assert abs(box_sdf(p1, np.zeros(3), R1, h1)) < 1e-9
+            assert abs(box_sdf(p2, x2, R2, h2)) < 1e-9
+            assert k["dist"] <= 1e-12  # zero margin: only touching or penetrating points
+        nface += len(c) > 1
+        nedge += len(c) == 1
+    assert nface > 20 and nedge > 5
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_interpreter_matches_oracle_on_box_scene(seed):
+    xml = scenes.boxes_xml(6, seed, solver="Newton")
+    B = 16
+    sim = emu.EmuSim(xml, B, maxcon=64, maxefc=256)
+    fields = ref64.fields_from(sim.field)
+    orc = ref64.Oracle(fields)
+    rng = np.random.default_rng(seed)
+    q = np.tile(sim.field("qpos0"), (B, 1))
+    qv = np.zeros((B, sim.nv))
+    qa = np.zeros((B, sim.nv))
+    lens = rng.integers(20, 120, size=B)  # drop and let the boxes settle for different times
+    for k in range(int(lens.max())):
+        idx = np.nonzero(lens > k)[0]
+        qs, vs, as_ = q[idx].copy(), qv[idx].copy(), qa[idx].copy()
+        orc.batch_step(qs, vs, as_, np.zeros((len(idx), 0)), 4)
+        q[idx], qv[idx], qa[idx] = qs, vs, as_
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    q, qv, qa = f32(q), f32(qv), f32(qa)
+    sim.set_state(q, qv, qa)
+    sim.step(None, nsub=1, dump=True)
+    st = sim.stats()
+    assert not (st["status"] & ~3).any(), st["status"]
+    ncon = worst = 0
+    for e in range(B):
+        states.oracle_step(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32))
+        con = sim.dump(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
+        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4)
+        assert ok, f"env {e}: {msg}"
+        ncon += len(con)
+        b = orc.get("qacc")
+        worst = max(worst, np.max(np.abs(sim.dump(2, e, "qacc") - b)) / max(1.0, np.max(np.abs(b))))
+    print(f"contacts {ncon}, worst qacc rel err {worst:.2e}")
+    assert ncon > B
+    assert worst < 1e-3

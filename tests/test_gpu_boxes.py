@@ -1,0 +1,57 @@
+"""Box scenes on the GPU (the step kernel built with the capsule-box and
+box-box narrow phases) against the FP64 oracle, through the C ABI."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import ref64
+import states
+from mjmlx_hip import capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _settled_states(model, B, seed, nbox):
+    orc = ref64.Oracle(ref64.fields_from(model.field))
+    rng = np.random.default_rng(seed)
+    q = np.tile(model.field("qpos0"), (B, 1))
+    qv = np.zeros((B, model.nv))
+    qa = np.zeros((B, model.nv))
+    lens = rng.integers(20, 120, size=B)
+    for k in range(int(lens.max())):
+        idx = np.nonzero(lens > k)[0]
+        qs, vs, as_ = q[idx].copy(), qv[idx].copy(), qa[idx].copy()
+        orc.batch_step(qs, vs, as_, np.zeros((len(idx), 0)), 4)
+        q[idx], qv[idx], qa[idx] = qs, vs, as_
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    return f32(q), f32(qv), f32(qa), orc
+
+
+@pytest.mark.parametrize("solver", ["Newton", "CG"])
+def test_box_scene_single_step_parity(solver):
+    nbox = 6
+    model = capi.Model(scenes.boxes_xml(nbox, 3, solver=solver, iterations=50))
+    B = 32
+    q, qv, qa, orc = _settled_states(model, B, 4, nbox)
+    sim = capi.BatchedSim(model, B)
+    sim.set_state(q, qv)
+    sim.set_qacc_warmstart(qa)
+    dump = sim.debug_step(None)
+    st = sim.stats()
+    assert not (st["status"] & ~3).any(), st["status"]
+    ncon = 0
+    worst = 0.0
+    for e in range(B):
+        states.oracle_step(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32))
+        con = dump.get(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
+        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4)
+        assert ok, f"env {e}: {msg}"
+        ncon += len(con)
+        b = orc.get("qacc")
+        worst = max(worst, np.max(np.abs(dump.get(2, e, "qacc") - b)) / max(1.0, np.max(np.abs(b))))
+    print(f"{solver}: contacts {ncon}, worst qacc rel err {worst:.2e}, mean iterations {st['niter'].mean():.2f}")
+    assert ncon > B
+    # Newton converges to the 1e-8 scaled residual; CG stops on its improvement
+    # test at an iterate that FP32 and FP64 reach at different iterations
+    assert worst < (1e-3 if solver == "Newton" else 5e-3)

@@ -151,3 +151,53 @@ def test_duplicate_envs_bitwise(model):
     for k in range(1, B // 8):
         assert np.array_equal(qf[:8], qf[8 * k: 8 * k + 8])
         assert np.array_equal(vf[:8], vf[8 * k: 8 * k + 8])
+
+
+def test_cfrc_ext_from_contact_forces(model):
+    """cfrc_ext after a step: each contact's normal force (sum of its rows'
+    forces) along the contact normal, +body2 / -body1, zero torque."""
+    import ref64
+    B = 64
+    q, qv, qa, ctrl, _ = states.quadruped_states(ref64.fields_from(model.field), B, seed=17)
+    sim = capi.BatchedSim(model, B)
+    sim.set_state(q, qv)
+    sim.set_qacc_warmstart(qa)
+    sim.xpos()  # turns the per-body outputs on
+    dump = sim.debug_step(ctrl)
+    st = sim.stats()
+    got = sim._getter("mjmlx_batched_get_cfrc_ext", (B, model.info.nbody, 6))
+    b1, b2 = model.field("cpair_b1").astype(int), model.field("cpair_b2").astype(int)
+    nonzero = 0
+    for e in range(B):
+        want = np.zeros((model.info.nbody, 6))
+        con = dump.get(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
+        force = dump.get(2, e, "efc_force")
+        for c in con:
+            p, nrow, r0 = (int(v) for v in c[13:16].view(np.int32))
+            fn = force[r0:r0 + nrow].sum() if r0 + nrow <= st["nefc"][e] else 0.0
+            want[b2[p], 3:] += fn * c[3:6]
+            want[b1[p], 3:] -= fn * c[3:6]
+        nonzero += int(np.abs(want).sum() > 0)
+        assert np.allclose(got[e], want, rtol=1e-5, atol=1e-4 * max(1.0, np.abs(want).max())), e
+    assert nonzero > B // 2
+
+
+def test_cg_solver_parity():
+    """The CG solver (MuJoCo's preconditioned nonlinear CG) against the oracle's."""
+    import ref64
+    m = capi.Model(scenes.quadruped_xml(self_collision=True, solver="CG", iterations=50))
+    B = 64
+    q, qv, qa, ctrl, orc = states.quadruped_states(ref64.fields_from(m.field), B, seed=23)
+    sim = capi.BatchedSim(m, B)
+    sim.set_state(q, qv)
+    sim.set_qacc_warmstart(qa)
+    dump = sim.debug_step(ctrl)
+    st = sim.stats()
+    assert (st["status"] & ~3 == 0).all(), st["status"]
+    worst = 0.0
+    for e in range(B):
+        states.oracle_step(orc, q[e], qv[e], qa[e], ctrl[e])
+        b = orc.get("qacc")
+        worst = max(worst, np.max(np.abs(dump.get(2, e, "qacc") - b)) / max(1.0, np.max(np.abs(b))))
+    print(f"CG worst qacc rel err {worst:.2e}, mean iterations {st['niter'].mean():.2f}")
+    assert worst < 5e-3  # CG's improvement test stops FP32 and FP64 at different iterates
