@@ -144,7 +144,7 @@ def run(args):
     # roofline of the fused step kernel (plain path: one k_step launch per control step)
     achieved = B * bpe / (plain["ms_per_step"] / 1e3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_quadruped{B}.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_quadruped{B}.txt")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")

@@ -112,6 +112,11 @@ MJMLX_API float mjmlx_mlp_fit(MjmlxMLP* mlp, int epochs, int batch, float lr, fl
 
 /* width of the warm-start features of this sim's model */
 MJMLX_API int mjmlx_batched_feature_dim(const MjmlxBatchedSim* sim);
+/* feature mode 1 appends the previous substep's converged qacc (nv values) to
+ * the features and makes the network residual: it is trained on, and
+ * predicts, the change of qacc from that previous value. Mode 0 (default) is
+ * the plain feature set. Changing the mode removes the installed network. */
+MJMLX_API int mjmlx_batched_set_feature_mode(MjmlxBatchedSim* sim, int mode);
 /* installs the network used by warm-start mode 2 (NULL removes it and leaves mode 2);
  * returns -1 when its widths do not match the model */
 MJMLX_API int mjmlx_batched_set_warmstart_mlp(MjmlxBatchedSim* sim, MjmlxMLP* mlp);

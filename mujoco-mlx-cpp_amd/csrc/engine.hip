@@ -138,6 +138,8 @@ struct MjmlxBatchedSim {
   float* d_pred = nullptr;
   int feat_cur = 0;
   bool feat_valid = false;
+  int feat_mode = 0;  // 1: features end with the previous substep's qacc and the network predicts the change
+  int nfeat() const { return dm.nfeat + (feat_mode ? nv : 0); }
   // scratch host buffers for dumps
   float *d_dump1 = nullptr, *d_dump2 = nullptr;
   size_t dump_cap = 0;
@@ -221,7 +223,7 @@ void launch(MjmlxBatchedSim* s, int nsub, float* dump1, float* dump2) {
 
 void ensure_learned(MjmlxBatchedSim* s) {
   if (s->d_pred) return;
-  size_t nf = (size_t)s->B * s->dm.nfeat;
+  size_t nf = (size_t)s->B * (s->dm.nfeat + s->nv);
   for (int i = 0; i < 2; i++) {
     HIPOK(hipMalloc(&s->d_feat[i], nf * sizeof(float)));
     HIPOK(hipMemsetAsync(s->d_feat[i], 0, nf * sizeof(float), s->stream));
@@ -243,15 +245,19 @@ long step_substeps(MjmlxBatchedSim* s, int n, MjmlxMLP* rec) {
   for (int k = 0; k < n; k++) {
     mjk::StepIO io = make_io(s, nullptr, nullptr);
     io.accumulate = k > 0;
+    int nf = s->nfeat();
+    const float* base = s->feat_mode ? s->d_feat[s->feat_cur] + s->dm.nfeat : nullptr;
     if (s->ws_mode == 2 && s->mlp && s->feat_valid) {
       mjmlp::pack_features(s->mlp, s->d_feat[s->feat_cur], s->d_ctrl, coff, s->nu, s->B, s->stream);
-      mjmlp::predict(s->mlp, s->B, s->d_pred, s->stream);
+      mjmlp::predict(s->mlp, s->B, s->d_pred, s->stream, base, nf);
       io.qacc_pred = s->d_pred;
     }
     io.feat = s->d_feat[s->feat_cur ^ 1];
+    io.feat_prev_qacc = s->feat_mode;
     launch_step(s, io, 1);
     HIPOK(hipGetLastError());
-    if (rec && s->feat_valid) rows += mjmlp::record(rec, s->d_feat[s->feat_cur], s->d_ctrl, coff, s->nu, s->d_qws, s->B, s->stream);
+    if (rec && s->feat_valid)
+      rows += mjmlp::record(rec, s->d_feat[s->feat_cur], s->d_ctrl, coff, s->nu, s->d_qws, s->B, s->stream, base, nf);
     s->feat_cur ^= 1;
     s->feat_valid = true;
   }
@@ -811,7 +817,20 @@ MJMLX_API int mjmlx_batched_profile_step(MjmlxBatchedSim* s, const float* ctrl, 
 #endif
 }
 
-MJMLX_API int mjmlx_batched_feature_dim(const MjmlxBatchedSim* s) { return s ? s->dm.nfeat : -1; }
+MJMLX_API int mjmlx_batched_feature_dim(const MjmlxBatchedSim* s) { return s ? s->nfeat() : -1; }
+
+MJMLX_API int mjmlx_batched_set_feature_mode(MjmlxBatchedSim* s, int mode) {
+  if (!s || mode < 0 || mode > 1) return -1;
+  if (mode != s->feat_mode) {
+    s->feat_mode = mode;
+    s->feat_valid = false;
+    if (s->mlp) {
+      s->mlp = nullptr;  // its input width no longer matches
+      if (s->ws_mode == 2) s->ws_mode = 0;
+    }
+  }
+  return 0;
+}
 
 MJMLX_API int mjmlx_batched_set_warmstart_mlp(MjmlxBatchedSim* s, MjmlxMLP* mlp) {
   if (!s) return -1;
@@ -822,9 +841,9 @@ MJMLX_API int mjmlx_batched_set_warmstart_mlp(MjmlxBatchedSim* s, MjmlxMLP* mlp)
   }
   int nin = 0, nout = 0;
   mjmlx_mlp_dims(mlp, &nin, nullptr, &nout, nullptr);
-  if (nin != s->dm.nfeat || nout != s->nv) {
+  if (nin != s->nfeat() || nout != s->nv) {
     std::fprintf(stderr, "mjmlx_batched_set_warmstart_mlp error: network is %d -> %d, the model needs %d -> %d\n", nin,
-                 nout, s->dm.nfeat, s->nv);
+                 nout, s->nfeat(), s->nv);
     return -1;
   }
   try {
@@ -838,7 +857,7 @@ MJMLX_API long mjmlx_batched_record(MjmlxBatchedSim* s, MjmlxMLP* mlp, const flo
   if (!s || !mlp || nsub <= 0) return -1;
   int nin = 0, nout = 0;
   mjmlx_mlp_dims(mlp, &nin, nullptr, &nout, nullptr);
-  if (nin != s->dm.nfeat || nout != s->nv) return -1;
+  if (nin != s->nfeat() || nout != s->nv) return -1;
   try {
     upload_ctrl(s, ctrl);
     return step_substeps(s, nsub, mlp);
@@ -849,10 +868,10 @@ MJMLX_API int mjmlx_batched_get_features(const MjmlxBatchedSim* cs, float* out) 
   auto* s = const_cast<MjmlxBatchedSim*>(cs);
   if (!s || !out || !s->d_pred || !s->feat_valid) return -1;
   try {
-    HIPOK(hipMemcpyAsync(out, s->d_feat[s->feat_cur], (size_t)s->B * s->dm.nfeat * sizeof(float), hipMemcpyDeviceToHost,
+    HIPOK(hipMemcpyAsync(out, s->d_feat[s->feat_cur], (size_t)s->B * s->nfeat() * sizeof(float), hipMemcpyDeviceToHost,
                          s->stream));
     HIPOK(hipStreamSynchronize(s->stream));
-    return s->dm.nfeat;
+    return s->nfeat();
   } catch (const std::exception& e) { report("mjmlx_batched_get_features", e); return -1; }
 }
 

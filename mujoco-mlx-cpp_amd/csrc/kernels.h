@@ -1683,7 +1683,8 @@ struct StepIO {
   int *niter, *status, *ncon, *nefc;
   int accumulate;            // add niter / OR status into the outputs instead of overwriting them
   const float* qacc_pred;    // learned warm start: predicted qacc [env * nv], or null
-  float* feat;               // learned warm start: raw features after the last substep [env * nfeat], or null
+  float* feat;               // learned warm start: raw features after the last substep [env * nfeat(+nv)], or null
+  int feat_prev_qacc;        // append the substep's converged qacc to the features
   float *dump1, *dump2;  // debug: workspace after constraint build / after solve (last substep)
   AuxOut aux;
   int has_aux;
@@ -1747,14 +1748,16 @@ KDEV void write_aux(const DModel& m, const Layout& L, const float* w, const AuxO
 // Learned warm-start input of one env after a substep: qpos, qvel, ctrl
 // (written as zeros here: the caller inserts the ctrl of the substep the
 // prediction is for), then the first FEAT_NCON contacts as (normal, depth =
-// -dist, one-hot of the limb class of either geom), zero-padded.
-KDEV void write_features(const DModel& m, const Layout& L, const float* w, float* feat, int env) {
-  int nq = m.nq, nv = m.nv, nu = m.nu, n0 = nq + nv + nu, nf = m.nfeat;
+// -dist, one-hot of the limb class of either geom), zero-padded; with `prev`
+// the substep's converged qacc follows (residual networks).
+KDEV void write_features(const DModel& m, const Layout& L, const float* w, float* feat, int env, int prev) {
+  int nq = m.nq, nv = m.nv, nu = m.nu, n0 = nq + nv + nu, nc = m.nfeat, nf = nc + (prev ? nv : 0);
   int ncon = reinterpret_cast<const int*>(w + L.misc)[MISC_NCON];
   for (int i = LANE; i < nf; i += WAVE) {
     float x = 0.f;
     if (i < nq) x = w[L.qpos + i];
     else if (i < nq + nv) x = w[L.qvel + i - nq];
+    else if (i >= nc) x = w[L.qacc_ws + i - nc];
     else if (i >= n0) {
       int j = i - n0, c = j / FEAT_PER_CON, k = j - c * FEAT_PER_CON;
       if (c < ncon) {
@@ -1845,7 +1848,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     io.qacc_ws[(size_t)env * nv + i] = w[L.qacc_ws + i];
   }
   if (wori(bad)) st |= ST_NONFINITE;
-  if (io.feat) write_features(m, L, w, io.feat, env);
+  if (io.feat) write_features(m, L, w, io.feat, env, io.feat_prev_qacc);
   KPROF_MARK(&pa, KP_IO);
   if (LANE == 0) {
     const int* misc = reinterpret_cast<const int*>(w + L.misc);

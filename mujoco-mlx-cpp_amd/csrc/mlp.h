@@ -47,11 +47,13 @@ namespace mjmlp {
 // [rows][nf] whose columns [ctrl_off, ctrl_off + nu) are replaced by ctrl[r]
 // (ctrl may be null: zeros)
 void pack_features(MjmlxMLP* n, const float* raw, const float* ctrl, int ctrl_off, int nu, int rows, hipStream_t s);
-// pred[r][0..nout) = de-normalised network output for the packed rows in xb
-void predict(MjmlxMLP* n, int rows, float* pred, hipStream_t s);
-// appends rows (raw features with ctrl inserted, raw targets) to the training set;
-// returns the number appended (limited by the remaining capacity)
+// pred[r][0..nout) = de-normalised network output for the packed rows in xb,
+// plus base[r * base_ld + c] when base is set (residual networks)
+void predict(MjmlxMLP* n, int rows, float* pred, hipStream_t s, const float* base = nullptr, int base_ld = 0);
+// appends rows (raw features with ctrl inserted, targets minus base[r * base_ld
+// + c] when base is set) to the training set; returns the number appended
+// (limited by the remaining capacity)
 long record(MjmlxMLP* n, const float* raw, const float* ctrl, int ctrl_off, int nu, const float* target, int rows,
-            hipStream_t s);
+            hipStream_t s, const float* base = nullptr, int base_ld = 0);
 
 }  // namespace mjmlp

@@ -60,6 +60,8 @@ struct Epi {
   float* Yf;                    // activation fp32 [M][ldc] or null
   int ldc, nstore;              // row stride of the outputs; columns c < nstore are written
   const float *oscale, *oshift; // Yf = y * oscale[c] + oshift[c] when oscale is set
+  const float* base;            // ... + base[r * ldbase + c] when set
+  int ldbase;
 };
 
 // Stages rows [r0, r0 + 64) x k [k0, k0 + 32) of an operand into S[row][k].
@@ -117,7 +119,11 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const uint16_
         if (ep.Z) ep.Z[o] = z;
         float y = ep.act ? silu(z) : z;
         if (ep.Y) ep.Y[o] = f2bf(y);
-        if (ep.Yf) ep.Yf[o] = ep.oscale ? y * ep.oscale[c] + ep.oshift[c] : y;
+        if (ep.Yf) {
+          float yo = ep.oscale ? y * ep.oscale[c] + ep.oshift[c] : y;
+          if (ep.base) yo += ep.base[(size_t)r * ep.ldbase + c];
+          ep.Yf[o] = yo;
+        }
       }
   }
 }
@@ -222,8 +228,8 @@ __global__ void k_gather(int rows, int nf, int ip, int nout, int op, const int* 
 
 // appends raw rows (ctrl columns inserted) and targets to the data set
 __global__ void k_record(int rows, int nf, const float* __restrict__ raw, const float* __restrict__ ctrl, int ctrl_off,
-                         int nu, const float* __restrict__ target, int nout, float* __restrict__ dsx,
-                         float* __restrict__ dsy) {
+                         int nu, const float* __restrict__ target, int nout, const float* __restrict__ base,
+                         int base_ld, float* __restrict__ dsx, float* __restrict__ dsy) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t nx = (size_t)rows * nf;
   if (i < nx) {
@@ -232,7 +238,10 @@ __global__ void k_record(int rows, int nf, const float* __restrict__ raw, const 
     if (c >= ctrl_off && c < ctrl_off + nu) x = ctrl ? ctrl[(size_t)r * nu + c - ctrl_off] : 0.f;
     dsx[i] = x;
   }
-  if (i < (size_t)rows * nout) dsy[i] = target[i];
+  if (i < (size_t)rows * nout) {
+    int r = (int)(i / nout), c = (int)(i % nout);
+    dsy[i] = target[i] - (base ? base[(size_t)r * base_ld + c] : 0.f);
+  }
 }
 
 // per-chunk column statistics about a shift (the chunk's first row):
@@ -405,21 +414,23 @@ void pack_features(MjmlxMLP* n, const float* raw, const float* ctrl, int ctrl_of
   HIPOK(hipGetLastError());
 }
 
-void predict(MjmlxMLP* n, int rows, float* pred, hipStream_t s) {
+void predict(MjmlxMLP* n, int rows, float* pred, hipStream_t s, const float* base, int base_ld) {
   hidden(n, n->xb, rows, s, false);
   Epi e = epi(n->bias(2), 0, nullptr, nullptr, pred, n->nout, n->nout);
   e.oscale = n->out_std;
   e.oshift = n->out_mean;
+  e.base = base;
+  e.ldbase = base_ld;
   gemm(s, false, false, rows, n->op, n->nh, n->h2b, n->nh, n->Wb(2), n->op, e);
 }
 
 long record(MjmlxMLP* n, const float* raw, const float* ctrl, int ctrl_off, int nu, const float* target, int rows,
-            hipStream_t s) {
+            hipStream_t s, const float* base, int base_ld) {
   long k = std::min<long>(rows, n->ds_cap - n->ds_n);
   if (k <= 0) return 0;
   size_t tot = (size_t)k * std::max(n->nin, n->nout);
   hipLaunchKernelGGL(k_record, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (int)k, n->nin, raw, ctrl,
-                     ctrl_off, nu, target, n->nout, n->ds_x + (size_t)n->ds_n * n->nin,
+                     ctrl_off, nu, target, n->nout, base, base_ld, n->ds_x + (size_t)n->ds_n * n->nin,
                      n->ds_y + (size_t)n->ds_n * n->nout);
   HIPOK(hipGetLastError());
   n->ds_n += k;

@@ -104,6 +104,7 @@ def lib() -> C.CDLL:
             "mjmlx_mlp_fit_normalization": (C.c_int, [C.c_void_p]),
             "mjmlx_mlp_fit": (C.c_float, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int]),
             "mjmlx_batched_feature_dim": (C.c_int, [C.c_void_p]),
+            "mjmlx_batched_set_feature_mode": (C.c_int, [C.c_void_p, C.c_int]),
             "mjmlx_batched_set_warmstart_mlp": (C.c_int, [C.c_void_p, C.c_void_p]),
             "mjmlx_batched_record": (C.c_long, [C.c_void_p, C.c_void_p, _F, C.c_int]),
             "mjmlx_batched_get_features": (C.c_int, [C.c_void_p, _F]),
@@ -246,6 +247,9 @@ class BatchedSim:
 
     def feature_dim(self) -> int:
         return lib().mjmlx_batched_feature_dim(self.h)
+
+    def set_feature_mode(self, mode: int):
+        assert lib().mjmlx_batched_set_feature_mode(self.h, mode) == 0
 
     def set_warmstart_mlp(self, mlp):
         self._mlp = mlp  # the sim borrows the network: keep it alive
