@@ -71,7 +71,7 @@ struct DModel {
 // Newton Hessian overlays it.
 struct Layout {
   int maxcon, maxefc, nv, total;
-  int qpos, qvel, ctrl, qacc_ws, qacc_pred, qacc, qM, qLD, qfrc_smooth, qacc_smooth, qfrc_constraint, act_force, red, misc;
+  int qpos, qvel, ctrl, qacc_ws, qacc_pred, qacc, qM, qLD, qfrc_smooth, qacc_smooth, qfrc_constraint, act_force, misc;
   int xpos, xquat, xmat, xipos, xanchor, xaxis, gxpos, gxmat, scom, cinert, cdof, cvel, cdofdot, crb, cacc, cfrc, buf6;
   int H;
   int con, efc_J, efc_D, efc_R, efc_aref, efc_Jaref, efc_Jv, efc_force, efc_floss, efc_pos, efc_margin,
@@ -91,7 +91,12 @@ enum { MISC_NCON = 0, MISC_NEFC = 1, MISC_STATUS = 2, MISC_NITER = 3, MISC_NFLOS
 
 MJK_HD inline int al2(int x) { return (x + 1) & ~1; }
 
-MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, int ngeom, int maxcon, int maxefc) {
+// qld_shared: the solver never reads the LDL factor of qM (Newton), so the
+// factor lives in the efc Jacobian's space: it is dead once qacc_smooth is
+// solved, before the Jacobian is built, and the integrator reuses the space
+// as scratch after the solve.
+MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, int ngeom, int maxcon, int maxefc,
+                                 bool qld_shared = false) {
   Layout L;
   L.maxcon = maxcon; L.maxefc = maxefc; L.nv = nv;
   int nvp = nv > 0 ? nv : 1, nup = nu > 0 ? nu : 1, njp = njnt > 0 ? njnt : 1, ngp = ngeom > 0 ? ngeom : 1;
@@ -103,12 +108,11 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.qacc_pred = o; o += al2(nvp);
   L.qacc = o; o += al2(nvp);
   L.qM = o; o += al2(nvp * nvp);
-  L.qLD = o; o += al2(nvp * nvp);
+  if (!qld_shared || maxefc < nvp) { L.qLD = o; o += al2(nvp * nvp); } else { L.qLD = -1; }
   L.qfrc_smooth = o; o += al2(nvp);
   L.qacc_smooth = o; o += al2(nvp);
   L.qfrc_constraint = o; o += al2(nvp);
   L.act_force = o; o += al2(nup);
-  L.red = o; o += 64;
   L.misc = o; o += MISC_SIZE;
   // region A
   int a0 = o;
@@ -145,12 +149,14 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.efc_Jv = o; o += maxefc;
   L.efc_force = o; o += maxefc;
   L.efc_floss = o; o += maxefc;
-  L.efc_pos = o; o += maxefc;
-  L.efc_margin = o; o += maxefc;
   L.efc_type = o; o += maxefc;
   L.efc_act = o; o += maxefc;
   L.efc_id = o; o += maxefc;
   L.efc_Dact = o; o += maxefc;
+  // contact distance and margin are read only while the rows are built; the
+  // solver-only vectors Jv and Dact take their place afterwards
+  L.efc_pos = L.efc_Jv;
+  L.efc_margin = L.efc_Dact;
   L.Ma = o; o += nvp;
   L.grad = o; o += nvp;
   L.Mgrad = o; o += nvp;
@@ -161,6 +167,7 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.tmpv = o; o += nvp;
   o = al2(o);
   L.dvec = o; o += 2 * nvp;
+  if (L.qLD < 0) L.qLD = al2(L.efc_J);
   L.total = al2(o);
   return L;
 }

@@ -47,7 +47,7 @@ __attribute__((visibility("default"))) void* mjemu_create(const char* xml, int s
     if (maxcon < 1) maxcon = 1;
     if (maxefc < 1) maxefc = 1;
     auto& m = s->model;
-    s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, maxcon, maxefc);
+    s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, maxcon, maxefc, m.solver == mjh::SOL_NEWTON);
     s->fields = mjh::layout_fields(m, s->L);
     s->B = B;
     s->qpos.resize((size_t)B * m.nq);

@@ -462,11 +462,14 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
       if ((t1 == mjk::GCAPSULE && t2 == mjk::GBOX) || (t1 == mjk::GBOX && t2 == mjk::GBOX)) s->npset = 1;
     }
     s->B = config->num_envs; s->nq = m.nq; s->nv = m.nv; s->nu = m.nu; s->nb = m.nbody;
-    int maxcon = std::max(1, std::min(mjh::max_contacts(m, pairs), 32));
-    int maxefc = std::max(1, std::min(mjh::max_rows(m, pairs), 96));
+    // rows per env: 72 for nv <= 32 and 256 above, the reference's caps
+    // (reference src/compat/mkx_kernels/solver.hpp:28-33); contacts 32 / 64
+    bool small = m.nv <= 32;
+    int maxcon = std::max(1, std::min(mjh::max_contacts(m, pairs), small ? 32 : 64));
+    int maxefc = std::max(1, std::min(mjh::max_rows(m, pairs), small ? 72 : 256));
     if (const char* v = std::getenv("MJMLX_HIP_MAXCON")) maxcon = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("MJMLX_HIP_MAXEFC")) maxefc = std::max(1, std::atoi(v));
-    s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, maxcon, maxefc);
+    s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, maxcon, maxefc, m.solver == mjh::SOL_NEWTON);
     s->fields = mjh::layout_fields(m, s->L);
     s->lds_bytes = (size_t)s->L.total * sizeof(float);
     int dev = 0;
