@@ -83,17 +83,38 @@ def timed_run(sim, pool, warmup, steps, offset=0):
     return float(np.mean(kernel_ms)), dt, float(np.mean(iters))
 
 
-def run(args):
+SCENES = {
+    "quadruped": "quadruped nq19 nv18 nu12, self-collision, condim 3 pyramidal, Newton 16 it, tol 1e-8, "
+                 "dt 0.005 x 4 substeps",
+    "humanoid": "humanoid nq28 nv27 nu21 (free joint + 21 hinges), self-collision, condim 6 elliptic, Newton 16 it, "
+                "tol 1e-8, dt 0.005 x 4 substeps, motor ctrl U(+-0.4)",
+}
+
+
+def scene_setup(name, B, rng):
+    """model, action pool and initial state of a bench scene"""
     from mjmlx_hip import capi, scenes
+    if name == "quadruped":
+        model = capi.Model(scenes.quadruped_xml(self_collision=True))
+        pool = [scenes.quadruped_actions(B, rng).astype(np.float32) for _ in range(16)]
+        q0 = scenes.quadruped_initial_qpos(B, 0).astype(np.float32)
+    else:
+        model = capi.Model(scenes.humanoid_xml())
+        pool = [rng.uniform(-0.4, 0.4, (B, model.nu)).astype(np.float32) for _ in range(16)]
+        q0 = np.tile(model.field("qpos0"), (B, 1)).astype(np.float32)
+    return model, pool, q0
+
+
+def run(args):
+    from mjmlx_hip import capi
 
     if capi.lib().mjmlx_hip_device_count() < 1:
         raise SystemExit("bench: no HIP device visible")
     B = args.envs
-    model = capi.Model(scenes.quadruped_xml(self_collision=True))
     rng = np.random.default_rng(1)
-    pool = [scenes.quadruped_actions(B, rng).astype(np.float32) for _ in range(16)]
+    model, pool, q0 = scene_setup(args.scene, B, rng)
     src = capi.BatchedSim(model, B)
-    src.set_state(scenes.quadruped_initial_qpos(B, 0).astype(np.float32), np.zeros((B, model.nv), np.float32))
+    src.set_state(q0, np.zeros((B, model.nv), np.float32))
     for k in range(args.warmup):
         src.step(pool[k % len(pool)], 4)
     learned = None
@@ -144,7 +165,7 @@ def run(args):
     # roofline of the fused step kernel (plain path: one k_step launch per control step)
     achieved = B * bpe / (plain["ms_per_step"] / 1e3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_quadruped{B}.txt")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.scene}{B}.txt")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -161,8 +182,7 @@ def run(args):
         "vs_baseline": None,
         "dtype": "f32 (FP64 Hessian Cholesky and free-joint quaternion integration)",
         "data": "synthetic",
-        "config": {"workload": "quadruped nq19 nv18 nu12, self-collision, condim 3 pyramidal, Newton 16 it, "
-                               "tol 1e-8, dt 0.005 x 4 substeps",
+        "config": {"workload": SCENES[args.scene],
                    "num_envs": B, "substeps_per_env_step": 4, "parallelism": "single GPU, 1 wavefront per env",
                    "warm_start": best, "launch": "one workgroup per env"},
         "plain": plain,
@@ -173,7 +193,7 @@ def run(args):
                      "bytes_per_env_step": bpe},
         "cpu_baseline": None,
     }
-    if not args.no_cpu:
+    if not args.no_cpu and args.scene == "quadruped":
         import ref64
         out["cpu_baseline"] = cpu_baseline(ref64.fields_from(model.field))
     print(json.dumps(out), flush=True)
@@ -186,6 +206,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=25)
     ap.add_argument("--envs", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scene", choices=sorted(SCENES), default="quadruped",
+                    help="quadruped is the headline workload; humanoid is the high-DoF elliptic-cone one")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="training pairs for the learned warm start")
     ap.add_argument("--epochs", type=int, default=8)

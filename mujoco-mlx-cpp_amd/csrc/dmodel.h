@@ -35,6 +35,7 @@ enum { MAX_DOF_DEPTH = 64 };
 struct DModel {
   int nq, nv, nu, nbody, njnt, ngeom, npair;
   int nlevel, ndoflevel, maxdepth, nfloss, nlim;
+  int nflev;  // levels of the level-parallel LDL of qM (deepest dof level first)
   int iterations, ls_iterations, solver, cone, integrator, disableflags;
   int has_damping;
   float timestep, gravity[3], tolerance, ls_tolerance, impratio, meaninertia;
@@ -52,6 +53,9 @@ struct DModel {
   // dofs
   const int *dof_bodyid, *dof_jntid, *dof_parentid, *dof_depth, *dof_subtree_end, *doflevel_adr,
       *doflevel_dof, *dof_chain, *dof_nchain, *dof_act_adr, *dof_act_num, *dof_act_list;
+  // level-parallel LDL of qM: level l updates entries [flev_adr[l], flev_adr[l+1]),
+  // entry e is (flev_i[e], flev_j[e]) and sums over dofs flev_k[flev_kadr[e] .. + flev_kcnt[e])
+  const int *flev_adr, *flev_i, *flev_j, *flev_kadr, *flev_kcnt, *flev_k;
   const float *dof_armature, *dof_damping, *dof_frictionloss, *dof_invweight0, *dof_solref, *dof_solimp;
   // geoms
   const int *geom_type, *geom_bodyid, *geom_fclass;

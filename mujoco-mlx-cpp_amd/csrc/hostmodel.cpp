@@ -79,7 +79,6 @@ int max_rows(const Model& m, const std::vector<CollisionPair>& pairs) {
 Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   using namespace mjk;
   if (m.nv > 64) throw std::runtime_error("HIP engine: models with nv > 64 are not supported by the dense-row kernel");
-  if (m.cone != CONE_PYRAMIDAL) throw std::runtime_error("HIP engine: elliptic friction cones are not implemented yet");
   if (m.solver == SOL_PGS) throw std::runtime_error("HIP engine: the PGS solver is not implemented (use Newton or CG)");
   if (m.integrator == INT_RK4) throw std::runtime_error("HIP engine: the RK4 integrator is not implemented");
   std::vector<CollisionPair> pairs = build_pairs(m);
@@ -141,6 +140,29 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
       if (!desc) throw std::runtime_error("HIP engine: dof subtree is not contiguous");
     }
   int ndl = nv > 0 ? maxdd + 1 : 0;
+  // level-parallel LDL of qM: for each dof level (deepest first, the root
+  // level has no ancestors to update) the entries (i, j), i and j strict
+  // ancestors of a dof k of the level with j in the chain of i, each with the
+  // list of such k
+  std::vector<int> flev_adr, flev_i, flev_j, flev_kadr, flev_kcnt, flev_k;
+  for (int lv = ndl - 1; lv >= 1; lv--) {
+    flev_adr.push_back((int)flev_i.size());
+    std::vector<std::vector<int>> ks((size_t)nv * nv);
+    for (int k = 0; k < nv; k++) {
+      if (ddepth[k] != lv) continue;
+      for (int i = m.dof_parentid[k]; i >= 0; i = m.dof_parentid[i])
+        for (int j = i; j >= 0; j = m.dof_parentid[j]) ks[(size_t)i * nv + j].push_back(k);
+    }
+    for (int i = 0; i < nv; i++)
+      for (int j = 0; j <= i; j++) {
+        auto& v = ks[(size_t)i * nv + j];
+        if (v.empty()) continue;
+        flev_i.push_back(i); flev_j.push_back(j);
+        flev_kadr.push_back((int)flev_k.size()); flev_kcnt.push_back((int)v.size());
+        for (int k : v) flev_k.push_back(k);
+      }
+  }
+  flev_adr.push_back((int)flev_i.size());
   std::vector<int> dl_adr(ndl + 1, 0), dl_dof;
   for (int lv = 0; lv < ndl; lv++) {
     dl_adr[lv] = (int)dl_dof.size();
@@ -197,7 +219,7 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   DModel& d = blob.proto;
   std::memset(&d, 0, sizeof(d));
   d.nq = m.nq; d.nv = nv; d.nu = m.nu; d.nbody = nb; d.njnt = m.njnt; d.ngeom = m.ngeom; d.npair = np;
-  d.nlevel = maxd; d.ndoflevel = ndl; d.maxdepth = maxdd; d.nfloss = (int)floss_dof.size(); d.nlim = (int)lim_jnt.size();
+  d.nlevel = maxd; d.ndoflevel = ndl; d.nflev = (int)flev_adr.size() - 1; d.maxdepth = maxdd; d.nfloss = (int)floss_dof.size(); d.nlim = (int)lim_jnt.size();
   d.iterations = m.iterations; d.ls_iterations = m.ls_iterations; d.solver = m.solver; d.cone = m.cone;
   d.integrator = m.integrator; d.disableflags = m.disableflags;
   d.has_damping = 0;
@@ -224,6 +246,8 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   ADDP(dof_depth, ddepth); ADDP(dof_subtree_end, dsend); ADDP(doflevel_adr, dl_adr); ADDP(doflevel_dof, dl_dof);
   ADDP(dof_chain, chain); ADDP(dof_nchain, nchain); ADDP(dof_act_adr, dof_act_adr); ADDP(dof_act_num, dof_act_num);
   ADDP(dof_act_list, dof_act_list);
+  ADDP(flev_adr, flev_adr); ADDP(flev_i, flev_i); ADDP(flev_j, flev_j); ADDP(flev_kadr, flev_kadr);
+  ADDP(flev_kcnt, flev_kcnt); ADDP(flev_k, flev_k);
   ADDP(dof_armature, tof(m.dof_armature)); ADDP(dof_damping, tof(m.dof_damping)); ADDP(dof_frictionloss, tof(m.dof_frictionloss));
   ADDP(dof_invweight0, tof(m.dof_invweight0)); ADDP(dof_solref, tof(m.dof_solref)); ADDP(dof_solimp, tof(m.dof_solimp));
   ADDP(geom_type, m.geom_type); ADDP(geom_bodyid, m.geom_bodyid); ADDP(geom_fclass, fclass);

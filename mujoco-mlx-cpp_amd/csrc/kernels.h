@@ -147,6 +147,7 @@ KDEV int wscan_excl(int v) {
 }
 
 KDEV int* iptr(float* w, int off) { return reinterpret_cast<int*>(w + off); }
+KDEV const int* iptr(const float* w, int off) { return reinterpret_cast<const int*>(w + off); }
 KDEV double* dptr(float* w, int off) { return reinterpret_cast<double*>(w + off); }
 
 // (row, col) with col <= row of lower-triangular entry t (row-major packing)
@@ -443,18 +444,21 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
   for (int t = LANE; t < nv * nv; t += WAVE) LD[t] = A[t];
   KSYNC();
   int bad = 0;
-  for (int k = nv - 1; k >= 0; k--) {
-    float dk = LD[k * nv + k];
-    if (!(dk > KMINVAL)) { bad = 1; dk = KMINVAL; }
-    const int* ch = m.dof_chain + k * MAX_DOF_DEPTH;
-    int na = m.dof_nchain[k] - 1;  // strict ancestors
-    int np = na * (na + 1) / 2;
-    float inv = 1.0f / dk;
-    for (int t = LANE; t < np; t += WAVE) {
-      int bb, aa;
-      tri_rc(t, &bb, &aa);
-      int i = ch[1 + aa], j = ch[1 + bb];  // j is an ancestor of (or equal to) i
-      LD[i * nv + j] -= LD[k * nv + i] * inv * LD[k * nv + j];
+  // dofs of one tree level are neither ancestors nor descendants of each
+  // other, so their steps run together: each entry (i, j) of the level's list
+  // sums the contributions of the level's dofs that have both i and j as
+  // strict ancestors. Levels go deepest first, so a dof's row is final when read.
+  for (int l = 0; l < m.nflev; l++) {
+    for (int e = m.flev_adr[l] + LANE; e < m.flev_adr[l + 1]; e += WAVE) {
+      int i = m.flev_i[e], j = m.flev_j[e];
+      float acc = 0.f;
+      for (int q = m.flev_kadr[e]; q < m.flev_kadr[e] + m.flev_kcnt[e]; q++) {
+        int k = m.flev_k[q];
+        float dk = LD[k * nv + k];
+        if (!(dk > KMINVAL)) { bad = 1; dk = KMINVAL; }
+        acc += LD[k * nv + i] * LD[k * nv + j] / dk;
+      }
+      LD[i * nv + j] -= acc;
     }
     KSYNC();
   }
@@ -463,10 +467,12 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
     if (i < k) {
       float dk = LD[k * nv + k];
       LD[t] = dk > KMINVAL ? LD[t] / dk : LD[t] / KMINVAL;
+    } else if (i == k && !(LD[t] > KMINVAL)) {
+      bad = 1;
     }
   }
   KSYNC();
-  return bad ? ST_M_NOT_PD : 0;
+  return wori(bad) ? ST_M_NOT_PD : 0;
 }
 
 // x <- (L^T D L)^-1 x with L unit lower (entries outside the dof tree are 0).
@@ -1150,7 +1156,9 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
         int dim = m.pair_condim[p];
         for (int k = 0; k < cnt; k++) {
           etype[r0 + k] = dim == 1 ? EFC_CONTACT_FRICTIONLESS : (m.cone == 0 ? EFC_CONTACT_PYRAMIDAL : EFC_CONTACT_ELLIPTIC);
-          eid[r0 + k] = c; epos[r0 + k] = cr[CON_DIST]; emarg[r0 + k] = m.pair_margin[p] - m.pair_gap[p]; floss[r0 + k] = 0.f;
+          bool fric = m.cone != 0 && dim > 1 && k > 0;  // elliptic friction rows carry no position term
+          eid[r0 + k] = c; epos[r0 + k] = fric ? 0.f : cr[CON_DIST];
+          emarg[r0 + k] = fric ? 0.f : m.pair_margin[p] - m.pair_gap[p]; floss[r0 + k] = 0.f;
         }
       }
       nrow += tot;
@@ -1160,7 +1168,6 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
     if (cut < nrow) { nrow = cut; st |= ST_EFC_OVERFLOW; }
     if (nrow > L.maxefc) { nrow = L.maxefc; st |= ST_EFC_OVERFLOW; }
   }
-  if (m.cone != 0) st |= ST_BAD_PAIR;  // elliptic cones are not supported by this kernel version
   KSYNC();
   // Jacobian rows
   for (int t = LANE; t < nrow * nv; t += WAVE) J[t] = 0.f;
@@ -1202,6 +1209,11 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
     float Jn = dot3(fr, jp);
     if (cnt == 1) { J[r0 * nv + k] = Jn; continue; }
     float Jd[5] = {dot3(fr + 3, jp), dot3(fr + 6, jp), dot3(fr, jr), dot3(fr + 3, jr), dot3(fr + 6, jr)};
+    if (m.cone != 0) {  // elliptic: normal, then t1, t2, spin, roll1, roll2
+      J[r0 * nv + k] = Jn;
+      for (int q = 1; q < cnt; q++) J[(r0 + q) * nv + k] = Jd[q - 1];
+      continue;
+    }
     const float* mu = m.pair_friction + 5 * p;
     for (int q = 0; q < cnt / 2; q++) {
       J[(r0 + 2 * q) * nv + k] = Jn + mu[q] * Jd[q];
@@ -1245,6 +1257,20 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
     aref[r] = -bb * v - kk * imp * x;
     float R = fmaxf(dA * (1.f - imp) / imp, KMINVAL);
     if (ty == EFC_CONTACT_PYRAMIDAL) R = fmaxf(2.f * mu0 * mu0 / m.impratio * R, KMINVAL);
+    if (ty == EFC_CONTACT_ELLIPTIC) {
+      // friction rows (MuJoCo mj_makeImpedance): R_1 = R_0 / impratio,
+      // R_k = R_1 mu_0^2 / mu_(k-1)^2, R_0 the normal row's regulariser
+      const float* cr = con + CON_STRIDE * eid[r];
+      int kr = r - iptr(cr, CON_ROW0)[0];
+      if (kr > 0) {
+        int p = iptr(cr, CON_PAIR)[0];
+        float imp0 = k_impedance(si, cr[CON_DIST] - (m.pair_margin[p] - m.pair_gap[p]));
+        float R0 = fmaxf(dA * (1.f - imp0) / imp0, KMINVAL);
+        float R1 = R0 / fmaxf(KMINVAL, m.impratio);
+        const float* f5 = m.pair_friction + 5 * p;
+        R = fmaxf(kr == 1 ? R1 : R1 * f5[0] * f5[0] / (f5[kr - 1] * f5[kr - 1]), KMINVAL);
+      }
+    }
     Rr[r] = R;
     D[r] = 1.f / R;
   }
@@ -1274,6 +1300,66 @@ KDEV void set_qacc(const DModel& m, const Layout& L, float* w, const float* x, i
 
 // constraint forces / active flags / cost at the current Jaref, gradient;
 // returns the total cost (wave-uniform)
+// Elliptic cone of one contact at row values x (rows r0 .. r0+dim-1; f the
+// pair's five friction coefficients, mu = f[0]): with U_0 = mu x_0,
+// U_j = f_(j-1) x_j, N = U_0, T = |U_1..|: zone 0 (top, N >= mu T) no force;
+// zone 1 (bottom, mu N + T <= 0) every row quadratic; zone 2 (middle) cost
+// 1/2 Dm (N - mu T)^2 with Dm = D_0 / (mu^2 (1 + mu^2)) (MuJoCo's primal
+// elliptic cone). Writes forces when force != null; returns the zone.
+KDEV int k_ellip(const float* x, const float* D, const float* f, int dim, float* cost, float* force) {
+  float mu = f[0], T = 0.f;
+  for (int j = 1; j < dim; j++) { float U = f[j - 1] * x[j]; T += U * U; }
+  T = sqrtf(T);
+  float N = mu * x[0];
+  if (N >= mu * T || (T <= 0.f && N >= 0.f)) {
+    *cost = 0.f;
+    if (force) for (int j = 0; j < dim; j++) force[j] = 0.f;
+    return 0;
+  }
+  if (mu * N + T <= 0.f || (T <= 0.f && N < 0.f)) {
+    float c = 0.f;
+    for (int j = 0; j < dim; j++) {
+      c += 0.5f * D[j] * x[j] * x[j];
+      if (force) force[j] = -D[j] * x[j];
+    }
+    *cost = c;
+    return 1;
+  }
+  float Dm = D[0] / (mu * mu * (1.f + mu * mu)), g = N - mu * T;
+  *cost = 0.5f * Dm * g * g;
+  if (force) {
+    float f0 = -Dm * g * mu;
+    force[0] = f0;
+    for (int j = 1; j < dim; j++) force[j] = -f0 * f[j - 1] * f[j - 1] * x[j] / T;
+  }
+  return 2;
+}
+
+// elliptic contact cost along x + alpha v: value, slope, curvature; returns the zone
+KDEV int k_ellip_line(const float* x0, const float* v, const float* D, const float* f, int dim, float alpha, float* c,
+                      float* d1, float* d2) {
+  float x[6];
+  for (int j = 0; j < 6; j++) x[j] = j < dim ? x0[j] + alpha * v[j] : 0.f;
+  int z = k_ellip(x, D, f, dim, c, nullptr);
+  *d1 = 0.f; *d2 = 0.f;
+  if (z == 1) {
+    for (int j = 0; j < dim; j++) { *d1 += D[j] * x[j] * v[j]; *d2 += D[j] * v[j] * v[j]; }
+  } else if (z == 2) {
+    float mu = f[0], T = 0.f, TV = 0.f, VV = 0.f;
+    for (int j = 1; j < dim; j++) {
+      float U = f[j - 1] * x[j], V = f[j - 1] * v[j];
+      T += U * U; TV += U * V; VV += V * V;
+    }
+    T = sqrtf(T);
+    float Tp = TV / T, Tpp = (VV - Tp * Tp) / T;
+    float Dm = D[0] / (mu * mu * (1.f + mu * mu));
+    float g = mu * x[0] - mu * T, gp = mu * v[0] - mu * Tp, gpp = -mu * Tpp;
+    *d1 = Dm * g * gp;
+    *d2 = Dm * (gp * gp + g * gpp);
+  }
+  return z;
+}
+
 KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* nchanged) {
   int nv = m.nv;
   const float *Jaref = w + L.efc_Jaref, *D = w + L.efc_D, *Rr = w + L.efc_R, *floss = w + L.efc_floss;
@@ -1284,6 +1370,7 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
   for (int r = LANE; r < nefc; r += WAVE) {
     float x = Jaref[r], d = D[r];
     int ty = etype[r];
+    if (ty == EFC_CONTACT_ELLIPTIC) continue;  // per contact, below
     int old = act[r];
     if (ty == EFC_EQUALITY) { act[r] = 1; force[r] = -d * x; part += 0.5f * d * x * x; }
     else if (ty == EFC_FRICTION_DOF) {
@@ -1297,6 +1384,26 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
     }
     nch += act[r] != old;
     Dact[r] = act[r] ? d : 0.f;
+  }
+  if (m.cone != 0) {
+    const int* misc = reinterpret_cast<const int*>(w + L.misc);
+    int ncon = misc[MISC_NCON];
+    const float* con = w + L.con;
+    for (int c = LANE; c < ncon; c += WAVE) {
+      const float* cr = con + CON_STRIDE * c;
+      int dim = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
+      if (dim < 2 || r0 + dim > nefc) continue;
+      const float* f5 = m.pair_friction + 5 * reinterpret_cast<const int*>(cr + CON_PAIR)[0];
+      float cst;
+      int z = k_ellip(Jaref + r0, D + r0, f5, dim, &cst, force + r0);
+      part += cst;
+      // the middle zone's Hessian moves with qacc: it always asks for a new factor
+      nch += (act[r0] != z) || z == 2;
+      for (int j = 0; j < dim; j++) {
+        act[r0 + j] = z;
+        Dact[r0 + j] = z == 1 ? D[r0 + j] : 0.f;
+      }
+    }
   }
   const float *Ma = w + L.Ma, *qfs = w + L.qfrc_smooth, *qacc = w + L.qacc, *qas = w + L.qacc_smooth;
   for (int i = LANE; i < nv; i += WAVE) part += 0.5f * (Ma[i] - qfs[i]) * (qacc[i] - qas[i]);
@@ -1313,7 +1420,7 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
 // H = M + J^T diag(D act) J (FP32 accumulate), factored in FP64 as
 // H = L D L^T (right-looking, one LDS phase per column, the column left
 // unscaled); the reciprocal pivots go to dvec.
-KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc) {
+KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KProfAcc* pa) {
   int nv = m.nv;
   double* H = dptr(w, L.H);
   double* idg = dptr(w, L.dvec);
@@ -1324,12 +1431,72 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc) {
     tri_rc(t, &i, &j);
     H[i * nv + j] = (double)(M[i * nv + j] + kdot3(Dact, J + i, nv, J + j, nv, nefc));
   }
+  if (m.cone != 0) {
+    // contacts in the middle zone of the elliptic cone: H += J_c^T Hc J_c with
+    // Hc = Dm (a a^T + g B), a = (mu, -mu f_j u_j), g = N - mu T,
+    // B_jk = -mu f_j f_k (delta_jk - u_j u_k) / T (u = U_t / T); per entry
+    // J_c^T Hc J_c = Dm [mu^2 (J0i - s_i)(J0l - s_l) - g mu / T (q_il - s_i s_l)],
+    // s_i = sum_j f_j^2 x_j J_ji / T, q_il = sum_j f_j^2 J_ji J_jl
+    const int* misc = reinterpret_cast<const int*>(w + L.misc);
+    const int* act = reinterpret_cast<const int*>(w + L.efc_act);
+    const float *Jaref = w + L.efc_Jaref, *D = w + L.efc_D, *con = w + L.con;
+    int ncon = misc[MISC_NCON];
+    for (int c = 0; c < ncon; c++) {
+      const float* cr = con + CON_STRIDE * c;
+      int dim = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
+      if (dim < 2 || r0 + dim > nefc || act[r0] != 2) continue;
+      const float* f5 = m.pair_friction + 5 * reinterpret_cast<const int*>(cr + CON_PAIR)[0];
+      const float* x = Jaref + r0;
+      float mu = f5[0], T = 0.f;
+      for (int j = 1; j < dim; j++) { float U = f5[j - 1] * x[j]; T += U * U; }
+      T = sqrtf(T);
+      float Dm = D[r0] / (mu * mu * (1.f + mu * mu)), g = mu * x[0] - mu * T, cg = g * mu / T;
+      const float* Jc = J + r0 * nv;
+      for (int t = LANE; t < ntri; t += WAVE) {
+        int i, l;
+        tri_rc(t, &i, &l);
+        float si = 0.f, sl = 0.f, q = 0.f;
+        for (int j = 1; j < dim; j++) {
+          float f2 = f5[j - 1] * f5[j - 1], ji = Jc[j * nv + i], jl = Jc[j * nv + l];
+          si += f2 * x[j] * ji; sl += f2 * x[j] * jl; q += f2 * ji * jl;
+        }
+        si /= T; sl /= T;
+        float hv = Dm * (mu * mu * (Jc[i] - si) * (Jc[l] - sl) - cg * (q - si * sl));
+        H[i * nv + l] += (double)hv;
+      }
+    }
+  }
   KSYNC();
+  KPROF_MARK(pa, KP_SPARE);  // profiling builds: the Hessian build is reported apart
   int bad = 0;
-  for (int k = 0; k < nv - 1; k++) {
-    double dkk = H[k * nv + k];
-    if (!(dkk > 0.0)) { bad = 1; dkk = 1e-30; }
-    double ik = 1.0 / dkk;
+  // Two columns per LDS phase. For the pair (k, k+1), with c = H[k+1][k] / d0
+  // and d1 = H[k+1][k+1] - c H[k+1][k], the trailing entries take both rank-1
+  // updates at once, computing column k+1's updated entries on the fly:
+  //   H[i][j] -= H[i][k] H[j][k] / d0 + b_i b_j / d1,  b_x = H[x][k+1] - c H[x][k].
+  // Column k+1 itself (its diagonal included) is updated in one pass at the end.
+  int k = 0;
+  for (; k + 1 < nv - 1; k += 2) {
+    double d0 = H[k * nv + k];
+    if (!(d0 > 0.0)) { bad = 1; d0 = 1e-30; }
+    double h10 = H[(k + 1) * nv + k], c = h10 / d0;
+    double d1 = H[(k + 1) * nv + k + 1] - c * h10;
+    if (!(d1 > 0.0)) { bad = 1; d1 = 1e-30; }
+    double i0 = 1.0 / d0, i1 = 1.0 / d1;
+    int mm = nv - k - 2, nt = mm * (mm + 1) / 2;
+    for (int t = LANE; t < nt; t += WAVE) {
+      int a, b;
+      tri_rc(t, &a, &b);
+      int i = k + 2 + a, j = k + 2 + b;
+      double ai = H[i * nv + k], aj = H[j * nv + k];
+      double bi = H[i * nv + k + 1] - c * ai, bj = H[j * nv + k + 1] - c * aj;
+      H[i * nv + j] -= ai * aj * i0 + bi * bj * i1;
+    }
+    KSYNC();
+  }
+  if (k < nv - 1) {  // one column left with trailing rows
+    double d0 = H[k * nv + k];
+    if (!(d0 > 0.0)) { bad = 1; d0 = 1e-30; }
+    double ik = 1.0 / d0;
     int nt = (nv - k - 1) * (nv - k) / 2;
     for (int t = LANE; t < nt; t += WAVE) {
       int a, b;
@@ -1339,14 +1506,19 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc) {
     }
     KSYNC();
   }
+  // deferred updates of the second column of each pair: H[i][k+1] -= c H[i][k], i >= k+1
+  for (int kk = 0; kk + 1 < nv - 1; kk += 2) {
+    double c = H[(kk + 1) * nv + kk] / H[kk * nv + kk];
+    for (int i = kk + 1 + LANE; i < nv; i += WAVE) H[i * nv + kk + 1] -= c * H[i * nv + kk];
+  }
+  KSYNC();
   for (int i = LANE; i < nv; i += WAVE) {
     double d = H[i * nv + i];
-    if (!(d > 0.0)) { d = 1e-30; }
+    if (!(d > 0.0)) { bad = 1; d = 1e-30; }
     idg[i] = 1.0 / d;
   }
-  int badl = !(H[(nv - 1) * nv + nv - 1] > 0.0);
   KSYNC();
-  return (bad || badl) ? ST_NOT_PD : 0;
+  return wori(bad) ? ST_NOT_PD : 0;
 }
 
 // Mgrad = H^-1 grad with the factor of newton_factor, in FP64; on the GPU the
@@ -1386,31 +1558,83 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
 
 struct LsPt { float alpha, f, d1, d2; int nchange; };
 
+// The rows a lane owns (r = LANE + q WAVE) held in registers for the whole
+// line search, when there are at most two per lane: every evaluation is then
+// register arithmetic plus the wave reductions, with no LDS traffic.
+struct LsRows { float jar[2], jv[2], d[2]; int ty[2]; int on; };
+
+KDEV void ls_rows(const Layout& L, const float* w, int nefc, LsRows* c) {
+  c->on = 0;
+#if WAVE == 64
+  if (nefc <= 2 * WAVE) {
+    c->on = 1;
+    const float *Jaref = w + L.efc_Jaref, *Jv = w + L.efc_Jv, *D = w + L.efc_D;
+    const int* etype = reinterpret_cast<const int*>(w + L.efc_type);
+    for (int q = 0; q < 2; q++) {
+      int r = LANE + q * WAVE;
+      bool in = r < nefc;
+      c->jar[q] = in ? Jaref[r] : 0.f;
+      c->jv[q] = in ? Jv[r] : 0.f;
+      c->d[q] = in ? D[r] : 0.f;
+      c->ty[q] = in ? etype[r] : -1;
+    }
+  }
+#else
+  (void)L; (void)w; (void)nefc;
+#endif
+}
+
+// cost terms of one row at alpha (x = Jaref + alpha Jv); zone change against ref_alpha
+KDEV void ls_row(const Layout& L, const float* w, int r, int ty, float jar, float jv, float d, float alpha,
+                 float ref_alpha, float& f, float& d1, float& d2, int& nch) {
+  if (ty == EFC_CONTACT_ELLIPTIC) return;  // evaluated per contact
+  float x = jar + alpha * jv, xr = jar + ref_alpha * jv;
+  if (ty == EFC_FRICTION_DOF) {
+    float fl = w[L.efc_floss + r], R = w[L.efc_R + r];
+    int z = x <= -R * fl ? 0 : (x >= R * fl ? 2 : 1);
+    int zr = xr <= -R * fl ? 0 : (xr >= R * fl ? 2 : 1);
+    nch += z != zr;
+    if (z == 0) { f += -fl * x - 0.5f * R * fl * fl; d1 += -fl * jv; }
+    else if (z == 2) { f += fl * x - 0.5f * R * fl * fl; d1 += fl * jv; }
+    else { f += 0.5f * d * x * x; d1 += d * x * jv; d2 += d * jv * jv; }
+  } else if (ty == EFC_EQUALITY) {
+    f += 0.5f * d * x * x; d1 += d * x * jv; d2 += d * jv * jv;
+  } else {
+    nch += (x < 0.f) != (xr < 0.f);
+    if (x < 0.f) { f += 0.5f * d * x * x; d1 += d * x * jv; d2 += d * jv * jv; }
+  }
+}
+
 // cost along search at alpha; nchange counts rows whose zone differs from
 // their zone at ref_alpha (0 means one quadratic piece spans both points)
-KDEV LsPt ls_eval(const Layout& L, float* w, int nefc, float alpha, float ref_alpha, const float* gq) {
-  const float *Jaref = w + L.efc_Jaref, *Jv = w + L.efc_Jv, *D = w + L.efc_D, *Rr = w + L.efc_R, *floss = w + L.efc_floss;
-  const int* etype = iptr(w, L.efc_type);
+KDEV LsPt ls_eval(const DModel& m, const Layout& L, float* w, int nefc, float alpha, float ref_alpha, const float* gq,
+                  const LsRows& c) {
   float f = 0.f, d1 = 0.f, d2 = 0.f;
   int nch = 0;
-  for (int r = LANE; r < nefc; r += WAVE) {
-    float jv = Jv[r];
-    float x = Jaref[r] + alpha * jv, xr = Jaref[r] + ref_alpha * jv, d = D[r];
-    int ty = etype[r];
-    if (ty == EFC_FRICTION_DOF) {
-      float fl = floss[r], R = Rr[r];
-      int z = x <= -R * fl ? 0 : (x >= R * fl ? 2 : 1);
-      int zr = xr <= -R * fl ? 0 : (xr >= R * fl ? 2 : 1);
-      nch += z != zr;
-      if (z == 0) { f += -fl * x - 0.5f * R * fl * fl; d1 += -fl * jv; }
-      else if (z == 2) { f += fl * x - 0.5f * R * fl * fl; d1 += fl * jv; }
-      else { f += 0.5f * d * x * x; d1 += d * x * jv; d2 += d * jv * jv; }
-    } else if (ty == EFC_EQUALITY) {
-      f += 0.5f * d * x * x; d1 += d * x * jv; d2 += d * jv * jv;
-    } else {
-      nch += (x < 0.f) != (xr < 0.f);
-      if (x < 0.f) { f += 0.5f * d * x * x; d1 += d * x * jv; d2 += d * jv * jv; }
+  if (m.cone != 0) {
+    // elliptic contacts; the middle zone is not quadratic, so it never counts as one exact piece
+    const int* misc = reinterpret_cast<const int*>(w + L.misc);
+    int ncon = misc[MISC_NCON];
+    const float *Jaref = w + L.efc_Jaref, *Jv = w + L.efc_Jv, *D = w + L.efc_D, *con = w + L.con;
+    for (int q = LANE; q < ncon; q += WAVE) {
+      const float* cr = con + CON_STRIDE * q;
+      int dim = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
+      if (dim < 2 || r0 + dim > nefc) continue;
+      const float* f5 = m.pair_friction + 5 * reinterpret_cast<const int*>(cr + CON_PAIR)[0];
+      float cv, g1, g2, cr_, h1, h2;
+      int z = k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, dim, alpha, &cv, &g1, &g2);
+      int zr = k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, dim, ref_alpha, &cr_, &h1, &h2);
+      nch += (z != zr) || z == 2;
+      f += cv; d1 += g1; d2 += g2;
     }
+  }
+  if (c.on) {
+    for (int q = 0; q < 2; q++)
+      if (c.ty[q] >= 0) ls_row(L, w, LANE + q * WAVE, c.ty[q], c.jar[q], c.jv[q], c.d[q], alpha, ref_alpha, f, d1, d2, nch);
+  } else {
+    const float *Jaref = w + L.efc_Jaref, *Jv = w + L.efc_Jv, *D = w + L.efc_D;
+    const int* etype = iptr(w, L.efc_type);
+    for (int r = LANE; r < nefc; r += WAVE) ls_row(L, w, r, etype[r], Jaref[r], Jv[r], D[r], alpha, ref_alpha, f, d1, d2, nch);
   }
   LsPt p;
   p.alpha = alpha;
@@ -1447,16 +1671,18 @@ KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int*
   }
   float gq[3] = {wsum(g0), wsum(g1), wsum(g2)};
   *exact = 0;
-  LsPt p0 = ls_eval(L, w, nefc, 0.f, 0.f, gq);
+  LsRows rows;
+  ls_rows(L, w, nefc, &rows);
+  LsPt p0 = ls_eval(m, L, w, nefc, 0.f, 0.f, gq, rows);
   if (!(p0.d1 < 0.f)) return 0.f;
   int it = 1, maxit = m.ls_iterations;
   LsPt lo = p0;
-  LsPt pt = ls_eval(L, w, nefc, -p0.d1 / p0.d2, 0.f, gq);
+  LsPt pt = ls_eval(m, L, w, nefc, -p0.d1 / p0.d2, 0.f, gq, rows);
   if (pt.nchange == 0) { *exact = 1; return pt.alpha; }
   if (fabsf(pt.d1) <= gtol) return pt.alpha;
   while (pt.d1 < 0.f && it < maxit) {
     lo = pt;
-    pt = ls_eval(L, w, nefc, pt.alpha - pt.d1 / pt.d2, lo.alpha, gq);
+    pt = ls_eval(m, L, w, nefc, pt.alpha - pt.d1 / pt.d2, lo.alpha, gq, rows);
     it++;
     if (pt.nchange == 0 || fabsf(pt.d1) <= gtol) return pt.alpha;
   }
@@ -1467,7 +1693,7 @@ KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int*
     bool newton = a > lo.alpha && a < hi.alpha;
     if (!newton) a = lo.alpha - lo.d1 * (hi.alpha - lo.alpha) / (hi.d1 - lo.d1);
     if (!(a > lo.alpha && a < hi.alpha)) a = 0.5f * (lo.alpha + hi.alpha);
-    pt = ls_eval(L, w, nefc, a, lo.alpha, gq);
+    pt = ls_eval(m, L, w, nefc, a, lo.alpha, gq, rows);
     it++;
     if ((newton && pt.nchange == 0) || fabsf(pt.d1) <= gtol) return pt.alpha;
     if (pt.d1 < 0.f) lo = pt; else hi = pt;
@@ -1522,7 +1748,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
   const float* Jv = w + L.efc_Jv;
   KPROF_MARK(pa, KP_SOLVE_INIT);
   if (newton) {
-    st |= newton_factor(m, L, w, nefc);
+    st |= newton_factor(m, L, w, nefc, pa);
     newton_solve(m, L, w);
   } else {
     for (int i = LANE; i < nv; i += WAVE) Mg[i] = grad[i];
@@ -1560,7 +1786,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     if (newton && exact) break;
     if (newton) {
       // H depends on qacc only through the active set: refactor only when it changed
-      if (nch) st |= newton_factor(m, L, w, nefc);
+      if (nch) st |= newton_factor(m, L, w, nefc, pa);
       newton_solve(m, L, w);
     } else {
       for (int i = LANE; i < nv; i += WAVE) Mg[i] = grad[i];
@@ -1581,11 +1807,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
   }
   KPROF_MARK(pa, KP_DIR);
   const float *J = w + L.efc_J, *force = w + L.efc_force;
-  for (int i = LANE; i < nv; i += WAVE) {
-    float s = 0.f;
-    for (int r = 0; r < nefc; r++) s += J[r * nv + i] * force[r];
-    qfc[i] = s;
-  }
+  for (int i = LANE; i < nv; i += WAVE) qfc[i] = kdot(J + i, nv, force, 1, nefc);
   KSYNC();
   KPROF_MARK(pa, KP_SOLVE_REST);
   return st;
@@ -1715,7 +1937,9 @@ KDEV void write_cfrc_ext(const DModel& m, const Layout& L, const float* w, const
       if (sgn == 0.f) continue;
       int nrow = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
       float fn = 0.f;
-      for (int r = r0; r < r0 + nrow && r < nefc; r++) fn += force[r];
+      if (m.cone != 0) fn = nrow > 0 && r0 < nefc ? force[r0] : 0.f;  // elliptic: the normal row
+      else
+        for (int r = r0; r < r0 + nrow && r < nefc; r++) fn += force[r];
       for (int k = 0; k < 3; k++) f[k] += sgn * fn * cr[CON_FRAME + k];
     }
     float* o = aux.cfrc_ext + ((size_t)env * nb + b) * 6;

@@ -1211,8 +1211,15 @@ static int make_constraint(const orc_prep_t* p, orc_data_t* d) {
           }
         }
       } else {
-        /* elliptic cones are outside the supported set of this oracle version */
-        status |= 8;
+        /* elliptic: normal row, then t1, t2, spin, roll1, roll2 (dim - 1 of them);
+         * friction rows carry no position term */
+        if (d->nefc + dim > ORC_MAXEFC) { status |= 2; free(big); break; }
+        const double* rows[6] = {Jrow[0], Jrow[1], Jrow[2], Jrow[3], Jrow[4], Jrow[5]};
+        for (int k = 0; k < dim; k++) {
+          int r = add_row(d, nv, EFC_CONTACT_ELLIPTIC, ci, k == 0 ? c->dist : 0.0, k == 0 ? c->includemargin : 0.0, tran, 0);
+          memcpy(d->efc_J + (size_t)r * nv, rows[k], sizeof(double) * nv);
+          memcpy(sref[r], c->solref, 2 * sizeof(double)); memcpy(simp[r], c->solimp, 5 * sizeof(double));
+        }
       }
       free(big);
     }
@@ -1250,8 +1257,97 @@ static int make_constraint(const orc_prep_t* p, orc_data_t* d) {
     for (int k = 0; k < nrow; k++) d->efc_R[r + k] = Rpy;
     r += nrow;
   }
+  /* elliptic contacts (MuJoCo mj_makeImpedance): R_1 = R_0 / impratio for the
+   * first friction row, R_k = R_1 mu_0^2 / mu_(k-1)^2 for the others */
+  for (int r = 0; r < d->nefc;) {
+    if (d->efc_type[r] != EFC_CONTACT_ELLIPTIC) { r++; continue; }
+    const orc_contact_t* c = &d->con[d->efc_id[r]];
+    if (c->dim > 1) {
+      double R1 = d->efc_R[r] / fmax(MINVAL, m->impratio);
+      for (int k = 1; k < c->dim; k++)
+        d->efc_R[r + k] = fmax(k == 1 ? R1 : R1 * c->friction[0] * c->friction[0] / (c->friction[k - 1] * c->friction[k - 1]), MINVAL);
+    }
+    r += c->dim;
+  }
   for (int r = 0; r < d->nefc; r++) d->efc_D[r] = 1.0 / d->efc_R[r];
   return status;
+}
+
+/* Elliptic cone of one contact (rows r .. r+dim-1, row values x): with
+ * U_0 = mu x_0, U_j = f_j x_j (f the contact's friction, mu = f_0),
+ * N = U_0 and T = |U_1..|:
+ *   top (N >= mu T, or T = 0 and N >= 0): no force, no cost;
+ *   bottom (mu N + T <= 0, or T = 0 and N < 0): every row quadratic;
+ *   middle: cost 1/2 Dm (N - mu T)^2, Dm = D_0 / (mu^2 (1 + mu^2))
+ * (MuJoCo's primal elliptic cone). Returns the zone (0 top, 1 bottom, 2
+ * middle) and writes the cost and, if asked, the forces and the dim x dim
+ * Hessian of the cost in the row values. */
+static int ellip_state(const orc_data_t* d, int r, int dim, const double* x, double* cost, double* force, double* Hc) {
+  const orc_contact_t* c = &d->con[d->efc_id[r]];
+  double mu = c->friction[0], U[6], T = 0;
+  U[0] = mu * x[0];
+  for (int j = 1; j < dim; j++) { U[j] = c->friction[j - 1] * x[j]; T += U[j] * U[j]; }
+  T = sqrt(T);
+  double N = U[0];
+  if (Hc) memset(Hc, 0, sizeof(double) * dim * dim);
+  if (N >= mu * T || (T <= 0 && N >= 0)) {
+    *cost = 0;
+    if (force) for (int j = 0; j < dim; j++) force[j] = 0;
+    return 0;
+  }
+  if (mu * N + T <= 0 || (T <= 0 && N < 0)) {
+    *cost = 0;
+    for (int j = 0; j < dim; j++) {
+      double D = d->efc_D[r + j];
+      *cost += 0.5 * D * x[j] * x[j];
+      if (force) force[j] = -D * x[j];
+      if (Hc) Hc[j * dim + j] = D;
+    }
+    return 1;
+  }
+  double Dm = d->efc_D[r] / (mu * mu * (1 + mu * mu)), g = N - mu * T;
+  *cost = 0.5 * Dm * g * g;
+  double a[6];
+  a[0] = mu;
+  for (int j = 1; j < dim; j++) a[j] = -mu * c->friction[j - 1] * U[j] / T;
+  if (force) for (int j = 0; j < dim; j++) force[j] = -Dm * g * a[j];
+  if (Hc) {
+    for (int i = 0; i < dim; i++)
+      for (int j = 0; j < dim; j++) Hc[i * dim + j] = Dm * a[i] * a[j];
+    for (int i = 1; i < dim; i++)
+      for (int j = 1; j < dim; j++) {
+        double B = -mu * c->friction[i - 1] * c->friction[j - 1] * ((i == j ? 1.0 : 0.0) - U[i] * U[j] / (T * T)) / T;
+        Hc[i * dim + j] += Dm * g * B;
+      }
+  }
+  return 2;
+}
+
+/* value, slope and curvature of one elliptic contact's cost along x + alpha v */
+static int ellip_line(const orc_data_t* d, int r, int dim, const double* x0, const double* v, double alpha, double* f,
+                      double* d1, double* d2) {
+  double x[6] = {0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < dim; j++) x[j] = x0[j] + alpha * v[j];
+  double cst;
+  int z = ellip_state(d, r, dim, x, &cst, NULL, NULL);
+  *f = cst; *d1 = 0; *d2 = 0;
+  if (z == 1) {
+    for (int j = 0; j < dim; j++) { double D = d->efc_D[r + j]; *d1 += D * x[j] * v[j]; *d2 += D * v[j] * v[j]; }
+  } else if (z == 2) {
+    const orc_contact_t* c = &d->con[d->efc_id[r]];
+    double mu = c->friction[0], T = 0, TV = 0, VV = 0;
+    for (int j = 1; j < dim; j++) {
+      double U = c->friction[j - 1] * x[j], V = c->friction[j - 1] * v[j];
+      T += U * U; TV += U * V; VV += V * V;
+    }
+    T = sqrt(T);
+    double Tp = TV / T, Tpp = (VV - Tp * Tp) / T;
+    double Dm = d->efc_D[r] / (mu * mu * (1 + mu * mu));
+    double g = mu * x[0] - mu * T, gp = mu * v[0] - mu * Tp, gpp = -mu * Tpp;
+    *d1 = Dm * g * gp;
+    *d2 = Dm * (gp * gp + g * gpp);
+  }
+  return z;
 }
 
 /* ------------------------------------------------------------------ solver */
@@ -1263,7 +1359,14 @@ static double update_constraint(orc_data_t* d) {
   for (int r = 0; r < d->nefc; r++) {
     double x = d->efc_Jaref[r], D = d->efc_D[r];
     int t = d->efc_type[r];
-    if (t == EFC_EQUALITY) {
+    if (t == EFC_CONTACT_ELLIPTIC) {
+      int dim = d->con[d->efc_id[r]].dim;
+      double cst;
+      int z = ellip_state(d, r, dim, d->efc_Jaref + r, &cst, d->efc_force + r, NULL);
+      for (int j = 0; j < dim; j++) d->efc_active[r + j] = z;
+      cost += cst;
+      r += dim - 1;
+    } else if (t == EFC_EQUALITY) {
       d->efc_active[r] = 1; d->efc_force[r] = -D * x; cost += 0.5 * D * x * x;
     } else if (t == EFC_FRICTION_DOF) {
       double f = d->efc_floss[r], R = d->efc_R[r];
@@ -1329,7 +1432,16 @@ static lspt_t ls_eval(const orc_data_t* d, double alpha, double ref, const doubl
     double jv = d->efc_Jv[r];
     double x = d->efc_Jaref[r] + alpha * jv, xr = d->efc_Jaref[r] + ref * jv, D = d->efc_D[r];
     int t = d->efc_type[r];
-    if (t == EFC_FRICTION_DOF) {
+    if (t == EFC_CONTACT_ELLIPTIC) {
+      /* the middle zone is not quadratic: it never counts as one exact piece */
+      int dim = d->con[d->efc_id[r]].dim;
+      double f, g1, g2, fr, h1, h2;
+      int z = ellip_line(d, r, dim, d->efc_Jaref + r, d->efc_Jv + r, alpha, &f, &g1, &g2);
+      int zr = ellip_line(d, r, dim, d->efc_Jaref + r, d->efc_Jv + r, ref, &fr, &h1, &h2);
+      pt.nchange += (z != zr) || z == 2;
+      pt.f += f; pt.d1 += g1; pt.d2 += g2;
+      r += dim - 1;
+    } else if (t == EFC_FRICTION_DOF) {
       double f = d->efc_floss[r], R = d->efc_R[r];
       int z = x <= -R * f ? 0 : (x >= R * f ? 2 : 1), zr = xr <= -R * f ? 0 : (xr >= R * f ? 2 : 1);
       pt.nchange += z != zr;
@@ -1440,6 +1552,22 @@ static int hessian_factor(const orc_prep_t* p, orc_data_t* d) {
   double* H = d->H;
   memcpy(H, d->qM, sizeof(double) * nv * nv);
   for (int r = 0; r < d->nefc; r++) {
+    if (d->efc_type[r] == EFC_CONTACT_ELLIPTIC) {
+      /* H += J_c^T Hc J_c over the contact's rows */
+      int dim = d->con[d->efc_id[r]].dim;
+      double Hc[36], cst;
+      ellip_state(d, r, dim, d->efc_Jaref + r, &cst, NULL, Hc);
+      for (int a = 0; a < dim; a++)
+        for (int b = 0; b < dim; b++) {
+          double h = Hc[a * dim + b];
+          if (h == 0) continue;
+          const double *Ja = d->efc_J + (size_t)(r + a) * nv, *Jb = d->efc_J + (size_t)(r + b) * nv;
+          for (int i = 0; i < nv; i++)
+            for (int j = 0; j <= i; j++) H[i * nv + j] += Ja[i] * h * Jb[j];
+        }
+      r += dim - 1;
+      continue;
+    }
     if (!d->efc_active[r]) continue;
     const double* J = d->efc_J + (size_t)r * nv;
     double D = d->efc_D[r];
@@ -1648,6 +1776,45 @@ int orc_step(const orc_prep_t* p, orc_data_t* d) {
 int orc_ncon(const orc_data_t* d) { return d->ncon; }
 const orc_contact_t* orc_contacts(const orc_data_t* d) { return d->con; }
 int orc_nefc(const orc_data_t* d) { return d->nefc; }
+/* Total cost, gradient and full Hessian (nv x nv) of the constraint problem
+ * built by the last orc_forward, at qacc. The constraint state (forces,
+ * zones in efc_active) is left at qacc. */
+int orc_cost(const orc_prep_t* p, orc_data_t* d, const double* qacc, double* cost, double* grad, double* Hout) {
+  int nv = p->m.nv;
+  set_qacc(p, d, qacc);
+  cost_grad(p, d);
+  if (cost) *cost = d->cost;
+  if (grad) memcpy(grad, d->grad, sizeof(double) * nv);
+  if (Hout) {
+    double* keep = (double*)malloc(sizeof(double) * nv * nv);
+    memcpy(keep, d->H, sizeof(double) * nv * nv);
+    double* H = d->H;
+    memcpy(H, d->qM, sizeof(double) * nv * nv);
+    for (int r = 0; r < d->nefc; r++) {
+      if (d->efc_type[r] == EFC_CONTACT_ELLIPTIC) {
+        int dim = d->con[d->efc_id[r]].dim;
+        double Hc[36], cst;
+        ellip_state(d, r, dim, d->efc_Jaref + r, &cst, NULL, Hc);
+        for (int a = 0; a < dim; a++)
+          for (int b = 0; b < dim; b++)
+            for (int i = 0; i < nv; i++)
+              for (int j = 0; j < nv; j++)
+                H[i * nv + j] += d->efc_J[(size_t)(r + a) * nv + i] * Hc[a * dim + b] * d->efc_J[(size_t)(r + b) * nv + j];
+        r += dim - 1;
+        continue;
+      }
+      if (!d->efc_active[r]) continue;
+      for (int i = 0; i < nv; i++)
+        for (int j = 0; j < nv; j++)
+          H[i * nv + j] += d->efc_D[r] * d->efc_J[(size_t)r * nv + i] * d->efc_J[(size_t)r * nv + j];
+    }
+    memcpy(Hout, H, sizeof(double) * nv * nv);
+    memcpy(d->H, keep, sizeof(double) * nv * nv);
+    free(keep);
+  }
+  return 0;
+}
+
 int orc_solver_niter(const orc_data_t* d) { return d->niter; }
 
 int orc_get(const orc_data_t* d, const char* name, double* out, int cap) {
@@ -1674,6 +1841,10 @@ int orc_get(const orc_data_t* d, const char* name, double* out, int cap) {
       if (out && n > 0) memcpy(out, f[i].p, sizeof(double) * n);
       return f[i].len;
     }
+  }
+  if (strcmp(name, "efc_active") == 0) {
+    for (int i = 0; i < d->nefc && i < cap; i++) out[i] = d->efc_active[i];
+    return d->nefc;
   }
   if (strcmp(name, "efc_type") == 0 || strcmp(name, "efc_id") == 0) {
     const int* src = name[4] == 't' ? d->efc_type : d->efc_id;

@@ -92,6 +92,8 @@ int orc_ncon(const orc_data_t* d);
 const orc_contact_t* orc_contacts(const orc_data_t* d);
 int orc_nefc(const orc_data_t* d);
 int orc_solver_niter(const orc_data_t* d);
+/* cost, gradient and Hessian of the last forward's constraint problem at qacc */
+int orc_cost(const orc_prep_t* p, orc_data_t* d, const double* qacc, double* cost, double* grad, double* H);
 
 /* Narrow-phase entry for unit tests: collide one prepared pair given the two
  * geoms' world poses; writes up to maxc contacts, returns the count. */

@@ -110,6 +110,7 @@ def lib() -> C.CDLL:
         L.orc_contacts.argtypes = [C.c_void_p]
         L.orc_nefc.argtypes = [C.c_void_p]
         L.orc_solver_niter.argtypes = [C.c_void_p]
+        L.orc_cost.argtypes = [C.c_void_p, C.c_void_p, _D, _D, _D, _D]
         L.orc_collide_pair.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D, C.POINTER(OrcContact), C.c_int]
         L.orc_batch_step.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D, C.c_int, C.c_int, _I]
         _lib = L
@@ -222,13 +223,23 @@ class Oracle:
         n = lib().orc_ncon(self.d)
         cs = lib().orc_contacts(self.d)
         return [dict(dist=cs[i].dist, pos=np.array(cs[i].pos), frame=np.array(cs[i].frame), geom1=cs[i].geom1,
-                     geom2=cs[i].geom2, pair=cs[i].pair, dim=cs[i].dim) for i in range(n)]
+                     geom2=cs[i].geom2, pair=cs[i].pair, dim=cs[i].dim, friction=np.array(cs[i].friction))
+                for i in range(n)]
 
     def nefc(self) -> int:
         return lib().orc_nefc(self.d)
 
     def niter(self) -> int:
         return lib().orc_solver_niter(self.d)
+
+    def cost(self, qacc):
+        """(cost, gradient, Hessian) of the last forward's constraint problem at qacc."""
+        q = np.ascontiguousarray(qacc, dtype=np.float64)
+        c = C.c_double()
+        g = np.zeros(self.nv)
+        H = np.zeros((self.nv, self.nv))
+        lib().orc_cost(self.p, self.d, _dp(q), C.byref(c), _dp(g), _dp(H))
+        return c.value, g, H
 
     def batch_step(self, qpos, qvel, qacc_ws, ctrl, nsub: int, nthreads: int = 1):
         """Step env-major arrays in place; returns per-env solver iterations."""
