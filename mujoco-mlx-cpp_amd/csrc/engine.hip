@@ -173,7 +173,9 @@ void free_sim(MjmlxBatchedSim* s) {
 
 // one workgroup (one wave) per env, with the narrow-phase set the model needs
 void launch_step(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
-  if (s->npset == 1)
+  if (s->npset == 2)
+    hipLaunchKernelGGL(k_step<2>, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub);
+  else if (s->npset == 1)
     hipLaunchKernelGGL(k_step<1>, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub);
   else
     hipLaunchKernelGGL(k_step<0>, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub);
@@ -459,7 +461,8 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     s->blob = mjh::build_blob(m, &pairs);
     for (auto& p : pairs) {
       int t1 = m.geom_type[p.g1], t2 = m.geom_type[p.g2];
-      if ((t1 == mjk::GCAPSULE && t2 == mjk::GBOX) || (t1 == mjk::GBOX && t2 == mjk::GBOX)) s->npset = 1;
+      if ((t1 == mjk::GCAPSULE && t2 == mjk::GBOX) || (t1 == mjk::GBOX && t2 == mjk::GBOX)) s->npset = std::max(s->npset, 1);
+      if (t1 == mjk::GELLIPSOID || t1 == mjk::GCYLINDER || t2 == mjk::GELLIPSOID || t2 == mjk::GCYLINDER) s->npset = 2;
     }
     s->B = config->num_envs; s->nq = m.nq; s->nv = m.nv; s->nu = m.nu; s->nb = m.nbody;
     // rows per env: 72 for nv <= 32 and 256 above, the reference's caps
@@ -482,6 +485,7 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     if (s->lds_bytes > 65536) {
       HIPOK(hipFuncSetAttribute((const void*)k_step<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
       HIPOK(hipFuncSetAttribute((const void*)k_step<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
+      HIPOK(hipFuncSetAttribute((const void*)k_step<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
       HIPOK(hipFuncSetAttribute((const void*)k_step_queue, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
     }
     HIPOK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));

@@ -28,6 +28,10 @@ std::vector<float> tof(const std::vector<double>& v) { return std::vector<float>
 
 bool pair_supported(int t1, int t2) {
   using namespace mjk;
+  bool cvx1 = t1 == GSPHERE || t1 == GCAPSULE || t1 == GELLIPSOID || t1 == GCYLINDER || t1 == GBOX;
+  bool cvx2 = t2 == GSPHERE || t2 == GCAPSULE || t2 == GELLIPSOID || t2 == GCYLINDER || t2 == GBOX;
+  if (t1 == GPLANE && (t2 == GELLIPSOID || t2 == GCYLINDER)) return true;
+  if ((t1 == GELLIPSOID || t1 == GCYLINDER || t2 == GELLIPSOID || t2 == GCYLINDER) && cvx1 && cvx2) return true;
   if (t1 == GPLANE) return t2 == GSPHERE || t2 == GCAPSULE || t2 == GBOX;
   if (t1 == GSPHERE) return t2 == GSPHERE || t2 == GCAPSULE || t2 == GBOX;
   if (t1 == GCAPSULE) return t2 == GCAPSULE || t2 == GBOX;

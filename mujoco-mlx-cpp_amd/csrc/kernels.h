@@ -934,6 +934,287 @@ KDEV int k_box_box(const float* x1, const float* R1, const float* h1, const floa
   return ns;
 }
 
+// ------------------------------------------------------------------ GJK / EPA
+// Convex primitives (sphere, capsule, ellipsoid, cylinder, box) through their
+// support functions: GJK (Gilbert, Johnson and Keerthi 1988; Ericson's
+// Voronoi-region nearest points on the simplex) gives the distance of
+// separated shapes; for overlapping ones EPA (van den Bergen 2001) grows the
+// final simplex into a polytope of the Minkowski difference A - B until its
+// face nearest the origin lies on the boundary. One contact; normal from A
+// (geom1) to B (geom2). The polytope lives in per-lane private memory.
+struct KCvx { int type; const float *x, *R, *s; };
+struct KSv { float w[3], a[3], b[3]; };
+
+KDEV void k_support(const KCvx& g, const float* d, float* out) {
+  float dl[3], pl[3] = {0.f, 0.f, 0.f};
+  mtv3(g.R, d, dl);
+  float n = sqrtf(dot3(dl, dl));
+  if (n < 1e-30f) { dl[0] = 1.f; dl[1] = dl[2] = 0.f; n = 1.f; }
+  const float* s = g.s;
+  if (g.type == GSPHERE || g.type == GCAPSULE) {
+    for (int i = 0; i < 3; i++) pl[i] = s[0] * dl[i] / n;
+    if (g.type == GCAPSULE) pl[2] += dl[2] >= 0.f ? s[1] : -s[1];
+  } else if (g.type == GELLIPSOID) {
+    float den = sqrtf(s[0] * s[0] * dl[0] * dl[0] + s[1] * s[1] * dl[1] * dl[1] + s[2] * s[2] * dl[2] * dl[2]);
+    for (int i = 0; i < 3; i++) pl[i] = s[i] * s[i] * dl[i] / den;
+  } else if (g.type == GCYLINDER) {
+    float r = sqrtf(dl[0] * dl[0] + dl[1] * dl[1]);
+    if (r > 1e-30f) { pl[0] = s[0] * dl[0] / r; pl[1] = s[0] * dl[1] / r; }
+    pl[2] = dl[2] >= 0.f ? s[1] : -s[1];
+  } else {
+    for (int i = 0; i < 3; i++) pl[i] = dl[i] >= 0.f ? s[i] : -s[i];
+  }
+  mv3(g.R, pl, out);
+  for (int i = 0; i < 3; i++) out[i] += g.x[i];
+}
+
+KDEV void k_msupport(const KCvx& A, const KCvx& B, const float* d, KSv* v) {
+  float nd[3] = {-d[0], -d[1], -d[2]};
+  k_support(A, d, v->a);
+  k_support(B, nd, v->b);
+  for (int i = 0; i < 3; i++) v->w[i] = v->a[i] - v->b[i];
+}
+
+// barycentric coordinates of the point of triangle (a, b, c) nearest the origin
+KDEV void k_tri_nearest(const float* a, const float* b, const float* c, float* l) {
+  float ab[3], ac[3], ap[3] = {-a[0], -a[1], -a[2]}, bp[3] = {-b[0], -b[1], -b[2]}, cp[3] = {-c[0], -c[1], -c[2]};
+  for (int i = 0; i < 3; i++) { ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i]; }
+  float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 <= 0.f && d2 <= 0.f) { l[0] = 1.f; l[1] = l[2] = 0.f; return; }
+  float d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 >= 0.f && d4 <= d3) { l[1] = 1.f; l[0] = l[2] = 0.f; return; }
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) { float v = d1 / (d1 - d3); l[0] = 1.f - v; l[1] = v; l[2] = 0.f; return; }
+  float d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  if (d6 >= 0.f && d5 <= d6) { l[2] = 1.f; l[0] = l[1] = 0.f; return; }
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) { float w = d2 / (d2 - d6); l[0] = 1.f - w; l[1] = 0.f; l[2] = w; return; }
+  float va = d3 * d6 - d5 * d4;
+  if (va <= 0.f && d4 - d3 >= 0.f && d5 - d6 >= 0.f) {
+    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    l[0] = 0.f; l[1] = 1.f - w; l[2] = w; return;
+  }
+  float den = 1.f / (va + vb + vc), v = vb * den, w = vc * den;
+  l[0] = 1.f - v - w; l[1] = v; l[2] = w;
+}
+
+// nearest point of the simplex to the origin; keeps the vertices with non-zero
+// weight; returns 1 when a tetrahedron holds the origin
+KDEV int k_simplex_nearest(KSv* S, int* n, float* lam, float* v) {
+  float l[4] = {0.f, 0.f, 0.f, 0.f};
+  if (*n == 1) {
+    l[0] = 1.f;
+  } else if (*n == 2) {
+    float e[3], ee = 0.f, t = 0.f;
+    for (int i = 0; i < 3; i++) { e[i] = S[1].w[i] - S[0].w[i]; ee += e[i] * e[i]; t -= S[0].w[i] * e[i]; }
+    t = ee > 0.f ? t / ee : 0.f;
+    if (t <= 0.f) l[0] = 1.f;
+    else if (t >= 1.f) l[1] = 1.f;
+    else { l[0] = 1.f - t; l[1] = t; }
+  } else if (*n == 3) {
+    k_tri_nearest(S[0].w, S[1].w, S[2].w, l);
+  } else {
+    const int F[4][4] = {{0, 1, 2, 3}, {0, 1, 3, 2}, {0, 2, 3, 1}, {1, 2, 3, 0}};
+    float best = 3.0e38f;
+    int outside = 0;
+    for (int f = 0; f < 4; f++) {
+      const float *a = S[F[f][0]].w, *b = S[F[f][1]].w, *c = S[F[f][2]].w, *o = S[F[f][3]].w;
+      float ab[3], ac[3], nrm[3], ao[3];
+      for (int i = 0; i < 3; i++) { ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i]; ao[i] = o[i] - a[i]; }
+      cross3(ab, ac, nrm);
+      float so = -dot3(nrm, a), sl = dot3(nrm, ao);
+      if (so * sl >= 0.f) continue;
+      outside = 1;
+      float tl[3], p[3];
+      k_tri_nearest(a, b, c, tl);
+      for (int i = 0; i < 3; i++) p[i] = tl[0] * a[i] + tl[1] * b[i] + tl[2] * c[i];
+      float d = dot3(p, p);
+      if (d < best) {
+        best = d;
+        l[0] = l[1] = l[2] = l[3] = 0.f;
+        l[F[f][0]] = tl[0]; l[F[f][1]] = tl[1]; l[F[f][2]] = tl[2];
+      }
+    }
+    if (!outside) { v[0] = v[1] = v[2] = 0.f; return 1; }
+  }
+  v[0] = v[1] = v[2] = 0.f;
+  int k = 0;
+  for (int i = 0; i < *n; i++) {
+    if (l[i] <= 0.f) continue;
+    for (int q = 0; q < 3; q++) v[q] += l[i] * S[i].w[q];
+    S[k] = S[i];
+    lam[k++] = l[i];
+  }
+  *n = k;
+  return 0;
+}
+
+enum { KEPA_MAXV = 64, KEPA_MAXF = 128 };
+
+KDEV void k_epa_face(const KSv* V, int a, int b, int c, int (*F)[3], float (*Fn)[4], int* nf) {
+  float e1[3], e2[3], n[3];
+  for (int q = 0; q < 3; q++) { e1[q] = V[b].w[q] - V[a].w[q]; e2[q] = V[c].w[q] - V[a].w[q]; }
+  cross3(e1, e2, n);
+  float l = sqrtf(dot3(n, n));
+  if (l > 1e-30f) for (int q = 0; q < 3; q++) n[q] /= l;
+  int f = *nf;
+  F[f][0] = a; F[f][1] = b; F[f][2] = c;
+  Fn[f][0] = n[0]; Fn[f][1] = n[1]; Fn[f][2] = n[2];
+  Fn[f][3] = dot3(n, V[a].w);
+  *nf = f + 1;
+}
+
+KDEV int k_gjk_epa(const KCvx& A, const KCvx& B, float margin, RawC* out) {
+  KSv S[4];
+  int n = 1;
+  float d0[3] = {B.x[0] - A.x[0], B.x[1] - A.x[1], B.x[2] - A.x[2]};
+  if (dot3(d0, d0) < 1e-20f) { d0[0] = 1.f; d0[1] = d0[2] = 0.f; }
+  k_msupport(A, B, d0, &S[0]);
+  float v[3] = {S[0].w[0], S[0].w[1], S[0].w[2]}, lam[4] = {1.f, 0.f, 0.f, 0.f};
+  int overlap = 0;
+  for (int it = 0; it < 64; it++) {
+    float vv = dot3(v, v);
+    if (vv < 1e-20f) { overlap = 1; break; }
+    KSv w;
+    float nd[3] = {-v[0], -v[1], -v[2]};
+    k_msupport(A, B, nd, &w);
+    if (vv - dot3(v, w.w) <= 1e-6f * vv) break;
+    bool dup = false;
+    for (int i = 0; i < n; i++)
+      if (fabsf(S[i].w[0] - w.w[0]) + fabsf(S[i].w[1] - w.w[1]) + fabsf(S[i].w[2] - w.w[2]) < 1e-9f) dup = true;
+    if (dup) break;
+    S[n++] = w;
+    if (k_simplex_nearest(S, &n, lam, v)) { overlap = 1; break; }
+  }
+  if (!overlap) {
+    float pa[3] = {0.f, 0.f, 0.f}, pb[3] = {0.f, 0.f, 0.f};
+    for (int i = 0; i < n; i++)
+      for (int q = 0; q < 3; q++) { pa[q] += lam[i] * S[i].a[q]; pb[q] += lam[i] * S[i].b[q]; }
+    float dv[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+    float dist = sqrtf(dot3(dv, dv));
+    if (dist > margin || dist <= 0.f) return 0;
+    out->dist = dist;
+    for (int q = 0; q < 3; q++) { out->n[q] = dv[q] / dist; out->pos[q] = 0.5f * (pa[q] + pb[q]); }
+    return 1;
+  }
+  KSv V[KEPA_MAXV];
+  int nvtx = n;
+  for (int i = 0; i < n; i++) V[i] = S[i];
+  const float ax[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  for (int k = 0; nvtx < 4 && k < 6; k++) {
+    float dir[3] = {ax[k][0], ax[k][1], ax[k][2]};
+    if (nvtx == 2) {
+      float e[3] = {V[1].w[0] - V[0].w[0], V[1].w[1] - V[0].w[1], V[1].w[2] - V[0].w[2]};
+      cross3(e, ax[k], dir);
+    } else if (nvtx == 3) {
+      float e1[3], e2[3];
+      for (int q = 0; q < 3; q++) { e1[q] = V[1].w[q] - V[0].w[q]; e2[q] = V[2].w[q] - V[0].w[q]; }
+      cross3(e1, e2, dir);
+      if (k & 1) for (int q = 0; q < 3; q++) dir[q] = -dir[q];
+    }
+    if (dot3(dir, dir) < 1e-20f) continue;
+    KSv w;
+    k_msupport(A, B, dir, &w);
+    float ok;
+    if (nvtx == 1) {
+      float e[3] = {w.w[0] - V[0].w[0], w.w[1] - V[0].w[1], w.w[2] - V[0].w[2]};
+      ok = dot3(e, e);
+    } else if (nvtx == 2) {
+      float e1[3], e2[3], c[3];
+      for (int q = 0; q < 3; q++) { e1[q] = V[1].w[q] - V[0].w[q]; e2[q] = w.w[q] - V[0].w[q]; }
+      cross3(e1, e2, c);
+      ok = dot3(c, c);
+    } else {
+      float e1[3], e2[3], e3[3], c[3];
+      for (int q = 0; q < 3; q++) { e1[q] = V[1].w[q] - V[0].w[q]; e2[q] = V[2].w[q] - V[0].w[q]; e3[q] = w.w[q] - V[0].w[q]; }
+      cross3(e1, e2, c);
+      ok = fabsf(dot3(c, e3));
+    }
+    if (ok > 1e-14f) V[nvtx++] = w;
+  }
+  if (nvtx < 4) {  // degenerate: touching
+    float dn = sqrtf(dot3(d0, d0));
+    out->dist = 0.f;
+    for (int q = 0; q < 3; q++) { out->n[q] = d0[q] / dn; out->pos[q] = 0.5f * (V[0].a[q] + V[0].b[q]); }
+    return 1;
+  }
+  // faces keep an outward winding: the first four face away from the opposite
+  // vertex, a new face takes the winding of the removed face it replaces
+  int F[KEPA_MAXF][3];
+  float Fn[KEPA_MAXF][4];
+  int nf = 0;
+  const int T[4][4] = {{0, 1, 2, 3}, {0, 3, 1, 2}, {0, 2, 3, 1}, {1, 3, 2, 0}};
+  for (int f = 0; f < 4; f++) {
+    int a = T[f][0], b = T[f][1], c = T[f][2], o = T[f][3];
+    float e1[3], e2[3], nn[3], eo[3];
+    for (int q = 0; q < 3; q++) { e1[q] = V[b].w[q] - V[a].w[q]; e2[q] = V[c].w[q] - V[a].w[q]; eo[q] = V[o].w[q] - V[a].w[q]; }
+    cross3(e1, e2, nn);
+    if (dot3(nn, eo) > 0.f) k_epa_face(V, a, c, b, F, Fn, &nf);
+    else k_epa_face(V, a, b, c, F, Fn, &nf);
+  }
+  int best = 0;
+  for (int it = 0; it < 48; it++) {
+    best = 0;
+    for (int f = 1; f < nf; f++) if (Fn[f][3] < Fn[best][3]) best = f;
+    KSv w;
+    k_msupport(A, B, Fn[best], &w);
+    float gap = dot3(Fn[best], w.w) - Fn[best][3];
+    if (gap <= 1e-5f * (1e-3f + Fn[best][3]) || nvtx >= KEPA_MAXV) break;
+    int E[KEPA_MAXF][2], ne = 0, keep = 0;
+    bool full = false;
+    for (int f = 0; f < nf; f++) {
+      float rel[3] = {w.w[0] - V[F[f][0]].w[0], w.w[1] - V[F[f][0]].w[1], w.w[2] - V[F[f][0]].w[2]};
+      if (dot3(Fn[f], rel) > 1e-9f) {
+        for (int e = 0; e < 3; e++) {
+          int a = F[f][e], b = F[f][(e + 1) % 3], found = -1;
+          for (int q = 0; q < ne; q++) if (E[q][0] == b && E[q][1] == a) { found = q; break; }
+          if (found >= 0) { E[found][0] = E[ne - 1][0]; E[found][1] = E[ne - 1][1]; ne--; }
+          else if (ne < KEPA_MAXF) { E[ne][0] = a; E[ne][1] = b; ne++; }
+          else full = true;
+        }
+      } else {
+        if (keep != f) {
+          for (int q = 0; q < 3; q++) F[keep][q] = F[f][q];
+          for (int q = 0; q < 4; q++) Fn[keep][q] = Fn[f][q];
+        }
+        keep++;
+      }
+    }
+    nf = keep;
+    if (full || nf + ne > KEPA_MAXF) break;
+    V[nvtx] = w;
+    for (int q = 0; q < ne; q++) k_epa_face(V, E[q][0], E[q][1], nvtx, F, Fn, &nf);
+    nvtx++;
+  }
+  for (int f = 1; f < nf; f++) if (Fn[f][3] < Fn[best][3]) best = f;
+  float depth = Fn[best][3];
+  float p[3] = {Fn[best][0] * depth, Fn[best][1] * depth, Fn[best][2] * depth};
+  const float *a = V[F[best][0]].w, *b = V[F[best][1]].w, *c = V[F[best][2]].w;
+  float v0[3], v1[3], v2[3];
+  for (int q = 0; q < 3; q++) { v0[q] = b[q] - a[q]; v1[q] = c[q] - a[q]; v2[q] = p[q] - a[q]; }
+  float d00 = dot3(v0, v0), d01 = dot3(v0, v1), d11 = dot3(v1, v1), d20 = dot3(v2, v0), d21 = dot3(v2, v1);
+  float den = d00 * d11 - d01 * d01;
+  float lv = den != 0.f ? (d11 * d20 - d01 * d21) / den : 0.f, lw = den != 0.f ? (d00 * d21 - d01 * d20) / den : 0.f;
+  float lu = 1.f - lv - lw;
+  float pa[3], pb[3];
+  for (int q = 0; q < 3; q++) {
+    pa[q] = lu * V[F[best][0]].a[q] + lv * V[F[best][1]].a[q] + lw * V[F[best][2]].a[q];
+    pb[q] = lu * V[F[best][0]].b[q] + lv * V[F[best][1]].b[q] + lw * V[F[best][2]].b[q];
+  }
+  // the supports along the normal give the depth h_A(n) + h_B(-n) to second
+  // order in the normal's error; the points move onto the supporting planes
+  KSv sw;
+  k_msupport(A, B, Fn[best], &sw);
+  float ha = dot3(Fn[best], sw.a), hb = dot3(Fn[best], sw.b);
+  float ea = ha - dot3(Fn[best], pa), eb = hb - dot3(Fn[best], pb);
+  out->dist = -(ha - hb);
+  for (int q = 0; q < 3; q++) {
+    out->n[q] = Fn[best][q];
+    out->pos[q] = 0.5f * (pa[q] + ea * Fn[best][q] + pb[q] + eb * Fn[best][q]);
+  }
+  return 1;
+}
+
 // narrow phase for one pair (geom1 has the lower type); -1 for unsupported
 template <int NPSET>
 KDEV int k_narrow(int t1, int t2, const float* s1, const float* s2, const float* x1, const float* R1,
@@ -941,6 +1222,12 @@ KDEV int k_narrow(int t1, int t2, const float* s1, const float* s2, const float*
   if (t1 == GPLANE) {
     float pn[3] = {R1[2], R1[5], R1[8]};
     if (t2 == GSPHERE) return k_plane_point(x1, pn, x2, s2[0], margin, out);
+    if (NPSET >= 2 && (t2 == GELLIPSOID || t2 == GCYLINDER)) {  // the deepest support point
+      KCvx g = {t2, x2, R2, s2};
+      float dn[3] = {-pn[0], -pn[1], -pn[2]}, sp[3];
+      k_support(g, dn, sp);
+      return k_plane_point(x1, pn, sp, 0.f, margin, out);
+    }
     if (t2 == GCAPSULE) {
       float p0[3], dd[3];
       k_capsule_ends(x2, R2, s2[1], p0, dd);
@@ -960,6 +1247,10 @@ KDEV int k_narrow(int t1, int t2, const float* s1, const float* s2, const float*
       return n;
     }
     return -1;
+  }
+  if (NPSET >= 2 && (t1 == GELLIPSOID || t1 == GCYLINDER || t2 == GELLIPSOID || t2 == GCYLINDER)) {
+    KCvx A = {t1, x1, R1, s1}, B = {t2, x2, R2, s2};
+    return k_gjk_epa(A, B, margin, out);
   }
   if (t1 == GSPHERE) {
     if (t2 == GSPHERE) return k_sphere_sphere(x1, s1[0], x2, s2[0], margin, out);
@@ -2000,7 +2291,8 @@ KDEV void write_features(const DModel& m, const Layout& L, const float* w, float
 }
 
 // NPSET selects the narrow phases compiled in (0: plane / sphere / capsule
-// pairs and box against plane and sphere; 1: also capsule-box and box-box),
+// pairs and box against plane and sphere; 1: also capsule-box and box-box;
+// 2: also ellipsoids and cylinders through GJK / EPA),
 // so models without the heavier pairs do not pay their registers.
 template <int NPSET>
 KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, int env, int nsub) {

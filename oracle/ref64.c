@@ -971,6 +971,290 @@ static int box_box(const double* x1, const double* R1, const double* h1, const d
   return ns;
 }
 
+/* ------------------------------------------------------------------ convex primitives: GJK / EPA
+ * Pairs involving an ellipsoid or a cylinder (and no plane) go through the
+ * support functions of the two convex primitives: GJK (Gilbert, Johnson and
+ * Keerthi 1988, with Ericson's Voronoi-region closest points on the simplex)
+ * gives the distance of separated shapes; for overlapping shapes EPA (van den
+ * Bergen 2001) expands the final GJK simplex into a polytope of the Minkowski
+ * difference until its face nearest the origin is on the boundary. One
+ * contact per pair; the normal points from geom1 to geom2. */
+typedef struct { int type; const double *x, *R, *s; } cvx_t;
+typedef struct { double w[3], a[3], b[3]; } svert_t;
+
+static void cvx_support(const cvx_t* g, const double d[3], double out[3]) {
+  double dl[3], pl[3] = {0, 0, 0};
+  mulmatTvec3(g->R, d, dl);
+  double n = sqrt(dot3(dl, dl));
+  if (n < 1e-300) { dl[0] = 1; dl[1] = dl[2] = 0; n = 1; }
+  const double* s = g->s;
+  if (g->type == GSPHERE || g->type == GCAPSULE) {
+    for (int i = 0; i < 3; i++) pl[i] = s[0] * dl[i] / n;
+    if (g->type == GCAPSULE) pl[2] += dl[2] >= 0 ? s[1] : -s[1];
+  } else if (g->type == GELLIPSOID) {
+    double q[3], den = 0;
+    for (int i = 0; i < 3; i++) { q[i] = s[i] * s[i] * dl[i]; den += s[i] * s[i] * dl[i] * dl[i]; }
+    den = sqrt(den);
+    for (int i = 0; i < 3; i++) pl[i] = q[i] / den;
+  } else if (g->type == GCYLINDER) {
+    double r = sqrt(dl[0] * dl[0] + dl[1] * dl[1]);
+    if (r > 1e-300) { pl[0] = s[0] * dl[0] / r; pl[1] = s[0] * dl[1] / r; }
+    pl[2] = dl[2] >= 0 ? s[1] : -s[1];
+  } else {  /* box */
+    for (int i = 0; i < 3; i++) pl[i] = dl[i] >= 0 ? s[i] : -s[i];
+  }
+  mulmatvec3(g->R, pl, out);
+  for (int i = 0; i < 3; i++) out[i] += g->x[i];
+}
+
+/* support of the Minkowski difference A - B in direction d */
+static void mink_support(const cvx_t* A, const cvx_t* B, const double d[3], svert_t* v) {
+  double nd[3] = {-d[0], -d[1], -d[2]};
+  cvx_support(A, d, v->a);
+  cvx_support(B, nd, v->b);
+  for (int i = 0; i < 3; i++) v->w[i] = v->a[i] - v->b[i];
+}
+
+/* barycentric coordinates of the point of triangle (a, b, c) nearest the origin */
+static void tri_nearest(const double a[3], const double b[3], const double c[3], double l[3]) {
+  double ab[3], ac[3], ap[3] = {-a[0], -a[1], -a[2]}, bp[3] = {-b[0], -b[1], -b[2]}, cp[3] = {-c[0], -c[1], -c[2]};
+  for (int i = 0; i < 3; i++) { ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i]; }
+  double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 <= 0 && d2 <= 0) { l[0] = 1; l[1] = l[2] = 0; return; }
+  double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 >= 0 && d4 <= d3) { l[1] = 1; l[0] = l[2] = 0; return; }
+  double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) { double v = d1 / (d1 - d3); l[0] = 1 - v; l[1] = v; l[2] = 0; return; }
+  double d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  if (d6 >= 0 && d5 <= d6) { l[2] = 1; l[0] = l[1] = 0; return; }
+  double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) { double w = d2 / (d2 - d6); l[0] = 1 - w; l[1] = 0; l[2] = w; return; }
+  double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && d4 - d3 >= 0 && d5 - d6 >= 0) {
+    double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    l[0] = 0; l[1] = 1 - w; l[2] = w; return;
+  }
+  double den = 1 / (va + vb + vc), v = vb * den, w = vc * den;
+  l[0] = 1 - v - w; l[1] = v; l[2] = w;
+}
+
+/* nearest point of the simplex to the origin; the simplex shrinks to the
+ * vertices with non-zero weight. Returns 1 when a tetrahedron holds the origin. */
+static int simplex_nearest(svert_t* S, int* n, double lam[4], double v[3]) {
+  double l[4] = {0, 0, 0, 0};
+  if (*n == 1) { l[0] = 1; }
+  else if (*n == 2) {
+    double e[3], ee = 0, t = 0;
+    for (int i = 0; i < 3; i++) { e[i] = S[1].w[i] - S[0].w[i]; ee += e[i] * e[i]; t -= S[0].w[i] * e[i]; }
+    t = ee > 0 ? t / ee : 0;
+    if (t <= 0) { l[0] = 1; } else if (t >= 1) { l[1] = 1; } else { l[0] = 1 - t; l[1] = t; }
+  } else if (*n == 3) {
+    tri_nearest(S[0].w, S[1].w, S[2].w, l);
+  } else {
+    static const int F[4][4] = {{0, 1, 2, 3}, {0, 1, 3, 2}, {0, 2, 3, 1}, {1, 2, 3, 0}};
+    double best = 1e300;
+    int outside = 0;
+    for (int f = 0; f < 4; f++) {
+      const double *a = S[F[f][0]].w, *b = S[F[f][1]].w, *c = S[F[f][2]].w, *o = S[F[f][3]].w;
+      double ab[3], ac[3], nrm[3], ao[3];
+      for (int i = 0; i < 3; i++) { ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i]; ao[i] = o[i] - a[i]; }
+      cross3(ab, ac, nrm);
+      double so = -dot3(nrm, a), sl = dot3(nrm, ao);
+      if (so * sl >= 0) continue;  /* origin on the same side as the opposite vertex */
+      outside = 1;
+      double tl[3], p[3];
+      tri_nearest(a, b, c, tl);
+      for (int i = 0; i < 3; i++) p[i] = tl[0] * a[i] + tl[1] * b[i] + tl[2] * c[i];
+      double d = dot3(p, p);
+      if (d < best) {
+        best = d;
+        l[0] = l[1] = l[2] = l[3] = 0;
+        l[F[f][0]] = tl[0]; l[F[f][1]] = tl[1]; l[F[f][2]] = tl[2];
+      }
+    }
+    if (!outside) {
+      v[0] = v[1] = v[2] = 0;
+      return 1;
+    }
+  }
+  v[0] = v[1] = v[2] = 0;
+  int k = 0;
+  for (int i = 0; i < *n; i++) {
+    if (l[i] <= 0) continue;
+    for (int q = 0; q < 3; q++) v[q] += l[i] * S[i].w[q];
+    S[k] = S[i];
+    lam[k++] = l[i];
+  }
+  *n = k;
+  return 0;
+}
+
+enum { EPA_MAXV = 64, EPA_MAXF = 128 };
+
+/* GJK distance; on overlap, EPA depth. Returns the number of contacts (0/1). */
+static int gjk_epa(const cvx_t* A, const cvx_t* B, double margin, rawc_t* out) {
+  svert_t S[4];
+  int n = 1;
+  double d0[3] = {B->x[0] - A->x[0], B->x[1] - A->x[1], B->x[2] - A->x[2]};
+  if (dot3(d0, d0) < 1e-24) { d0[0] = 1; d0[1] = d0[2] = 0; }
+  mink_support(A, B, d0, &S[0]);
+  double v[3] = {S[0].w[0], S[0].w[1], S[0].w[2]}, lam[4] = {1, 0, 0, 0};
+  int overlap = 0;
+  for (int it = 0; it < 100; it++) {
+    double vv = dot3(v, v);
+    if (vv < 1e-24) { overlap = 1; break; }
+    svert_t w;
+    double nv_[3] = {-v[0], -v[1], -v[2]};
+    mink_support(A, B, nv_, &w);
+    if (vv - dot3(v, w.w) <= 1e-12 * vv) break;  /* no progress: v is the nearest point */
+    int dup = 0;
+    for (int i = 0; i < n; i++)
+      if (fabs(S[i].w[0] - w.w[0]) + fabs(S[i].w[1] - w.w[1]) + fabs(S[i].w[2] - w.w[2]) < 1e-14) dup = 1;
+    if (dup) break;
+    S[n++] = w;
+    if (simplex_nearest(S, &n, lam, v)) { overlap = 1; break; }
+  }
+  if (!overlap) {
+    double pa[3] = {0, 0, 0}, pb[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++)
+      for (int q = 0; q < 3; q++) { pa[q] += lam[i] * S[i].a[q]; pb[q] += lam[i] * S[i].b[q]; }
+    double dv[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+    double dist = sqrt(dot3(dv, dv));
+    if (dist > margin) return 0;
+    out->dist = dist;
+    for (int q = 0; q < 3; q++) { out->n[q] = dv[q] / dist; out->pos[q] = 0.5 * (pa[q] + pb[q]); }
+    return 1;
+  }
+  /* EPA: a tetrahedron around the origin from the final simplex */
+  svert_t V[EPA_MAXV];
+  int nvtx = n;
+  for (int i = 0; i < n; i++) V[i] = S[i];
+  static const double ax[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  for (int k = 0; nvtx < 4 && k < 6; k++) {
+    double dir[3] = {ax[k][0], ax[k][1], ax[k][2]};
+    if (nvtx == 2) {
+      double e[3] = {V[1].w[0] - V[0].w[0], V[1].w[1] - V[0].w[1], V[1].w[2] - V[0].w[2]};
+      cross3(e, ax[k], dir);
+    } else if (nvtx == 3) {
+      double e1[3], e2[3];
+      for (int q = 0; q < 3; q++) { e1[q] = V[1].w[q] - V[0].w[q]; e2[q] = V[2].w[q] - V[0].w[q]; }
+      cross3(e1, e2, dir);
+      if (k & 1) for (int q = 0; q < 3; q++) dir[q] = -dir[q];
+    }
+    if (dot3(dir, dir) < 1e-24) continue;
+    svert_t w;
+    mink_support(A, B, dir, &w);
+    /* accept only a point that adds a dimension */
+    double ok = 0;
+    if (nvtx == 1) { double e[3] = {w.w[0] - V[0].w[0], w.w[1] - V[0].w[1], w.w[2] - V[0].w[2]}; ok = dot3(e, e); }
+    else if (nvtx == 2) {
+      double e1[3], e2[3], c[3];
+      for (int q = 0; q < 3; q++) { e1[q] = V[1].w[q] - V[0].w[q]; e2[q] = w.w[q] - V[0].w[q]; }
+      cross3(e1, e2, c);
+      ok = dot3(c, c);
+    } else {
+      double e1[3], e2[3], e3[3], c[3];
+      for (int q = 0; q < 3; q++) { e1[q] = V[1].w[q] - V[0].w[q]; e2[q] = V[2].w[q] - V[0].w[q]; e3[q] = w.w[q] - V[0].w[q]; }
+      cross3(e1, e2, c);
+      ok = fabs(dot3(c, e3));
+    }
+    if (ok > 1e-18) V[nvtx++] = w;
+  }
+  if (nvtx < 4) {  /* degenerate: touching */
+    out->dist = 0;
+    double dn = sqrt(dot3(d0, d0));
+    for (int q = 0; q < 3; q++) { out->n[q] = d0[q] / dn; out->pos[q] = 0.5 * (V[0].a[q] + V[0].b[q]); }
+    return margin >= 0 ? 1 : 0;
+  }
+  int F[EPA_MAXF][3], nf = 0;
+  double Fn[EPA_MAXF][3], Fd[EPA_MAXF];
+  /* faces keep their winding outward: the initial ones are oriented away from
+   * the opposite vertex, a new face takes the winding of the removed face
+   * whose horizon edge it closes */
+#define EPA_FACE(i0, i1, i2)                                                              \
+  do {                                                                                    \
+    int a_ = (i0), b_ = (i1), c_ = (i2);                                                  \
+    double e1_[3], e2_[3], n_[3];                                                         \
+    for (int q = 0; q < 3; q++) { e1_[q] = V[b_].w[q] - V[a_].w[q]; e2_[q] = V[c_].w[q] - V[a_].w[q]; } \
+    cross3(e1_, e2_, n_);                                                                 \
+    double l_ = sqrt(dot3(n_, n_));                                                       \
+    if (l_ > 1e-300) for (int q = 0; q < 3; q++) n_[q] /= l_;                            \
+    F[nf][0] = a_; F[nf][1] = b_; F[nf][2] = c_;                                          \
+    for (int q = 0; q < 3; q++) Fn[nf][q] = n_[q];                                        \
+    Fd[nf] = dot3(n_, V[a_].w); nf++;                                                     \
+  } while (0)
+  {
+    static const int T[4][4] = {{0, 1, 2, 3}, {0, 3, 1, 2}, {0, 2, 3, 1}, {1, 3, 2, 0}};
+    for (int f = 0; f < 4; f++) {
+      int a = T[f][0], b = T[f][1], c = T[f][2], o = T[f][3];
+      double e1[3], e2[3], nn[3], eo[3];
+      for (int q = 0; q < 3; q++) { e1[q] = V[b].w[q] - V[a].w[q]; e2[q] = V[c].w[q] - V[a].w[q]; eo[q] = V[o].w[q] - V[a].w[q]; }
+      cross3(e1, e2, nn);
+      if (dot3(nn, eo) > 0) EPA_FACE(a, c, b); else EPA_FACE(a, b, c);
+    }
+  }
+  int best = 0;
+  for (int it = 0; it < 64; it++) {
+    best = 0;
+    for (int f = 1; f < nf; f++) if (Fd[f] < Fd[best]) best = f;
+    svert_t w;
+    mink_support(A, B, Fn[best], &w);
+    double gap = dot3(Fn[best], w.w) - Fd[best];
+    if (gap <= 1e-10 * (1 + Fd[best]) || nvtx >= EPA_MAXV) break;
+    /* remove the faces w sees, keep their unshared edges (the horizon) */
+    int E[3 * EPA_MAXF][2], ne = 0;
+    int keep = 0;
+    for (int f = 0; f < nf; f++) {
+      double rel[3] = {w.w[0] - V[F[f][0]].w[0], w.w[1] - V[F[f][0]].w[1], w.w[2] - V[F[f][0]].w[2]};
+      if (dot3(Fn[f], rel) > 1e-14) {
+        for (int e = 0; e < 3; e++) {
+          int a = F[f][e], b = F[f][(e + 1) % 3], found = -1;
+          for (int k = 0; k < ne; k++) if (E[k][0] == b && E[k][1] == a) { found = k; break; }
+          if (found >= 0) { E[found][0] = E[ne - 1][0]; E[found][1] = E[ne - 1][1]; ne--; }
+          else { E[ne][0] = a; E[ne][1] = b; ne++; }
+        }
+      } else {
+        if (keep != f) {
+          memcpy(F[keep], F[f], sizeof(F[f])); memcpy(Fn[keep], Fn[f], sizeof(Fn[f])); Fd[keep] = Fd[f];
+        }
+        keep++;
+      }
+    }
+    nf = keep;
+    if (nf + ne > EPA_MAXF) break;
+    V[nvtx] = w;
+    for (int k = 0; k < ne; k++) EPA_FACE(E[k][0], E[k][1], nvtx);
+    nvtx++;
+  }
+#undef EPA_FACE
+  for (int f = 1; f < nf; f++) if (Fd[f] < Fd[best]) best = f;
+  double depth = Fd[best], p[3] = {Fn[best][0] * depth, Fn[best][1] * depth, Fn[best][2] * depth};
+  /* barycentric coordinates of p on the face */
+  const double *a = V[F[best][0]].w, *b = V[F[best][1]].w, *c = V[F[best][2]].w;
+  double v0[3], v1[3], v2[3];
+  for (int q = 0; q < 3; q++) { v0[q] = b[q] - a[q]; v1[q] = c[q] - a[q]; v2[q] = p[q] - a[q]; }
+  double d00 = dot3(v0, v0), d01 = dot3(v0, v1), d11 = dot3(v1, v1), d20 = dot3(v2, v0), d21 = dot3(v2, v1);
+  double den = d00 * d11 - d01 * d01;
+  double lv = den != 0 ? (d11 * d20 - d01 * d21) / den : 0, lw = den != 0 ? (d00 * d21 - d01 * d20) / den : 0, lu = 1 - lv - lw;
+  double pa[3], pb[3];
+  for (int q = 0; q < 3; q++) {
+    pa[q] = lu * V[F[best][0]].a[q] + lv * V[F[best][1]].a[q] + lw * V[F[best][2]].a[q];
+    pb[q] = lu * V[F[best][0]].b[q] + lv * V[F[best][1]].b[q] + lw * V[F[best][2]].b[q];
+  }
+  /* the face distance bounds the depth from below with a first-order error in
+   * the normal; the supports along the normal give the second-order value
+   * h_A(n) + h_B(-n), and the points move onto the two supporting planes */
+  svert_t sw;
+  mink_support(A, B, Fn[best], &sw);
+  double ha = dot3(Fn[best], sw.a), hb = dot3(Fn[best], sw.b);
+  double ea = ha - dot3(Fn[best], pa), eb = hb - dot3(Fn[best], pb);
+  for (int q = 0; q < 3; q++) { pa[q] += ea * Fn[best][q]; pb[q] += eb * Fn[best][q]; }
+  depth = ha - hb;
+  out->dist = -depth;
+  for (int q = 0; q < 3; q++) { out->n[q] = Fn[best][q]; out->pos[q] = 0.5 * (pa[q] + pb[q]); }
+  return 1;
+}
+
 /* narrow phase for one pair; geom1 has the lower type. Returns the number of
  * raw contacts written (normal from geom1 to geom2). */
 static int narrowphase(int t1, int t2, const double* s1, const double* s2, const double* x1, const double* R1,
@@ -979,6 +1263,12 @@ static int narrowphase(int t1, int t2, const double* s1, const double* s2, const
   if (t1 == GPLANE) {
     double pn[3] = {R1[2], R1[5], R1[8]};
     if (t2 == GSPHERE) return plane_point(x1, pn, x2, s2[0], margin, out);
+    if (t2 == GELLIPSOID || t2 == GCYLINDER) {  /* the deepest support point */
+      cvx_t g = {t2, x2, R2, s2};
+      double dn[3] = {-pn[0], -pn[1], -pn[2]}, sp[3];
+      cvx_support(&g, dn, sp);
+      return plane_point(x1, pn, sp, 0.0, margin, out);
+    }
     if (t2 == GCAPSULE) {
       double p0[3], dd[3];
       capsule_ends(x2, R2, s2[1], p0, dd);
@@ -999,6 +1289,10 @@ static int narrowphase(int t1, int t2, const double* s1, const double* s2, const
       return n;
     }
     return -1;
+  }
+  if (t1 == GELLIPSOID || t1 == GCYLINDER || t2 == GELLIPSOID || t2 == GCYLINDER) {
+    cvx_t A = {t1, x1, R1, s1}, B = {t2, x2, R2, s2};
+    return gjk_epa(&A, &B, margin, out);
   }
   if (t1 == GSPHERE) {
     if (t2 == GSPHERE) return sphere_sphere(x1, s1[0], x2, s2[0], margin, out);
