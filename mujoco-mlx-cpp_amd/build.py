@@ -63,7 +63,9 @@ def build_mjb():
 def build_emu():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libmjmlx_emu.so")
-    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", f"-I{CSRC}", "-o", out, "emu.cpp", *HOST_SRC])
+    # a*b+c contracted into fused multiply-adds, as hipcc does in device code
+    _run(["g++", "-std=c++17", "-O2", "-mfma", "-ffp-contract=fast", "-fPIC", "-shared", f"-I{CSRC}", "-o", out,
+          "emu.cpp", *HOST_SRC])
     return out
 
 

@@ -43,7 +43,6 @@ struct DModel {
   // bodies
   const int *body_parentid, *body_rootid, *body_jntnum, *body_jntadr, *body_dofnum, *body_dofadr,
       *body_subtree_end, *level_adr, *level_body, *body_lastdof;
-  const uint64_t* body_dofmask;
   const float *body_pos, *body_quat, *body_ipos, *body_iquat, *body_mass, *body_inertia,
       *body_subtreemass, *body_invweight0;
   // joints
@@ -74,7 +73,10 @@ struct DModel {
 // intermediates) is dead once the constraint Jacobian is built, so the FP64
 // Newton Hessian overlays it.
 struct Layout {
-  int maxcon, maxefc, nv, total;
+  // maxcon / maxefc: buffer sizes; efc_cap: the row budget that admits rows
+  // (limit rows while fewer than efc_cap - 4 rows exist, a contact while fewer
+  // than efc_cap - 10), maxefc >= min(rows the model can produce, efc_cap)
+  int maxcon, maxefc, efc_cap, nv, total;
   int qpos, qvel, ctrl, qacc_ws, qacc_pred, qacc, qM, qLD, qfrc_smooth, qacc_smooth, qfrc_constraint, act_force, misc;
   int xpos, xquat, xmat, xipos, xanchor, xaxis, gxpos, gxmat, scom, cinert, cdof, cvel, cdofdot, crb, cacc, cfrc, buf6;
   int H;
@@ -102,7 +104,7 @@ MJK_HD inline int al2(int x) { return (x + 1) & ~1; }
 MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, int ngeom, int maxcon, int maxefc,
                                  bool qld_shared = false) {
   Layout L;
-  L.maxcon = maxcon; L.maxefc = maxefc; L.nv = nv;
+  L.maxcon = maxcon; L.maxefc = maxefc; L.efc_cap = maxefc; L.nv = nv;
   int nvp = nv > 0 ? nv : 1, nup = nu > 0 ? nu : 1, njp = njnt > 0 ? njnt : 1, ngp = ngeom > 0 ? ngeom : 1;
   int o = 0;
   L.qpos = o; o += al2(nq > 0 ? nq : 1);
@@ -170,7 +172,7 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.Mgrad_old = o; o += nvp;
   L.tmpv = o; o += nvp;
   o = al2(o);
-  L.dvec = o; o += 2 * nvp;
+  L.dvec = o; o += 4 * nvp;  // two FP64 vectors: inverse pivots and a solve vector
   if (L.qLD < 0) L.qLD = al2(L.efc_J);
   L.total = al2(o);
   return L;

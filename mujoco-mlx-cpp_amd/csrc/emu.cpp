@@ -42,12 +42,12 @@ __attribute__((visibility("default"))) void* mjemu_create(const char* xml, int s
     std::vector<mjh::CollisionPair> pairs;
     s->blob = mjh::build_blob(s->model, &pairs);
     s->dm = s->blob.bind(s->blob.bytes.data());
-    if (maxcon <= 0) maxcon = std::min(mjh::max_contacts(s->model, pairs), 64);
-    if (maxefc <= 0) maxefc = std::min(mjh::max_rows(s->model, pairs), 128);
-    if (maxcon < 1) maxcon = 1;
-    if (maxefc < 1) maxefc = 1;
+    // maxcon / maxefc > 0 replace the contact cap and the row budget
+    mjh::Caps caps = mjh::default_caps(s->model, pairs, maxcon, maxefc);
     auto& m = s->model;
-    s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, maxcon, maxefc, m.solver == mjh::SOL_NEWTON);
+    s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, caps.maxcon, caps.maxefc,
+                            m.solver == mjh::SOL_NEWTON);
+    s->L.efc_cap = caps.efc_cap;
     s->fields = mjh::layout_fields(m, s->L);
     s->B = B;
     s->qpos.resize((size_t)B * m.nq);
@@ -134,6 +134,20 @@ __attribute__((visibility("default"))) void mjemu_stats(void* p, int* niter, int
   if (status) std::memcpy(status, s->status.data(), s->B * sizeof(int));
   if (ncon) std::memcpy(ncon, s->ncon.data(), s->B * sizeof(int));
   if (nefc) std::memcpy(nefc, s->nefc.data(), s->B * sizeof(int));
+}
+
+// The kernel's narrow phase for one geom pair (geom types as in the model,
+// the lower type first); out holds up to 8 contacts of (dist, pos[3], n[3]).
+__attribute__((visibility("default"))) int mjemu_narrow(int t1, int t2, const float* s1, const float* s2,
+                                                       const float* x1, const float* R1, const float* x2,
+                                                       const float* R2, float margin, float* out) {
+  mjk::RawC rc[8];
+  int n = mjk::k_narrow<2>(t1, t2, s1, s2, x1, R1, x2, R2, margin, rc);
+  for (int k = 0; k < n && k < 8; k++) {
+    out[7 * k] = rc[k].dist;
+    for (int q = 0; q < 3; q++) { out[7 * k + 1 + q] = rc[k].pos[q]; out[7 * k + 4 + q] = rc[k].n[q]; }
+  }
+  return n;
 }
 
 }  // extern "C"

@@ -43,12 +43,15 @@ void report(const char* fn, const std::exception& e) { std::fprintf(stderr, "%s 
 }  // namespace
 
 // ------------------------------------------------------------------ kernels
-template <int NPSET>
-__global__ void __launch_bounds__(64) k_step(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub) {
+// GWS: the per-env workspace lives in device memory (gws + env * L.total)
+// instead of LDS, for models whose workspace exceeds the LDS of a workgroup
+template <int NPSET, int GWS>
+__global__ void __launch_bounds__(64) k_step(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub, float* gws) {
   extern __shared__ __align__(16) float lds[];
   int env = blockIdx.x;
   if (env >= B) return;
-  mjk::run_env<NPSET>(m, L, lds, io, env, nsub);
+  float* w = GWS ? gws + (size_t)env * L.total : lds;
+  mjk::run_env<NPSET>(m, L, w, io, env, nsub);
 }
 
 __global__ void __launch_bounds__(64) k_step_queue(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub,
@@ -127,6 +130,7 @@ struct MjmlxBatchedSim {
   bool timed = false, ctrl_pending = false;
   int persistent = 0, grid = 0, ncu = 0;
   int npset = 0;  // narrow-phase set compiled into the step kernel
+  float* d_gws = nullptr;  // per-env workspaces in device memory when they exceed the LDS of a workgroup
   int ws_mode = 0;
   // packed readback
   int* d_pack_bodies = nullptr;
@@ -157,7 +161,7 @@ void free_sim(MjmlxBatchedSim* s) {
   if (s->stream) hipStreamSynchronize(s->stream);
   void* dev[] = {s->d_blob, s->d_qpos, s->d_qvel, s->d_qws, s->d_ctrl, s->d_qpos0, s->d_niter, s->d_status,
                  s->d_ncon, s->d_nefc, s->d_queue, s->d_mask, s->d_pack_bodies, s->d_pack, s->d_dump1, s->d_dump2,
-                 s->d_feat[0], s->d_feat[1], s->d_pred};
+                 s->d_feat[0], s->d_feat[1], s->d_pred, s->d_gws};
   for (void* p : dev) if (p) hipFree(p);
   for (int i = 0; i < 7; i++) { if (s->d_aux[i]) hipFree(s->d_aux[i]); if (s->h_aux[i]) hipHostFree(s->h_aux[i]); }
   if (s->h_qpos) hipHostFree(s->h_qpos);
@@ -173,12 +177,18 @@ void free_sim(MjmlxBatchedSim* s) {
 
 // one workgroup (one wave) per env, with the narrow-phase set the model needs
 void launch_step(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
-  if (s->npset == 2)
-    hipLaunchKernelGGL(k_step<2>, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub);
-  else if (s->npset == 1)
-    hipLaunchKernelGGL(k_step<1>, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub);
-  else
-    hipLaunchKernelGGL(k_step<0>, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub);
+  dim3 g(s->B), b(64);
+  size_t sh = s->d_gws ? 0 : s->lds_bytes;
+  float* gw = s->d_gws;
+  if (s->d_gws) {
+    if (s->npset == 2) hipLaunchKernelGGL((k_step<2, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
+    else if (s->npset == 1) hipLaunchKernelGGL((k_step<1, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
+    else hipLaunchKernelGGL((k_step<0, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
+  } else {
+    if (s->npset == 2) hipLaunchKernelGGL((k_step<2, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
+    else if (s->npset == 1) hipLaunchKernelGGL((k_step<1, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
+    else hipLaunchKernelGGL((k_step<0, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
+  }
 }
 
 mjk::StepIO make_io(MjmlxBatchedSim* s, float* dump1, float* dump2) {
@@ -210,11 +220,11 @@ void upload_ctrl(MjmlxBatchedSim* s, const float* ctrl) {
 void launch(MjmlxBatchedSim* s, int nsub, float* dump1, float* dump2) {
   mjk::StepIO io = make_io(s, dump1, dump2);
   HIPOK(hipEventRecord(s->ev0, s->stream));
-  if (s->persistent) {
+  if (s->persistent && !s->d_gws) {
     int grid = s->grid > 0 ? s->grid : std::max(1, s->ncu * 8);
     grid = std::min(grid, s->B);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s->stream, s->d_queue, 1);
-    hipLaunchKernelGGL(k_step_queue, dim3(grid), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub, s->d_queue);
+    hipLaunchKernelGGL(k_step_queue, dim3(grid), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub, s->d_queue);  // LDS workspaces only
   } else {
     launch_step(s, io, nsub);
   }
@@ -465,14 +475,14 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
       if (t1 == mjk::GELLIPSOID || t1 == mjk::GCYLINDER || t2 == mjk::GELLIPSOID || t2 == mjk::GCYLINDER) s->npset = 2;
     }
     s->B = config->num_envs; s->nq = m.nq; s->nv = m.nv; s->nu = m.nu; s->nb = m.nbody;
-    // rows per env: 72 for nv <= 32 and 256 above, the reference's caps
-    // (reference src/compat/mkx_kernels/solver.hpp:28-33); contacts 32 / 64
-    bool small = m.nv <= 32;
-    int maxcon = std::max(1, std::min(mjh::max_contacts(m, pairs), small ? 32 : 64));
-    int maxefc = std::max(1, std::min(mjh::max_rows(m, pairs), small ? 72 : 256));
-    if (const char* v = std::getenv("MJMLX_HIP_MAXCON")) maxcon = std::max(1, std::atoi(v));
-    if (const char* v = std::getenv("MJMLX_HIP_MAXEFC")) maxefc = std::max(1, std::atoi(v));
-    s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, maxcon, maxefc, m.solver == mjh::SOL_NEWTON);
+    // per-env capacities: the reference's contact cap and row budget
+    int con_over = 0, efc_over = 0;
+    if (const char* v = std::getenv("MJMLX_HIP_MAXCON")) con_over = std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("MJMLX_HIP_MAXEFC")) efc_over = std::max(1, std::atoi(v));
+    mjh::Caps caps = mjh::default_caps(m, pairs, con_over, efc_over);
+    s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, caps.maxcon, caps.maxefc,
+                            m.solver == mjh::SOL_NEWTON);
+    s->L.efc_cap = caps.efc_cap;
     s->fields = mjh::layout_fields(m, s->L);
     s->lds_bytes = (size_t)s->L.total * sizeof(float);
     int dev = 0;
@@ -480,14 +490,10 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     hipDeviceProp_t prop;
     HIPOK(hipGetDeviceProperties(&prop, dev));
     s->ncu = prop.multiProcessorCount;
-    if (s->lds_bytes > (size_t)prop.sharedMemPerBlock)
-      throw std::runtime_error("per-env workspace (" + std::to_string(s->lds_bytes) + " B) exceeds the LDS per workgroup");
-    if (s->lds_bytes > 65536) {
-      HIPOK(hipFuncSetAttribute((const void*)k_step<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
-      HIPOK(hipFuncSetAttribute((const void*)k_step<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
-      HIPOK(hipFuncSetAttribute((const void*)k_step<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
-      HIPOK(hipFuncSetAttribute((const void*)k_step_queue, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
-    }
+    // workspaces beyond 64 KB live in device memory (L2-cached) instead of LDS
+    bool use_gws = s->lds_bytes > 65536 || s->lds_bytes > (size_t)prop.sharedMemPerBlock;
+    if (const char* v = std::getenv("MJMLX_HIP_GLOBAL_WORKSPACE")) use_gws = use_gws || std::atoi(v) != 0;
+    if (use_gws) HIPOK(hipMalloc(&s->d_gws, (size_t)s->B * s->L.total * sizeof(float)));
     HIPOK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIPOK(hipEventCreate(&s->ev0));
     HIPOK(hipEventCreate(&s->ev1));

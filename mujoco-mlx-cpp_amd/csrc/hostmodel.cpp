@@ -1,8 +1,10 @@
 // Kernel tables from a compiled model. Static derived data mirrors what the
 // reference precomputes on the host: the collision pair table
 // (reference src/batched.cpp:317-367), per-pair solver properties (:529-548),
-// limit/friction tables (:378-409,458-478) and per-body dof masks (:551-561).
+// limit/friction tables (:378-409,458-478) and per-body dof chains (:551-561).
 #include "hostmodel.h"
+
+#include <cstdlib>
 
 #include <algorithm>
 #include <cstddef>
@@ -80,9 +82,22 @@ int max_rows(const Model& m, const std::vector<CollisionPair>& pairs) {
   return s;
 }
 
+Caps default_caps(const Model& m, const std::vector<CollisionPair>& pairs, int maxcon_override,
+                  int efc_cap_override) {
+  Caps c;
+  int con_cap = maxcon_override > 0 ? maxcon_override : 128;
+  c.efc_cap = efc_cap_override > 0 ? efc_cap_override : (m.nv <= 32 ? 72 : 256);
+  if (const char* v = std::getenv("MJMLX_MAX_EFC")) if (efc_cap_override <= 0 && std::atoi(v) > 0) c.efc_cap = std::atoi(v);
+  c.maxcon = std::max(1, std::min(max_contacts(m, pairs), con_cap));
+  // admitted rows never exceed efc_cap - 1 (a contact starts below efc_cap - 10
+  // and adds at most 10 rows)
+  c.maxefc = std::max(1, std::min(max_rows(m, pairs), c.efc_cap));
+  return c;
+}
+
 Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   using namespace mjk;
-  if (m.nv > 64) throw std::runtime_error("HIP engine: models with nv > 64 are not supported by the dense-row kernel");
+  if (m.nv > 256) throw std::runtime_error("HIP engine: models with nv > 256 are not supported by the dense-row kernel");
   if (m.solver == SOL_PGS) throw std::runtime_error("HIP engine: the PGS solver is not implemented (use Newton or CG)");
   if (m.integrator == INT_RK4) throw std::runtime_error("HIP engine: the RK4 integrator is not implemented");
   std::vector<CollisionPair> pairs = build_pairs(m);
@@ -109,16 +124,12 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
     for (int b = 1; b < nb; b++) if (depth[b] == lv) level_body.push_back(b);
   }
   level_adr[maxd] = (int)level_body.size();
-  // last dof of each body's kinematic chain and the dof mask
+  // last dof of each body's kinematic chain (-1: none)
   std::vector<int> lastdof(nb, -1);
-  std::vector<uint64_t> mask(nb, 0);
   for (int b = 1; b < nb; b++) {
     int bb = b;
     while (bb > 0 && m.body_dofnum[bb] == 0) bb = m.body_parentid[bb];
-    if (bb > 0) {
-      lastdof[b] = m.body_dofadr[bb] + m.body_dofnum[bb] - 1;
-      for (int d = lastdof[b]; d >= 0; d = m.dof_parentid[d]) mask[b] |= (uint64_t)1 << d;
-    }
+    if (bb > 0) lastdof[b] = m.body_dofadr[bb] + m.body_dofnum[bb] - 1;
   }
   // dof tree
   std::vector<int> ddepth(nv, 0), dsend(nv), nchain(nv), chain((size_t)std::max(nv, 1) * MAX_DOF_DEPTH, -1);
@@ -238,7 +249,6 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   ADDP(body_parentid, m.body_parentid); ADDP(body_rootid, m.body_rootid); ADDP(body_jntnum, m.body_jntnum);
   ADDP(body_jntadr, m.body_jntadr); ADDP(body_dofnum, m.body_dofnum); ADDP(body_dofadr, m.body_dofadr);
   ADDP(body_subtree_end, send); ADDP(level_adr, level_adr); ADDP(level_body, level_body); ADDP(body_lastdof, lastdof);
-  ADDP(body_dofmask, mask);
   ADDP(body_pos, tof(m.body_pos)); ADDP(body_quat, tof(m.body_quat)); ADDP(body_ipos, tof(m.body_ipos));
   ADDP(body_iquat, tof(m.body_iquat)); ADDP(body_mass, tof(m.body_mass)); ADDP(body_inertia, tof(m.body_inertia));
   ADDP(body_subtreemass, subtreemass); ADDP(body_invweight0, tof(m.body_invweight0));

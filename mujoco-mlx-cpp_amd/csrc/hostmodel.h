@@ -26,6 +26,15 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out);
 int max_contacts(const Model& m, const std::vector<CollisionPair>& pairs);
 int max_rows(const Model& m, const std::vector<CollisionPair>& pairs);
 
+// Per-env capacities, the reference's (src/batched.cpp:290 MAX_CONTACTS_PER_ENV
+// = 128; src/compat/mkx_kernels/solver.hpp:28-33 row budget 72 for nv <= 32,
+// else 256, MJMLX_MAX_EFC overriding it). Buffers are sized to what the model
+// can produce under those budgets. maxcon_override / efc_cap_override > 0
+// replace the contact cap and the row budget.
+struct Caps { int maxcon, maxefc, efc_cap; };
+Caps default_caps(const Model& m, const std::vector<CollisionPair>& pairs, int maxcon_override = 0,
+                  int efc_cap_override = 0);
+
 // workspace field table for debug dumps: name -> (offset in floats, length, is_int)
 struct LayoutField { const char* name; int off; int len; int kind; };  // kind 0 float, 1 int, 2 double
 std::vector<LayoutField> layout_fields(const Model& m, const mjk::Layout& L);

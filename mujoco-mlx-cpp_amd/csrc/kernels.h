@@ -481,6 +481,21 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
 KDEV void solve_ld(const DModel& m, const float* LD, float* x) {
   int nv = m.nv;
 #if WAVE == 64
+  if (nv > WAVE) {  // more dofs than lanes: one workspace phase per column
+    for (int k = nv - 1; k >= 1; k--) {
+      float xk = x[k];
+      for (int i = LANE; i < k; i += WAVE) x[i] -= LD[k * nv + i] * xk;
+      KSYNC();
+    }
+    for (int i = LANE; i < nv; i += WAVE) x[i] /= LD[i * nv + i];
+    KSYNC();
+    for (int i = 0; i < nv - 1; i++) {
+      float xi = x[i];
+      for (int k = i + 1 + LANE; k < nv; k += WAVE) x[k] -= LD[k * nv + i] * xi;
+      KSYNC();
+    }
+    return;
+  }
   float v = LANE < nv ? x[LANE] : 0.f;
   for (int k = nv - 1; k >= 1; k--) {  // L^T y = x: column k updates rows i < k
     float xk = rl_f(v, k);
@@ -1071,20 +1086,43 @@ KDEV int k_gjk_epa(const KCvx& A, const KCvx& B, float margin, RawC* out) {
   if (dot3(d0, d0) < 1e-20f) { d0[0] = 1.f; d0[1] = d0[2] = 0.f; }
   k_msupport(A, B, d0, &S[0]);
   float v[3] = {S[0].w[0], S[0].w[1], S[0].w[2]}, lam[4] = {1.f, 0.f, 0.f, 0.f};
+  // FP32 resolves the distance to about 1e-6 of the difference body's extent:
+  // the loop stops once the distance bound |v| - v.w/|v| is below 1e-5 of it
+  // (or 1e-6 relative), before a near-coplanar vertex can make a tetrahedron
+  // that seems to hold the origin
+  // An origin within 1e-6 of the extent of the simplex counts as touching.
+  // Every point s of the current simplex has v.s >= v.v, so the new simplex can
+  // only hold the origin when v.w < 0, and |v| only shrinks: a containment
+  // claimed with v.w > 0, or a step that does not shrink |v|, is rounding,
+  // and the previous simplex is kept as the separated answer.
+  float gscale = fmaxf(sqrtf(dot3(v, v)), 1e-6f);
+  float touch = 1e-12f * gscale * gscale;
   int overlap = 0;
   for (int it = 0; it < 64; it++) {
     float vv = dot3(v, v);
-    if (vv < 1e-20f) { overlap = 1; break; }
+    if (vv < touch) { overlap = 1; break; }
     KSv w;
     float nd[3] = {-v[0], -v[1], -v[2]};
     k_msupport(A, B, nd, &w);
-    if (vv - dot3(v, w.w) <= 1e-6f * vv) break;
+    float vw = dot3(v, w.w);
+    if (vv - vw <= fmaxf(1e-6f * vv, 1e-5f * gscale * sqrtf(vv))) break;
     bool dup = false;
     for (int i = 0; i < n; i++)
-      if (fabsf(S[i].w[0] - w.w[0]) + fabsf(S[i].w[1] - w.w[1]) + fabsf(S[i].w[2] - w.w[2]) < 1e-9f) dup = true;
+      if (fabsf(S[i].w[0] - w.w[0]) + fabsf(S[i].w[1] - w.w[1]) + fabsf(S[i].w[2] - w.w[2]) < 1e-7f * gscale) dup = true;
     if (dup) break;
+    KSv Sp[4];
+    float lp[4] = {lam[0], lam[1], lam[2], lam[3]}, vp[3] = {v[0], v[1], v[2]};
+    int np = n;
+    for (int i = 0; i < n; i++) Sp[i] = S[i];
     S[n++] = w;
-    if (k_simplex_nearest(S, &n, lam, v)) { overlap = 1; break; }
+    int inside = k_simplex_nearest(S, &n, lam, v);
+    if (inside && vw < 0.f) { overlap = 1; break; }
+    if (inside || !(dot3(v, v) < vv)) {
+      n = np;
+      for (int i = 0; i < np; i++) { S[i] = Sp[i]; lam[i] = lp[i]; }
+      for (int q = 0; q < 3; q++) v[q] = vp[q];
+      break;
+    }
   }
   if (!overlap) {
     float pa[3] = {0.f, 0.f, 0.f}, pb[3] = {0.f, 0.f, 0.f};
@@ -1152,19 +1190,33 @@ KDEV int k_gjk_epa(const KCvx& A, const KCvx& B, float margin, RawC* out) {
     if (dot3(nn, eo) > 0.f) k_epa_face(V, a, c, b, F, Fn, &nf);
     else k_epa_face(V, a, b, c, F, Fn, &nf);
   }
-  int best = 0;
+  // FP32 termination: the support gap is resolved to about 1e-7 of the
+  // difference body's extent, so the loop stops at a gap of 1e-6 of it (or
+  // 1e-5 of the depth); the nearest-face distance is a lower bound on the depth
+  // that only grows in exact arithmetic, so a drop marks rounding trouble and
+  // the previous face is kept
+  float scale = 0.f;
+  for (int i = 0; i < 4; i++) scale = fmaxf(scale, sqrtf(dot3(V[i].w, V[i].w)));
+  float gtol = 1e-6f * scale, vtol = 1e-7f * scale;
+  int best = 0, pv[3] = {0, 0, 0};
+  float pn[4] = {0.f, 0.f, 0.f, -3.0e38f};
   for (int it = 0; it < 48; it++) {
     best = 0;
     for (int f = 1; f < nf; f++) if (Fn[f][3] < Fn[best][3]) best = f;
+    if (Fn[best][3] < pn[3] - gtol) { best = -1; break; }
+    for (int q = 0; q < 3; q++) pv[q] = F[best][q];
+    for (int q = 0; q < 4; q++) pn[q] = Fn[best][q];
     KSv w;
     k_msupport(A, B, Fn[best], &w);
-    float gap = dot3(Fn[best], w.w) - Fn[best][3];
-    if (gap <= 1e-5f * (1e-3f + Fn[best][3]) || nvtx >= KEPA_MAXV) break;
+    const float* wa = V[F[best][0]].w;
+    float dw[3] = {w.w[0] - wa[0], w.w[1] - wa[1], w.w[2] - wa[2]};
+    float gap = dot3(Fn[best], dw);
+    if (gap <= fmaxf(1e-5f * Fn[best][3], gtol) || nvtx >= KEPA_MAXV) break;
     int E[KEPA_MAXF][2], ne = 0, keep = 0;
     bool full = false;
     for (int f = 0; f < nf; f++) {
       float rel[3] = {w.w[0] - V[F[f][0]].w[0], w.w[1] - V[F[f][0]].w[1], w.w[2] - V[F[f][0]].w[2]};
-      if (dot3(Fn[f], rel) > 1e-9f) {
+      if (dot3(Fn[f], rel) > vtol) {
         for (int e = 0; e < 3; e++) {
           int a = F[f][e], b = F[f][(e + 1) % 3], found = -1;
           for (int q = 0; q < ne; q++) if (E[q][0] == b && E[q][1] == a) { found = q; break; }
@@ -1181,15 +1233,17 @@ KDEV int k_gjk_epa(const KCvx& A, const KCvx& B, float margin, RawC* out) {
       }
     }
     nf = keep;
-    if (full || nf + ne > KEPA_MAXF) break;
+    if (full || nf + ne > KEPA_MAXF) { best = -1; break; }
     V[nvtx] = w;
     for (int q = 0; q < ne; q++) k_epa_face(V, E[q][0], E[q][1], nvtx, F, Fn, &nf);
     nvtx++;
+    best = -1;  // the polytope changed: the last accepted face stands unless a new one is chosen
   }
-  for (int f = 1; f < nf; f++) if (Fn[f][3] < Fn[best][3]) best = f;
-  float depth = Fn[best][3];
-  float p[3] = {Fn[best][0] * depth, Fn[best][1] * depth, Fn[best][2] * depth};
-  const float *a = V[F[best][0]].w, *b = V[F[best][1]].w, *c = V[F[best][2]].w;
+  // the face used: the last accepted nearest face (pv, pn)
+  (void)best;
+  float depth = pn[3];
+  float p[3] = {pn[0] * depth, pn[1] * depth, pn[2] * depth};
+  const float *a = V[pv[0]].w, *b = V[pv[1]].w, *c = V[pv[2]].w;
   float v0[3], v1[3], v2[3];
   for (int q = 0; q < 3; q++) { v0[q] = b[q] - a[q]; v1[q] = c[q] - a[q]; v2[q] = p[q] - a[q]; }
   float d00 = dot3(v0, v0), d01 = dot3(v0, v1), d11 = dot3(v1, v1), d20 = dot3(v2, v0), d21 = dot3(v2, v1);
@@ -1198,19 +1252,19 @@ KDEV int k_gjk_epa(const KCvx& A, const KCvx& B, float margin, RawC* out) {
   float lu = 1.f - lv - lw;
   float pa[3], pb[3];
   for (int q = 0; q < 3; q++) {
-    pa[q] = lu * V[F[best][0]].a[q] + lv * V[F[best][1]].a[q] + lw * V[F[best][2]].a[q];
-    pb[q] = lu * V[F[best][0]].b[q] + lv * V[F[best][1]].b[q] + lw * V[F[best][2]].b[q];
+    pa[q] = lu * V[pv[0]].a[q] + lv * V[pv[1]].a[q] + lw * V[pv[2]].a[q];
+    pb[q] = lu * V[pv[0]].b[q] + lv * V[pv[1]].b[q] + lw * V[pv[2]].b[q];
   }
   // the supports along the normal give the depth h_A(n) + h_B(-n) to second
   // order in the normal's error; the points move onto the supporting planes
   KSv sw;
-  k_msupport(A, B, Fn[best], &sw);
-  float ha = dot3(Fn[best], sw.a), hb = dot3(Fn[best], sw.b);
-  float ea = ha - dot3(Fn[best], pa), eb = hb - dot3(Fn[best], pb);
+  k_msupport(A, B, pn, &sw);
+  float ha = dot3(pn, sw.a), hb = dot3(pn, sw.b);
+  float ea = ha - dot3(pn, pa), eb = hb - dot3(pn, pb);
   out->dist = -(ha - hb);
   for (int q = 0; q < 3; q++) {
-    out->n[q] = Fn[best][q];
-    out->pos[q] = 0.5f * (pa[q] + ea * Fn[best][q] + pb[q] + eb * Fn[best][q]);
+    out->n[q] = pn[q];
+    out->pos[q] = 0.5f * (pa[q] + ea * pn[q] + pb[q] + eb * pn[q]);
   }
   return 1;
 }
@@ -1389,8 +1443,13 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
     return 0;
   }
   // dof friction rows
+  // row budget (reference src/compat/mkx_kernels/solver.hpp:299,349,403):
+  // friction and limit rows while fewer than efc_cap - 4 rows exist, a
+  // contact while fewer than efc_cap - 10
+  int lim_cap = L.efc_cap - 4 < L.maxefc ? L.efc_cap - 4 : L.maxefc;
+  if (lim_cap < 0) lim_cap = 0;
   int nf = (m.disableflags & DS_FRICTIONLOSS) ? 0 : m.nfloss;
-  if (nf > L.maxefc) { nf = L.maxefc; st |= ST_EFC_OVERFLOW; }
+  if (nf > lim_cap) { nf = lim_cap; st |= ST_EFC_OVERFLOW; }
   for (int t = LANE; t < nf; t += WAVE) {
     int k = m.floss_dof[t];
     etype[t] = EFC_FRICTION_DOF; eid[t] = k; epos[t] = 0.f; emarg[t] = 0.f; floss[t] = m.dof_frictionloss[k];
@@ -1414,12 +1473,12 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
       }
       int off = wscan_excl(cnt), tot = wsumi(cnt);
       int r = nrow + off;
-      if (s0 && r < L.maxefc) { etype[r] = EFC_LIMIT_JOINT; eid[r] = 2 * j; epos[r] = d0; emarg[r] = m.jnt_margin[j]; floss[r] = 0.f; }
+      if (s0 && r < lim_cap) { etype[r] = EFC_LIMIT_JOINT; eid[r] = 2 * j; epos[r] = d0; emarg[r] = m.jnt_margin[j]; floss[r] = 0.f; }
       if (s0) r++;
-      if (s1 && r < L.maxefc) { etype[r] = EFC_LIMIT_JOINT; eid[r] = 2 * j + 1; epos[r] = d1; emarg[r] = m.jnt_margin[j]; floss[r] = 0.f; }
+      if (s1 && r < lim_cap) { etype[r] = EFC_LIMIT_JOINT; eid[r] = 2 * j + 1; epos[r] = d1; emarg[r] = m.jnt_margin[j]; floss[r] = 0.f; }
       nrow += tot;
     }
-    if (nrow > L.maxefc) { nrow = L.maxefc; st |= ST_EFC_OVERFLOW; }
+    if (nrow > lim_cap) { nrow = lim_cap; st |= ST_EFC_OVERFLOW; }
   }
   // contact rows: count per contact, compact
   int ncon = misc[MISC_NCON];
@@ -1440,7 +1499,7 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
       if (c < ncon) {
         float* cr = con + CON_STRIDE * c;
         int r0 = nrow + off;
-        if (r0 + cnt > L.maxefc) { if (cnt > 0) dropr0 = r0; cnt = 0; }
+        if (cnt > 0 && (r0 >= L.efc_cap - 10 || r0 + cnt > L.maxefc)) { dropr0 = r0; cnt = 0; }
         iptr(cr, CON_NROW)[0] = cnt;
         iptr(cr, CON_ROW0)[0] = r0;
         int p = iptr(cr, CON_PAIR)[0];
@@ -1483,14 +1542,15 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
     int b1 = m.pair_b1[p], b2 = m.pair_b2[p];
     float jp[3] = {0.f, 0.f, 0.f}, jr[3] = {0.f, 0.f, 0.f};
     const float* cd = cdof + 6 * k;
-    uint64_t bit = (uint64_t)1 << k;
-    if (m.body_dofmask[b2] & bit) {
+    // dof k moves body b iff it is in the chain of b's last dof: k <= d < subtree_end(k)
+    int d2 = m.body_lastdof[b2], d1 = m.body_lastdof[b1];
+    if (d2 >= k && d2 < m.dof_subtree_end[k]) {
       const float* cc = scom + 3 * m.body_rootid[b2];
       float off[3] = {cr[0] - cc[0], cr[1] - cc[1], cr[2] - cc[2]}, tt[3];
       cross3(cd, off, tt);
       for (int i = 0; i < 3; i++) { jp[i] += cd[3 + i] + tt[i]; jr[i] += cd[i]; }
     }
-    if (m.body_dofmask[b1] & bit) {
+    if (d1 >= k && d1 < m.dof_subtree_end[k]) {
       const float* cc = scom + 3 * m.body_rootid[b1];
       float off[3] = {cr[0] - cc[0], cr[1] - cc[1], cr[2] - cc[2]}, tt[3];
       cross3(cd, off, tt);
@@ -1717,10 +1777,45 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
   double* idg = dptr(w, L.dvec);
   const float *M = w + L.qM, *J = w + L.efc_J, *Dact = w + L.efc_Dact;
   int ntri = nv * (nv + 1) / 2;
-  for (int t = LANE; t < ntri; t += WAVE) {
-    int i, j;
-    tri_rc(t, &i, &j);
-    H[i * nv + j] = (double)(M[i * nv + j] + kdot3(Dact, J + i, nv, J + j, nv, nefc));
+#if WAVE == 64
+  if (nv <= 32) {
+    // J^T diag(Dact) J on the matrix cores: 16 x 16 x 4 FP32 MFMA tiles over the
+    // 32 x 32 padding, four constraint rows per step. Operand layout: lane l
+    // supplies A[l % 16][l / 16] (= J[r + l / 16][row]) and B[l / 16][l % 16]
+    // (= Dact J[r + l / 16][col]) and holds D[4 (l / 16) + q][l % 16].
+    typedef float v4f_t __attribute__((ext_vector_type(4)));
+    v4f_t t00 = {0.f, 0.f, 0.f, 0.f}, t10 = t00, t11 = t00;
+    int lr = LANE & 15, lk = LANE >> 4;
+    for (int r0 = 0; r0 < nefc; r0 += 4) {
+      int r = r0 + lk;
+      bool in = r < nefc;
+      float d = in ? Dact[r] : 0.f;
+      float a0 = (in && lr < nv) ? J[r * nv + lr] : 0.f;
+      float a1 = (in && lr + 16 < nv) ? J[r * nv + 16 + lr] : 0.f;
+      float b0 = a0 * d, b1 = a1 * d;
+      t00 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, t00, 0, 0, 0);
+      if (nv > 16) {
+        t10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, t10, 0, 0, 0);
+        t11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, t11, 0, 0, 0);
+      }
+    }
+    int col = LANE & 15, row0 = 4 * (LANE >> 4);
+    for (int q = 0; q < 4; q++) {
+      int i = row0 + q;
+      if (i < nv && col <= i) H[i * nv + col] = (double)(M[i * nv + col] + t00[q]);
+      if (16 + i < nv && col < nv) H[(16 + i) * nv + col] = (double)(M[(16 + i) * nv + col] + t10[q]);
+      if (16 + i < nv && 16 + col <= 16 + i) H[(16 + i) * nv + 16 + col] = (double)(M[(16 + i) * nv + 16 + col] + t11[q]);
+    }
+    KSYNC();
+  } else
+#endif
+  {
+    for (int t = LANE; t < ntri; t += WAVE) {
+      int i, j;
+      tri_rc(t, &i, &j);
+      H[i * nv + j] = (double)(M[i * nv + j] + kdot3(Dact, J + i, nv, J + j, nv, nefc));
+    }
+    KSYNC();
   }
   if (m.cone != 0) {
     // contacts in the middle zone of the elliptic cone: H += J_c^T Hc J_c with
@@ -1821,6 +1916,26 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
   const float* grad = w + L.grad;
   float* Mg = w + L.Mgrad;
 #if WAVE == 64
+  if (nv > WAVE) {
+    double* y = dptr(w, L.dvec) + nv;
+    for (int i = LANE; i < nv; i += WAVE) y[i] = grad[i];
+    KSYNC();
+    for (int k = 0; k < nv - 1; k++) {
+      double yk = y[k] * idg[k];
+      for (int i = k + 1 + LANE; i < nv; i += WAVE) y[i] -= H[i * nv + k] * yk;
+      KSYNC();
+    }
+    for (int i = LANE; i < nv; i += WAVE) y[i] *= idg[i];
+    KSYNC();
+    for (int k = nv - 1; k >= 1; k--) {
+      double xk = y[k];
+      for (int i = LANE; i < k; i += WAVE) y[i] -= H[k * nv + i] * idg[i] * xk;
+      KSYNC();
+    }
+    for (int i = LANE; i < nv; i += WAVE) Mg[i] = (float)y[i];
+    KSYNC();
+    return;
+  }
   double v = LANE < nv ? (double)grad[LANE] : 0.0;
   double id = LANE < nv ? idg[LANE] : 1.0;
   for (int k = 0; k < nv - 1; k++) {          // L y = g, L_ik = H_ik / H_kk
@@ -1835,7 +1950,7 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
   KSYNC();
   if (LANE < nv) Mg[LANE] = (float)v;
 #else
-  double y[64];
+  double* y = dptr(w, L.dvec) + nv;
   for (int i = 0; i < nv; i++) y[i] = grad[i];
   for (int k = 0; k < nv - 1; k++)
     for (int i = k + 1; i < nv; i++) y[i] -= H[i * nv + k] * idg[k] * y[k];

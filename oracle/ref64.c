@@ -48,6 +48,7 @@ struct orc_prep {
   double* dof_invweight0;
   double* body_subtreemass;
   double meaninertia;
+  int con_cap, efc_cap; /* 0: unbounded (ORC_MAXCON / ORC_MAXEFC) */
 };
 
 struct orc_data {
@@ -333,6 +334,15 @@ orc_prep_t* orc_prepare(const orc_model_t* m) {
   free(J); free(e); free(Minv);
   orc_data_free(d);
   return p;
+}
+
+/* Per-env capacities of the reference (src/batched.cpp:290 contacts;
+ * src/compat/mkx_kernels/solver.hpp:299,349,403 row budget: friction and limit
+ * rows while fewer than efc_cap - 4 rows exist, a contact while fewer than
+ * efc_cap - 10). 0 leaves a cap unbounded. */
+void orc_set_caps(orc_prep_t* p, int con_cap, int efc_cap) {
+  p->con_cap = con_cap;
+  p->efc_cap = efc_cap;
 }
 
 void orc_free(orc_prep_t* p) {
@@ -1366,7 +1376,7 @@ static int collision(const orc_prep_t* p, orc_data_t* d) {
     int n = orc_collide_pair(p, pi, x1, d->geom_xmat + 9 * pr->g1, x2, d->geom_xmat + 9 * pr->g2, tmp, 8);
     if (n < 0) { status |= 4; continue; }
     for (int k = 0; k < n; k++) {
-      if (d->ncon >= ORC_MAXCON) { status |= 1; break; }
+      if (d->ncon >= (p->con_cap > 0 && p->con_cap < ORC_MAXCON ? p->con_cap : ORC_MAXCON)) { status |= 1; break; }
       d->con[d->ncon++] = tmp[k];
     }
   }
@@ -1436,6 +1446,7 @@ static int make_constraint(const orc_prep_t* p, orc_data_t* d) {
   if (!(m->disableflags & DS_FRICTIONLOSS)) {
     for (int k = 0; k < nv; k++) {
       if (m->dof_frictionloss[k] <= 0) continue;
+      if (p->efc_cap > 0 && d->nefc >= p->efc_cap - 4) { status |= 2; break; }
       int r = add_row(d, nv, EFC_FRICTION_DOF, k, 0, 0, p->dof_invweight0[k], m->dof_frictionloss[k]);
       if (r < 0) { status |= 2; break; }
       d->efc_J[(size_t)r * nv + k] = 1;
@@ -1452,6 +1463,7 @@ static int make_constraint(const orc_prep_t* p, orc_data_t* d) {
       for (int side = 0; side < 2; side++) {
         double dist = side == 0 ? v - m->jnt_range[2 * j] : m->jnt_range[2 * j + 1] - v;
         if (dist < m->jnt_margin[j]) {
+          if (p->efc_cap > 0 && d->nefc >= p->efc_cap - 4) { status |= 2; break; }
           int r = add_row(d, nv, EFC_LIMIT_JOINT, j, dist, m->jnt_margin[j], p->dof_invweight0[da], 0);
           if (r < 0) { status |= 2; break; }
           d->efc_J[(size_t)r * nv + da] = side == 0 ? 1.0 : -1.0;
@@ -1466,6 +1478,7 @@ static int make_constraint(const orc_prep_t* p, orc_data_t* d) {
     for (int ci = 0; ci < d->ncon; ci++) {
       orc_contact_t* c = &d->con[ci];
       if (!(c->dist < c->includemargin)) continue;
+      if (p->efc_cap > 0 && d->nefc >= p->efc_cap - 10) { status |= 2; break; }
       const orc_pair_t* pr = &p->pairs[c->pair];
       memset(jac, 0, sizeof(double) * 6 * nv);
       jac_point(p, d, pr->b2, c->pos, jac, jac + 3 * nv, 1.0);

@@ -62,6 +62,7 @@ typedef struct orc_prep orc_prep_t;
 
 orc_prep_t* orc_prepare(const orc_model_t* m);
 void orc_free(orc_prep_t* p);
+void orc_set_caps(orc_prep_t* p, int con_cap, int efc_cap);
 int orc_npair(const orc_prep_t* p);
 const orc_pair_t* orc_pairs(const orc_prep_t* p);
 const double* orc_body_invweight0(const orc_prep_t* p);

@@ -87,6 +87,7 @@ def lib() -> C.CDLL:
         L.orc_prepare.restype = C.c_void_p
         L.orc_prepare.argtypes = [C.POINTER(OrcModel)]
         L.orc_free.argtypes = [C.c_void_p]
+        L.orc_set_caps.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_npair.argtypes = [C.c_void_p]
         L.orc_pairs.restype = C.POINTER(OrcPair)
         L.orc_pairs.argtypes = [C.c_void_p]
@@ -168,6 +169,14 @@ class Oracle:
                 L.orc_free(self.p)
         except Exception:
             pass
+
+    def set_caps(self, con_cap: int = 128, efc_cap: int | None = None):
+        """Per-env contact cap and row budget (0: unbounded); the defaults are
+        the reference's: 128 contacts, 72 rows for nv <= 32, else 256."""
+        if efc_cap is None:
+            efc_cap = 72 if self.nv <= 32 else 256
+        lib().orc_set_caps(self.p, int(con_cap), int(efc_cap))
+        return self
 
     # ---- derived tables
     def pairs(self):

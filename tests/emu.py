@@ -39,6 +39,7 @@ def lib():
         L.mjemu_model_field.argtypes = [C.c_void_p, C.c_char_p, _D, C.c_int]
         L.mjemu_features.argtypes = [C.c_void_p, _F]
         L.mjemu_set_pred.argtypes = [C.c_void_p, _F]
+        L.mjemu_narrow.argtypes = [C.c_int, C.c_int, _F, _F, _F, _F, _F, _F, C.c_float, _F]
         L.mjemu_compile.restype = C.c_void_p
         L.mjemu_compile.argtypes = [C.c_char_p, C.c_int, C.c_int]
         _lib = L
@@ -119,6 +120,17 @@ class EmuSim:
         if kind == 1:
             return seg.view(np.int32)
         return seg
+
+
+def narrow(t1, t2, s1, s2, x1, R1, x2, R2, margin):
+    """The kernel's narrow phase for one pair: list of (dist, pos, normal)."""
+    f = lambda a: np.ascontiguousarray(np.asarray(a, np.float32).ravel())
+    args = [f(a) for a in (s1, s2, x1, R1, x2, R2)]
+    out = np.zeros(56, np.float32)
+    n = lib().mjemu_narrow(int(t1), int(t2), *(_fp(a) for a in args), float(margin), _fp(out))
+    if n < 0:
+        raise ValueError("unsupported pair")
+    return [(float(out[7 * k]), out[7 * k + 1:7 * k + 4].copy(), out[7 * k + 4:7 * k + 7].copy()) for k in range(n)]
 
 
 class CompiledModel:

@@ -160,23 +160,31 @@ def test_contact_points_lie_on_both_surfaces(orc):
     assert nface > 20 and nedge > 5
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_interpreter_matches_oracle_on_box_scene(seed):
-    xml = scenes.boxes_xml(6, seed, solver="Newton")
-    B = 16
-    sim = emu.EmuSim(xml, B, maxcon=64, maxefc=256)
+@pytest.mark.parametrize("nbox,solver,seed", [(6, "Newton", 0), (6, "Newton", 1), (14, "Newton", 2), (32, "CG", 3)])
+def test_interpreter_matches_oracle_on_box_scene(nbox, solver, seed):
+    """Kernel source (serial interpreter) against the oracle, both under the
+    reference's capacities (128 contacts; row budget 72 for nv <= 32, else 256:
+    a contact gets rows while fewer than 246 rows exist). 14 boxes have nv 84,
+    beyond one wave's 64 lanes; 32 boxes (nv 192) overflow the row budget."""
+    xml = scenes.boxes_xml(nbox, seed, solver=solver, iterations=50 if solver == "Newton" else 16)
+    B = 16 if nbox <= 14 else 8
+    sim = emu.EmuSim(xml, B)
     fields = ref64.fields_from(sim.field)
-    orc = ref64.Oracle(fields)
+    orc = ref64.Oracle(fields).set_caps()
     rng = np.random.default_rng(seed)
     q = np.tile(sim.field("qpos0"), (B, 1))
     qv = np.zeros((B, sim.nv))
     qa = np.zeros((B, sim.nv))
-    lens = rng.integers(20, 120, size=B)  # drop and let the boxes settle for different times
+    # drop and let the boxes settle for different times; 32 boxes are taken
+    # around the step where the landed boxes' contacts outgrow the row budget
+    # (later, boxes whose contacts get no rows sink through the floor)
+    lens, nsub = (rng.integers(20, 120, size=B), 4) if nbox <= 14 else (rng.integers(95, 125, size=B), 1)
     for k in range(int(lens.max())):
         idx = np.nonzero(lens > k)[0]
         qs, vs, as_ = q[idx].copy(), qv[idx].copy(), qa[idx].copy()
-        orc.batch_step(qs, vs, as_, np.zeros((len(idx), 0)), 4)
+        orc.batch_step(qs, vs, as_, np.zeros((len(idx), 0)), nsub)
         q[idx], qv[idx], qa[idx] = qs, vs, as_
+    assert np.isfinite(qv).all()
     f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
     q, qv, qa = f32(q), f32(qv), f32(qa)
     sim.set_state(q, qv, qa)
@@ -184,14 +192,21 @@ def test_interpreter_matches_oracle_on_box_scene(seed):
     st = sim.stats()
     assert not (st["status"] & ~3).any(), st["status"]
     ncon = worst = 0
+    nover = 0
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32))
         con = sim.dump(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
         ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4)
         assert ok, f"env {e}: {msg}"
         ncon += len(con)
+        assert st["nefc"][e] == orc.nefc(), (e, st["nefc"][e], orc.nefc())
+        nover += bool(st["status"][e] & 2)
         b = orc.get("qacc")
         worst = max(worst, np.max(np.abs(sim.dump(2, e, "qacc") - b)) / max(1.0, np.max(np.abs(b))))
-    print(f"contacts {ncon}, worst qacc rel err {worst:.2e}")
+    print(f"nbox {nbox} nv {sim.nv}: contacts {ncon}, row-budget overflows {nover}, worst qacc rel err {worst:.2e}")
     assert ncon > B
-    assert worst < 1e-3
+    if nbox == 32:
+        assert nover > 0 and st["nefc"].max() <= 249
+    # Newton converges to the 1e-8 scaled residual; CG stops on its improvement
+    # test at an iterate that FP32 and FP64 reach at different iterations
+    assert worst < (1e-3 if solver == "Newton" else 5e-3)

@@ -13,26 +13,31 @@ pytestmark = pytest.mark.gpu
 
 
 def _settled_states(model, B, seed, nbox):
-    orc = ref64.Oracle(ref64.fields_from(model.field))
+    orc = ref64.Oracle(ref64.fields_from(model.field)).set_caps()
     rng = np.random.default_rng(seed)
     q = np.tile(model.field("qpos0"), (B, 1))
     qv = np.zeros((B, model.nv))
     qa = np.zeros((B, model.nv))
-    lens = rng.integers(20, 120, size=B)
+    # 32 boxes are taken around the step where the landed boxes' contacts
+    # outgrow the row budget (later, boxes whose contacts get no rows sink
+    # through the floor)
+    lens, nsub = (rng.integers(20, 120, size=B), 4) if nbox <= 14 else (rng.integers(95, 125, size=B), 1)
     for k in range(int(lens.max())):
         idx = np.nonzero(lens > k)[0]
         qs, vs, as_ = q[idx].copy(), qv[idx].copy(), qa[idx].copy()
-        orc.batch_step(qs, vs, as_, np.zeros((len(idx), 0)), 4)
+        orc.batch_step(qs, vs, as_, np.zeros((len(idx), 0)), nsub)
         q[idx], qv[idx], qa[idx] = qs, vs, as_
+    assert np.isfinite(qv).all()
     f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
     return f32(q), f32(qv), f32(qa), orc
 
 
-@pytest.mark.parametrize("solver", ["Newton", "CG"])
-def test_box_scene_single_step_parity(solver):
-    nbox = 6
-    model = capi.Model(scenes.boxes_xml(nbox, 3, solver=solver, iterations=50))
-    B = 32
+@pytest.mark.parametrize("nbox,solver", [(6, "Newton"), (6, "CG"), (14, "Newton"), (32, "CG")])
+def test_box_scene_single_step_parity(nbox, solver):
+    """6 boxes run from LDS; 14 (nv 84) and 32 (nv 192, row budget overflowing)
+    from the device-memory workspace. The oracle applies the same capacities."""
+    model = capi.Model(scenes.boxes_xml(nbox, 3, solver=solver, iterations=50 if solver == "Newton" else 16))
+    B = 32 if nbox <= 14 else 16
     q, qv, qa, orc = _settled_states(model, B, 4, nbox)
     sim = capi.BatchedSim(model, B)
     sim.set_state(q, qv)
@@ -50,8 +55,33 @@ def test_box_scene_single_step_parity(solver):
         ncon += len(con)
         b = orc.get("qacc")
         worst = max(worst, np.max(np.abs(dump.get(2, e, "qacc") - b)) / max(1.0, np.max(np.abs(b))))
-    print(f"{solver}: contacts {ncon}, worst qacc rel err {worst:.2e}, mean iterations {st['niter'].mean():.2f}")
+        assert st["nefc"][e] == orc.nefc(), (e, st["nefc"][e], orc.nefc())
+    print(f"nbox {nbox} {solver}: contacts {ncon}, worst qacc rel err {worst:.2e}, "
+          f"mean iterations {st['niter'].mean():.2f}, lds {sim.capacity()}")
     assert ncon > B
     # Newton converges to the 1e-8 scaled residual; CG stops on its improvement
     # test at an iterate that FP32 and FP64 reach at different iterations
     assert worst < (1e-3 if solver == "Newton" else 5e-3)
+
+
+def test_device_workspace_matches_lds(monkeypatch):
+    """The same kernel with its per-env workspace in device memory instead of
+    LDS runs the same arithmetic: 20 steps of the quadruped agree bit for bit."""
+    model = capi.Model(scenes.quadruped_xml())
+    B = 256
+    q0 = scenes.quadruped_initial_qpos(B, 3)
+    rng = np.random.default_rng(3)
+    acts = [scenes.quadruped_actions(B, rng).astype(np.float32) for _ in range(20)]
+    out = []
+    for gws in ("0", "1"):
+        monkeypatch.setenv("MJMLX_HIP_GLOBAL_WORKSPACE", gws)
+        sim = capi.BatchedSim(model, B)
+        sim.set_state(q0.astype(np.float32), np.zeros((B, model.nv), np.float32))
+        for a in acts:
+            sim.step(a)
+        out.append(sim.state())
+        st = sim.stats()
+        assert not (st["status"] & ~3).any()
+        del sim
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
