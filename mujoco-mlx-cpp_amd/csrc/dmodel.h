@@ -75,8 +75,10 @@ struct DModel {
 struct Layout {
   // maxcon / maxefc: buffer sizes; efc_cap: the row budget that admits rows
   // (limit rows while fewer than efc_cap - 4 rows exist, a contact while fewer
-  // than efc_cap - 10), maxefc >= min(rows the model can produce, efc_cap)
-  int maxcon, maxefc, efc_cap, nv, total;
+  // than efc_cap - 10), maxefc >= min(rows the model can produce, efc_cap);
+  // con_cap: contacts counted per env (maxcon <= con_cap of them are stored:
+  // at least every contact the row budget can admit)
+  int maxcon, maxefc, efc_cap, con_cap, nv, total;
   int qpos, qvel, ctrl, qacc_ws, qacc_pred, qacc, qM, qLD, qfrc_smooth, qacc_smooth, qfrc_constraint, act_force, misc;
   int xpos, xquat, xmat, xipos, xanchor, xaxis, gxpos, gxmat, scom, cinert, cdof, cvel, cdofdot, crb, cacc, cfrc, buf6;
   int H;
@@ -87,7 +89,7 @@ struct Layout {
 
 // misc ints (stored as int bits in the float workspace)
 enum { MISC_NCON = 0, MISC_NEFC = 1, MISC_STATUS = 2, MISC_NITER = 3, MISC_NFLOSSROWS = 4, MISC_NLIMROWS = 5,
-       MISC_CONSUB = 6, MISC_SIZE = 8 };
+       MISC_CONSUB = 6, MISC_NCON_ALL = 7, MISC_SIZE = 8 };
 
 #if defined(__HIPCC__) || defined(__CUDACC__)
 #define MJK_HD __host__ __device__
@@ -104,7 +106,7 @@ MJK_HD inline int al2(int x) { return (x + 1) & ~1; }
 MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, int ngeom, int maxcon, int maxefc,
                                  bool qld_shared = false) {
   Layout L;
-  L.maxcon = maxcon; L.maxefc = maxefc; L.efc_cap = maxefc; L.nv = nv;
+  L.maxcon = maxcon; L.maxefc = maxefc; L.efc_cap = maxefc; L.con_cap = maxcon; L.nv = nv;
   int nvp = nv > 0 ? nv : 1, nup = nu > 0 ? nu : 1, njp = njnt > 0 ? njnt : 1, ngp = ngeom > 0 ? ngeom : 1;
   int o = 0;
   L.qpos = o; o += al2(nq > 0 ? nq : 1);

@@ -48,6 +48,7 @@ __attribute__((visibility("default"))) void* mjemu_create(const char* xml, int s
     s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, caps.maxcon, caps.maxefc,
                             m.solver == mjh::SOL_NEWTON);
     s->L.efc_cap = caps.efc_cap;
+    s->L.con_cap = caps.con_cap;
     s->fields = mjh::layout_fields(m, s->L);
     s->B = B;
     s->qpos.resize((size_t)B * m.nq);

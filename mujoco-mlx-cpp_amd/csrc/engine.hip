@@ -483,6 +483,7 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, caps.maxcon, caps.maxefc,
                             m.solver == mjh::SOL_NEWTON);
     s->L.efc_cap = caps.efc_cap;
+    s->L.con_cap = caps.con_cap;
     s->fields = mjh::layout_fields(m, s->L);
     s->lds_bytes = (size_t)s->L.total * sizeof(float);
     int dev = 0;

@@ -88,10 +88,22 @@ Caps default_caps(const Model& m, const std::vector<CollisionPair>& pairs, int m
   int con_cap = maxcon_override > 0 ? maxcon_override : 128;
   c.efc_cap = efc_cap_override > 0 ? efc_cap_override : (m.nv <= 32 ? 72 : 256);
   if (const char* v = std::getenv("MJMLX_MAX_EFC")) if (efc_cap_override <= 0 && std::atoi(v) > 0) c.efc_cap = std::atoi(v);
+  c.con_cap = con_cap;
   c.maxcon = std::max(1, std::min(max_contacts(m, pairs), con_cap));
   // admitted rows never exceed efc_cap - 1 (a contact starts below efc_cap - 10
   // and adds at most 10 rows)
   c.maxefc = std::max(1, std::min(max_rows(m, pairs), c.efc_cap));
+  // when every stored contact is active (no pair with a gap), contacts get
+  // rows in list order until efc_cap - 10 rows exist: with r rows per contact
+  // at least, no contact past the first ceil((efc_cap - 10) / r) gets rows
+  bool gap = false;
+  int rmin = 10;
+  for (auto& p : pairs) {
+    gap = gap || p.gap > 0;
+    int per = p.condim == 1 ? 1 : (m.cone == CONE_PYRAMIDAL ? 2 * (p.condim - 1) : p.condim);
+    rmin = std::min(rmin, per);
+  }
+  if (!gap && rmin > 0) c.maxcon = std::max(1, std::min(c.maxcon, (std::max(c.efc_cap - 10, 0) + rmin - 1) / rmin));
   return c;
 }
 

@@ -29,9 +29,10 @@ int max_rows(const Model& m, const std::vector<CollisionPair>& pairs);
 // Per-env capacities, the reference's (src/batched.cpp:290 MAX_CONTACTS_PER_ENV
 // = 128; src/compat/mkx_kernels/solver.hpp:28-33 row budget 72 for nv <= 32,
 // else 256, MJMLX_MAX_EFC overriding it). Buffers are sized to what the model
-// can produce under those budgets. maxcon_override / efc_cap_override > 0
+// can produce under those budgets: contacts beyond the ones the row budget
+// can admit are counted, not stored. maxcon_override / efc_cap_override > 0
 // replace the contact cap and the row budget.
-struct Caps { int maxcon, maxefc, efc_cap; };
+struct Caps { int maxcon, maxefc, efc_cap, con_cap; };
 Caps default_caps(const Model& m, const std::vector<CollisionPair>& pairs, int maxcon_override = 0,
                   int efc_cap_override = 0);
 

@@ -1363,7 +1363,7 @@ KDEV int st_collision(const DModel& m, const Layout& L, float* w) {
   int* misc = iptr(w, L.misc);
   int st = 0;
   if (m.disableflags & (DS_CONTACT | DS_CONSTRAINT)) {
-    if (LANE == 0) misc[MISC_NCON] = 0;
+    if (LANE == 0) { misc[MISC_NCON] = 0; misc[MISC_NCON_ALL] = 0; }
     KSYNC();
     return 0;
   }
@@ -1403,8 +1403,12 @@ KDEV int st_collision(const DModel& m, const Layout& L, float* w) {
     }
     base += tot;
   }
-  if (base > L.maxcon) { st |= ST_CON_OVERFLOW; base = L.maxcon; }
-  if (LANE == 0) misc[MISC_NCON] = base;
+  // all contacts count up to con_cap (the reference's list cap); the first
+  // maxcon, every one the row budget can admit, are stored
+  int all = base;
+  if (all > L.con_cap) { st |= ST_CON_OVERFLOW; all = L.con_cap; }
+  if (all > L.maxcon) st |= ST_EFC_OVERFLOW;  // unstored contacts are past the row budget
+  if (LANE == 0) { misc[MISC_NCON] = base < L.maxcon ? base : L.maxcon; misc[MISC_NCON_ALL] = all; }
   KSYNC();
   return st;
 }
@@ -2487,7 +2491,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
       for (int i = 0; i < KP_N; i++) io.prof[(size_t)env * KP_N + i] = pa.t[i];
     if (io.niter) io.niter[env] = io.accumulate ? io.niter[env] + niter : niter;
     if (io.status) io.status[env] = io.accumulate ? (io.status[env] | st) : st;
-    if (io.ncon) io.ncon[env] = misc[MISC_NCON];
+    if (io.ncon) io.ncon[env] = misc[MISC_NCON_ALL];
     if (io.nefc) io.nefc[env] = misc[MISC_NEFC];
   }
 }

@@ -171,6 +171,8 @@ def test_interpreter_matches_oracle_on_box_scene(nbox, solver, seed):
     sim = emu.EmuSim(xml, B)
     fields = ref64.fields_from(sim.field)
     orc = ref64.Oracle(fields).set_caps()
+    fopt = dict(fields, solver=2, iterations=200, tolerance=1e-14)
+    orc_opt = ref64.Oracle(fopt).set_caps()
     rng = np.random.default_rng(seed)
     q = np.tile(sim.field("qpos0"), (B, 1))
     qv = np.zeros((B, sim.nv))
@@ -196,17 +198,22 @@ def test_interpreter_matches_oracle_on_box_scene(nbox, solver, seed):
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32))
         con = sim.dump(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
-        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4)
+        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4, ncon=st["ncon"][e])
         assert ok, f"env {e}: {msg}"
         ncon += len(con)
         assert st["nefc"][e] == orc.nefc(), (e, st["nefc"][e], orc.nefc())
         nover += bool(st["status"][e] & 2)
         b = orc.get("qacc")
         worst = max(worst, np.max(np.abs(sim.dump(2, e, "qacc") - b)) / max(1.0, np.max(np.abs(b))))
+        if solver == "CG":
+            ok, gk, g16, sc = states.cg_progress_ok(orc, orc_opt, q[e], qv[e], qa[e], np.zeros(0, np.float32),
+                                                    sim.dump(2, e, "qacc").astype(np.float64))
+            assert ok, (e, gk, g16, sc)
     print(f"nbox {nbox} nv {sim.nv}: contacts {ncon}, row-budget overflows {nover}, worst qacc rel err {worst:.2e}")
     assert ncon > B
     if nbox == 32:
         assert nover > 0 and st["nefc"].max() <= 249
-    # Newton converges to the 1e-8 scaled residual; CG stops on its improvement
-    # test at an iterate that FP32 and FP64 reach at different iterations
-    assert worst < (1e-3 if solver == "Newton" else 5e-3)
+    # Newton converges to the 1e-8 scaled residual. CG at its 16-iteration
+    # budget is unconverged: FP32 and FP64 CG end at different iterates of
+    # about equal cost (checked above), up to a few 1e-2 apart in qacc
+    assert worst < (1e-3 if solver == "Newton" else 5e-2)

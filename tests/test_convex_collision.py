@@ -239,7 +239,7 @@ def test_interpreter_matches_oracle_on_convex_drop():
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32))
         con = sim.dump(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
-        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4)
+        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4, ncon=st["ncon"][e])
         assert ok, f"env {e}: {msg}"
         # FP32 EPA against FP64 EPA: same pair list, depths to 1e-5 m, normals to 0.2 degree
         oc = sorted((c["pair"], c["dist"], tuple(c["frame"][:3])) for c in orc.contacts())

@@ -32,6 +32,12 @@ def _settled_states(model, B, seed, nbox):
     return f32(q), f32(qv), f32(qa), orc
 
 
+def _newton_optimum(getter):
+    f = ref64.fields_from(getter)
+    f.update(solver=2, iterations=200, tolerance=1e-14)
+    return ref64.Oracle(f).set_caps()
+
+
 @pytest.mark.parametrize("nbox,solver", [(6, "Newton"), (6, "CG"), (14, "Newton"), (32, "CG")])
 def test_box_scene_single_step_parity(nbox, solver):
     """6 boxes run from LDS; 14 (nv 84) and 32 (nv 192, row budget overflowing)
@@ -39,6 +45,8 @@ def test_box_scene_single_step_parity(nbox, solver):
     model = capi.Model(scenes.boxes_xml(nbox, 3, solver=solver, iterations=50 if solver == "Newton" else 16))
     B = 32 if nbox <= 14 else 16
     q, qv, qa, orc = _settled_states(model, B, 4, nbox)
+    orc_opt = _newton_optimum(model.field)
+    worstp = 0.0
     sim = capi.BatchedSim(model, B)
     sim.set_state(q, qv)
     sim.set_qacc_warmstart(qa)
@@ -50,18 +58,25 @@ def test_box_scene_single_step_parity(nbox, solver):
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32))
         con = dump.get(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
-        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4)
+        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4, ncon=st["ncon"][e])
         assert ok, f"env {e}: {msg}"
         ncon += len(con)
         b = orc.get("qacc")
         worst = max(worst, np.max(np.abs(dump.get(2, e, "qacc") - b)) / max(1.0, np.max(np.abs(b))))
         assert st["nefc"][e] == orc.nefc(), (e, st["nefc"][e], orc.nefc())
+        if solver == "CG":
+            ok, gk, g16, sc = states.cg_progress_ok(orc, orc_opt, q[e], qv[e], qa[e], np.zeros(0, np.float32),
+                                                    dump.get(2, e, "qacc").astype(np.float64))
+            worstp = max(worstp, gk / max(g16, 1e-6 * sc))
+            assert ok, (e, gk, g16, sc)
     print(f"nbox {nbox} {solver}: contacts {ncon}, worst qacc rel err {worst:.2e}, "
-          f"mean iterations {st['niter'].mean():.2f}, lds {sim.capacity()}")
+          f"mean iterations {st['niter'].mean():.2f}, lds {sim.capacity()}"
+          + (f", worst remaining-gap ratio to FP64 CG {worstp:.2f}" if solver == "CG" else ""))
     assert ncon > B
-    # Newton converges to the 1e-8 scaled residual; CG stops on its improvement
-    # test at an iterate that FP32 and FP64 reach at different iterations
-    assert worst < (1e-3 if solver == "Newton" else 5e-3)
+    # Newton converges to the 1e-8 scaled residual. CG at its 16-iteration
+    # budget is unconverged: FP32 and FP64 CG end at different iterates of
+    # about equal cost (checked above), up to a few 1e-2 apart in qacc
+    assert worst < (1e-3 if solver == "Newton" else 5e-2)
 
 
 def test_device_workspace_matches_lds(monkeypatch):

@@ -23,19 +23,29 @@ def test_convex_drop_single_step_parity():
     st = sim.stats()
     assert not (st["status"] & ~3).any(), st["status"]
     ncon = 0
-    worst = 0.0
+    worst = worstn = worstc = 0.0
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32))
         con = dump.get(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
-        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4)
+        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4, ncon=st["ncon"][e])
         assert ok, f"env {e}: {msg}"
-        oc = sorted((c["pair"], c["dist"]) for c in orc.contacts())
-        kc = sorted((int(c[13:14].view(np.int32)[0]), float(c[12])) for c in con)
-        for (p1, d1), (p2, d2) in zip(oc, kc):
+        oc = sorted((c["pair"], c["dist"], tuple(c["frame"][:3])) for c in orc.contacts())
+        kc = sorted((int(c[13:14].view(np.int32)[0]), float(c[12]), tuple(c[3:6])) for c in con)
+        for (p1, d1, n1), (p2, d2, n2) in zip(oc, kc):
             assert p1 == p2 and abs(d1 - d2) < 1e-5, (p1, d1, d2)
+            worstn = max(worstn, np.degrees(np.arccos(np.clip(np.dot(n1, n2), -1, 1))))
         ncon += len(con)
         b = orc.get("qacc")
-        worst = max(worst, np.max(np.abs(dump.get(2, e, "qacc") - b)) / max(1.0, np.max(np.abs(b))))
-    print(f"convex drop: contacts {ncon}, worst qacc rel err {worst:.2e}")
+        qk = dump.get(2, e, "qacc")
+        worst = max(worst, np.max(np.abs(qk - b)) / max(1.0, np.max(np.abs(b))))
+        c_opt = orc.cost(b)[0]
+        worstc = max(worstc, (orc.cost(qk.astype(np.float64))[0] - c_opt) / states.cost_scale(orc, c_opt))
+    print(f"convex drop: contacts {ncon}, worst qacc rel err {worst:.2e}, worst normal {worstn:.3f} deg, "
+          f"worst cost gap {worstc:.2e}")
     assert ncon > B
-    assert worst < 1e-2
+    # FP32 EPA normals of curved pairs differ from FP64 ones by up to ~1 degree
+    # (the support gap it can resolve); on centimetre-deep stiff contacts that
+    # moves the optimum by a few 1e-2 of qacc, the kernel's own problem being
+    # solved to convergence
+    assert worstn < 1.5
+    assert worst < 5e-2 and worstc < 2e-3

@@ -44,7 +44,7 @@ def test_humanoid_single_step_parity():
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], ctrl[e])
         con = dump.get(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
-        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4)
+        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4, ncon=st["ncon"][e])
         assert ok, f"env {e}: {msg}"
         ncon += len(con)
         b = orc.get("qacc")
@@ -52,7 +52,7 @@ def test_humanoid_single_step_parity():
         worst = max(worst, np.max(np.abs(qk - b)) / max(1.0, np.max(np.abs(b))))
         if orc.nefc():
             c_opt = orc.cost(b)[0]
-            worstc = max(worstc, (orc.cost(qk.astype(np.float64))[0] - c_opt) / max(1.0, abs(c_opt)))
+            worstc = max(worstc, (orc.cost(qk.astype(np.float64))[0] - c_opt) / states.cost_scale(orc, c_opt))
     print(f"humanoid: contacts {ncon}, worst qacc rel err {worst:.2e}, cost gap {worstc:.2e}, "
           f"mean iterations {st['niter'].mean():.2f}")
     assert ncon > B // 2

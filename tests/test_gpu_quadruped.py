@@ -42,7 +42,7 @@ def test_single_step_parity(model):
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], ctrl[e])
         con = dump.get(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
-        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts())
+        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), ncon=st["ncon"][e])
         assert ok, f"env {e}: {msg}"
         ncontact += len(orc.contacts())
         qa_gpu, qa_orc = dump.get(2, e, "qacc"), orc.get("qacc")

@@ -29,7 +29,7 @@ def test_quadruped_single_step(self_collision):
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], ctrl[e])
         con = sim.dump(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
-        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts())
+        ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), ncon=st["ncon"][e])
         assert ok, f"env {e}: {msg}"
         b = orc.get("qacc")
         worst_q = max(worst_q, np.max(np.abs(sim.dump(2, e, "qacc") - b)) / max(1.0, np.max(np.abs(b))))
