@@ -83,11 +83,90 @@ def timed_run(sim, pool, warmup, steps, offset=0):
     return float(np.mean(kernel_ms)), dt, float(np.mean(iters))
 
 
+def run_boxes(args):
+    """Uneven-contact batching: every env is a plane plus N in U{4..32} free
+    boxes (half-sizes U(0.02, 0.1), densities U(500, 2000), friction
+    U(0.3, 1.0), drop heights U(0.1, 1.0) above the box's bounding radius,
+    random orientations), condim 3 pyramidal, CG 16 iterations. Envs with the
+    same N share one model (one sim and stream per N); the groups' launches
+    are queued together, so value = all envs x steps / wall time of the timed
+    steps with everything resident."""
+    from mjmlx_hip import capi, scenes
+
+    if capi.lib().mjmlx_hip_device_count() < 1:
+        raise SystemExit("bench: no HIP device visible")
+    B = args.envs
+    rng = np.random.default_rng(3)
+    ns = rng.integers(4, 33, size=B)
+    groups = []
+    for n in range(4, 33):
+        nb = int((ns == n).sum())
+        if nb == 0:
+            continue
+        model = capi.Model(scenes.boxes_xml(n, 100 + n, solver="CG", iterations=16))
+        q = np.tile(model.field("qpos0"), (nb, 1))
+        hs = model.field("geom_size").reshape(-1, 3)[1:]
+        for b in range(n):
+            a = 7 * b
+            q[:, a] += rng.uniform(-0.05, 0.05, nb)
+            q[:, a + 1] += rng.uniform(-0.05, 0.05, nb)
+            q[:, a + 2] = rng.uniform(0.1, 1.0, nb) + np.linalg.norm(hs[b])
+            qq = rng.normal(size=(nb, 4))
+            q[:, a + 3:a + 7] = qq / np.linalg.norm(qq, axis=1, keepdims=True)
+        sim = capi.BatchedSim(model, nb)
+        sim.set_state(q.astype(np.float32), np.zeros((nb, model.nv), np.float32))
+        groups.append((n, nb, model, sim))
+    for k in range(args.warmup):
+        for _, _, _, sim in groups:
+            sim.step(None, 4)
+    for _, _, _, sim in groups:
+        sim.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        for _, _, _, sim in groups:
+            sim.step(None, 4)
+    for _, _, _, sim in groups:
+        sim.sync()
+    wall = time.perf_counter() - t0
+    # per-group kernel time and statistics, each group timed alone
+    per = []
+    for n, nb, model, sim in groups:
+        ms = []
+        for k in range(5):
+            sim.step(None, 4)
+            ms.append(sim.last_step_ms())
+        st = sim.stats()
+        per.append({"n_boxes": n, "envs": nb, "nv": model.nv, "ms_per_step_alone": float(np.mean(ms)),
+                    "lds_bytes_per_env": sim.capacity()[2], "mean_ncon": float(st["ncon"].mean()),
+                    "mean_nefc": float(st["nefc"].mean()), "row_budget_overflow_envs": int(((st["status"] & 2) > 0).sum()),
+                    "mean_iters": float(st["niter"].mean())})
+    value = B * args.steps / wall
+    bpe = float(np.mean([bytes_per_env_step(7 * n, 6 * n, 0) for n in ns]))
+    out = {
+        "metric": "physics env-steps/s @16384 envs, plane + U{4..32} boxes",
+        "value": value, "unit": "env-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plane + N in U{4..32} free boxes, condim 3 pyramidal, CG 16 it, tol 1e-8, "
+                               "dt 0.005 x 4 substeps; reference capacities (128 contacts, 72/256 rows)",
+                   "num_envs": B, "substeps_per_env_step": 4, "groups": len(groups),
+                   "parallelism": "single GPU, 1 wavefront per env, one sim (stream) per N",
+                   "workspace": "LDS up to the LDS limit, device memory beyond"},
+        "roofline": {"bound": "hbm", "achieved": B * bpe / (wall / args.steps) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": B * bpe / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_step", "bytes_per_env_step": bpe},
+        "groups": per,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 SCENES = {
     "quadruped": "quadruped nq19 nv18 nu12, self-collision, condim 3 pyramidal, Newton 16 it, tol 1e-8, "
                  "dt 0.005 x 4 substeps",
     "humanoid": "humanoid nq28 nv27 nu21 (free joint + 21 hinges), self-collision, condim 6 elliptic, Newton 16 it, "
                 "tol 1e-8, dt 0.005 x 4 substeps, motor ctrl U(+-0.4)",
+    "boxes": "plane + N in U{4..32} free boxes, CG 16 it",
 }
 
 
@@ -207,7 +286,8 @@ def main():
     ap.add_argument("--envs", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scene", choices=sorted(SCENES), default="quadruped",
-                    help="quadruped is the headline workload; humanoid is the high-DoF elliptic-cone one")
+                    help="quadruped is the headline workload; humanoid is the high-DoF elliptic-cone one; "
+                         "boxes the uneven-contact one")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="training pairs for the learned warm start")
     ap.add_argument("--epochs", type=int, default=8)
@@ -215,7 +295,10 @@ def main():
     if args.gpus != 1:
         raise SystemExit("bench.py measures one GPU only: the batched step has no multi-GPU path "
                          "(envs are independent, so each GPU would step its own share of them)")
-    run(args)
+    if args.scene == "boxes":
+        run_boxes(args)
+    else:
+        run(args)
 
 
 if __name__ == "__main__":

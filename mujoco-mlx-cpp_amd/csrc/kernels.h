@@ -476,46 +476,51 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
 }
 
 // x <- (L^T D L)^-1 x with L unit lower (entries outside the dof tree are 0).
-// GPU: lane i holds x_i; each column step broadcasts one solved entry with
-// readlane and updates the other lanes, with no LDS round trip per step.
+// GPU, nv <= 64: lane i holds x_i; each column step broadcasts one solved entry
+// with readlane and updates the other lanes, with no LDS round trip per step.
+// Otherwise one workspace phase per tree level: L^T y = x deepest level first,
+// each dof gathering from its (final) descendants, a contiguous dof range;
+// L z = y from the roots, each dof gathering from its (final) ancestors.
 KDEV void solve_ld(const DModel& m, const float* LD, float* x) {
   int nv = m.nv;
 #if WAVE == 64
-  if (nv > WAVE) {  // more dofs than lanes: one workspace phase per column
-    for (int k = nv - 1; k >= 1; k--) {
-      float xk = x[k];
-      for (int i = LANE; i < k; i += WAVE) x[i] -= LD[k * nv + i] * xk;
-      KSYNC();
+  if (nv <= WAVE) {
+    float v = LANE < nv ? x[LANE] : 0.f;
+    for (int k = nv - 1; k >= 1; k--) {  // L^T y = x: column k updates rows i < k
+      float xk = rl_f(v, k);
+      if (LANE < k) v -= LD[k * nv + LANE] * xk;
     }
-    for (int i = LANE; i < nv; i += WAVE) x[i] /= LD[i * nv + i];
+    if (LANE < nv) v /= LD[LANE * nv + LANE];
+    for (int i = 0; i < nv - 1; i++) {   // L z = y: solved entry i updates rows k > i
+      float xi = rl_f(v, i);
+      if (LANE > i && LANE < nv) v -= LD[LANE * nv + i] * xi;
+    }
     KSYNC();
-    for (int i = 0; i < nv - 1; i++) {
-      float xi = x[i];
-      for (int k = i + 1 + LANE; k < nv; k += WAVE) x[k] -= LD[k * nv + i] * xi;
-      KSYNC();
-    }
+    if (LANE < nv) x[LANE] = v;
+    KSYNC();
     return;
   }
-  float v = LANE < nv ? x[LANE] : 0.f;
-  for (int k = nv - 1; k >= 1; k--) {  // L^T y = x: column k updates rows i < k
-    float xk = rl_f(v, k);
-    if (LANE < k) v -= LD[k * nv + LANE] * xk;
-  }
-  if (LANE < nv) v /= LD[LANE * nv + LANE];
-  for (int i = 0; i < nv - 1; i++) {   // L z = y: solved entry i updates rows k > i
-    float xi = rl_f(v, i);
-    if (LANE > i && LANE < nv) v -= LD[LANE * nv + i] * xi;
-  }
-  KSYNC();
-  if (LANE < nv) x[LANE] = v;
-  KSYNC();
-#else
-  for (int k = nv - 1; k >= 1; k--)
-    for (int i = 0; i < k; i++) x[i] -= LD[k * nv + i] * x[k];
-  for (int i = 0; i < nv; i++) x[i] /= LD[i * nv + i];
-  for (int i = 0; i < nv - 1; i++)
-    for (int k = i + 1; k < nv; k++) x[k] -= LD[k * nv + i] * x[i];
 #endif
+  for (int lv = m.ndoflevel - 2; lv >= 0; lv--) {
+    for (int t = m.doflevel_adr[lv] + LANE; t < m.doflevel_adr[lv + 1]; t += WAVE) {
+      int i = m.doflevel_dof[t];
+      float acc = 0.f;
+      for (int k = i + 1; k < m.dof_subtree_end[i]; k++) acc += LD[k * nv + i] * x[k];
+      x[i] -= acc;
+    }
+    KSYNC();
+  }
+  for (int i = LANE; i < nv; i += WAVE) x[i] /= LD[i * nv + i];
+  KSYNC();
+  for (int lv = 1; lv < m.ndoflevel; lv++) {
+    for (int t = m.doflevel_adr[lv] + LANE; t < m.doflevel_adr[lv + 1]; t += WAVE) {
+      int k = m.doflevel_dof[t];
+      float acc = 0.f;
+      for (int i = m.dof_parentid[k]; i >= 0; i = m.dof_parentid[i]) acc += LD[k * nv + i] * x[i];
+      x[k] -= acc;
+    }
+    KSYNC();
+  }
 }
 
 // mj_comVel with MuJoCo's grouping of free / ball dofs (reference forward.hpp:287-332)
@@ -1636,9 +1641,21 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
 
 // ------------------------------------------------------------------ solver
 // y = M x (dense qM), lanes over rows
-KDEV void mulM(const DModel& m, const float* M, const float* x, float* y) {
+// row i of M x. Up to 64 dofs the dense row (LDS-resident, no dependent
+// loads); beyond, only the dof tree's nonzeros: ancestors (the parent chain),
+// the diagonal and the descendants (a contiguous dof range)
+KDEV float mulM_row(const DModel& m, const float* M, const float* x, int i) {
   int nv = m.nv;
-  for (int i = LANE; i < nv; i += WAVE) y[i] = kdot(M + i * nv, 1, x, 1, nv);
+  const float* Mi = M + i * nv;
+  if (nv <= 64) return kdot(Mi, 1, x, 1, nv);
+  float acc = 0.f;
+  for (int j = m.dof_parentid[i]; j >= 0; j = m.dof_parentid[j]) acc += Mi[j] * x[j];
+  for (int k = i; k < m.dof_subtree_end[i]; k++) acc += Mi[k] * x[k];
+  return acc;
+}
+
+KDEV void mulM(const DModel& m, const float* M, const float* x, float* y) {
+  for (int i = LANE; i < m.nv; i += WAVE) y[i] = mulM_row(m, M, x, i);
 }
 
 // qacc <- x; Ma = M x; Jaref = J x - aref
@@ -1646,7 +1663,7 @@ KDEV void set_qacc(const DModel& m, const Layout& L, float* w, const float* x, i
   int nv = m.nv;
   float *qacc = w + L.qacc, *Ma = w + L.Ma, *Jaref = w + L.efc_Jaref;
   const float *J = w + L.efc_J, *aref = w + L.efc_aref, *M = w + L.qM;
-  for (int i = LANE; i < nv; i += WAVE) Ma[i] = kdot(M + i * nv, 1, x, 1, nv);
+  for (int i = LANE; i < nv; i += WAVE) Ma[i] = mulM_row(m, M, x, i);
   for (int r = LANE; r < nefc; r += WAVE) Jaref[r] = kdot(J + r * nv, 1, x, 1, nv) - aref[r];
   KSYNC();
   for (int i = LANE; i < nv; i += WAVE) qacc[i] = x[i];
