@@ -441,7 +441,16 @@ KDEV void st_crb(const DModel& m, const Layout& L, float* w) {
 // final pass. Returns ST_M_NOT_PD on a non-positive pivot.
 KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
   int nv = m.nv;
-  for (int t = LANE; t < nv * nv; t += WAVE) LD[t] = A[t];
+  // beyond 64 dofs only the tree's entries are copied and scaled (row k:
+  // its diagonal and its ancestors), all its readers touch; up to 64 the
+  // dense matrix, whose zeros the register solve reads
+  bool sparse = nv > 64;
+  if (sparse) {
+    for (int k = LANE; k < nv; k += WAVE)
+      for (int i = k; i >= 0; i = i == k ? m.dof_parentid[k] : m.dof_parentid[i]) LD[k * nv + i] = A[k * nv + i];
+  } else {
+    for (int t = LANE; t < nv * nv; t += WAVE) LD[t] = A[t];
+  }
   KSYNC();
   int bad = 0;
   // dofs of one tree level are neither ancestors nor descendants of each
@@ -462,13 +471,21 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
     }
     KSYNC();
   }
-  for (int t = LANE; t < nv * nv; t += WAVE) {
-    int k = t / nv, i = t - k * nv;
-    if (i < k) {
+  if (sparse) {
+    for (int k = LANE; k < nv; k += WAVE) {
       float dk = LD[k * nv + k];
-      LD[t] = dk > KMINVAL ? LD[t] / dk : LD[t] / KMINVAL;
-    } else if (i == k && !(LD[t] > KMINVAL)) {
-      bad = 1;
+      if (!(dk > KMINVAL)) { bad = 1; dk = KMINVAL; }
+      for (int i = m.dof_parentid[k]; i >= 0; i = m.dof_parentid[i]) LD[k * nv + i] /= dk;
+    }
+  } else {
+    for (int t = LANE; t < nv * nv; t += WAVE) {
+      int k = t / nv, i = t - k * nv;
+      if (i < k) {
+        float dk = LD[k * nv + k];
+        LD[t] = dk > KMINVAL ? LD[t] / dk : LD[t] / KMINVAL;
+      } else if (i == k && !(LD[t] > KMINVAL)) {
+        bad = 1;
+      }
     }
   }
   KSYNC();
@@ -1658,13 +1675,77 @@ KDEV void mulM(const DModel& m, const float* M, const float* x, float* y) {
   for (int i = LANE; i < m.nv; i += WAVE) y[i] = mulM_row(m, M, x, i);
 }
 
+// row r of J x. Beyond 64 dofs only the row's possible nonzeros: a friction
+// or limit row's dof, a contact row's two body chains (body2's, then body1's
+// up to the first dof body2's chain shares); rows of other types are dense.
+KDEV float jrow_dot(const DModel& m, const Layout& L, const float* w, int r, const float* x) {
+  int nv = m.nv;
+  const float* Jr = w + L.efc_J + r * nv;
+  if (nv <= 64) return kdot(Jr, 1, x, 1, nv);
+  int ty = reinterpret_cast<const int*>(w + L.efc_type)[r], id = reinterpret_cast<const int*>(w + L.efc_id)[r];
+  if (ty == EFC_FRICTION_DOF) return Jr[id] * x[id];
+  if (ty == EFC_LIMIT_JOINT) { int d = m.jnt_dofadr[id >> 1]; return Jr[d] * x[d]; }
+  if (ty != EFC_CONTACT_FRICTIONLESS && ty != EFC_CONTACT_PYRAMIDAL && ty != EFC_CONTACT_ELLIPTIC)
+    return kdot(Jr, 1, x, 1, nv);
+  int p = reinterpret_cast<const int*>(w + L.con + CON_STRIDE * id + CON_PAIR)[0];
+  int l2 = m.body_lastdof[m.pair_b2[p]];
+  float acc = 0.f;
+  for (int d = l2; d >= 0; d = m.dof_parentid[d]) acc += Jr[d] * x[d];
+  for (int d = m.body_lastdof[m.pair_b1[p]]; d >= 0; d = m.dof_parentid[d]) {
+    if (l2 >= d && l2 < m.dof_subtree_end[d]) break;  // d is on body2's chain
+    acc += Jr[d] * x[d];
+  }
+  return acc;
+}
+
+// y = J^T f. GPU beyond 64 dofs: lane l keeps dofs l, l + 64, .. in registers
+// and streams the rows with nonzero force, coalesced across lanes.
+KDEV void mul_jt(const DModel& m, const Layout& L, const float* w, const float* f, int nefc, float* y) {
+  int nv = m.nv;
+  const float* J = w + L.efc_J;
+#if WAVE == 64
+  if (nv > WAVE) {
+    // eight rows per step: up to 32 independent loads in flight per lane
+    float g[4] = {0.f, 0.f, 0.f, 0.f};
+    int r = 0;
+    for (; r + 8 <= nefc; r += 8) {
+      float fr[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) fr[u] = f[r + u];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int i = LANE + q * WAVE;
+        if (i < nv) {
+          float a[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) a[u] = J[(r + u) * nv + i];
+#pragma unroll
+          for (int u = 0; u < 8; u++) g[q] += a[u] * fr[u];
+        }
+      }
+    }
+    for (; r < nefc; r++) {
+      float fr = f[r];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (LANE + q * WAVE < nv) g[q] += J[r * nv + LANE + q * WAVE] * fr;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (LANE + q * WAVE < nv) y[LANE + q * WAVE] = g[q];
+    return;
+  }
+#endif
+  for (int i = LANE; i < nv; i += WAVE) y[i] = kdot(J + i, nv, f, 1, nefc);
+}
+
 // qacc <- x; Ma = M x; Jaref = J x - aref
 KDEV void set_qacc(const DModel& m, const Layout& L, float* w, const float* x, int nefc) {
   int nv = m.nv;
   float *qacc = w + L.qacc, *Ma = w + L.Ma, *Jaref = w + L.efc_Jaref;
-  const float *J = w + L.efc_J, *aref = w + L.efc_aref, *M = w + L.qM;
+  const float *aref = w + L.efc_aref, *M = w + L.qM;
   for (int i = LANE; i < nv; i += WAVE) Ma[i] = mulM_row(m, M, x, i);
-  for (int r = LANE; r < nefc; r += WAVE) Jaref[r] = kdot(J + r * nv, 1, x, 1, nv) - aref[r];
+  for (int r = LANE; r < nefc; r += WAVE) Jaref[r] = jrow_dot(m, L, w, r, x) - aref[r];
   KSYNC();
   for (int i = LANE; i < nv; i += WAVE) qacc[i] = x[i];
   KSYNC();
@@ -1782,9 +1863,10 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
   float cost = wsum(part);
   *nchanged = wsumi(nch);
   KSYNC();
-  const float* J = w + L.efc_J;
   float* grad = w + L.grad;
-  for (int i = LANE; i < nv; i += WAVE) grad[i] = Ma[i] - qfs[i] - kdot(J + i, nv, force, 1, nefc);
+  mul_jt(m, L, w, force, nefc, grad);
+  KSYNC();
+  for (int i = LANE; i < nv; i += WAVE) grad[i] = Ma[i] - qfs[i] - grad[i];
   KSYNC();
   return cost;
 }
@@ -2078,7 +2160,7 @@ KDEV LsPt ls_eval(const DModel& m, const Layout& L, float* w, int nefc, float al
 // in the same quadratic piece as its start is the exact minimiser.
 KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int* exact) {
   int nv = m.nv;
-  const float *s = w + L.search, *M = w + L.qM, *J = w + L.efc_J;
+  const float *s = w + L.search, *M = w + L.qM;
   float *Mv = w + L.Mv, *Jv = w + L.efc_Jv;
   const float *Ma = w + L.Ma, *qfs = w + L.qfrc_smooth, *qacc = w + L.qacc, *qas = w + L.qacc_smooth;
   float p0s = 0.f;
@@ -2088,7 +2170,7 @@ KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int*
   float scale = 1.f / (m.meaninertia * (float)(nv > 1 ? nv : 1));
   float gtol = m.tolerance * m.ls_tolerance * snorm / scale;
   mulM(m, M, s, Mv);
-  for (int r = LANE; r < nefc; r += WAVE) Jv[r] = kdot(J + r * nv, 1, s, 1, nv);
+  for (int r = LANE; r < nefc; r += WAVE) Jv[r] = jrow_dot(m, L, w, r, s);
   KSYNC();
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   for (int i = LANE; i < nv; i += WAVE) {
@@ -2233,8 +2315,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     KSYNC();
   }
   KPROF_MARK(pa, KP_DIR);
-  const float *J = w + L.efc_J, *force = w + L.efc_force;
-  for (int i = LANE; i < nv; i += WAVE) qfc[i] = kdot(J + i, nv, force, 1, nefc);
+  mul_jt(m, L, w, w + L.efc_force, nefc, qfc);
   KSYNC();
   KPROF_MARK(pa, KP_SOLVE_REST);
   return st;
