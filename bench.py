@@ -43,8 +43,9 @@ def bytes_per_env_step(nq: int, nv: int, nu: int) -> int:
     return 4 * (2 * nq + 4 * nv + nu)
 
 
-def cpu_baseline(xml_fields, nenv=4096, warm=5, steps=50):
-    """FP64 oracle on the host cores: env-steps/s on a bounded sample."""
+def cpu_baseline(xml_fields, nenv=4096, warm=5, target_s=12.0):
+    """FP64 oracle on the host cores: env-steps/s on a bounded sample of about
+    target_s seconds (the step count is sized from a 5-step pilot)."""
     import ref64
     from mjmlx_hip import scenes
 
@@ -56,10 +57,16 @@ def cpu_baseline(xml_fields, nenv=4096, warm=5, steps=50):
     qa = np.zeros((nenv, orc.nv))
     for _ in range(warm):
         orc.batch_step(q, qv, qa, scenes.quadruped_actions(nenv, rng), 4, nthreads)
-    ctrls = [np.ascontiguousarray(scenes.quadruped_actions(nenv, rng)) for _ in range(steps)]
     t0 = time.perf_counter()
-    for c in ctrls:
-        orc.batch_step(q, qv, qa, c, 4, nthreads)
+    for _ in range(5):
+        orc.batch_step(q, qv, qa, scenes.quadruped_actions(nenv, rng), 4, nthreads)
+    pilot = (time.perf_counter() - t0) / 5
+    steps = int(min(2000, max(20, target_s / max(pilot, 1e-6))))
+    warm += 5
+    ctrls = [np.ascontiguousarray(scenes.quadruped_actions(nenv, rng)) for _ in range(16)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        orc.batch_step(q, qv, qa, ctrls[k % 16], 4, nthreads)
     dt = time.perf_counter() - t0
     return {"value": nenv * steps / dt, "unit": "env-steps/s", "cores": nthreads, "kind": "port",
             "sample": f"own FP64 restatement (oracle/ref64.c, not MuJoCo C): {nenv} envs x {steps} control steps "
@@ -159,6 +166,66 @@ def run_boxes(args):
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
+
+
+STAGES = [("kin", "kinematics"), ("compos", "subtree com"), ("crb", "CRB"), ("factor", "LDL of qM"),
+          ("comvel", "com velocities"), ("smooth", "smooth forces (RNE, passive, actuation)"),
+          ("coll", "collision"), ("constr", "constraint rows"), ("solve_init", "solver set-up"),
+          ("ls", "line search"), ("costgrad", "cost / gradient"), ("dir", "search direction (factor + solve)"),
+          ("solve_rest", "other solver work"), ("integrate", "integration"), ("io", "state in/out"),
+          ("hbuild", "Newton Hessian build")]
+
+
+def run_stages(args):
+    """Per-stage cycle shares from the profiling build (libmjmlx_prof.so:
+    s_memtime counters around each stage of the step kernel), one 4-substep
+    control step of every env after the warm-up, mean per env."""
+    import ctypes as C
+    from mjmlx_hip import capi, scenes
+    capi.LIBPATH = os.path.join(capi.LIBDIR, "libmjmlx_prof.so")
+    L = capi.lib()
+    L.mjmlx_batched_profile_step.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_ulonglong)]
+    rng = np.random.default_rng(1)
+    B = args.envs
+    if args.scene == "boxes":
+        model = capi.Model(scenes.boxes_xml(args.nbox, 100 + args.nbox, solver="CG", iterations=16))
+        pool = [None]
+        q0 = np.tile(model.field("qpos0"), (B, 1)).astype(np.float32)
+    else:
+        model, pool, q0 = scene_setup(args.scene, B, rng)
+    sim = capi.BatchedSim(model, B)
+    sim.set_state(q0, np.zeros((B, model.nv), np.float32))
+    for k in range(args.warmup):
+        sim.step(pool[k % len(pool)], 4)
+    sim.sync()
+    out = np.zeros((B, len(STAGES)), np.uint64)
+    c = pool[0]
+    cp = None if c is None else c.ctypes.data_as(C.POINTER(C.c_float))
+    assert L.mjmlx_batched_profile_step(sim.h, cp, 4, out.ctypes.data_as(C.POINTER(C.c_ulonglong))) == 0
+    st = sim.stats()
+    tot = out.sum(1).astype(np.float64)
+    name = args.scene if args.scene != "boxes" else f"boxes N={args.nbox}"
+    print(f"{name}, {B} envs: mean solver iterations per control step {st['niter'].mean():.2f}, "
+          f"mean contacts {st['ncon'].mean():.1f}, rows {st['nefc'].mean():.1f}, "
+          f"{tot.mean():.0f} cycles per env-step")
+    frac = out.sum(0) / max(1, out.sum())
+    print("| stage | share | cycles per substep |\n|---|---|---|")
+    for i in np.argsort(-frac):
+        if frac[i] > 0.001:
+            print(f"| {STAGES[i][1]} | {frac[i] * 100:.1f}% | {out[:, i].mean() / 4:.0f} |")
+
+
+def copy_calibration():
+    """A plain device copy of known size (3 x 256 MiB, fp32) whose FETCH_SIZE /
+    WRITE_SIZE counters calibrate those of the step kernel."""
+    import torch
+    n = 256 * 1024 * 1024 // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copied 3 x 256 MiB", flush=True)
 
 
 SCENES = {
@@ -291,11 +358,20 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="training pairs for the learned warm start")
     ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--stages", action="store_true", help="per-stage cycle table from the profiling build")
+    ap.add_argument("--nbox", type=int, default=16, help="boxes per env for --stages --scene boxes")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 3 x 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         raise SystemExit("bench.py measures one GPU only: the batched step has no multi-GPU path "
                          "(envs are independent, so each GPU would step its own share of them)")
-    if args.scene == "boxes":
+    if args.stages:
+        run_stages(args)
+    elif args.scene == "boxes":
         run_boxes(args)
     else:
         run(args)

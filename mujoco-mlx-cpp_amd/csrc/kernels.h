@@ -2065,7 +2065,9 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
   KSYNC();
 }
 
-struct LsPt { float alpha, f, d1, d2; int nchange; };
+// d1n: sum of the magnitudes of the slope's terms, the scale of its FP32
+// rounding
+struct LsPt { float alpha, f, d1, d2, d1n; int nchange; };
 
 // The rows a lane owns (r = LANE + q WAVE) held in registers for the whole
 // line search, when there are at most two per lane: every evaluation is then
@@ -2118,7 +2120,7 @@ KDEV void ls_row(const Layout& L, const float* w, int r, int ty, float jar, floa
 // their zone at ref_alpha (0 means one quadratic piece spans both points)
 KDEV LsPt ls_eval(const DModel& m, const Layout& L, float* w, int nefc, float alpha, float ref_alpha, const float* gq,
                   const LsRows& c) {
-  float f = 0.f, d1 = 0.f, d2 = 0.f;
+  float f = 0.f, d1 = 0.f, d2 = 0.f, d1a = 0.f;
   int nch = 0;
   if (m.cone != 0) {
     // elliptic contacts; the middle zone is not quadratic, so it never counts as one exact piece
@@ -2134,25 +2136,38 @@ KDEV LsPt ls_eval(const DModel& m, const Layout& L, float* w, int nefc, float al
       int z = k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, dim, alpha, &cv, &g1, &g2);
       int zr = k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, dim, ref_alpha, &cr_, &h1, &h2);
       nch += (z != zr) || z == 2;
-      f += cv; d1 += g1; d2 += g2;
+      f += cv; d1 += g1; d2 += g2; d1a += fabsf(g1);
     }
   }
   if (c.on) {
     for (int q = 0; q < 2; q++)
-      if (c.ty[q] >= 0) ls_row(L, w, LANE + q * WAVE, c.ty[q], c.jar[q], c.jv[q], c.d[q], alpha, ref_alpha, f, d1, d2, nch);
+      if (c.ty[q] >= 0) {
+        float t = d1;
+        ls_row(L, w, LANE + q * WAVE, c.ty[q], c.jar[q], c.jv[q], c.d[q], alpha, ref_alpha, f, d1, d2, nch);
+        d1a += fabsf(d1 - t);
+      }
   } else {
     const float *Jaref = w + L.efc_Jaref, *Jv = w + L.efc_Jv, *D = w + L.efc_D;
     const int* etype = iptr(w, L.efc_type);
-    for (int r = LANE; r < nefc; r += WAVE) ls_row(L, w, r, etype[r], Jaref[r], Jv[r], D[r], alpha, ref_alpha, f, d1, d2, nch);
+    for (int r = LANE; r < nefc; r += WAVE) {
+      float t = d1;
+      ls_row(L, w, r, etype[r], Jaref[r], Jv[r], D[r], alpha, ref_alpha, f, d1, d2, nch);
+      d1a += fabsf(d1 - t);
+    }
   }
   LsPt p;
   p.alpha = alpha;
   p.f = wsum(f) + gq[0] + alpha * gq[1] + 0.5f * alpha * alpha * gq[2];
   p.d1 = wsum(d1) + gq[1] + alpha * gq[2];
+  p.d1n = wsum(d1a) + fabsf(gq[1]) + fabsf(alpha * gq[2]);
   p.d2 = wsum(d2) + gq[2];
   p.nchange = wsumi(nch);
   return p;
 }
+
+// the slope is zero to the tolerance, or to FP32 resolution (a few 1e-7 of
+// its terms' magnitudes, below which further steps only chase rounding)
+KDEV bool ls_flat(const LsPt& p, float gtol) { return fabsf(p.d1) <= fmaxf(gtol, 1e-6f * p.d1n); }
 
 // Exact line search on the convex piecewise-quadratic cost (same algorithm as
 // the oracle): Newton steps until the derivative changes sign, then Newton
@@ -2188,12 +2203,12 @@ KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int*
   LsPt lo = p0;
   LsPt pt = ls_eval(m, L, w, nefc, -p0.d1 / p0.d2, 0.f, gq, rows);
   if (pt.nchange == 0) { *exact = 1; return pt.alpha; }
-  if (fabsf(pt.d1) <= gtol) return pt.alpha;
+  if (ls_flat(pt, gtol)) return pt.alpha;
   while (pt.d1 < 0.f && it < maxit) {
     lo = pt;
     pt = ls_eval(m, L, w, nefc, pt.alpha - pt.d1 / pt.d2, lo.alpha, gq, rows);
     it++;
-    if (pt.nchange == 0 || fabsf(pt.d1) <= gtol) return pt.alpha;
+    if (pt.nchange == 0 || ls_flat(pt, gtol)) return pt.alpha;
   }
   if (pt.d1 < 0.f) return pt.alpha;
   LsPt hi = pt;
@@ -2204,7 +2219,7 @@ KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int*
     if (!(a > lo.alpha && a < hi.alpha)) a = 0.5f * (lo.alpha + hi.alpha);
     pt = ls_eval(m, L, w, nefc, a, lo.alpha, gq, rows);
     it++;
-    if ((newton && pt.nchange == 0) || fabsf(pt.d1) <= gtol) return pt.alpha;
+    if ((newton && pt.nchange == 0) || ls_flat(pt, gtol)) return pt.alpha;
     if (pt.d1 < 0.f) lo = pt; else hi = pt;
     if (!(hi.alpha > lo.alpha)) break;
   }
