@@ -491,9 +491,11 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     hipDeviceProp_t prop;
     HIPOK(hipGetDeviceProperties(&prop, dev));
     s->ncu = prop.multiProcessorCount;
-    // workspaces beyond lds_max (64 KB unless MJMLX_HIP_LDS_MAX says otherwise,
-    // at most the workgroup's LDS) live in device memory instead of LDS
-    size_t lds_max = 65536;
+    // workspaces beyond lds_max (144 KB unless MJMLX_HIP_LDS_MAX says
+    // otherwise, at most the workgroup's LDS) live in device memory instead of
+    // LDS. Measured on the box workloads: one LDS-resident env per CU still
+    // beats device memory up to ~140 KB workspaces, not at ~158 KB.
+    size_t lds_max = 147456;
     if (const char* v = std::getenv("MJMLX_HIP_LDS_MAX")) lds_max = (size_t)std::max(0L, std::atol(v));
     bool use_gws = s->lds_bytes > lds_max || s->lds_bytes > (size_t)prop.sharedMemPerBlock;
     if (const char* v = std::getenv("MJMLX_HIP_GLOBAL_WORKSPACE")) use_gws = use_gws || std::atoi(v) != 0;
