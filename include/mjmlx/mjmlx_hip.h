@@ -51,8 +51,9 @@ MJMLX_API int mjmlx_batched_set_launch(MjmlxBatchedSim* sim, int persistent, int
  * events on the sim's stream (0 before the first step). */
 MJMLX_API float mjmlx_batched_last_step_ms(const MjmlxBatchedSim* sim);
 
-/* Capacity per env: contacts and constraint rows kept in the per-env LDS
- * workspace (beyond them the status word reports an overflow). */
+/* Capacity per env: contacts stored and constraint rows kept in the per-env
+ * workspace, and the workspace size in bytes (LDS up to 144 KB, else device
+ * memory). Overflows set status bits. */
 MJMLX_API int mjmlx_batched_capacity(const MjmlxBatchedSim* sim, int* maxcon, int* maxefc, int* lds_bytes);
 
 /* Debug: one physics step like mjmlx_batched_step that also copies each env's
