@@ -30,6 +30,9 @@ struct MjmlxMLP {
         *dh1 = nullptr, *d2f = nullptr, *d1f = nullptr, *rowloss = nullptr;
   int* d_idx = nullptr;
   long idx_cap = 0;
+  // split-K partial tiles of the weight gradients and partial column sums
+  float* part = nullptr;
+  size_t part_cap = 0;
   // device-resident training set: raw (unnormalised) rows
   float *ds_x = nullptr, *ds_y = nullptr;
   long ds_cap = 0, ds_n = 0;

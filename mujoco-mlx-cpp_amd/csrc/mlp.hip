@@ -84,18 +84,22 @@ __device__ __forceinline__ void stage(uint16_t (*S)[KPAD], const uint16_t* __res
 
 // C = op(A) op(B) with the epilogue. op(A) is M x K (TA: A stored K x M),
 // op(B) is K x N (TB: B stored N x K). K is a multiple of 32.
+// Split K (gridDim.z > 1): block z covers k in [z kchunk, (z + 1) kchunk) and
+// stores its raw tile in part[z][M][N]; k_ksum adds the slices.
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
-                                              const uint16_t* __restrict__ B, int ldb, Epi ep) {
+                                              const uint16_t* __restrict__ B, int ldb, Epi ep, int kchunk,
+                                              float* __restrict__ part) {
   __shared__ __align__(16) uint16_t As[BM][KPAD];
   __shared__ __align__(16) uint16_t Bs[BN][KPAD];
   int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   int wm = wave >> 1, wn = wave & 1;
   int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  int kb = blockIdx.z * kchunk, ke = kb + kchunk < K ? kb + kchunk : K;
   v4f acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  for (int k0 = kb; k0 < ke; k0 += BK) {
     stage<!TA>(As, A, lda, m0, k0, M, tid);
     stage<TB>(Bs, B, ldb, n0, k0, N, tid);
     __syncthreads();
@@ -105,6 +109,19 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const uint16_
     for (int i = 0; i < 2; i++)
       for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     __syncthreads();
+  }
+  if (part) {
+    float* P = part + (size_t)blockIdx.z * M * N;
+    for (int j = 0; j < 2; j++) {
+      int c = n0 + 32 * wn + 16 * j + (lane & 15);
+      if (c >= N) continue;
+      for (int i = 0; i < 2; i++)
+        for (int t = 0; t < 4; t++) {
+          int r = m0 + 32 * wm + 16 * i + 4 * (lane >> 4) + t;
+          if (r < M) P[(size_t)r * N + c] = acc[i][j][t];
+        }
+    }
+    return;
   }
   for (int j = 0; j < 2; j++) {
     int c = n0 + 32 * wn + 16 * j + (lane & 15);
@@ -139,6 +156,17 @@ __global__ void k_silu_bwd(int n, const float* __restrict__ dY, const float* __r
   dZb[i] = f2bf(g);
 }
 
+// Y[r][c] = sum over the nz split-K slices of part[z][r][c], c < nstore
+__global__ void k_ksum(int M, int N, int nz, const float* __restrict__ part, float* __restrict__ Y, int ldc, int nstore) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M * N) return;
+  int r = i / N, c = i - r * N;
+  if (c >= nstore) return;
+  float s = 0.f;
+  for (int z = 0; z < nz; z++) s += part[(size_t)z * M * N + i];
+  Y[(size_t)r * ldc + c] = s;
+}
+
 // column sums of an R x C row-major matrix: 64 columns x 4 row groups per block
 __global__ void __launch_bounds__(256) k_colsum(int R, int C, const float* __restrict__ X, float* __restrict__ out) {
   __shared__ float part[4][64];
@@ -150,6 +178,20 @@ __global__ void __launch_bounds__(256) k_colsum(int R, int C, const float* __res
   part[ry][cx] = s;
   __syncthreads();
   if (ry == 0 && c < C) out[c] = (part[0][cx] + part[1][cx]) + (part[2][cx] + part[3][cx]);
+}
+
+// partial column sums over 128-row slices: part[slice][c]
+__global__ void __launch_bounds__(256) k_colsum_part(int R, int C, const float* __restrict__ X, float* __restrict__ part) {
+  __shared__ float ps[4][64];
+  int cx = threadIdx.x & 63, ry = threadIdx.x >> 6;
+  int c = blockIdx.x * 64 + cx;
+  int r0 = blockIdx.y * 128, r1 = r0 + 128 < R ? r0 + 128 : R;
+  float s = 0.f;
+  if (c < C)
+    for (int r = r0 + ry; r < r1; r += 4) s += X[(size_t)r * C + c];
+  ps[ry][cx] = s;
+  __syncthreads();
+  if (ry == 0 && c < C) part[(size_t)blockIdx.y * C + c] = (ps[0][cx] + ps[1][cx]) + (ps[2][cx] + ps[3][cx]);
 }
 
 // squared error of the first nvalid columns of the first rvalid rows; gradient
@@ -265,14 +307,20 @@ __global__ void k_colstats(long R, int C, const float* __restrict__ X, long chun
 int rup(int x, int a) { return (x + a - 1) / a * a; }
 
 void gemm(hipStream_t s, bool ta, bool tb, int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B, int ldb,
-          const Epi& ep) {
+          const Epi& ep, int nz = 1, float* part = nullptr) {
   if (K % BK) throw std::runtime_error("gemm: K must be a multiple of 32");
   if ((ta && M % 8) || (!tb && N % 8)) throw std::runtime_error("gemm: row-contiguous operands need a multiple of 8");
-  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
-  if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, ep);
-  else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, ep);
-  else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, ep);
-  else hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, ep);
+  int kchunk = nz > 1 ? rup((K + nz - 1) / nz, BK) : K;
+  if (nz > 1) nz = (K + kchunk - 1) / kchunk;
+  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, nz);
+  float* pp = nz > 1 ? part : nullptr;
+  if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, ep, kchunk, pp);
+  else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, ep, kchunk, pp);
+  else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, ep, kchunk, pp);
+  else hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, ep, kchunk, pp);
+  if (nz > 1)
+    hipLaunchKernelGGL(k_ksum, dim3((M * N + 255) / 256), dim3(256), 0, s, M, N, nz, (const float*)part, ep.Yf, ep.ldc,
+                       ep.nstore);
   HIPOK(hipGetLastError());
 }
 
@@ -281,6 +329,41 @@ Epi epi(const float* bias, int act, float* Z, uint16_t* Y, float* Yf, int ldc, i
   e.bias = bias; e.act = act; e.Z = Z; e.Y = Y; e.Yf = Yf; e.ldc = ldc; e.nstore = nstore;
   return e;
 }
+
+// scratch of at least `floats` in n->part
+float* part_scratch(MjmlxMLP* n, size_t floats) {
+  if (floats > n->part_cap) {
+    if (n->part) HIPOK(hipFree(n->part));
+    HIPOK(hipMalloc(&n->part, floats * 4));
+    n->part_cap = floats;
+  }
+  return n->part;
+}
+
+// weight gradient Yf = A^T B over the batch (K = rows): few output tiles and a
+// long K, so K is split across enough blocks to cover the chip, the slices
+// summed by k_ksum
+void gemm_wgrad(MjmlxMLP* n, hipStream_t s, int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B,
+                int ldb, float* Yf, int ldc) {
+  int tiles = ((N + BN - 1) / BN) * ((M + BM - 1) / BM);
+  int nz = std::max(1, std::min(K / (4 * BK), 512 / std::max(tiles, 1)));
+  float* part = nz > 1 ? part_scratch(n, (size_t)nz * M * N) : nullptr;
+  gemm(s, true, false, M, N, K, A, lda, B, ldb, epi(nullptr, 0, nullptr, nullptr, Yf, ldc, ldc), nz, part);
+}
+
+// column sums of the first R rows of X [R][C] into out: 128-row slices in
+// parallel, then one pass over the slices
+void colsum(MjmlxMLP* n, hipStream_t s, int R, int C, const float* X, float* out) {
+  int nch = (R + 127) / 128;
+  if (nch <= 1) {
+    hipLaunchKernelGGL(k_colsum, dim3((C + 63) / 64), dim3(256), 0, s, R, C, X, out);
+    return;
+  }
+  float* part = part_scratch(n, (size_t)nch * C);
+  hipLaunchKernelGGL(k_colsum_part, dim3((C + 63) / 64, nch), dim3(256), 0, s, R, C, X, part);
+  hipLaunchKernelGGL(k_colsum, dim3((C + 63) / 64), dim3(256), 0, s, nch, C, (const float*)part, out);
+}
+
 
 // hidden layers for `rows` packed inputs x [rows][ip]
 void hidden(MjmlxMLP* n, const uint16_t* x, int rows, hipStream_t s, bool keep_z) {
@@ -303,16 +386,16 @@ void backward(MjmlxMLP* n, const uint16_t* x, int rows, int rvalid, hipStream_t 
   float scale = 1.f / (float)((size_t)rvalid * n->nout);
   hipLaunchKernelGGL(k_mse, dim3((B + 255) / 256), dim3(256), 0, s, B, O, rvalid, n->nout, n->out, n->tgt, scale,
                      n->dout, n->d3b, n->rowloss);
-  gemm(s, true, false, H, O, B, n->h2b, H, n->d3b, O, epi(nullptr, 0, nullptr, nullptr, n->gW(2), O, O));
-  hipLaunchKernelGGL(k_colsum, dim3((O + 63) / 64), dim3(256), 0, s, B, O, n->dout, n->gb(2));
+  gemm_wgrad(n, s, H, O, B, n->h2b, H, n->d3b, O, n->gW(2), O);
+  colsum(n, s, B, O, n->dout, n->gb(2));
   gemm(s, false, true, B, H, O, n->d3b, O, n->Wb(2), O, epi(nullptr, 0, nullptr, nullptr, n->dh2, H, H));
   hipLaunchKernelGGL(k_silu_bwd, dim3((B * H + 255) / 256), dim3(256), 0, s, B * H, n->dh2, n->z2, n->d2b, n->d2f);
-  gemm(s, true, false, H, H, B, n->h1b, H, n->d2b, H, epi(nullptr, 0, nullptr, nullptr, n->gW(1), H, H));
-  hipLaunchKernelGGL(k_colsum, dim3((H + 63) / 64), dim3(256), 0, s, B, H, n->d2f, n->gb(1));
+  gemm_wgrad(n, s, H, H, B, n->h1b, H, n->d2b, H, n->gW(1), H);
+  colsum(n, s, B, H, n->d2f, n->gb(1));
   gemm(s, false, true, B, H, H, n->d2b, H, n->Wb(1), H, epi(nullptr, 0, nullptr, nullptr, n->dh1, H, H));
   hipLaunchKernelGGL(k_silu_bwd, dim3((B * H + 255) / 256), dim3(256), 0, s, B * H, n->dh1, n->z1, n->d1b, n->d1f);
-  gemm(s, true, false, I, H, B, x, I, n->d1b, H, epi(nullptr, 0, nullptr, nullptr, n->gW(0), H, H));
-  hipLaunchKernelGGL(k_colsum, dim3((H + 63) / 64), dim3(256), 0, s, B, H, n->d1f, n->gb(0));
+  gemm_wgrad(n, s, I, H, B, x, I, n->d1b, H, n->gW(0), H);
+  colsum(n, s, B, H, n->d1f, n->gb(0));
   HIPOK(hipGetLastError());
 }
 
@@ -500,7 +583,7 @@ MJMLX_API void mjmlx_mlp_free(MjmlxMLP* n) {
   if (n->stream) (void)hipStreamSynchronize(n->stream);
   void* ptrs[] = {n->p, n->g, n->m, n->v, n->pb, n->in_mean, n->in_istd, n->out_mean, n->out_std,
                   n->xb, n->h1b, n->h2b, n->d3b, n->d2b, n->d1b, n->z1, n->z2, n->out, n->tgt, n->dout, n->dh2,
-                  n->dh1, n->d2f, n->d1f, n->rowloss, n->d_idx, n->ds_x, n->ds_y};
+                  n->dh1, n->d2f, n->d1f, n->rowloss, n->d_idx, n->ds_x, n->ds_y, n->part};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   if (n->stream) (void)hipStreamDestroy(n->stream);
   delete n;
