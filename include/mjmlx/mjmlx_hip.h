@@ -56,6 +56,15 @@ MJMLX_API float mjmlx_batched_last_step_ms(const MjmlxBatchedSim* sim);
  * memory). Overflows set status bits. */
 MJMLX_API int mjmlx_batched_capacity(const MjmlxBatchedSim* sim, int* maxcon, int* maxefc, int* lds_bytes);
 
+/* Autotuner: when the workspace fits in LDS, drains the device, times `steps`
+ * control steps from the current state with the workspace in LDS and in device
+ * memory (best of two rounds after a warm-up round), keeps the faster
+ * (choice: 0 LDS, 1 device memory) and restores the state (qpos,
+ * qvel, warm start; ctrl of the timed steps is zero, the solver statistics
+ * are those of the last timed step). No reference counterpart: the
+ * reference's kernels have one workspace placement per size class. */
+MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* sim, int steps, int* choice);
+
 /* Debug: one physics step like mjmlx_batched_step that also copies each env's
  * workspace after the constraint build (dump1) and after the solver (dump2),
  * float[num_envs * total] each (total from mjmlx_batched_layout("total")). */

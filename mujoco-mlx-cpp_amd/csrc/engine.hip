@@ -131,6 +131,8 @@ struct MjmlxBatchedSim {
   int persistent = 0, grid = 0, ncu = 0;
   int npset = 0;  // narrow-phase set compiled into the step kernel
   float* d_gws = nullptr;  // per-env workspaces in device memory when they exceed the LDS of a workgroup
+  bool use_gws = false;    // launch the device-memory workspace variant
+  bool lds_ok = false;     // the workspace fits the LDS of a workgroup
   int ws_mode = 0;
   // packed readback
   int* d_pack_bodies = nullptr;
@@ -178,9 +180,9 @@ void free_sim(MjmlxBatchedSim* s) {
 // one workgroup (one wave) per env, with the narrow-phase set the model needs
 void launch_step(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
   dim3 g(s->B), b(64);
-  size_t sh = s->d_gws ? 0 : s->lds_bytes;
+  size_t sh = s->use_gws ? 0 : s->lds_bytes;
   float* gw = s->d_gws;
-  if (s->d_gws) {
+  if (s->use_gws) {
     if (s->npset == 2) hipLaunchKernelGGL((k_step<2, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
     else if (s->npset == 1) hipLaunchKernelGGL((k_step<1, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
     else hipLaunchKernelGGL((k_step<0, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
@@ -220,7 +222,7 @@ void upload_ctrl(MjmlxBatchedSim* s, const float* ctrl) {
 void launch(MjmlxBatchedSim* s, int nsub, float* dump1, float* dump2) {
   mjk::StepIO io = make_io(s, dump1, dump2);
   HIPOK(hipEventRecord(s->ev0, s->stream));
-  if (s->persistent && !s->d_gws) {
+  if (s->persistent && !s->use_gws) {
     int grid = s->grid > 0 ? s->grid : std::max(1, s->ncu * 8);
     grid = std::min(grid, s->B);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s->stream, s->d_queue, 1);
@@ -499,6 +501,8 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     if (const char* v = std::getenv("MJMLX_HIP_LDS_MAX")) lds_max = (size_t)std::max(0L, std::atol(v));
     bool use_gws = s->lds_bytes > lds_max || s->lds_bytes > (size_t)prop.sharedMemPerBlock;
     if (const char* v = std::getenv("MJMLX_HIP_GLOBAL_WORKSPACE")) use_gws = use_gws || std::atoi(v) != 0;
+    s->lds_ok = s->lds_bytes <= (size_t)prop.sharedMemPerBlock;
+    s->use_gws = use_gws;
     if (use_gws) HIPOK(hipMalloc(&s->d_gws, (size_t)s->B * s->L.total * sizeof(float)));
     HIPOK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIPOK(hipEventCreate(&s->ev0));
@@ -771,6 +775,54 @@ MJMLX_API int mjmlx_batched_set_launch(MjmlxBatchedSim* s, int persistent, int g
   s->persistent = 0;
   s->grid = grid;
   return 0;
+}
+
+MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice) {
+  if (!s || steps < 1) return -1;
+  try {
+    if (s->lds_ok) {
+      size_t B = s->B, nq = std::max(1, s->nq), nv = std::max(1, s->nv);
+      float *q = nullptr, *v = nullptr, *a = nullptr;
+      HIPOK(hipMalloc(&q, B * nq * sizeof(float)));
+      HIPOK(hipMalloc(&v, B * nv * sizeof(float)));
+      HIPOK(hipMalloc(&a, B * nv * sizeof(float)));
+      auto copy = [&](float* dst, const float* src, size_t n) {
+        HIPOK(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+      };
+      copy(q, s->d_qpos, B * nq); copy(v, s->d_qvel, B * nv); copy(a, s->d_qws, B * nv);
+      if (!s->d_gws) HIPOK(hipMalloc(&s->d_gws, B * s->L.total * sizeof(float)));
+      HIPOK(hipMemsetAsync(s->d_ctrl, 0, B * std::max(1, s->nu) * sizeof(float), s->stream));
+      hipEvent_t e0, e1;
+      HIPOK(hipEventCreate(&e0));
+      HIPOK(hipEventCreate(&e1));
+      float ms[2] = {3.0e38f, 3.0e38f};
+      mjk::StepIO io = make_io(s, nullptr, nullptr);
+      // the device is drained first so that other work does not share the timing
+      HIPOK(hipDeviceSynchronize());
+      for (int rep = 0; rep < 3; rep++)
+        for (int g = 0; g < 2; g++) {
+          s->use_gws = g == 1;
+          copy(s->d_qpos, q, B * nq); copy(s->d_qvel, v, B * nv); copy(s->d_qws, a, B * nv);
+          HIPOK(hipEventRecord(e0, s->stream));
+          for (int k = 0; k < steps; k++) launch_step(s, io, 4);
+          HIPOK(hipEventRecord(e1, s->stream));
+          HIPOK(hipGetLastError());
+          HIPOK(hipEventSynchronize(e1));
+          float t = 0.f;
+          HIPOK(hipEventElapsedTime(&t, e0, e1));
+          if (rep > 0) ms[g] = std::min(ms[g], t);  // the first round warms caches and code
+        }
+      s->use_gws = ms[1] < ms[0];
+      copy(s->d_qpos, q, B * nq); copy(s->d_qvel, v, B * nv); copy(s->d_qws, a, B * nv);
+      HIPOK(hipStreamSynchronize(s->stream));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      HIPOK(hipFree(q)); HIPOK(hipFree(v)); HIPOK(hipFree(a));
+      if (!s->use_gws) { HIPOK(hipFree(s->d_gws)); s->d_gws = nullptr; }
+    }
+    if (choice) *choice = s->use_gws ? 1 : 0;
+    return 0;
+  } catch (const std::exception& e) { report("mjmlx_batched_autotune", e); return -1; }
 }
 
 MJMLX_API float mjmlx_batched_last_step_ms(const MjmlxBatchedSim* s) {

@@ -100,3 +100,26 @@ def test_device_workspace_matches_lds(monkeypatch):
         del sim
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def test_autotune_keeps_state():
+    """The workspace autotuner times both placements from the current state and
+    restores it: the tuned sim then steps bit-identically to an untuned one."""
+    model = capi.Model(scenes.boxes_xml(6, 3, solver="CG", iterations=16))
+    B = 128
+    q = np.tile(model.field("qpos0"), (B, 1)).astype(np.float32)
+    out = []
+    for tune in (False, True):
+        sim = capi.BatchedSim(model, B)
+        sim.set_state(q, np.zeros((B, model.nv), np.float32))
+        for _ in range(20):
+            sim.step(None, 4)
+        if tune:
+            choice = sim.autotune(2)
+            assert choice in (0, 1)
+        for _ in range(5):
+            sim.step(None, 4)
+        out.append(sim.state())
+        del sim
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
