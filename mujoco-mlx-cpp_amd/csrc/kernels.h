@@ -1220,12 +1220,12 @@ KDEV int k_gjk_epa(const KCvx& A, const KCvx& B, float margin, RawC* out) {
   float scale = 0.f;
   for (int i = 0; i < 4; i++) scale = fmaxf(scale, sqrtf(dot3(V[i].w, V[i].w)));
   float gtol = 1e-6f * scale, vtol = 1e-7f * scale;
-  int best = 0, pv[3] = {0, 0, 0};
+  int pv[3] = {0, 0, 0};
   float pn[4] = {0.f, 0.f, 0.f, -3.0e38f};
   for (int it = 0; it < 48; it++) {
-    best = 0;
+    int best = 0;
     for (int f = 1; f < nf; f++) if (Fn[f][3] < Fn[best][3]) best = f;
-    if (Fn[best][3] < pn[3] - gtol) { best = -1; break; }
+    if (Fn[best][3] < pn[3] - gtol) break;
     for (int q = 0; q < 3; q++) pv[q] = F[best][q];
     for (int q = 0; q < 4; q++) pn[q] = Fn[best][q];
     KSv w;
@@ -1255,14 +1255,12 @@ KDEV int k_gjk_epa(const KCvx& A, const KCvx& B, float margin, RawC* out) {
       }
     }
     nf = keep;
-    if (full || nf + ne > KEPA_MAXF) { best = -1; break; }
+    if (full || nf + ne > KEPA_MAXF) break;
     V[nvtx] = w;
     for (int q = 0; q < ne; q++) k_epa_face(V, E[q][0], E[q][1], nvtx, F, Fn, &nf);
     nvtx++;
-    best = -1;  // the polytope changed: the last accepted face stands unless a new one is chosen
   }
   // the face used: the last accepted nearest face (pv, pn)
-  (void)best;
   float depth = pn[3];
   float p[3] = {pn[0] * depth, pn[1] * depth, pn[2] * depth};
   const float *a = V[pv[0]].w, *b = V[pv[1]].w, *c = V[pv[2]].w;

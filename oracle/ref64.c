@@ -9,8 +9,9 @@
  * src/compat/mkx_kernels/collision.hpp:493-500, solver.hpp:41-45,592) this file
  * follows MuJoCo instead.
  *
- * Parity status of each part (how it is pinned) is listed in DESIGN.md of the
- * repository and checked by tests/test_oracle.py.
+ * How each part is pinned (closed forms, brute force, optimality, the
+ * reference's known-answer tests) is checked by tests/test_oracle.py and the
+ * per-feature tests beside it; the parts no check pins are listed in ref64.h.
  */
 #include "ref64.h"
 

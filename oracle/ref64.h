@@ -9,6 +9,22 @@
  * env; tests/test_gpu_vs_cpu.cpp compares against it), because MuJoCo C itself
  * is absent from this environment. Each function cites the reference file:line
  * whose semantics it follows, or states the MuJoCo behaviour it restates.
+ *
+ * Pinned by checks under tests/: kinematics, mass matrix, bias forces and
+ * integration (closed forms, brute force, finite differences of the
+ * Lagrangian); the reference's math and Cholesky known-answer tests;
+ * primitive, box and convex collision (closed forms, sampling, the
+ * support-function duality); impedance and constraint rows (direct
+ * evaluation); the solvers (KKT optimality, Newton == CG, finite differences
+ * of the elliptic cone cost); the reference's capacity rules (row counts).
+ *
+ * Parity unpinned (MuJoCo C would be needed to check them): MuJoCo's own
+ * contact points, counts and frames for box-box, capsule-box and convex
+ * (ccd) pairs beyond the closed forms; its line-search details and tolerance
+ * scaling; warm-start selection; the elliptic cone's mu regularisation with
+ * impratio and the friction rows' R; the implicitfast derivative terms;
+ * mj_setConst's invweight0 / meaninertia (checked only against this file's
+ * own FP64 setConst).
  */
 #ifndef REF64_H
 #define REF64_H

@@ -1,6 +1,6 @@
 """GPU parity of the quadruped step against the FP64 oracle (through the C ABI).
 
-Bars (BASELINE north star): contact pair lists and geom ids identical;
+Bars: contact pair lists and geom ids identical;
 single-step qacc and efc_force within 1e-4 relative (relative to the largest
 magnitude of the oracle's vector, floored at 1); determinism of duplicate envs
 and of the two launch shapes; a 1000-step rollout stays finite, with its
