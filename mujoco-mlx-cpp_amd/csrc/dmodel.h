@@ -108,7 +108,9 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   Layout L;
   L.maxcon = maxcon; L.maxefc = maxefc; L.efc_cap = maxefc; L.con_cap = maxcon; L.nv = nv;
   int nvp = nv > 0 ? nv : 1, nup = nu > 0 ? nu : 1, njp = njnt > 0 ? njnt : 1, ngp = ngeom > 0 ? ngeom : 1;
+  int ncon = maxcon > 0 ? maxcon : 1;
   int o = 0;
+  // persistent across the substep
   L.qpos = o; o += al2(nq > 0 ? nq : 1);
   L.qvel = o; o += al2(nvp);
   L.ctrl = o; o += al2(nup);
@@ -116,39 +118,46 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.qacc_pred = o; o += al2(nvp);
   L.qacc = o; o += al2(nvp);
   L.qM = o; o += al2(nvp * nvp);
-  if (!qld_shared || maxefc < nvp) { L.qLD = o; o += al2(nvp * nvp); } else { L.qLD = -1; }
+  if (!qld_shared) { L.qLD = o; o += al2(nvp * nvp); }
   L.qfrc_smooth = o; o += al2(nvp);
   L.qacc_smooth = o; o += al2(nvp);
   L.qfrc_constraint = o; o += al2(nvp);
   L.act_force = o; o += al2(nup);
   L.misc = o; o += MISC_SIZE;
-  // region A
-  int a0 = o;
+  o = al2(o);
+  // The rest is packed by lifetime within a substep:
+  //   S  scom, cdof          com positions -> constraint rows
+  //   G  geom poses          kinematics -> collision
+  //   KC the other kinematics and dynamics intermediates, kinematics -> smooth
+  //   con, rows              collision / constraint rows -> end of the substep
+  //   V  solver vectors      solve (tmpv also integration)
+  //   H  FP64 Hessian        solve; under Newton the LDL factor of qM (factor
+  //                          -> smooth, and the integrator's factor) uses it
+  // con starts where KC starts (KC is dead once collision runs); the rows
+  // follow con; V takes S and G's space when it fits (both dead in the solve).
+  int r0 = o;
+  L.scom = o; o += 3 * nbody;
+  L.cdof = o; o += 6 * nvp;
+  L.gxpos = o; o += 3 * ngp;
+  L.gxmat = o; o += 9 * ngp;
+  o = al2(o);
+  int kc0 = o;
   L.xpos = o; o += 3 * nbody;
   L.xquat = o; o += 4 * nbody;
   L.xmat = o; o += 9 * nbody;
   L.xipos = o; o += 3 * nbody;
   L.xanchor = o; o += 3 * njp;
   L.xaxis = o; o += 3 * njp;
-  L.gxpos = o; o += 3 * ngp;
-  L.gxmat = o; o += 9 * ngp;
-  L.scom = o; o += 3 * nbody;
   L.cinert = o; o += 10 * nbody;
-  L.cdof = o; o += 6 * nvp;
   L.cvel = o; o += 6 * nbody;
   L.cdofdot = o; o += 6 * nvp;
   L.crb = o; o += 10 * nbody;
   L.cacc = o; o += 6 * nbody;
   L.cfrc = o; o += 6 * nbody;
   L.buf6 = o; o += 6 * nvp;
-  o = al2(o);
-  int aend = o;
-  L.H = al2(a0);
-  int hend = L.H + 2 * nvp * nvp;
-  o = aend > hend ? aend : hend;
-  o = al2(o);
-  // region B
-  L.con = o; o += CON_STRIDE * (maxcon > 0 ? maxcon : 1);
+  int kc_end = al2(o);
+  o = kc0;
+  L.con = o; o += CON_STRIDE * ncon;
   L.efc_J = o; o += maxefc * nvp;
   L.efc_D = o; o += maxefc;
   L.efc_R = o; o += maxefc;
@@ -165,17 +174,24 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   // solver-only vectors Jv and Dact take their place afterwards
   L.efc_pos = L.efc_Jv;
   L.efc_margin = L.efc_Dact;
-  L.Ma = o; o += nvp;
-  L.grad = o; o += nvp;
-  L.Mgrad = o; o += nvp;
-  L.search = o; o += nvp;
-  L.Mv = o; o += nvp;
-  L.grad_old = o; o += nvp;
-  L.Mgrad_old = o; o += nvp;
-  L.tmpv = o; o += nvp;
-  o = al2(o);
-  L.dvec = o; o += 4 * nvp;  // two FP64 vectors: inverse pivots and a solve vector
-  if (L.qLD < 0) L.qLD = al2(L.efc_J);
+  int rows_end = al2(o);
+  o = al2(kc_end > rows_end ? kc_end : rows_end);
+  L.H = o; o += al2(2 * nvp * nvp);
+  if (qld_shared) L.qLD = L.H;
+  int nvec = al2(8 * nvp) + 4 * nvp;  // eight FP32 vectors, then two FP64 ones
+  int vbase = nvec <= kc0 - r0 ? r0 : o;
+  if (vbase == o) o += nvec;
+  int v = vbase;
+  L.Ma = v; v += nvp;
+  L.grad = v; v += nvp;
+  L.Mgrad = v; v += nvp;
+  L.search = v; v += nvp;
+  L.Mv = v; v += nvp;
+  L.grad_old = v; v += nvp;
+  L.Mgrad_old = v; v += nvp;
+  L.tmpv = v; v += nvp;
+  v = al2(v);
+  L.dvec = v;  // two FP64 vectors: inverse pivots and a solve vector
   L.total = al2(o);
   return L;
 }

@@ -54,6 +54,18 @@ __global__ void __launch_bounds__(64) k_step(mjk::DModel m, mjk::Layout L, mjk::
   mjk::run_env<NPSET>(m, L, w, io, env, nsub);
 }
 
+// The same step capped at 168 VGPRs (three waves per SIMD, a few hundred
+// bytes of spills per lane), for workspaces small enough that LDS admits more
+// than eight envs per CU.
+template <int NPSET>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_step_occ(mjk::DModel m, mjk::Layout L,
+                                                                                     mjk::StepIO io, int B, int nsub) {
+  extern __shared__ __align__(16) float lds[];
+  int env = blockIdx.x;
+  if (env >= B) return;
+  mjk::run_env<NPSET>(m, L, lds, io, env, nsub);
+}
+
 __global__ void __launch_bounds__(64) k_step_queue(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub,
                                                    int* queue) {
   extern __shared__ __align__(16) float lds[];
@@ -133,6 +145,7 @@ struct MjmlxBatchedSim {
   float* d_gws = nullptr;  // per-env workspaces in device memory when they exceed the LDS of a workgroup
   bool use_gws = false;    // launch the device-memory workspace variant
   bool lds_ok = false;     // the workspace fits the LDS of a workgroup
+  bool high_occ = false;   // LDS admits more than 8 envs per CU: the 168-VGPR kernel
   int ws_mode = 0;
   // packed readback
   int* d_pack_bodies = nullptr;
@@ -186,6 +199,10 @@ void launch_step(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
     if (s->npset == 2) hipLaunchKernelGGL((k_step<2, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
     else if (s->npset == 1) hipLaunchKernelGGL((k_step<1, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
     else hipLaunchKernelGGL((k_step<0, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
+  } else if (s->high_occ) {
+    if (s->npset == 2) hipLaunchKernelGGL((k_step_occ<2>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
+    else if (s->npset == 1) hipLaunchKernelGGL((k_step_occ<1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
+    else hipLaunchKernelGGL((k_step_occ<0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
   } else {
     if (s->npset == 2) hipLaunchKernelGGL((k_step<2, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
     else if (s->npset == 1) hipLaunchKernelGGL((k_step<1, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
@@ -503,6 +520,9 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     if (const char* v = std::getenv("MJMLX_HIP_GLOBAL_WORKSPACE")) use_gws = use_gws || std::atoi(v) != 0;
     s->lds_ok = s->lds_bytes <= (size_t)prop.sharedMemPerBlock;
     s->use_gws = use_gws;
+    size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)prop.maxSharedMemoryPerMultiProcessor : 163840;
+    s->high_occ = !use_gws && lds_cu / std::max<size_t>(s->lds_bytes, 1) > 8;
+    if (const char* v = std::getenv("MJMLX_HIP_HIGH_OCC")) s->high_occ = !use_gws && std::atoi(v) != 0;
     if (use_gws) HIPOK(hipMalloc(&s->d_gws, (size_t)s->B * s->L.total * sizeof(float)));
     HIPOK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIPOK(hipEventCreate(&s->ev0));

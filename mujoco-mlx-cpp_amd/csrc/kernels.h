@@ -2543,6 +2543,9 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     if (nv > 0) factor_ld(m, w + L.qM, w + L.qLD);
     st_com_vel(m, L, w);
     st_smooth(m, L, w);
+    // the kinematics and dynamics intermediates are intact only here: the
+    // constraint rows of a full substep take their space
+    dump_ws(L, w, io.dump1, env);
     if (io.has_aux) {
       write_aux(m, L, w, io.aux, env);
       write_cfrc_ext(m, L, w, io.aux, env, false);
