@@ -344,7 +344,8 @@ def run(args):
         "learned": learned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_step", "kernel_ms_avg": plain["ms_per_step"],
+                     "kernel": "k_step (fused 4-substep step; k_step_occ, its 168-VGPR variant, for small workspaces)",
+                     "kernel_ms_avg": plain["ms_per_step"],
                      "bytes_per_env_step": bpe},
         "cpu_baseline": None,
     }
