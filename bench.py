@@ -301,6 +301,7 @@ def run(args):
         if mode == "learned":
             sim.set_warmstart_mlp(net)
             assert sim.set_warmstart_mode(2) == 0
+            sim.set_graph(True)  # the per-substep launches replayed as one captured graph
         sim.set_state(q, qv)
         sim.set_qacc_warmstart(qa)
         ms, wall, it = timed_run(sim, pool, 5, args.steps, offset=3)

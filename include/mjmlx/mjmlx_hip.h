@@ -65,6 +65,14 @@ MJMLX_API int mjmlx_batched_capacity(const MjmlxBatchedSim* sim, int* maxcon, in
  * reference's kernels have one workspace placement per size class. */
 MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* sim, int steps, int* choice);
 
+/* Learned warm start (mode 2): replay each control step's launches (features,
+ * network, one step launch per substep) as a captured HIP graph instead of
+ * issuing them one by one (on != 0). The graph is captured on first use and
+ * again whenever the substep count, network or kernel variant changes; steps
+ * without a network or without features yet run eagerly. No reference
+ * counterpart (the reference has no learned warm start). */
+MJMLX_API int mjmlx_batched_set_graph(MjmlxBatchedSim* sim, int on);
+
 /* Debug: one physics step like mjmlx_batched_step that also copies each env's
  * workspace after the constraint build (dump1) and after the solver (dump2),
  * float[num_envs * total] each (total from mjmlx_batched_layout("total")). */

@@ -147,6 +147,14 @@ struct MjmlxBatchedSim {
   bool lds_ok = false;     // the workspace fits the LDS of a workgroup
   bool high_occ = false;   // LDS admits more than 8 envs per CU: the 168-VGPR kernel
   int ws_mode = 0;
+  // learned control step captured as a HIP graph (mjmlx_batched_set_graph):
+  // the captured launch sequence and the inputs it was captured for
+  bool graph_on = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_n = -1, g_cur = -1, g_mode = -1;
+  const void *g_mlp = nullptr, *g_xb = nullptr;
+  bool g_gws = false, g_occ = false;
   // packed readback
   int* d_pack_bodies = nullptr;
   float* d_pack = nullptr;
@@ -186,6 +194,8 @@ void free_sim(MjmlxBatchedSim* s) {
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
   if (s->ev_ctrl) hipEventDestroy(s->ev_ctrl);
+  if (s->gexec) hipGraphExecDestroy(s->gexec);
+  if (s->graph) hipGraphDestroy(s->graph);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
 }
@@ -268,8 +278,54 @@ void ensure_learned(MjmlxBatchedSim* s) {
 // substep's qacc from the previous substep's features and this call's ctrl
 // before the launch; with `rec` every substep that has input features appends
 // (features, converged qacc) rows to rec's training set.
+// The learned control step's launches (features, network, one step launch per
+// substep), issued on the sim's stream; feat_cur alternates per substep.
+void issue_learned(MjmlxBatchedSim* s, int n, int cur) {
+  int coff = s->nq + s->nv;
+  for (int k = 0; k < n; k++) {
+    mjk::StepIO io = make_io(s, nullptr, nullptr);
+    io.accumulate = k > 0;
+    int nf = s->nfeat();
+    const float* base = s->feat_mode ? s->d_feat[cur] + s->dm.nfeat : nullptr;
+    mjmlp::pack_features(s->mlp, s->d_feat[cur], s->d_ctrl, coff, s->nu, s->B, s->stream);
+    mjmlp::predict(s->mlp, s->B, s->d_pred, s->stream, base, nf);
+    io.qacc_pred = s->d_pred;
+    io.feat = s->d_feat[cur ^ 1];
+    io.feat_prev_qacc = s->feat_mode;
+    launch_step(s, io, 1);
+    cur ^= 1;
+  }
+}
+
+// Replays the learned control step as a HIP graph, captured again whenever
+// what its launches bake in changes (substep count, feature buffer parity,
+// network and its buffers, feature mode, kernel variant). Returns false when
+// the graph path does not apply (no network or no features yet).
+bool step_graph(MjmlxBatchedSim* s, int n) {
+  if (!s->graph_on || s->ws_mode != 2 || !s->mlp || !s->feat_valid) return false;
+  s->mlp->ensure(s->B);  // allocations happen before any capture
+  if (!s->gexec || s->g_n != n || s->g_cur != s->feat_cur || s->g_mlp != s->mlp || s->g_xb != s->mlp->xb ||
+      s->g_mode != s->feat_mode || s->g_gws != s->use_gws || s->g_occ != s->high_occ) {
+    if (s->gexec) { HIPOK(hipGraphExecDestroy(s->gexec)); s->gexec = nullptr; }
+    if (s->graph) { HIPOK(hipGraphDestroy(s->graph)); s->graph = nullptr; }
+    HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    issue_learned(s, n, s->feat_cur);
+    HIPOK(hipStreamEndCapture(s->stream, &s->graph));
+    HIPOK(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+    s->g_n = n; s->g_cur = s->feat_cur; s->g_mlp = s->mlp; s->g_xb = s->mlp->xb;
+    s->g_mode = s->feat_mode; s->g_gws = s->use_gws; s->g_occ = s->high_occ;
+  }
+  HIPOK(hipEventRecord(s->ev0, s->stream));
+  HIPOK(hipGraphLaunch(s->gexec, s->stream));
+  HIPOK(hipEventRecord(s->ev1, s->stream));
+  s->timed = true;
+  s->feat_cur ^= n & 1;
+  return true;
+}
+
 long step_substeps(MjmlxBatchedSim* s, int n, MjmlxMLP* rec) {
   ensure_learned(s);
+  if (!rec && step_graph(s, n)) return 0;
   long rows = 0;
   int coff = s->nq + s->nv;
   HIPOK(hipEventRecord(s->ev0, s->stream));
@@ -843,6 +899,12 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
     if (choice) *choice = s->use_gws ? 1 : 0;
     return 0;
   } catch (const std::exception& e) { report("mjmlx_batched_autotune", e); return -1; }
+}
+
+MJMLX_API int mjmlx_batched_set_graph(MjmlxBatchedSim* s, int on) {
+  if (!s) return -1;
+  s->graph_on = on != 0;
+  return 0;
 }
 
 MJMLX_API float mjmlx_batched_last_step_ms(const MjmlxBatchedSim* s) {

@@ -86,6 +86,7 @@ def lib() -> C.CDLL:
             "mjmlx_batched_last_step_ms": (C.c_float, [C.c_void_p]),
             "mjmlx_batched_capacity": (C.c_int, [C.c_void_p, _I, _I, _I]),
             "mjmlx_batched_autotune": (C.c_int, [C.c_void_p, C.c_int, _I]),
+            "mjmlx_batched_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
             "mjmlx_batched_debug_step": (C.c_int, [C.c_void_p, _F, _F, _F]),
             "mjmlx_batched_layout": (C.c_int, [C.c_void_p, C.c_char_p, _I, _I, _I]),
             "mjmlx_mlp_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -226,6 +227,10 @@ class BatchedSim:
 
     def last_step_ms(self) -> float:
         return lib().mjmlx_batched_last_step_ms(self.h)
+
+    def set_graph(self, on: bool = True):
+        """Replay the learned control step as a captured HIP graph."""
+        assert lib().mjmlx_batched_set_graph(self.h, int(on)) == 0
 
     def autotune(self, steps: int = 3) -> int:
         """Time both workspace placements from the current state and keep the

@@ -165,3 +165,34 @@ def test_learned_warm_start_keeps_results_and_reports_iterations():
             it_l += sl["niter"].mean()
     print(f"mean solver iterations per env-step: plain {it_p / 3:.3f}, learned {it_l / 3:.3f}")
     assert it_l <= 1.1 * it_p
+
+
+def test_graph_replay_matches_eager():
+    """The learned control step replayed as a captured HIP graph gives the
+    eager launches' results bit for bit, on every replay, with eager work
+    (state readback, statistics) and synchronisations between the replays,
+    and after a change of feature-buffer parity forces a new capture."""
+    B = 512
+    model, src = _quadruped_sim(B, 3)
+    net = capi.MLP(src.feature_dim(), NH, model.nv, seed=2)
+    q, qv = src.state()
+    qa = src.qacc_warmstart()
+    sims = []
+    for graph in (False, True):
+        s = capi.BatchedSim(model, B)
+        s.set_warmstart_mlp(net)
+        assert s.set_warmstart_mode(2) == 0
+        s.set_graph(graph)
+        s.set_state(q, qv)
+        s.set_qacc_warmstart(qa)
+        sims.append(s)
+    rng = np.random.default_rng(11)
+    for k in range(8):
+        c = scenes.quadruped_actions(B, rng).astype(np.float32)
+        nsub = 3 if k == 5 else 4  # an odd count flips the feature parity
+        for s in sims:
+            s.step(c, nsub)
+        (qe, ve), (qg, vg) = sims[0].state(), sims[1].state()
+        assert np.array_equal(qe, qg) and np.array_equal(ve, vg), k
+        se, sg = sims[0].stats(), sims[1].stats()
+        assert np.array_equal(se["niter"], sg["niter"]), k
