@@ -136,7 +136,9 @@ MJMLX_API int mjmlx_batched_feature_dim(const MjmlxBatchedSim* sim);
  * the plain feature set. Changing the mode removes the installed network. */
 MJMLX_API int mjmlx_batched_set_feature_mode(MjmlxBatchedSim* sim, int mode);
 /* installs the network used by warm-start mode 2 (NULL removes it and leaves mode 2);
- * returns -1 when its widths do not match the model */
+ * returns -1 when its widths do not match the model. The network's activation
+ * buffers serve one sim at a time: sims sharing a network must not step
+ * concurrently on different streams. */
 MJMLX_API int mjmlx_batched_set_warmstart_mlp(MjmlxBatchedSim* sim, MjmlxMLP* mlp);
 /* steps like step_n and appends one (features, converged qacc) row per env and
  * substep to mlp's training set (the first substep after a state change has no
