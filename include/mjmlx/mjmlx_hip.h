@@ -42,9 +42,10 @@ MJMLX_API int mjmlx_batched_set_qacc_warmstart(MjmlxBatchedSim* sim, const float
  * 64 non-finite state), ncon / nefc of the last substep. */
 MJMLX_API int mjmlx_batched_solver_stats(const MjmlxBatchedSim* sim, int* niter, int* status, int* ncon, int* nefc);
 
-/* Launch shape: persistent != 0 runs a grid of `grid` workgroups (0 = chosen
- * from the CU count) that pull envs from a device work queue; 0 launches one
- * workgroup per env. Returns 0 or -1. */
+/* Launch shape: 0 launches one workgroup per env (returns 0). persistent != 0
+ * would run a grid of `grid` workgroups pulling envs from a device work queue;
+ * that variant is switched off (it hung once with no cause found) and returns
+ * -1 with a message. */
 MJMLX_API int mjmlx_batched_set_launch(MjmlxBatchedSim* sim, int persistent, int grid);
 
 /* Time in milliseconds of the last step call's kernel, measured with HIP
