@@ -192,6 +192,7 @@ def test_graph_replay_matches_eager():
         nsub = 3 if k == 5 else 4  # an odd count flips the feature parity
         for s in sims:
             s.step(c, nsub)
+            s.sync()  # the two sims share the network's activation buffers
         (qe, ve), (qg, vg) = sims[0].state(), sims[1].state()
         assert np.array_equal(qe, qg) and np.array_equal(ve, vg), k
         se, sg = sims[0].stats(), sims[1].stats()
