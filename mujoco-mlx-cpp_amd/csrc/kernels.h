@@ -502,15 +502,25 @@ KDEV void solve_ld(const DModel& m, const float* LD, float* x) {
   int nv = m.nv;
 #if WAVE == 64
   if (nv <= WAVE) {
+    // the next step's factor entry is loaded ahead of the current step's
+    // readlane -> FMA chain, so the LDS latency overlaps it
+    int ln = LANE < nv ? LANE : 0;
     float v = LANE < nv ? x[LANE] : 0.f;
+    float ln_next = nv > 1 ? LD[(nv - 1) * nv + ln] : 0.f;
     for (int k = nv - 1; k >= 1; k--) {  // L^T y = x: column k updates rows i < k
+      float l = ln_next;
+      if (k > 1) ln_next = LD[(k - 1) * nv + ln];
       float xk = rl_f(v, k);
-      if (LANE < k) v -= LD[k * nv + LANE] * xk;
+      if (LANE < k) v -= l * xk;
     }
     if (LANE < nv) v /= LD[LANE * nv + LANE];
+    const float* LDr = LD + ln * nv;
+    ln_next = LDr[0];
     for (int i = 0; i < nv - 1; i++) {   // L z = y: solved entry i updates rows k > i
+      float l = ln_next;
+      if (i + 1 < nv - 1) ln_next = LDr[i + 1];
       float xi = rl_f(v, i);
-      if (LANE > i && LANE < nv) v -= LD[LANE * nv + i] * xi;
+      if (LANE > i && LANE < nv) v -= l * xi;
     }
     KSYNC();
     if (LANE < nv) x[LANE] = v;
@@ -1965,10 +1975,11 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
   for (; k + 1 < nv - 1; k += 2) {
     double d0 = H[k * nv + k];
     if (!(d0 > 0.0)) { bad = 1; d0 = 1e-30; }
-    double h10 = H[(k + 1) * nv + k], c = h10 / d0;
+    double i0 = 1.0 / d0, h10 = H[(k + 1) * nv + k], c = h10 * i0;
     double d1 = H[(k + 1) * nv + k + 1] - c * h10;
     if (!(d1 > 0.0)) { bad = 1; d1 = 1e-30; }
-    double i0 = 1.0 / d0, i1 = 1.0 / d1;
+    double i1 = 1.0 / d1;
+    if (LANE == 0) idg[k] = i0;  // the deferred pass below takes c from it
     int mm = nv - k - 2, nt = mm * (mm + 1) / 2;
     for (int t = LANE; t < nt; t += WAVE) {
       int a, b;
@@ -1995,7 +2006,7 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
   }
   // deferred updates of the second column of each pair: H[i][k+1] -= c H[i][k], i >= k+1
   for (int kk = 0; kk + 1 < nv - 1; kk += 2) {
-    double c = H[(kk + 1) * nv + kk] / H[kk * nv + kk];
+    double c = H[(kk + 1) * nv + kk] * idg[kk];
     for (int i = kk + 1 + LANE; i < nv; i += WAVE) H[i * nv + kk + 1] -= c * H[i * nv + kk];
   }
   KSYNC();
@@ -2037,16 +2048,25 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
     KSYNC();
     return;
   }
+  // each step's factor entry is loaded one step ahead (see solve_ld)
+  int ln = LANE < nv ? LANE : 0;
   double v = LANE < nv ? (double)grad[LANE] : 0.0;
   double id = LANE < nv ? idg[LANE] : 1.0;
+  const double* Hr = H + ln * nv;
+  double hn = Hr[0];
   for (int k = 0; k < nv - 1; k++) {          // L y = g, L_ik = H_ik / H_kk
+    double h = hn;
+    if (k + 1 < nv - 1) hn = Hr[k + 1];
     double yk = rl_d(v, k) * rl_d(id, k);
-    if (LANE > k && LANE < nv) v -= H[LANE * nv + k] * yk;
+    if (LANE > k && LANE < nv) v -= h * yk;
   }
   v *= id;                                      // D z = y
+  hn = nv > 1 ? H[(nv - 1) * nv + ln] : 0.0;
   for (int k = nv - 1; k >= 1; k--) {          // L^T x = z
+    double h = hn;
+    if (k > 1) hn = H[(k - 1) * nv + ln];
     double xk = rl_d(v, k);
-    if (LANE < k) v -= H[k * nv + LANE] * id * xk;
+    if (LANE < k) v -= h * id * xk;
   }
   KSYNC();
   if (LANE < nv) Mg[LANE] = (float)v;
