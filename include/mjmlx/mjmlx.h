@@ -121,7 +121,14 @@ MJMLX_API int mjmlx_batched_nsensordata(const MjmlxBatchedSim* sim);            
 MJMLX_API const float* mjmlx_batched_get_sensordata(const MjmlxBatchedSim* sim, int* n_out);     /* :221 */
 MJMLX_API void mjmlx_batched_set_env_qpos(MjmlxBatchedSim* sim, int env_idx, const float* qpos, int nq); /* :224 */
 MJMLX_API void mjmlx_batched_set_env_qvel(MjmlxBatchedSim* sim, int env_idx, const float* qvel, int nv); /* :225 */
-/* per-env physics randomisation (:233); not implemented: returns -1 */
+/* per-env physics randomisation of a sim created with domain_rand = 1 (:233-238):
+ * mass_scale [n][nbody] scales each body's mass and inertia, com_delta
+ * [n][nbody][3] shifts its COM in the body frame, friction_scale [n] scales
+ * every contact friction coefficient, gain_scale [n][nu] scales each
+ * actuator's force, drag_k [n][num_free_joints] adds F = -k |v| v on each
+ * free joint's translation. Rows go to env_idx[i] (env i when env_idx is
+ * NULL and n = num_envs); NULL arrays are kept. 0 ok, -1 bad handle or no
+ * domain_rand, -2 env index out of range, -3 (drag) no free joint. */
 MJMLX_API int mjmlx_batched_set_env_params(MjmlxBatchedSim* sim, const int* env_idx, int n, const float* mass_scale,
                                            const float* com_delta, const float* friction_scale, const float* gain_scale);
 MJMLX_API int mjmlx_batched_set_free_drag(MjmlxBatchedSim* sim, const int* env_idx, int n, const float* drag_k); /* :237 */

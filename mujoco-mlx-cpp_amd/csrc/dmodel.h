@@ -79,6 +79,7 @@ struct Layout {
   // con_cap: contacts counted per env (maxcon <= con_cap of them are stored:
   // at least every contact the row budget can admit)
   int maxcon, maxefc, efc_cap, con_cap, nv, total;
+  int rand;  // per-env physical parameters (rand_size floats), or -1 without domain randomisation
   int qpos, qvel, ctrl, qacc_ws, qacc_pred, qacc, qM, qLD, qfrc_smooth, qacc_smooth, qfrc_constraint, act_force, misc;
   int xpos, xquat, xmat, xipos, xanchor, xaxis, gxpos, gxmat, scom, cinert, cdof, cvel, cdofdot, crb, cacc, cfrc, buf6;
   int H;
@@ -99,12 +100,22 @@ enum { MISC_NCON = 0, MISC_NEFC = 1, MISC_STATUS = 2, MISC_NITER = 3, MISC_NFLOS
 
 MJK_HD inline int al2(int x) { return (x + 1) & ~1; }
 
+// Per-env physical parameters of domain randomisation, one block per env:
+// mass scale per body [nbody], body-frame COM shift per body [nbody][3],
+// friction scale [1], actuator force scale per actuator [nu], quadratic drag
+// coefficient per free joint [nfree] (in joint order).
+MJK_HD inline int rand_com(int nbody) { return nbody; }
+MJK_HD inline int rand_fric(int nbody) { return 4 * nbody; }
+MJK_HD inline int rand_gain(int nbody) { return 4 * nbody + 1; }
+MJK_HD inline int rand_drag(int nbody, int nu) { return 4 * nbody + 1 + nu; }
+MJK_HD inline int rand_size(int nbody, int nu, int nfree) { return 4 * nbody + 1 + nu + nfree; }
+
 // qld_shared: the solver never reads the LDL factor of qM (Newton), so the
 // factor lives in the efc Jacobian's space: it is dead once qacc_smooth is
 // solved, before the Jacobian is built, and the integrator reuses the space
 // as scratch after the solve.
 MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, int ngeom, int maxcon, int maxefc,
-                                 bool qld_shared = false) {
+                                 bool qld_shared = false, int nrand = 0) {
   Layout L;
   L.maxcon = maxcon; L.maxefc = maxefc; L.efc_cap = maxefc; L.con_cap = maxcon; L.nv = nv;
   int nvp = nv > 0 ? nv : 1, nup = nu > 0 ? nu : 1, njp = njnt > 0 ? njnt : 1, ngp = ngeom > 0 ? ngeom : 1;
@@ -124,6 +135,8 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.qfrc_constraint = o; o += al2(nvp);
   L.act_force = o; o += al2(nup);
   L.misc = o; o += MISC_SIZE;
+  L.rand = nrand > 0 ? o : -1;
+  o += nrand;
   o = al2(o);
   // The rest is packed by lifetime within a substep:
   //   S  scom, cdof          com positions -> constraint rows

@@ -23,8 +23,10 @@ struct EmuSim {
   mjk::Layout L;
   std::vector<mjh::LayoutField> fields;
   int B;
-  std::vector<float> qpos, qvel, qacc_ws, ws, dump1, dump2, feat, pred;
+  std::vector<float> qpos, qvel, qacc_ws, ws, dump1, dump2, feat, pred, rand;
   bool use_pred = false;
+  mjh::Caps caps;
+  int nrand = 0;
   std::vector<int> niter, status, ncon, nefc;
 };
 }  // namespace
@@ -44,6 +46,7 @@ __attribute__((visibility("default"))) void* mjemu_create(const char* xml, int s
     s->dm = s->blob.bind(s->blob.bytes.data());
     // maxcon / maxefc > 0 replace the contact cap and the row budget
     mjh::Caps caps = mjh::default_caps(s->model, pairs, maxcon, maxefc);
+    s->caps = caps;
     auto& m = s->model;
     s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, caps.maxcon, caps.maxefc,
                             m.solver == mjh::SOL_NEWTON);
@@ -62,6 +65,25 @@ __attribute__((visibility("default"))) void* mjemu_create(const char* xml, int s
     std::fprintf(stderr, "mjemu_create: %s\n", e.what());
     return nullptr;
   }
+}
+
+// Domain randomisation: per-env parameter blocks [B][rand_size] in the HIP
+// engine's order (mass scales, COM shifts, friction scale, force scales, free
+// joint drag); NULL switches it off. Returns the block size.
+__attribute__((visibility("default"))) int mjemu_set_rand(void* p, const float* blocks) {
+  auto* s = static_cast<EmuSim*>(p);
+  auto& m = s->model;
+  int nfree = 0;
+  for (int j = 0; j < m.njnt; j++) nfree += m.jnt_type[j] == mjk::JFREE;
+  s->nrand = blocks ? mjk::rand_size(m.nbody, m.nu, nfree) : 0;
+  s->rand.assign(blocks, blocks ? blocks + (size_t)s->B * s->nrand : blocks);
+  s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, s->caps.maxcon, s->caps.maxefc,
+                          m.solver == mjh::SOL_NEWTON, s->nrand);
+  s->L.efc_cap = s->caps.efc_cap;
+  s->L.con_cap = s->caps.con_cap;
+  s->fields = mjh::layout_fields(m, s->L);
+  s->ws.assign(s->L.total, 0.f);
+  return s->nrand;
 }
 
 __attribute__((visibility("default"))) void mjemu_free(void* p) { delete static_cast<EmuSim*>(p); }
@@ -87,6 +109,7 @@ __attribute__((visibility("default"))) int mjemu_step(void* p, const float* ctrl
   io.niter = s->niter.data(); io.status = s->status.data(); io.ncon = s->ncon.data(); io.nefc = s->nefc.data();
   s->feat.assign((size_t)s->B * s->dm.nfeat, 0.f);
   io.feat = s->feat.data();
+  if (s->nrand > 0) { io.rand = s->rand.data(); io.nrand = s->nrand; }
   if (s->use_pred) io.qacc_pred = s->pred.data();
   if (dump) {
     s->dump1.assign((size_t)s->B * s->L.total, 0.f);

@@ -167,6 +167,11 @@ struct MjmlxBatchedSim {
   bool feat_valid = false;
   int feat_mode = 0;  // 1: features end with the previous substep's qacc and the network predicts the change
   int nfeat() const { return dm.nfeat + (feat_mode ? nv : 0); }
+  // domain randomisation: per-env parameter blocks (mjk::rand_size floats per
+  // env, env-major), the host copy the setters edit, and the free-joint count
+  float* d_rand = nullptr;
+  std::vector<float> h_rand;
+  int nrand = 0, nfree = 0;
   // scratch host buffers for dumps
   float *d_dump1 = nullptr, *d_dump2 = nullptr;
   size_t dump_cap = 0;
@@ -184,7 +189,7 @@ void free_sim(MjmlxBatchedSim* s) {
   if (s->stream) hipStreamSynchronize(s->stream);
   void* dev[] = {s->d_blob, s->d_qpos, s->d_qvel, s->d_qws, s->d_ctrl, s->d_qpos0, s->d_niter, s->d_status,
                  s->d_ncon, s->d_nefc, s->d_queue, s->d_mask, s->d_pack_bodies, s->d_pack, s->d_dump1, s->d_dump2,
-                 s->d_feat[0], s->d_feat[1], s->d_pred, s->d_gws};
+                 s->d_feat[0], s->d_feat[1], s->d_pred, s->d_gws, s->d_rand};
   for (void* p : dev) if (p) hipFree(p);
   for (int i = 0; i < 7; i++) { if (s->d_aux[i]) hipFree(s->d_aux[i]); if (s->h_aux[i]) hipHostFree(s->h_aux[i]); }
   if (s->h_qpos) hipHostFree(s->h_qpos);
@@ -229,6 +234,7 @@ mjk::StepIO make_io(MjmlxBatchedSim* s, float* dump1, float* dump2) {
   io.aux.xpos = s->d_aux[0]; io.aux.xquat = s->d_aux[1]; io.aux.subtree_com = s->d_aux[2];
   io.aux.cinert = s->d_aux[3]; io.aux.cvel = s->d_aux[4]; io.aux.qfrc_actuator = s->d_aux[5];
   io.aux.cfrc_ext = s->d_aux[6];
+  io.rand = s->d_rand; io.nrand = s->nrand;
   return io;
 }
 
@@ -529,7 +535,6 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     if (config->num_envs <= 0) throw std::runtime_error("num_envs must be positive");
     if (!config->use_gpu)
       throw std::runtime_error("use_gpu=0 requests the MuJoCo C CPU path, which is not part of the HIP engine");
-    if (config->domain_rand) throw std::runtime_error("domain randomisation is not implemented by the HIP engine");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw std::runtime_error("no HIP device available");
     s = new MjmlxBatchedSim();
@@ -555,8 +560,12 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     if (const char* v = std::getenv("MJMLX_HIP_MAXCON")) con_over = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("MJMLX_HIP_MAXEFC")) efc_over = std::max(1, std::atoi(v));
     mjh::Caps caps = mjh::default_caps(m, pairs, con_over, efc_over);
+    // domain randomisation (reference src/batched.cpp:1348-1361): each env's
+    // parameter block rides in its workspace
+    for (int j = 0; j < m.njnt; j++) s->nfree += m.jnt_type[j] == mjk::JFREE;
+    if (config->domain_rand) s->nrand = mjk::rand_size(m.nbody, m.nu, s->nfree);
     s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, caps.maxcon, caps.maxefc,
-                            m.solver == mjh::SOL_NEWTON);
+                            m.solver == mjh::SOL_NEWTON, s->nrand);
     s->L.efc_cap = caps.efc_cap;
     s->L.con_cap = caps.con_cap;
     s->fields = mjh::layout_fields(m, s->L);
@@ -614,6 +623,18 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     HIPOK(hipMemset(s->d_status, 0, B * sizeof(int)));
     HIPOK(hipMemset(s->d_ncon, 0, B * sizeof(int)));
     HIPOK(hipMemset(s->d_nefc, 0, B * sizeof(int)));
+    if (s->nrand > 0) {
+      // identity parameters: unit mass, friction and force scales, no COM shift, no drag
+      s->h_rand.assign(B * s->nrand, 0.f);
+      for (size_t e = 0; e < B; e++) {
+        float* r = s->h_rand.data() + e * s->nrand;
+        for (int b = 0; b < m.nbody; b++) r[b] = 1.f;
+        r[mjk::rand_fric(m.nbody)] = 1.f;
+        for (int u = 0; u < m.nu; u++) r[mjk::rand_gain(m.nbody) + u] = 1.f;
+      }
+      HIPOK(hipMalloc(&s->d_rand, s->h_rand.size() * sizeof(float)));
+      HIPOK(hipMemcpy(s->d_rand, s->h_rand.data(), s->h_rand.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     return s;
   } catch (const std::exception& e) {
     report("mjmlx_batched_create", e);
@@ -731,12 +752,62 @@ MJMLX_API void mjmlx_batched_set_env_qvel(MjmlxBatchedSim* s, int env, const flo
   } catch (const std::exception& e) { report("mjmlx_batched_set_env_qvel", e); }
 }
 
-MJMLX_API int mjmlx_batched_set_env_params(MjmlxBatchedSim*, const int*, int, const float*, const float*, const float*,
-                                           const float*) {
-  return -1;
+namespace {
+// uploads the host parameter blocks once the steps queued before have read the old ones
+void upload_rand(MjmlxBatchedSim* s) {
+  HIPOK(hipStreamSynchronize(s->stream));
+  HIPOK(hipMemcpy(s->d_rand, s->h_rand.data(), s->h_rand.size() * sizeof(float), hipMemcpyHostToDevice));
 }
-MJMLX_API int mjmlx_batched_set_free_drag(MjmlxBatchedSim*, const int*, int, const float*) { return -1; }
-MJMLX_API int mjmlx_batched_num_free_joints(const MjmlxBatchedSim*) { return 0; }
+}  // namespace
+
+// Per-env physical parameters of a sim created with domain_rand (reference
+// src/batched.cpp:1508-1526): rows i of the arrays go to env env_idx[i] (env i
+// when env_idx is NULL, then n must be num_envs); NULL arrays are left as
+// they are. Returns 0, -1 (bad handle, no domain_rand, bad n), -2 (env index
+// out of range).
+MJMLX_API int mjmlx_batched_set_env_params(MjmlxBatchedSim* s, const int* env_idx, int n, const float* mass_scale,
+                                           const float* com_delta, const float* friction_scale,
+                                           const float* gain_scale) {
+  if (!s || s->nrand == 0 || n < 0) return -1;
+  if (!env_idx && n != s->B) return -1;
+  for (int i = 0; i < n; i++)
+    if (env_idx && (env_idx[i] < 0 || env_idx[i] >= s->B)) return -2;
+  try {
+    int nb = s->nb, nu = s->nu;
+    for (int i = 0; i < n; i++) {
+      float* r = s->h_rand.data() + (size_t)(env_idx ? env_idx[i] : i) * s->nrand;
+      if (mass_scale) std::memcpy(r, mass_scale + (size_t)i * nb, nb * sizeof(float));
+      if (com_delta) std::memcpy(r + mjk::rand_com(nb), com_delta + (size_t)i * nb * 3, nb * 3 * sizeof(float));
+      if (friction_scale) r[mjk::rand_fric(nb)] = friction_scale[i];
+      if (gain_scale && nu > 0) std::memcpy(r + mjk::rand_gain(nb), gain_scale + (size_t)i * nu, nu * sizeof(float));
+    }
+    upload_rand(s);
+    return 0;
+  } catch (const std::exception& e) { report("mjmlx_batched_set_env_params", e); return -1; }
+}
+
+// Quadratic drag coefficient k of each free joint (F = -k |v| v), drag_k
+// [n][num_free_joints] (reference src/batched.cpp:1528-1541). Returns 0, -1
+// (bad handle / arguments, no domain_rand), -2 (env index out of range), -3
+// (the model has no free joint).
+MJMLX_API int mjmlx_batched_set_free_drag(MjmlxBatchedSim* s, const int* env_idx, int n, const float* drag_k) {
+  if (!s || s->nrand == 0 || n < 0 || !drag_k) return -1;
+  if (s->nfree == 0) return -3;
+  if (!env_idx && n != s->B) return -1;
+  for (int i = 0; i < n; i++)
+    if (env_idx && (env_idx[i] < 0 || env_idx[i] >= s->B)) return -2;
+  try {
+    for (int i = 0; i < n; i++) {
+      float* r = s->h_rand.data() + (size_t)(env_idx ? env_idx[i] : i) * s->nrand;
+      std::memcpy(r + mjk::rand_drag(s->nb, s->nu), drag_k + (size_t)i * s->nfree, s->nfree * sizeof(float));
+    }
+    upload_rand(s);
+    return 0;
+  } catch (const std::exception& e) { report("mjmlx_batched_set_free_drag", e); return -1; }
+}
+
+// free joints of the model when the sim has domain_rand, else 0 (reference src/batched.cpp:1543-1545)
+MJMLX_API int mjmlx_batched_num_free_joints(const MjmlxBatchedSim* s) { return s && s->nrand > 0 ? s->nfree : 0; }
 
 MJMLX_API int mjmlx_batched_set_pack_spec(MjmlxBatchedSim* s, const int* body_ids, int nsel, int dof_lo, int ndof) {
   if (!s || nsel < 0 || ndof < 0 || (nsel > 0 && !body_ids)) return -1;

@@ -311,6 +311,11 @@ std::vector<LayoutField> layout_fields(const Model& m, const mjk::Layout& L) {
       {"efc_pos", L.efc_pos, L.maxefc, 0}, {"efc_margin", L.efc_margin, L.maxefc, 0},
       {"efc_type", L.efc_type, L.maxefc, 1}, {"efc_id", L.efc_id, L.maxefc, 1}, {"efc_floss", L.efc_floss, L.maxefc, 0},
   };
+  if (L.rand >= 0) {  // domain randomisation: the env's parameter block
+    int nfree = 0;
+    for (int j = 0; j < nj; j++) nfree += m.jnt_type[j] == mjk::JFREE;
+    f.push_back({"rand", L.rand, mjk::rand_size(nb, m.nu, nfree), 0});
+  }
   return f;
 }
 

@@ -316,7 +316,13 @@ KDEV void st_kinematics(const DModel& m, const Layout& L, float* w) {
       quat2mat(q, R);
       for (int k = 0; k < 9; k++) xmat[9 * b + k] = R[k];
       float t[3];
-      mv3(R, m.body_ipos + 3 * b, t);
+      if (L.rand >= 0) {  // randomised COM: the body-frame shift moves the inertial frame
+        const float* sh = w + L.rand + rand_com(m.nbody) + 3 * b;
+        float ip[3] = {m.body_ipos[3 * b] + sh[0], m.body_ipos[3 * b + 1] + sh[1], m.body_ipos[3 * b + 2] + sh[2]};
+        mv3(R, ip, t);
+      } else {
+        mv3(R, m.body_ipos + 3 * b, t);
+      }
       for (int k = 0; k < 3; k++) xipos[3 * b + k] = pos[k] + t[k];
     }
     KSYNC();
@@ -339,13 +345,15 @@ KDEV void st_kinematics(const DModel& m, const Layout& L, float* w) {
 KDEV void st_com_pos(const DModel& m, const Layout& L, float* w) {
   const float *xipos = w + L.xipos, *xquat = w + L.xquat, *xmat = w + L.xmat;
   float *scom = w + L.scom, *cinert = w + L.cinert, *cdof = w + L.cdof;
+  const float* msc = L.rand >= 0 ? w + L.rand : nullptr;  // randomised mass scales (mass and inertia)
   for (int b = LANE; b < m.nbody; b += WAVE) {
-    float s[3] = {0.f, 0.f, 0.f};
+    float s[3] = {0.f, 0.f, 0.f}, sm = 0.f;
     for (int k = b; k < m.body_subtree_end[b]; k++) {
-      float mk = m.body_mass[k];
+      float mk = msc ? m.body_mass[k] * msc[k] : m.body_mass[k];
       s[0] += mk * xipos[3 * k]; s[1] += mk * xipos[3 * k + 1]; s[2] += mk * xipos[3 * k + 2];
+      sm += mk;
     }
-    float sm = m.body_subtreemass[b];
+    if (!msc) sm = m.body_subtreemass[b];
     for (int k = 0; k < 3; k++) scom[3 * b + k] = sm < KMINVAL ? xipos[3 * b + k] : s[k] / sm;
   }
   KSYNC();
@@ -355,8 +363,9 @@ KDEV void st_com_pos(const DModel& m, const Layout& L, float* w) {
     float qi[4], R[9];
     mulquat(xquat + 4 * b, m.body_iquat + 4 * b, qi);
     quat2mat(qi, R);
-    const float* I = m.body_inertia + 3 * b;
-    float mass = m.body_mass[b];
+    float sc = msc ? msc[b] : 1.f;
+    const float I[3] = {m.body_inertia[3 * b] * sc, m.body_inertia[3 * b + 1] * sc, m.body_inertia[3 * b + 2] * sc};
+    float mass = m.body_mass[b] * sc;
     const float* c = scom + 3 * m.body_rootid[b];
     float d[3] = {xipos[3 * b] - c[0], xipos[3 * b + 1] - c[1], xipos[3 * b + 2] - c[2]};
     float A[9];
@@ -587,6 +596,31 @@ KDEV void st_com_vel(const DModel& m, const Layout& L, float* w) {
 // mj_rne (zero acceleration) -> qfrc_bias, then passive and actuator forces,
 // qfrc_smooth and qacc_smooth = M^-1 qfrc_smooth.
 // Reference forward.hpp:338-469.
+// Friction coefficients of pair p; under domain randomisation all five are
+// multiplied by the env's friction scale (into tmp) (reference
+// src/compat/mkx_kernels/solver.hpp:146,435-436,530).
+KDEV const float* pair_mu(const DModel& m, const Layout& L, const float* w, int p, float* tmp) {
+  const float* f5 = m.pair_friction + 5 * p;
+  if (L.rand < 0) return f5;
+  float sc = w[L.rand + rand_fric(m.nbody)];
+  for (int i = 0; i < 5; i++) tmp[i] = f5[i] * sc;
+  return tmp;
+}
+
+// Quadratic drag of a free joint's translation under domain randomisation:
+// F = -k |v| v on its three translational dofs, k per env and free joint
+// (reference src/compat/mkx_kernels/forward.hpp:455-465). Returns the
+// force on dof k (0 for other dofs).
+KDEV float free_drag(const DModel& m, const Layout& L, const float* w, int k) {
+  int j = m.dof_jntid[k], i = k - m.jnt_dofadr[j];
+  if (m.jnt_type[j] != JFREE || i >= 3) return 0.f;
+  int f = 0;
+  for (int t = 0; t < j; t++) f += m.jnt_type[t] == JFREE;
+  const float* v = w + L.qvel + m.jnt_dofadr[j];
+  float kd = -w[L.rand + rand_drag(m.nbody, m.nu) + f] * sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  return kd * v[i];
+}
+
 KDEV int st_smooth(const DModel& m, const Layout& L, float* w) {
   float *cacc = w + L.cacc, *cfrc = w + L.cfrc;
   const float *cinert = w + L.cinert, *cvel = w + L.cvel, *cdd = w + L.cdofdot, *cdof = w + L.cdof;
@@ -646,12 +680,14 @@ KDEV int st_smooth(const DModel& m, const Layout& L, float* w) {
         float len = gear * qpos[m.act_qadr[u]], vel = gear * qvel[k];
         const float *g = m.act_gain + 3 * u, *bi = m.act_bias + 3 * u;
         frc = g[0] * c + bi[0] + bi[1] * len + bi[2] * vel;
+        if (L.rand >= 0) frc *= w[L.rand + rand_gain(m.nbody) + u];  // randomised actuator strength
         if (m.act_forcelimited[u]) frc = clampf(frc, m.act_forcerange[2 * u], m.act_forcerange[2 * u + 1]);
         act += gear * frc;
       }
       af[u] = frc;
     }
     float s = pas - bias + act;
+    if (L.rand >= 0) s += free_drag(m, L, w, k);
     qfs[k] = s;
     qas[k] = s;
   }
@@ -1599,7 +1635,8 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
       for (int q = 1; q < cnt; q++) J[(r0 + q) * nv + k] = Jd[q - 1];
       continue;
     }
-    const float* mu = m.pair_friction + 5 * p;
+    float mtmp[5];
+    const float* mu = pair_mu(m, L, w, p, mtmp);
     for (int q = 0; q < cnt / 2; q++) {
       J[(r0 + 2 * q) * nv + k] = Jn + mu[q] * Jd[q];
       J[(r0 + 2 * q + 1) * nv + k] = Jn - mu[q] * Jd[q];
@@ -1622,7 +1659,8 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
       int p = iptr(con + CON_STRIDE * eid[r], CON_PAIR)[0];
       sr = m.pair_solref + 2 * p; si = m.pair_solimp + 5 * p;
       float tran = m.body_invweight0[2 * m.pair_b1[p]] + m.body_invweight0[2 * m.pair_b2[p]];
-      mu0 = m.pair_friction[5 * p];
+      float mtmp[5];
+      mu0 = pair_mu(m, L, w, p, mtmp)[0];
       dA = ty == EFC_CONTACT_PYRAMIDAL ? tran * (1.f + mu0 * mu0) : tran;
     }
     float v = kdot(J + r * nv, 1, qvel, 1, nv);
@@ -1652,7 +1690,8 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
         float imp0 = k_impedance(si, cr[CON_DIST] - (m.pair_margin[p] - m.pair_gap[p]));
         float R0 = fmaxf(dA * (1.f - imp0) / imp0, KMINVAL);
         float R1 = R0 / fmaxf(KMINVAL, m.impratio);
-        const float* f5 = m.pair_friction + 5 * p;
+        float mtmp[5];
+        const float* f5 = pair_mu(m, L, w, p, mtmp);
         R = fmaxf(kr == 1 ? R1 : R1 * f5[0] * f5[0] / (f5[kr - 1] * f5[kr - 1]), KMINVAL);
       }
     }
@@ -1854,7 +1893,8 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
       const float* cr = con + CON_STRIDE * c;
       int dim = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
       if (dim < 2 || r0 + dim > nefc) continue;
-      const float* f5 = m.pair_friction + 5 * reinterpret_cast<const int*>(cr + CON_PAIR)[0];
+      float mtmp[5];
+      const float* f5 = pair_mu(m, L, w, reinterpret_cast<const int*>(cr + CON_PAIR)[0], mtmp);
       float cst;
       int z = k_ellip(Jaref + r0, D + r0, f5, dim, &cst, force + r0);
       part += cst;
@@ -1942,7 +1982,8 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
       const float* cr = con + CON_STRIDE * c;
       int dim = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
       if (dim < 2 || r0 + dim > nefc || act[r0] != 2) continue;
-      const float* f5 = m.pair_friction + 5 * reinterpret_cast<const int*>(cr + CON_PAIR)[0];
+      float mtmp[5];
+      const float* f5 = pair_mu(m, L, w, reinterpret_cast<const int*>(cr + CON_PAIR)[0], mtmp);
       const float* x = Jaref + r0;
       float mu = f5[0], T = 0.f;
       for (int j = 1; j < dim; j++) { float U = f5[j - 1] * x[j]; T += U * U; }
@@ -2149,7 +2190,8 @@ KDEV LsPt ls_eval(const DModel& m, const Layout& L, float* w, int nefc, float al
       const float* cr = con + CON_STRIDE * q;
       int dim = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
       if (dim < 2 || r0 + dim > nefc) continue;
-      const float* f5 = m.pair_friction + 5 * reinterpret_cast<const int*>(cr + CON_PAIR)[0];
+      float mtmp[5];
+      const float* f5 = pair_mu(m, L, w, reinterpret_cast<const int*>(cr + CON_PAIR)[0], mtmp);
       float cv, g1, g2, cr_, h1, h2;
       int z = k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, dim, alpha, &cv, &g1, &g2);
       int zr = k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, dim, ref_alpha, &cr_, &h1, &h2);
@@ -2452,6 +2494,8 @@ struct StepIO {
   AuxOut aux;
   int has_aux;
   unsigned long long* prof;  // profiling builds: per-env stage cycles [env * KP_N + stage]
+  const float* rand;         // domain randomisation: per-env parameter blocks [env * nrand], or null
+  int nrand;
 };
 
 KDEV void dump_ws(const Layout& L, const float* w, float* dst, int env) {
@@ -2554,6 +2598,8 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     if (io.qacc_pred) w[L.qacc_pred + i] = io.qacc_pred[(size_t)env * nv + i];
   }
   for (int i = LANE; i < nu; i += WAVE) w[L.ctrl + i] = io.ctrl ? io.ctrl[(size_t)env * nu + i] : 0.f;
+  if (L.rand >= 0)
+    for (int i = LANE; i < io.nrand; i += WAVE) w[L.rand + i] = io.rand[(size_t)env * io.nrand + i];
   KSYNC();
   if (nsub == 0) {
     // derived quantities of the current state only (no solve, no integration)

@@ -75,6 +75,9 @@ def lib() -> C.CDLL:
             "mjmlx_batched_get_cfrc_ext": (_F, [C.c_void_p, _I]),
             "mjmlx_batched_set_env_qpos": (None, [C.c_void_p, C.c_int, _F, C.c_int]),
             "mjmlx_batched_set_env_qvel": (None, [C.c_void_p, C.c_int, _F, C.c_int]),
+            "mjmlx_batched_set_env_params": (C.c_int, [C.c_void_p, _I, C.c_int, _F, _F, _F, _F]),
+            "mjmlx_batched_set_free_drag": (C.c_int, [C.c_void_p, _I, C.c_int, _F]),
+            "mjmlx_batched_num_free_joints": (C.c_int, [C.c_void_p]),
             "mjmlx_batched_set_pack_spec": (C.c_int, [C.c_void_p, _I, C.c_int, C.c_int, C.c_int]),
             "mjmlx_batched_get_pack": (C.c_int, [C.c_void_p, _F]),
             "mjmlx_batched_eval_state": (None, [C.c_void_p]),
@@ -123,6 +126,10 @@ def _fp(a):
     return None if a is None else a.ctypes.data_as(_F)
 
 
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(_I)
+
+
 class Model:
     """Compiled MJCF model (mjmlx_load_model_from_string_ex)."""
 
@@ -165,10 +172,11 @@ INTEGRATORS = {"Euler": 0, "RK4": 1, "implicit": 2, "implicitfast": 3}
 class BatchedSim:
     """num_envs copies of a model stepped on the GPU (mjmlx_batched_*)."""
 
-    def __init__(self, model: Model, num_envs: int, integrator: int | None = None, solver_iterations: int = 0):
+    def __init__(self, model: Model, num_envs: int, integrator: int | None = None, solver_iterations: int = 0,
+                 domain_rand: bool = False):
         if integrator is None:
             integrator = int(model.field("integrator")[0])
-        cfg = BatchedConfig(num_envs, integrator, 1, solver_iterations, 0)
+        cfg = BatchedConfig(num_envs, integrator, 1, solver_iterations, 1 if domain_rand else 0)
         self.model = model
         self.h = lib().mjmlx_batched_create(model.h, C.byref(cfg))
         if not self.h:
@@ -189,6 +197,23 @@ class BatchedSim:
 
     def sync(self):
         lib().mjmlx_batched_eval_state(self.h)
+
+    def set_env_params(self, env_idx=None, mass_scale=None, com_delta=None, friction_scale=None, gain_scale=None):
+        """Per-env physical parameters (sim created with domain_rand); returns the C status."""
+        idx = None if env_idx is None else np.ascontiguousarray(env_idx, dtype=np.int32)
+        n = self.B if idx is None else len(idx)
+        f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+        ms, cd, fs, gs = f(mass_scale), f(com_delta), f(friction_scale), f(gain_scale)
+        return lib().mjmlx_batched_set_env_params(self.h, _ip(idx), n, _fp(ms), _fp(cd), _fp(fs), _fp(gs))
+
+    def set_free_drag(self, drag_k, env_idx=None):
+        idx = None if env_idx is None else np.ascontiguousarray(env_idx, dtype=np.int32)
+        n = self.B if idx is None else len(idx)
+        d = np.ascontiguousarray(drag_k, dtype=np.float32)
+        return lib().mjmlx_batched_set_free_drag(self.h, _ip(idx), n, _fp(d))
+
+    def num_free_joints(self) -> int:
+        return lib().mjmlx_batched_num_free_joints(self.h)
 
     def state(self):
         qp = np.zeros((self.B, self.nq), np.float32)

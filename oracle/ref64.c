@@ -73,6 +73,11 @@ struct orc_data {
   /* scratch */
   double *Ma, *grad, *Mgrad, *search, *Mv, *H, *tmp1, *tmp2;
   double cost;
+  /* domain randomisation (reference src/batched.cpp:1508-1538): per-body mass
+   * scale and body-frame COM shift, friction scale, per-actuator force scale,
+   * per-free-joint quadratic drag; off until orc_set_env_params is called */
+  int has_rand, nfree;
+  double *rand_mass, *rand_com, *rand_gain, *rand_drag, rand_fric;
 };
 
 /* ------------------------------------------------------------------ math */
@@ -242,6 +247,13 @@ static void data_alloc(orc_data_t* d, const orc_model_t* m) {
   AL(actuator_force, nu);
   AL(efc_J, (size_t)ORC_MAXEFC * nv);
   AL(Ma, nv); AL(grad, nv); AL(Mgrad, nv); AL(search, nv); AL(Mv, nv); AL(H, nv * nv); AL(tmp1, nv); AL(tmp2, nv);
+  d->nfree = 0;
+  for (int j = 0; j < m->njnt; j++) d->nfree += m->jnt_type[j] == JFREE;
+  AL(rand_mass, nb); AL(rand_com, 3 * nb); AL(rand_gain, nu); AL(rand_drag, d->nfree > 0 ? d->nfree : 1);
+  for (int b = 0; b < nb; b++) d->rand_mass[b] = 1;
+  for (int u = 0; u < nu; u++) d->rand_gain[u] = 1;
+  d->rand_fric = 1;
+  d->has_rand = 0;
 #undef AL
 }
 
@@ -250,7 +262,8 @@ static void free_data_arrays(orc_data_t* d) {
                     d->xipos, d->ximat, d->xanchor, d->xaxis, d->geom_xpos, d->geom_xmat, d->subtree_com,
                     d->cinert, d->cdof, d->cvel, d->cdof_dot, d->crb, d->cacc, d->cfrc, d->qM, d->qLD,
                     d->qfrc_bias, d->qfrc_passive, d->qfrc_actuator, d->qfrc_smooth, d->qfrc_constraint,
-                    d->actuator_force, d->efc_J, d->Ma, d->grad, d->Mgrad, d->search, d->Mv, d->H, d->tmp1, d->tmp2};
+                    d->actuator_force, d->efc_J, d->Ma, d->grad, d->Mgrad, d->search, d->Mv, d->H, d->tmp1, d->tmp2,
+                    d->rand_mass, d->rand_com, d->rand_gain, d->rand_drag};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
 }
 
@@ -374,6 +387,16 @@ void orc_set_state(orc_data_t* d, const double* qpos, const double* qvel, const 
   if (qacc_ws) memcpy(d->qacc_ws, qacc_ws, sizeof(double) * d->nv);
   if (ctrl) memcpy(d->ctrl, ctrl, sizeof(double) * d->nu);
 }
+void orc_set_env_params(orc_data_t* d, const double* mass_scale, const double* com_delta, const double* friction_scale,
+                        const double* gain_scale, const double* drag) {
+  d->has_rand = 1;
+  if (mass_scale) memcpy(d->rand_mass, mass_scale, sizeof(double) * d->nbody);
+  if (com_delta) memcpy(d->rand_com, com_delta, sizeof(double) * 3 * d->nbody);
+  if (friction_scale) d->rand_fric = friction_scale[0];
+  if (gain_scale) memcpy(d->rand_gain, gain_scale, sizeof(double) * d->nu);
+  if (drag) memcpy(d->rand_drag, drag, sizeof(double) * d->nfree);
+}
+int orc_nfree(const orc_data_t* d) { return d->nfree; }
 void orc_get_state(const orc_data_t* d, double* qpos, double* qvel, double* qacc_ws) {
   if (qpos) memcpy(qpos, d->qpos, sizeof(double) * d->nq);
   if (qvel) memcpy(qvel, d->qvel, sizeof(double) * d->nv);
@@ -433,8 +456,10 @@ static void kinematics(const orc_prep_t* p, orc_data_t* d) {
     memcpy(d->xpos + 3 * b, pos, 3 * sizeof(double));
     memcpy(d->xquat + 4 * b, q, 4 * sizeof(double));
     quat2mat(q, d->xmat + 9 * b);
-    double t[3], qi[4];
-    mulmatvec3(d->xmat + 9 * b, m->body_ipos + 3 * b, t);
+    double t[3], qi[4], ip[3];
+    /* randomised COM: reference forward.hpp:113-114 adds xmat * com_delta to xipos */
+    for (int i = 0; i < 3; i++) ip[i] = m->body_ipos[3 * b + i] + (d->has_rand ? d->rand_com[3 * b + i] : 0.0);
+    mulmatvec3(d->xmat + 9 * b, ip, t);
     for (int i = 0; i < 3; i++) d->xipos[3 * b + i] = pos[i] + t[i];
     mulquat(q, m->body_iquat + 4 * b, qi);
     quat2mat(qi, d->ximat + 9 * b);
@@ -454,20 +479,27 @@ static void kinematics(const orc_prep_t* p, orc_data_t* d) {
 static void com_pos(const orc_prep_t* p, orc_data_t* d) {
   const orc_model_t* m = &p->m;
   int nb = m->nbody;
-  for (int b = 0; b < nb; b++) for (int i = 0; i < 3; i++) d->subtree_com[3 * b + i] = m->body_mass[b] * d->xipos[3 * b + i];
+  /* randomised masses scale mass and inertia (reference forward.hpp:147,167,173) */
+  double smass[nb > 0 ? nb : 1];
+  for (int b = 0; b < nb; b++) {
+    smass[b] = m->body_mass[b] * (d->has_rand ? d->rand_mass[b] : 1.0);
+    for (int i = 0; i < 3; i++) d->subtree_com[3 * b + i] = smass[b] * d->xipos[3 * b + i];
+  }
   for (int b = nb - 1; b > 0; b--) {
     int par = m->body_parentid[b];
     for (int i = 0; i < 3; i++) d->subtree_com[3 * par + i] += d->subtree_com[3 * b + i];
+    smass[par] += smass[b];
   }
   for (int b = 0; b < nb; b++) {
-    double sm = p->body_subtreemass[b];
+    double sm = d->has_rand ? smass[b] : p->body_subtreemass[b];
     for (int i = 0; i < 3; i++) d->subtree_com[3 * b + i] = sm < MINVAL ? d->xipos[3 * b + i] : d->subtree_com[3 * b + i] / sm;
   }
   memset(d->cinert, 0, 10 * sizeof(double));
   for (int b = 1; b < nb; b++) {
     const double* R = d->ximat + 9 * b;
-    const double* I = m->body_inertia + 3 * b;
-    double mass = m->body_mass[b];
+    double sc = d->has_rand ? d->rand_mass[b] : 1.0;
+    double I[3] = {m->body_inertia[3 * b] * sc, m->body_inertia[3 * b + 1] * sc, m->body_inertia[3 * b + 2] * sc};
+    double mass = m->body_mass[b] * sc;
     const double* c = d->subtree_com + 3 * m->body_rootid[b];
     double dd[3] = {d->xipos[3 * b] - c[0], d->xipos[3 * b + 1] - c[1], d->xipos[3 * b + 2] - c[2]};
     double A[9];
@@ -642,6 +674,19 @@ static void passive(const orc_prep_t* p, orc_data_t* d) {
   for (int k = 0; k < m->nv; k++) d->qfrc_passive[k] -= m->dof_damping[k] * d->qvel[k];
 }
 
+/* quadratic drag F = -k |v| v on each free joint's translational dofs, k per
+ * free joint in joint order (reference forward.hpp:455-465) */
+static void free_drag(const orc_prep_t* p, orc_data_t* d) {
+  const orc_model_t* m = &p->m;
+  int f = 0;
+  for (int j = 0; j < m->njnt; j++) {
+    if (m->jnt_type[j] != JFREE) continue;
+    const double* v = d->qvel + m->jnt_dofadr[j];
+    double kd = -d->rand_drag[f++] * sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int i = 0; i < 3; i++) d->qfrc_smooth[m->jnt_dofadr[j] + i] += kd * v[i];
+  }
+}
+
 /* mj_fwdActuation for joint transmissions: force = gain*ctrl + b0 + b1*length
  * + b2*velocity, ctrl and force clamping, qfrc_actuator = moment * force.
  * Reference forward.hpp:411-435. */
@@ -659,6 +704,7 @@ static void actuation(const orc_prep_t* p, orc_data_t* d) {
     double len = gear * d->qpos[qa], vel = gear * d->qvel[da];
     const double *g = m->actuator_gainprm + 3 * u, *bi = m->actuator_biasprm + 3 * u;
     double f = g[0] * c + bi[0] + bi[1] * len + bi[2] * vel;
+    if (d->has_rand) f *= d->rand_gain[u]; /* reference forward.hpp:427 */
     if (m->actuator_forcelimited[u]) f = clampd(f, m->actuator_forcerange[2 * u], m->actuator_forcerange[2 * u + 1]);
     d->actuator_force[u] = f;
     d->qfrc_actuator[da] += gear * f;
@@ -1378,6 +1424,8 @@ static int collision(const orc_prep_t* p, orc_data_t* d) {
     if (n < 0) { status |= 4; continue; }
     for (int k = 0; k < n; k++) {
       if (d->ncon >= (p->con_cap > 0 && p->con_cap < ORC_MAXCON ? p->con_cap : ORC_MAXCON)) { status |= 1; break; }
+      if (d->has_rand) /* reference solver.hpp:435-436,530: every friction coefficient scaled */
+        for (int i = 0; i < 5; i++) tmp[k].friction[i] *= d->rand_fric;
       d->con[d->ncon++] = tmp[k];
     }
   }
@@ -1990,6 +2038,7 @@ int orc_forward(const orc_prep_t* p, orc_data_t* d) {
   rne_bias(p, d);
   actuation(p, d);
   for (int k = 0; k < nv; k++) d->qfrc_smooth[k] = d->qfrc_passive[k] - d->qfrc_bias[k] + d->qfrc_actuator[k];
+  if (d->has_rand) free_drag(p, d);
   memcpy(d->qacc_smooth, d->qfrc_smooth, sizeof(double) * nv);
   if (nv > 0) solve_m(p, d, d->qacc_smooth);
   /* aref needs efc_vel which needs qvel only: already computed in make_constraint */

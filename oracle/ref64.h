@@ -93,6 +93,15 @@ void orc_data_free(orc_data_t* d);
 /* state in / out */
 void orc_set_state(orc_data_t* d, const double* qpos, const double* qvel, const double* qacc_ws, const double* ctrl);
 void orc_get_state(const orc_data_t* d, double* qpos, double* qvel, double* qacc_ws);
+/* domain randomisation of this data (reference src/batched.cpp:1508-1538):
+ * mass_scale [nbody] (mass and inertia), com_delta [nbody][3] (body-frame COM
+ * shift), friction_scale [1] (all contact friction coefficients),
+ * gain_scale [nu] (actuator force), drag [nfree] (F = -k |v| v on each free
+ * joint's translation). NULL leaves a parameter at its current value
+ * (initially the identity). */
+void orc_set_env_params(orc_data_t* d, const double* mass_scale, const double* com_delta, const double* friction_scale,
+                        const double* gain_scale, const double* drag);
+int orc_nfree(const orc_data_t* d);
 
 /* One physics step (mj_step): forward pipeline, solver, integrator.
  * Returns 0, or a negative status (>0 capacity overflow bits). */

@@ -102,6 +102,8 @@ def lib() -> C.CDLL:
         L.orc_data_free.argtypes = [C.c_void_p]
         L.orc_set_state.argtypes = [C.c_void_p, _D, _D, _D, _D]
         L.orc_get_state.argtypes = [C.c_void_p, _D, _D, _D]
+        L.orc_set_env_params.argtypes = [C.c_void_p, _D, _D, _D, _D, _D]
+        L.orc_nfree.argtypes = [C.c_void_p]
         L.orc_step.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_forward.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_kinematics.argtypes = [C.c_void_p, C.c_void_p]
@@ -205,6 +207,27 @@ class Oracle:
         qp, qv, qa, ct = arr(qpos, self.nq), arr(qvel, self.nv), arr(qacc_ws, self.nv), arr(ctrl, self.nu)
         self._st = (qp, qv, qa, ct)
         lib().orc_set_state(self.d, *(None if a is None else _dp(a) for a in (qp, qv, qa, ct)))
+
+    def set_env_params(self, mass_scale=None, com_delta=None, friction_scale=None, gain_scale=None, drag=None):
+        """Domain randomisation of the oracle's env (reference
+        src/batched.cpp:1508-1538): mass_scale [nbody], com_delta [nbody, 3],
+        friction_scale (scalar), gain_scale [nu], drag [nfree]; None keeps a
+        parameter (initially the identity)."""
+        nfree = lib().orc_nfree(self.d)
+        def arr(x, n):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).ravel())
+            assert a.size == n, (a.size, n)
+            return a
+        args = (arr(mass_scale, self.nbody), arr(com_delta, 3 * self.nbody), arr(friction_scale, 1),
+                arr(gain_scale, self.nu), arr(drag, nfree))
+        self._rand = args
+        lib().orc_set_env_params(self.d, *(None if a is None else _dp(a) for a in args))
+        return self
+
+    def nfree(self) -> int:
+        return lib().orc_nfree(self.d)
 
     def get_state(self):
         qp, qv, qa = np.zeros(self.nq), np.zeros(max(self.nv, 1)), np.zeros(max(self.nv, 1))

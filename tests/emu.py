@@ -39,6 +39,7 @@ def lib():
         L.mjemu_model_field.argtypes = [C.c_void_p, C.c_char_p, _D, C.c_int]
         L.mjemu_features.argtypes = [C.c_void_p, _F]
         L.mjemu_set_pred.argtypes = [C.c_void_p, _F]
+        L.mjemu_set_rand.argtypes = [C.c_void_p, _F]
         L.mjemu_narrow.argtypes = [C.c_int, C.c_int, _F, _F, _F, _F, _F, _F, C.c_float, _F]
         L.mjemu_compile.restype = C.c_void_p
         L.mjemu_compile.argtypes = [C.c_char_p, C.c_int, C.c_int]
@@ -111,6 +112,17 @@ class EmuSim:
         """Predicted qacc [B, nv] offered to the solver as a warm-start candidate (None: none)."""
         self._pred = None if pred is None else np.ascontiguousarray(pred, dtype=np.float32)
         lib().mjemu_set_pred(self.h, _fp(self._pred))
+
+    def set_rand(self, mass_scale, com_delta, friction_scale, gain_scale, drag):
+        """Domain randomisation of every env: mass_scale [B, nbody], com_delta
+        [B, nbody, 3], friction_scale [B], gain_scale [B, nu], drag [B, nfree]
+        (packed into the engine's per-env parameter blocks)."""
+        cols = [np.asarray(a, np.float32).reshape(self.B, -1)
+                for a in (mass_scale, com_delta, friction_scale, gain_scale, drag)]
+        self._rand = np.ascontiguousarray(np.concatenate(cols, axis=1))
+        n = lib().mjemu_set_rand(self.h, _fp(self._rand))
+        assert n == self._rand.shape[1], (n, self._rand.shape)
+        self.total = self.layout("total")[1]
 
     def dump(self, which: int, env: int, name: str) -> np.ndarray:
         off, ln, kind = self.layout(name)
