@@ -137,7 +137,11 @@ MJMLX_API int mjmlx_batched_num_free_joints(const MjmlxBatchedSim* sim);        
  * squares and of absolute values of qfrc_actuator over a dof range] (:241-242) */
 MJMLX_API int mjmlx_batched_set_pack_spec(MjmlxBatchedSim* sim, const int* body_ids, int nbody_sel, int dof_lo, int ndof);
 MJMLX_API int mjmlx_batched_get_pack(MjmlxBatchedSim* sim, float* out);
-/* mj_ray per env and ray (:248); not implemented: returns -1 */
+/* mj_ray per env and ray at each env's current qpos (:248): pnt / vec
+ * [num_envs][nray][3]; dist_out (units of |vec|) and geomid_out [num_envs][nray],
+ * -1 / -1 for no hit. Skips geoms of body bodyexclude, of groups off in
+ * geomgroup[6] (NULL: all on) and, when flg_static is 0, of bodies welded to
+ * the world. Planes are one-sided. Returns 0, or -1 for bad arguments. */
 MJMLX_API int mjmlx_batched_raycast(MjmlxBatchedSim* sim, const float* pnt, const float* vec, int nray, int bodyexclude,
                                    const unsigned char* geomgroup, int flg_static, float* dist_out, int* geomid_out);
 /* completes all queued work of the sim (:252) */

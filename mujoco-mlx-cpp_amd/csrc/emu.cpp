@@ -174,6 +174,22 @@ __attribute__((visibility("default"))) int mjemu_narrow(int t1, int t2, const fl
   return n;
 }
 
+// the kernel's ray-primitive intersection (ray_geom) for one ray in the geom frame
+__attribute__((visibility("default"))) float mjemu_ray_geom(int type, const float* size, const float* p, const float* v) {
+  return mjk::ray_geom(type, size, p, v);
+}
+
+// the kernel's per-env ray routine: rays [B][nray][6] at the sim's current
+// qpos, geom filter gmask [ngeom]; dist / gid [B][nray]
+__attribute__((visibility("default"))) void mjemu_rays(void* p, const float* rays, int nray, const int* gmask,
+                                                       float* dist, int* gid) {
+  auto* s = static_cast<EmuSim*>(p);
+  for (int e = 0; e < s->B; e++) {
+    std::fill(s->ws.begin(), s->ws.end(), 0.f);
+    mjk::run_rays(s->dm, s->L, s->ws.data(), s->qpos.data(), s->rand.data(), s->nrand, rays, nray, gmask, dist, gid, e);
+  }
+}
+
 }  // extern "C"
 
 extern "C" __attribute__((visibility("default"))) int mjemu_model_field(void* p, const char* name, double* out, int cap) {

@@ -79,6 +79,20 @@ __global__ void __launch_bounds__(64) k_step_queue(mjk::DModel m, mjk::Layout L,
   }
 }
 
+// mj_ray for every (env, ray): one wave per env computes the env's geom
+// poses in its workspace, its lanes take the rays (reference
+// src/batched.cpp:1808-1893)
+template <int GWS>
+__global__ void __launch_bounds__(64) k_raycast(mjk::DModel m, mjk::Layout L, const float* qpos, const float* rand,
+                                                int nrand, const float* rays, int nray, const int* gmask, float* dist,
+                                                int* gid, int B, float* gws) {
+  extern __shared__ __align__(16) float lds[];
+  int env = blockIdx.x;
+  if (env >= B) return;
+  float* w = GWS ? gws + (size_t)env * L.total : lds;
+  mjk::run_rays(m, L, w, qpos, rand, nrand, rays, nray, gmask, dist, gid, env);
+}
+
 __global__ void k_zero_int(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -172,6 +186,10 @@ struct MjmlxBatchedSim {
   float* d_rand = nullptr;
   std::vector<float> h_rand;
   int nrand = 0, nfree = 0;
+  // raycast buffers: rays [B][nray][6], results, geom filter (grown on demand)
+  float *d_rays = nullptr, *d_rdist = nullptr;
+  int *d_rgid = nullptr, *d_gmask = nullptr;
+  size_t ray_cap = 0;
   // scratch host buffers for dumps
   float *d_dump1 = nullptr, *d_dump2 = nullptr;
   size_t dump_cap = 0;
@@ -189,7 +207,8 @@ void free_sim(MjmlxBatchedSim* s) {
   if (s->stream) hipStreamSynchronize(s->stream);
   void* dev[] = {s->d_blob, s->d_qpos, s->d_qvel, s->d_qws, s->d_ctrl, s->d_qpos0, s->d_niter, s->d_status,
                  s->d_ncon, s->d_nefc, s->d_queue, s->d_mask, s->d_pack_bodies, s->d_pack, s->d_dump1, s->d_dump2,
-                 s->d_feat[0], s->d_feat[1], s->d_pred, s->d_gws, s->d_rand};
+                 s->d_feat[0], s->d_feat[1], s->d_pred, s->d_gws, s->d_rand, s->d_rays, s->d_rdist,
+                 s->d_rgid, s->d_gmask};
   for (void* p : dev) if (p) hipFree(p);
   for (int i = 0; i < 7; i++) { if (s->d_aux[i]) hipFree(s->d_aux[i]); if (s->h_aux[i]) hipHostFree(s->h_aux[i]); }
   if (s->h_qpos) hipHostFree(s->h_qpos);
@@ -839,9 +858,58 @@ MJMLX_API int mjmlx_batched_get_pack(MjmlxBatchedSim* s, float* out) {
   } catch (const std::exception& e) { report("mjmlx_batched_get_pack", e); return -1; }
 }
 
-MJMLX_API int mjmlx_batched_raycast(MjmlxBatchedSim*, const float*, const float*, int, int, const unsigned char*, int, float*,
-                                   int*) {
-  return -1;
+// mj_ray from pnt along vec ([num_envs][nray][3] each) against each env's
+// geoms at its current qpos: dist in units of |vec| and the geom id, or -1 /
+// -1. Geoms of body bodyexclude, of groups switched off in geomgroup[6] (NULL:
+// all groups) and, when flg_static is 0, of bodies welded to the world are
+// skipped. Returns 0, or -1 (bad arguments). Reference
+// src/batched.cpp:1846-1893.
+MJMLX_API int mjmlx_batched_raycast(MjmlxBatchedSim* s, const float* pnt, const float* vec, int nray, int bodyexclude,
+                                   const unsigned char* geomgroup, int flg_static, float* dist_out, int* geomid_out) {
+  if (!s || !pnt || !vec || !dist_out || !geomid_out || nray <= 0) return -1;
+  try {
+    const auto& m = s->model;
+    size_t total = (size_t)s->B * nray;
+    if (m.ngeom == 0) {
+      std::fill(dist_out, dist_out + total, -1.f);
+      std::fill(geomid_out, geomid_out + total, -1);
+      return 0;
+    }
+    std::vector<int> mask(m.ngeom);
+    for (int g = 0; g < m.ngeom; g++) {
+      int b = m.geom_bodyid[g], grp = m.geom_group[g];
+      bool group_ok = !geomgroup || (grp >= 0 && grp < 6 && geomgroup[grp]);
+      bool static_ok = flg_static || m.body_weldid[b] != 0;
+      mask[g] = b != bodyexclude && group_ok && static_ok;
+    }
+    if (total > s->ray_cap) {
+      for (void* p : {(void*)s->d_rays, (void*)s->d_rdist, (void*)s->d_rgid}) if (p) HIPOK(hipFree(p));
+      s->d_rays = nullptr; s->d_rdist = nullptr; s->d_rgid = nullptr; s->ray_cap = 0;
+      HIPOK(hipMalloc(&s->d_rays, total * 6 * sizeof(float)));
+      HIPOK(hipMalloc(&s->d_rdist, total * sizeof(float)));
+      HIPOK(hipMalloc(&s->d_rgid, total * sizeof(int)));
+      s->ray_cap = total;
+    }
+    if (!s->d_gmask) HIPOK(hipMalloc(&s->d_gmask, m.ngeom * sizeof(int)));
+    std::vector<float> rays(total * 6);
+    for (size_t i = 0; i < total; i++)
+      for (int k = 0; k < 3; k++) { rays[6 * i + k] = pnt[3 * i + k]; rays[6 * i + 3 + k] = vec[3 * i + k]; }
+    // the queued steps finish before the host buffers are reused
+    HIPOK(hipStreamSynchronize(s->stream));
+    HIPOK(hipMemcpyAsync(s->d_rays, rays.data(), rays.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIPOK(hipMemcpyAsync(s->d_gmask, mask.data(), mask.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    if (s->use_gws)
+      hipLaunchKernelGGL(k_raycast<1>, dim3(s->B), dim3(64), 0, s->stream, s->dm, s->L, s->d_qpos, s->d_rand,
+                         s->nrand, s->d_rays, nray, s->d_gmask, s->d_rdist, s->d_rgid, s->B, s->d_gws);
+    else
+      hipLaunchKernelGGL(k_raycast<0>, dim3(s->B), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, s->d_qpos,
+                         s->d_rand, s->nrand, s->d_rays, nray, s->d_gmask, s->d_rdist, s->d_rgid, s->B, nullptr);
+    HIPOK(hipGetLastError());
+    HIPOK(hipMemcpyAsync(dist_out, s->d_rdist, total * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPOK(hipMemcpyAsync(geomid_out, s->d_rgid, total * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPOK(hipStreamSynchronize(s->stream));
+    return 0;
+  } catch (const std::exception& e) { report("mjmlx_batched_raycast", e); return -1; }
 }
 
 MJMLX_API void mjmlx_batched_eval_state(const MjmlxBatchedSim* s) {

@@ -368,7 +368,7 @@ int model_field(const Model& m, const char* name, double* out, int cap) {
   F(jnt_pos) F(jnt_axis) F(jnt_range) F(jnt_margin) F(jnt_solref) F(jnt_solimp) F(jnt_stiffness)
   I(dof_bodyid) I(dof_jntid) I(dof_parentid)
   F(dof_armature) F(dof_damping) F(dof_frictionloss) F(dof_invweight0) F(dof_solref) F(dof_solimp)
-  I(geom_type) I(geom_contype) I(geom_conaffinity) I(geom_condim) I(geom_bodyid) I(geom_priority)
+  I(geom_type) I(geom_contype) I(geom_conaffinity) I(geom_condim) I(geom_bodyid) I(geom_priority) I(geom_group)
   F(geom_size) F(geom_pos) F(geom_quat) F(geom_friction) F(geom_solref) F(geom_solimp) F(geom_margin) F(geom_gap)
   F(geom_solmix) F(geom_rbound)
   I(actuator_trnid) I(actuator_ctrllimited) I(actuator_forcelimited)

@@ -2482,6 +2482,116 @@ struct AuxOut {
 
 // Global arrays of one launch. State arrays are env-major (env * dim + i), as
 // the C API exchanges them (reference include/mjmlx/mjmlx.h:137-138,206-216).
+// ------------------------------------------------------------------ rays
+// mj_ray against one primitive in its own frame: the smallest x >= 0 with
+// p + x v on the surface, or -1 (v need not be unit length; x is in units of
+// |v|). Planes are hit from their +z side only, within their half extents
+// when those are nonzero; a ray starting inside a closed primitive hits its
+// exit point. Reference src/batched.cpp:1739-1806.
+KDEV float ray_quadric(float a, float b, float c) {  // smallest x >= 0 of a x^2 + 2 b x + c
+  float disc = b * b - a * c;
+  if (disc < 1e-12f || fabsf(a) < 1e-12f) return -1.f;
+  float s = sqrtf(disc), x0 = (-b - s) / a, x1 = (-b + s) / a;
+  return x0 >= 0.f ? x0 : (x1 >= 0.f ? x1 : -1.f);
+}
+KDEV float ray_closer(float best, float x) { return x >= 0.f && (best < 0.f || x < best) ? x : best; }
+KDEV float ray_geom(int type, const float* sz, const float* p, const float* v) {
+  switch (type) {
+    case GPLANE: {
+      if (v[2] > -1e-9f) return -1.f;
+      float x = -p[2] / v[2];
+      if (x < 0.f) return -1.f;
+      float hx = p[0] + x * v[0], hy = p[1] + x * v[1];
+      if ((sz[0] > 0.f && fabsf(hx) > sz[0]) || (sz[1] > 0.f && fabsf(hy) > sz[1])) return -1.f;
+      return x;
+    }
+    case GSPHERE:
+      return ray_quadric(dot3(v, v), dot3(p, v), dot3(p, p) - sz[0] * sz[0]);
+    case GELLIPSOID: {
+      float ps[3] = {p[0] / sz[0], p[1] / sz[1], p[2] / sz[2]}, vs[3] = {v[0] / sz[0], v[1] / sz[1], v[2] / sz[2]};
+      return ray_quadric(dot3(vs, vs), dot3(ps, vs), dot3(ps, ps) - 1.f);
+    }
+    case GCAPSULE:
+    case GCYLINDER: {
+      float r = sz[0], h = sz[1], best = -1.f;
+      // lateral surface |(x, y)| = r within |z| <= h
+      float a = v[0] * v[0] + v[1] * v[1], b = p[0] * v[0] + p[1] * v[1], c = p[0] * p[0] + p[1] * p[1] - r * r;
+      float disc = b * b - a * c;
+      if (a > 1e-12f && disc >= 0.f) {
+        float sd = sqrtf(disc);
+        for (int k = -1; k <= 1; k += 2) {
+          float x = (-b + k * sd) / a;
+          if (fabsf(p[2] + x * v[2]) <= h) best = ray_closer(best, x);
+        }
+      }
+      for (int e = -1; e <= 1; e += 2) {
+        if (type == GCYLINDER) {  // flat end discs
+          if (fabsf(v[2]) <= 1e-12f) continue;
+          float x = (e * h - p[2]) / v[2];
+          float qx = p[0] + x * v[0], qy = p[1] + x * v[1];
+          if (x >= 0.f && qx * qx + qy * qy <= r * r) best = ray_closer(best, x);
+        } else {  // hemispherical ends: the outer half of the sphere at z = e h
+          float pc[3] = {p[0], p[1], p[2] - e * h};
+          float a2 = dot3(v, v), b2 = dot3(pc, v), c2 = dot3(pc, pc) - r * r, d2 = b2 * b2 - a2 * c2;
+          if (d2 < 0.f) continue;
+          float sd = sqrtf(d2);
+          for (int k = -1; k <= 1; k += 2) {
+            float x = (-b2 + k * sd) / a2;
+            if (e * (p[2] + x * v[2] - e * h) >= 0.f) best = ray_closer(best, x);
+          }
+        }
+      }
+      return best;
+    }
+    case GBOX: {  // slabs
+      float tn = -1e30f, tf = 1e30f;
+      for (int i = 0; i < 3; i++) {
+        if (fabsf(v[i]) < 1e-12f) {
+          if (fabsf(p[i]) > sz[i]) return -1.f;
+          continue;
+        }
+        float t0 = (-sz[i] - p[i]) / v[i], t1 = (sz[i] - p[i]) / v[i];
+        tn = fmaxf(tn, fminf(t0, t1));
+        tf = fminf(tf, fmaxf(t0, t1));
+      }
+      if (tn > tf || tf < 0.f) return -1.f;
+      return tn >= 0.f ? tn : tf;
+    }
+  }
+  return -1.f;
+}
+
+// Rays of one env against its geoms at the current qpos: the env's kinematics
+// into the workspace, then one lane per ray over the geoms allowed by gmask.
+// rays [env][nray][pnt 3, vec 3]; dist / gid [env][nray] (-1: no hit).
+KDEV void run_rays(const DModel& m, const Layout& L, float* w, const float* qpos, const float* rand, int nrand,
+                   const float* rays, int nray, const int* gmask, float* dist, int* gid, int env) {
+  for (int i = LANE; i < m.nq; i += WAVE) w[L.qpos + i] = qpos[(size_t)env * m.nq + i];
+  if (L.rand >= 0)
+    for (int i = LANE; i < nrand; i += WAVE) w[L.rand + i] = rand[(size_t)env * nrand + i];
+  KSYNC();
+  st_kinematics(m, L, w);
+  const float *gx = w + L.gxpos, *gm = w + L.gxmat;
+  for (int r = LANE; r < nray; r += WAVE) {
+    const float* ry = rays + ((size_t)env * nray + r) * 6;
+    float best = -1.f;
+    int bg = -1;
+    for (int g = 0; g < m.ngeom; g++) {
+      if (!gmask[g]) continue;
+      const float *x = gx + 3 * g, *R = gm + 9 * g;
+      float d[3] = {ry[0] - x[0], ry[1] - x[1], ry[2] - x[2]}, lp[3], lv[3];
+      for (int k = 0; k < 3; k++) {
+        lp[k] = R[k] * d[0] + R[3 + k] * d[1] + R[6 + k] * d[2];
+        lv[k] = R[k] * ry[3] + R[3 + k] * ry[4] + R[6 + k] * ry[5];
+      }
+      float t = ray_geom(m.geom_type[g], m.geom_size + 3 * g, lp, lv);
+      if (t >= 0.f && (best < 0.f || t < best)) { best = t; bg = g; }
+    }
+    dist[(size_t)env * nray + r] = best;
+    gid[(size_t)env * nray + r] = bg;
+  }
+}
+
 struct StepIO {
   float *qpos, *qvel, *qacc_ws;
   const float* ctrl;

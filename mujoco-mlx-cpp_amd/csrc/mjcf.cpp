@@ -556,6 +556,7 @@ void Compiler::add_geom(const XNode& n, int b, const std::string& cls, bool robo
   m_.geom_conaffinity.push_back(conaff);
   m_.geom_condim.push_back(condim);
   m_.geom_priority.push_back((int)num(a, "priority", 0));
+  m_.geom_group.push_back((int)num(a, "group", 0));
   double df[3] = {1, 0.005, 0.0001}, dsr[2] = {0.02, 1}, dsi[5] = {0.9, 0.95, 0.001, 0.5, 2};
   double f[3], sr[2], si[5];
   vec(a, "friction", f, 3, df); vec(a, "solref", sr, 2, dsr); vec(a, "solimp", si, 5, dsi);

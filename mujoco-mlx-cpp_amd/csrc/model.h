@@ -59,7 +59,7 @@ struct Model {
 
   // geoms
   std::vector<int> geom_type, geom_contype, geom_conaffinity, geom_condim, geom_bodyid,
-      geom_priority;
+      geom_priority, geom_group;
   std::vector<double> geom_size, geom_pos, geom_quat, geom_friction, geom_solref,
       geom_solimp, geom_margin, geom_gap, geom_solmix, geom_rbound;
   std::vector<std::string> geom_name;

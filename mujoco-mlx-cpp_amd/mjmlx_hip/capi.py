@@ -78,6 +78,8 @@ def lib() -> C.CDLL:
             "mjmlx_batched_set_env_params": (C.c_int, [C.c_void_p, _I, C.c_int, _F, _F, _F, _F]),
             "mjmlx_batched_set_free_drag": (C.c_int, [C.c_void_p, _I, C.c_int, _F]),
             "mjmlx_batched_num_free_joints": (C.c_int, [C.c_void_p]),
+            "mjmlx_batched_raycast": (C.c_int, [C.c_void_p, _F, _F, C.c_int, C.c_int, C.POINTER(C.c_ubyte), C.c_int,
+                                                 _F, _I]),
             "mjmlx_batched_set_pack_spec": (C.c_int, [C.c_void_p, _I, C.c_int, C.c_int, C.c_int]),
             "mjmlx_batched_get_pack": (C.c_int, [C.c_void_p, _F]),
             "mjmlx_batched_eval_state": (None, [C.c_void_p]),
@@ -211,6 +213,20 @@ class BatchedSim:
         n = self.B if idx is None else len(idx)
         d = np.ascontiguousarray(drag_k, dtype=np.float32)
         return lib().mjmlx_batched_set_free_drag(self.h, _ip(idx), n, _fp(d))
+
+    def raycast(self, pnt, vec, bodyexclude: int = -1, geomgroup=None, flg_static: bool = True):
+        """mj_ray per env and ray: pnt, vec [B, nray, 3]; returns (dist, geom id) [B, nray]."""
+        p = np.ascontiguousarray(pnt, dtype=np.float32)
+        v = np.ascontiguousarray(vec, dtype=np.float32)
+        nray = p.shape[1]
+        dist = np.zeros((self.B, nray), np.float32)
+        gid = np.zeros((self.B, nray), np.int32)
+        gg = None if geomgroup is None else np.ascontiguousarray(geomgroup, dtype=np.uint8)
+        rc = lib().mjmlx_batched_raycast(self.h, _fp(p), _fp(v), nray, int(bodyexclude),
+                                         None if gg is None else gg.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                         int(bool(flg_static)), _fp(dist), _ip(gid))
+        assert rc == 0, rc
+        return dist, gid
 
     def num_free_joints(self) -> int:
         return lib().mjmlx_batched_num_free_joints(self.h)

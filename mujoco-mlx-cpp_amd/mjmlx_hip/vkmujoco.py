@@ -8,8 +8,8 @@ the reference, so ``import mjmlx_hip.vkmujoco as vkmujoco`` replaces
 (scripts/examples/go2/mjmlx_go2_env.py). Arrays are float32 / int32,
 C-contiguous, shaped (num_envs, dim). Failures raise RuntimeError, as the
 reference's do; there is no CPU fallback (``use_gpu=False`` fails, as
-MuJoCo C is not part of this engine). Rendering and raycasts are outside
-the engine: ``create_camera`` and ``BatchedSim.raycast`` raise.
+MuJoCo C is not part of this engine). Rendering is outside the engine:
+``create_camera`` raises.
 """
 from __future__ import annotations
 
@@ -244,7 +244,8 @@ class BatchedSim:
         return _copy_out(p, n.value, self.num_envs)
 
     def raycast(self, pnt, vec, bodyexclude: int = -1, geomgroup=None, flg_static: bool = True):
-        """mj_ray per env and ray; not part of this engine (the C call returns -1)."""
+        """mj_ray per env and ray at the current state: pnt / vec (num_envs, nray, 3)
+        or shared (nray, 3); returns (dist, geom id), each (num_envs, nray), -1 for no hit."""
         p, v = _f32(pnt, "pnt"), _f32(vec, "vec")
 
         def per_env(a, name):

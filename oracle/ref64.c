@@ -2046,6 +2046,98 @@ int orc_forward(const orc_prep_t* p, orc_data_t* d) {
   return status;
 }
 
+/* ------------------------------------------------------------------ rays */
+/* mj_ray against one primitive in its own frame: smallest x >= 0 with p + x v
+ * on the surface, else -1 (x in units of |v|). Planes are hit from +z only,
+ * inside nonzero half extents; from inside a closed primitive the exit point
+ * is hit. Reference src/batched.cpp:1739-1806. */
+static double ray_quadric(double a, double b, double c) {
+  double disc = b * b - a * c;
+  if (disc < 1e-12 || fabs(a) < 1e-12) return -1;
+  double s = sqrt(disc), x0 = (-b - s) / a, x1 = (-b + s) / a;
+  return x0 >= 0 ? x0 : (x1 >= 0 ? x1 : -1);
+}
+static double ray_closer(double best, double x) { return x >= 0 && (best < 0 || x < best) ? x : best; }
+double orc_ray_geom(int type, const double* sz, const double* p, const double* v) {
+  if (type == GPLANE) {
+    if (v[2] > -1e-9) return -1;
+    double x = -p[2] / v[2];
+    if (x < 0) return -1;
+    double hx = p[0] + x * v[0], hy = p[1] + x * v[1];
+    if ((sz[0] > 0 && fabs(hx) > sz[0]) || (sz[1] > 0 && fabs(hy) > sz[1])) return -1;
+    return x;
+  }
+  if (type == GSPHERE) return ray_quadric(dot3(v, v), dot3(p, v), dot3(p, p) - sz[0] * sz[0]);
+  if (type == GELLIPSOID) {
+    double ps[3], vs[3];
+    for (int i = 0; i < 3; i++) { ps[i] = p[i] / sz[i]; vs[i] = v[i] / sz[i]; }
+    return ray_quadric(dot3(vs, vs), dot3(ps, vs), dot3(ps, ps) - 1);
+  }
+  if (type == GCAPSULE || type == GCYLINDER) {
+    double r = sz[0], h = sz[1], best = -1;
+    double a = v[0] * v[0] + v[1] * v[1], b = p[0] * v[0] + p[1] * v[1], c = p[0] * p[0] + p[1] * p[1] - r * r;
+    double disc = b * b - a * c;
+    if (a > 1e-12 && disc >= 0) {
+      for (int k = -1; k <= 1; k += 2) {
+        double x = (-b + k * sqrt(disc)) / a;
+        if (fabs(p[2] + x * v[2]) <= h) best = ray_closer(best, x);
+      }
+    }
+    for (int e = -1; e <= 1; e += 2) {
+      if (type == GCYLINDER) {
+        if (fabs(v[2]) <= 1e-12) continue;
+        double x = (e * h - p[2]) / v[2], qx = p[0] + x * v[0], qy = p[1] + x * v[1];
+        if (x >= 0 && qx * qx + qy * qy <= r * r) best = ray_closer(best, x);
+      } else {
+        double pc[3] = {p[0], p[1], p[2] - e * h};
+        double a2 = dot3(v, v), b2 = dot3(pc, v), c2 = dot3(pc, pc) - r * r, d2 = b2 * b2 - a2 * c2;
+        if (d2 < 0) continue;
+        for (int k = -1; k <= 1; k += 2) {
+          double x = (-b2 + k * sqrt(d2)) / a2;
+          if (e * (p[2] + x * v[2] - e * h) >= 0) best = ray_closer(best, x);
+        }
+      }
+    }
+    return best;
+  }
+  if (type == GBOX) {
+    double tn = -1e300, tf = 1e300;
+    for (int i = 0; i < 3; i++) {
+      if (fabs(v[i]) < 1e-12) {
+        if (fabs(p[i]) > sz[i]) return -1;
+        continue;
+      }
+      double t0 = (-sz[i] - p[i]) / v[i], t1 = (sz[i] - p[i]) / v[i];
+      tn = fmax(tn, fmin(t0, t1));
+      tf = fmin(tf, fmax(t0, t1));
+    }
+    if (tn > tf || tf < 0) return -1;
+    return tn >= 0 ? tn : tf;
+  }
+  return -1;
+}
+
+/* nearest geom allowed by geom_ok along pnt + x vec at d's geom poses (call
+ * orc_kinematics first); returns x or -1 and the geom id or -1 */
+double orc_ray(const orc_prep_t* p, const orc_data_t* d, const double* pnt, const double* vec, const int* geom_ok,
+               int* geomid) {
+  const orc_model_t* m = &p->m;
+  double best = -1;
+  *geomid = -1;
+  for (int g = 0; g < m->ngeom; g++) {
+    if (!geom_ok[g]) continue;
+    const double *x = d->geom_xpos + 3 * g, *R = d->geom_xmat + 9 * g;
+    double dd[3] = {pnt[0] - x[0], pnt[1] - x[1], pnt[2] - x[2]}, lp[3], lv[3];
+    for (int k = 0; k < 3; k++) {
+      lp[k] = R[k] * dd[0] + R[3 + k] * dd[1] + R[6 + k] * dd[2];
+      lv[k] = R[k] * vec[0] + R[3 + k] * vec[1] + R[6 + k] * vec[2];
+    }
+    double t = orc_ray_geom(m->geom_type[g], m->geom_size + 3 * g, lp, lv);
+    if (t >= 0 && (best < 0 || t < best)) { best = t; *geomid = g; }
+  }
+  return best;
+}
+
 void orc_kinematics(const orc_prep_t* p, orc_data_t* d) {
   kinematics(p, d);
   com_pos(p, d);

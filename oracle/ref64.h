@@ -102,6 +102,11 @@ void orc_get_state(const orc_data_t* d, double* qpos, double* qvel, double* qacc
 void orc_set_env_params(orc_data_t* d, const double* mass_scale, const double* com_delta, const double* friction_scale,
                         const double* gain_scale, const double* drag);
 int orc_nfree(const orc_data_t* d);
+/* mj_ray: one primitive in its own frame (x along p + x v, or -1), and the
+ * nearest geom with geom_ok[g] != 0 at d's geom poses (after orc_kinematics) */
+double orc_ray_geom(int type, const double* size, const double* p, const double* v);
+double orc_ray(const orc_prep_t* p, const orc_data_t* d, const double* pnt, const double* vec, const int* geom_ok,
+               int* geomid);
 
 /* One physics step (mj_step): forward pipeline, solver, integrator.
  * Returns 0, or a negative status (>0 capacity overflow bits). */

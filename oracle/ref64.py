@@ -104,6 +104,10 @@ def lib() -> C.CDLL:
         L.orc_get_state.argtypes = [C.c_void_p, _D, _D, _D]
         L.orc_set_env_params.argtypes = [C.c_void_p, _D, _D, _D, _D, _D]
         L.orc_nfree.argtypes = [C.c_void_p]
+        L.orc_ray_geom.restype = C.c_double
+        L.orc_ray_geom.argtypes = [C.c_int, _D, _D, _D]
+        L.orc_ray.restype = C.c_double
+        L.orc_ray.argtypes = [C.c_void_p, C.c_void_p, _D, _D, _I, C.POINTER(C.c_int)]
         L.orc_step.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_forward.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_kinematics.argtypes = [C.c_void_p, C.c_void_p]
@@ -226,6 +230,17 @@ class Oracle:
         lib().orc_set_env_params(self.d, *(None if a is None else _dp(a) for a in args))
         return self
 
+    def ray(self, pnt, vec, geom_ok):
+        """mj_ray at the current geom poses (call kinematics() first): (x, geom id)
+        of the nearest allowed geom along pnt + x vec, or (-1, -1)."""
+        p = np.ascontiguousarray(np.asarray(pnt, np.float64).ravel())
+        v = np.ascontiguousarray(np.asarray(vec, np.float64).ravel())
+        ok = np.ascontiguousarray(np.asarray(geom_ok, np.int32).ravel())
+        assert ok.size == self.ngeom
+        g = C.c_int()
+        x = lib().orc_ray(self.p, self.d, _dp(p), _dp(v), _ip(ok), C.byref(g))
+        return x, g.value
+
     def nfree(self) -> int:
         return lib().orc_nfree(self.d)
 
@@ -298,6 +313,13 @@ def fields_from(getter) -> dict:
         else:
             out[name] = v
     return out
+
+
+def ray_geom(gtype: int, size, p, v) -> float:
+    """mj_ray against one primitive in its own frame: x along p + x v, or -1."""
+    f = lambda a: np.ascontiguousarray(np.asarray(a, np.float64).ravel())
+    s, pp, vv = f(size), f(p), f(v)
+    return lib().orc_ray_geom(int(gtype), _dp(s), _dp(pp), _dp(vv))
 
 
 def math(op: str, inp, nout: int = 64) -> np.ndarray:

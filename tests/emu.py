@@ -40,6 +40,9 @@ def lib():
         L.mjemu_features.argtypes = [C.c_void_p, _F]
         L.mjemu_set_pred.argtypes = [C.c_void_p, _F]
         L.mjemu_set_rand.argtypes = [C.c_void_p, _F]
+        L.mjemu_ray_geom.restype = C.c_float
+        L.mjemu_ray_geom.argtypes = [C.c_int, _F, _F, _F]
+        L.mjemu_rays.argtypes = [C.c_void_p, _F, C.c_int, _I, _F, _I]
         L.mjemu_narrow.argtypes = [C.c_int, C.c_int, _F, _F, _F, _F, _F, _F, C.c_float, _F]
         L.mjemu_compile.restype = C.c_void_p
         L.mjemu_compile.argtypes = [C.c_char_p, C.c_int, C.c_int]
@@ -124,6 +127,17 @@ class EmuSim:
         assert n == self._rand.shape[1], (n, self._rand.shape)
         self.total = self.layout("total")[1]
 
+    def rays(self, rays, gmask):
+        """The kernel's per-env rays at the current qpos: rays [B, nray, 6]
+        (pnt, vec), gmask [ngeom]; returns dist, gid [B, nray]."""
+        r = np.ascontiguousarray(rays, dtype=np.float32)
+        nray = r.shape[1]
+        g = np.ascontiguousarray(gmask, dtype=np.int32)
+        dist = np.zeros((self.B, nray), np.float32)
+        gid = np.zeros((self.B, nray), np.int32)
+        lib().mjemu_rays(self.h, _fp(r), nray, g.ctypes.data_as(_I), _fp(dist), gid.ctypes.data_as(_I))
+        return dist, gid
+
     def dump(self, which: int, env: int, name: str) -> np.ndarray:
         off, ln, kind = self.layout(name)
         p = lib().mjemu_dump(self.h, which)
@@ -143,6 +157,13 @@ def narrow(t1, t2, s1, s2, x1, R1, x2, R2, margin):
     if n < 0:
         raise ValueError("unsupported pair")
     return [(float(out[7 * k]), out[7 * k + 1:7 * k + 4].copy(), out[7 * k + 4:7 * k + 7].copy()) for k in range(n)]
+
+
+def ray_geom(gtype, size, p, v) -> float:
+    """The kernel's ray-primitive intersection (FP32) in the geom frame."""
+    f = lambda a: np.ascontiguousarray(np.asarray(a, np.float32).ravel())
+    a = [f(x) for x in (size, p, v)]
+    return float(lib().mjemu_ray_geom(int(gtype), *(_fp(x) for x in a)))
 
 
 class CompiledModel:

@@ -73,8 +73,16 @@ def test_argument_checks_and_domain_rand(tmp_path):
         sim.set_env_params(mass_scale=np.ones((B, model.nbody + 1), np.float32))
     with pytest.raises(RuntimeError, match="call set_pack_spec first"):
         sim.read_pack()
-    with pytest.raises(RuntimeError, match="raycast failed"):
-        sim.raycast(np.zeros((1, 3), np.float32), np.ones((1, 3), np.float32))
+    # shared rays (nray, 3) are the same rays for every env
+    pnt = np.array([[0, 0, 2.0], [0.5, 0, 2.0]], np.float32)
+    vec = np.array([[0, 0, -1.0], [0, 0, -1.0]], np.float32)
+    d, g = sim.raycast(pnt, vec)
+    d2, g2 = sim.raycast(np.ascontiguousarray(np.broadcast_to(pnt, (B, 2, 3))),
+                         np.ascontiguousarray(np.broadcast_to(vec, (B, 2, 3))))
+    assert d.shape == g.shape == (B, 2) and np.array_equal(d, d2) and np.array_equal(g, g2)
+    assert (g >= 0).all()
+    with pytest.raises(RuntimeError, match="pnt must have shape"):
+        sim.raycast(np.zeros((2, 2), np.float32), np.ones((2, 3), np.float32))
     with pytest.raises(RuntimeError):
         vkmujoco.create_camera(sim, 64, 64)
     assert sim.num_free_joints == 1
