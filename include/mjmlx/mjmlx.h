@@ -51,7 +51,12 @@ MJMLX_API void mjmlx_model_set_opt_timestep(MjmlxModel* model, float dt);      /
 MJMLX_API float mjmlx_model_body_mass(const MjmlxModel* model, int body_id);   /* :78 */
 /* obj_type: MuJoCo mjtObj (1 body, 2 joint, 5 geom, 19 actuator) (:83) */
 MJMLX_API int mjmlx_name2id(const MjmlxModel* model, int obj_type, const char* name);
-MJMLX_API int mjmlx_model_nsensor(const MjmlxModel* model);                    /* :86 (no sensors: 0) */
+/* sensors of the MJCF <sensor> section (reference src/sensor.cpp:362-377):
+ * jointpos/vel, actuatorpos/vel, jointactuatorfrc, ballquat/angvel, frame
+ * pos/quat/x,y,z axis/linvel/angvel of a site, geom, body or xbody, gyro,
+ * velocimeter, accelerometer, magnetometer, subtreecom; types are MuJoCo's
+ * mjtSensor ids */
+MJMLX_API int mjmlx_model_nsensor(const MjmlxModel* model);                    /* :86 */
 MJMLX_API int mjmlx_model_nsensordata(const MjmlxModel* model);                /* :87 */
 MJMLX_API const char* mjmlx_model_sensor_name(const MjmlxModel* model, int sensor_id); /* :88 */
 MJMLX_API int mjmlx_model_sensor_type(const MjmlxModel* model, int sensor_id); /* :89 */
@@ -118,6 +123,9 @@ MJMLX_API const float* mjmlx_batched_get_cvel(const MjmlxBatchedSim* sim, int* n
 MJMLX_API const float* mjmlx_batched_get_qfrc_actuator(const MjmlxBatchedSim* sim, int* n_out);
 MJMLX_API const float* mjmlx_batched_get_cfrc_ext(const MjmlxBatchedSim* sim, int* n_out);
 MJMLX_API int mjmlx_batched_nsensordata(const MjmlxBatchedSim* sim);                            /* :220 */
+/* [num_envs][nsensordata] of the state the last step's last substep started
+ * from (as the reference's outputs); the accelerometer differences its frame's
+ * velocity over one substep. NULL without sensors. */
 MJMLX_API const float* mjmlx_batched_get_sensordata(const MjmlxBatchedSim* sim, int* n_out);     /* :221 */
 MJMLX_API void mjmlx_batched_set_env_qpos(MjmlxBatchedSim* sim, int env_idx, const float* qpos, int nq); /* :224 */
 MJMLX_API void mjmlx_batched_set_env_qvel(MjmlxBatchedSim* sim, int env_idx, const float* qvel, int nv); /* :225 */

@@ -40,6 +40,9 @@ struct DModel {
   int has_damping;
   float timestep, gravity[3], tolerance, ls_tolerance, impratio, meaninertia;
   int nfeat;  // learned warm-start input width: nq + nv + nu + FEAT_NCON * FEAT_PER_CON
+  int nsensor, nsensordata, nsens_acc;  // sensors, sensordata width, accelerometers
+  float magnetic[3];
+  // pointers only from here on: Blob::bind relocates every word after body_parentid
   // bodies
   const int *body_parentid, *body_rootid, *body_jntnum, *body_jntadr, *body_dofnum, *body_dofadr,
       *body_subtree_end, *level_adr, *level_body, *body_lastdof;
@@ -67,7 +70,18 @@ struct DModel {
   const float *act_gear, *act_gain, *act_bias, *act_ctrlrange, *act_forcerange;
   // constraint source tables
   const int *floss_dof, *lim_jnt;
+  // sensors (MuJoCo mjtSensor ids): sensordata address and width, frame or
+  // subtree body (-1 for joint sensors) with the frame's pose in that body,
+  // qpos / dof address and gear of joint and actuator sensors, accelerometer
+  // slot (-1 for other sensors)
+  const int *sens_type, *sens_adr, *sens_dim, *sens_body, *sens_qadr, *sens_dadr, *sens_acc;
+  const float *sens_pos, *sens_quat, *sens_gear;
 };
+// sensor kinds (MuJoCo mjtSensor)
+enum { SENS_ACCELEROMETER = 1, SENS_VELOCIMETER = 2, SENS_GYRO = 3, SENS_MAGNETOMETER = 6, SENS_JOINTPOS = 9,
+       SENS_JOINTVEL = 10, SENS_ACTUATORPOS = 13, SENS_ACTUATORVEL = 14, SENS_JOINTACTFRC = 16, SENS_BALLQUAT = 18,
+       SENS_BALLANGVEL = 19, SENS_FRAMEPOS = 26, SENS_FRAMEQUAT = 27, SENS_FRAMEXAXIS = 28, SENS_FRAMEYAXIS = 29,
+       SENS_FRAMEZAXIS = 30, SENS_FRAMELINVEL = 31, SENS_FRAMEANGVEL = 32, SENS_SUBTREECOM = 35 };
 
 // Per-env workspace layout in floats. Region A (kinematics / dynamics
 // intermediates) is dead once the constraint Jacobian is built, so the FP64

@@ -6,6 +6,7 @@
 // (libmjmlx_emu.so) that only tests load; the engine library libmjmlx.so never
 // contains or calls it.
 #define MJ_EMU 1
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -23,7 +24,7 @@ struct EmuSim {
   mjk::Layout L;
   std::vector<mjh::LayoutField> fields;
   int B;
-  std::vector<float> qpos, qvel, qacc_ws, ws, dump1, dump2, feat, pred, rand;
+  std::vector<float> qpos, qvel, qacc_ws, ws, dump1, dump2, feat, pred, rand, sensordata, sens_prev;
   bool use_pred = false;
   mjh::Caps caps;
   int nrand = 0;
@@ -60,6 +61,8 @@ __attribute__((visibility("default"))) void* mjemu_create(const char* xml, int s
     s->qacc_ws.assign((size_t)B * m.nv, 0.f);
     s->ws.assign(s->L.total, 0.f);
     s->niter.assign(B, 0); s->status.assign(B, 0); s->ncon.assign(B, 0); s->nefc.assign(B, 0);
+    s->sensordata.assign((size_t)B * std::max(1, m.nsensordata), 0.f);
+    s->sens_prev.assign((size_t)B * (3 * s->dm.nsens_acc + 1), 0.f);
     return s;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "mjemu_create: %s\n", e.what());
@@ -110,6 +113,7 @@ __attribute__((visibility("default"))) int mjemu_step(void* p, const float* ctrl
   s->feat.assign((size_t)s->B * s->dm.nfeat, 0.f);
   io.feat = s->feat.data();
   if (s->nrand > 0) { io.rand = s->rand.data(); io.nrand = s->nrand; }
+  io.sensordata = s->sensordata.data(); io.sens_prev = s->sens_prev.data();
   if (s->use_pred) io.qacc_pred = s->pred.data();
   if (dump) {
     s->dump1.assign((size_t)s->B * s->L.total, 0.f);
@@ -172,6 +176,14 @@ __attribute__((visibility("default"))) int mjemu_narrow(int t1, int t2, const fl
     for (int q = 0; q < 3; q++) { out[7 * k + 1 + q] = rc[k].pos[q]; out[7 * k + 4 + q] = rc[k].n[q]; }
   }
   return n;
+}
+
+// sensordata of the last step [B][nsensordata]; returns nsensordata
+__attribute__((visibility("default"))) int mjemu_sensordata(void* p, float* out) {
+  auto* s = static_cast<EmuSim*>(p);
+  int nsd = s->model.nsensordata;
+  if (out && nsd > 0) std::memcpy(out, s->sensordata.data(), (size_t)s->B * nsd * sizeof(float));
+  return nsd;
 }
 
 // the kernel's ray-primitive intersection (ray_geom) for one ray in the geom frame

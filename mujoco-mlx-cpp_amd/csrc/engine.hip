@@ -98,9 +98,13 @@ __global__ void k_zero_int(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
-__global__ void k_reset(const int* mask, float* qpos, float* qvel, float* qws, const float* qpos0, int B, int nq, int nv) {
+// sprev: accelerometer state [B][pstride] whose last entry (previous velocity
+// valid) a reset clears, or null
+__global__ void k_reset(const int* mask, float* qpos, float* qvel, float* qws, const float* qpos0, int B, int nq, int nv,
+                        float* sprev, int pstride) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= B || !mask[e]) return;
+  if (sprev) sprev[(size_t)e * pstride + pstride - 1] = 0.f;
   for (int i = 0; i < nq; i++) qpos[(size_t)e * nq + i] = qpos0[i];
   for (int i = 0; i < nv; i++) { qvel[(size_t)e * nv + i] = 0.f; qws[(size_t)e * nv + i] = 0.f; }
 }
@@ -186,6 +190,9 @@ struct MjmlxBatchedSim {
   float* d_rand = nullptr;
   std::vector<float> h_rand;
   int nrand = 0, nfree = 0;
+  // sensors: sensordata [B][nsensordata] (device and a pinned host copy) and
+  // the accelerometers' previous frame velocities [B][3 * nacc + 1]
+  float *d_sensordata = nullptr, *h_sensordata = nullptr, *d_sens_prev = nullptr;
   // raycast buffers: rays [B][nray][6], results, geom filter (grown on demand)
   float *d_rays = nullptr, *d_rdist = nullptr;
   int *d_rgid = nullptr, *d_gmask = nullptr;
@@ -208,13 +215,14 @@ void free_sim(MjmlxBatchedSim* s) {
   void* dev[] = {s->d_blob, s->d_qpos, s->d_qvel, s->d_qws, s->d_ctrl, s->d_qpos0, s->d_niter, s->d_status,
                  s->d_ncon, s->d_nefc, s->d_queue, s->d_mask, s->d_pack_bodies, s->d_pack, s->d_dump1, s->d_dump2,
                  s->d_feat[0], s->d_feat[1], s->d_pred, s->d_gws, s->d_rand, s->d_rays, s->d_rdist,
-                 s->d_rgid, s->d_gmask};
+                 s->d_rgid, s->d_gmask, s->d_sensordata, s->d_sens_prev};
   for (void* p : dev) if (p) hipFree(p);
   for (int i = 0; i < 7; i++) { if (s->d_aux[i]) hipFree(s->d_aux[i]); if (s->h_aux[i]) hipHostFree(s->h_aux[i]); }
   if (s->h_qpos) hipHostFree(s->h_qpos);
   if (s->h_qvel) hipHostFree(s->h_qvel);
   if (s->h_ctrl) hipHostFree(s->h_ctrl);
   if (s->h_mask) hipHostFree(s->h_mask);
+  if (s->h_sensordata) hipHostFree(s->h_sensordata);
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
   if (s->ev_ctrl) hipEventDestroy(s->ev_ctrl);
@@ -254,6 +262,7 @@ mjk::StepIO make_io(MjmlxBatchedSim* s, float* dump1, float* dump2) {
   io.aux.cinert = s->d_aux[3]; io.aux.cvel = s->d_aux[4]; io.aux.qfrc_actuator = s->d_aux[5];
   io.aux.cfrc_ext = s->d_aux[6];
   io.rand = s->d_rand; io.nrand = s->nrand;
+  io.sensordata = s->d_sensordata; io.sens_prev = s->d_sens_prev;
   return io;
 }
 
@@ -491,12 +500,22 @@ MJMLX_API int mjmlx_name2id(const MjmlxModel* model, int obj_type, const char* n
   if (!model || !name) return -1;
   return model->m.name2id(obj_type, name);
 }
-MJMLX_API int mjmlx_model_nsensor(const MjmlxModel*) { return 0; }
-MJMLX_API int mjmlx_model_nsensordata(const MjmlxModel*) { return 0; }
-MJMLX_API const char* mjmlx_model_sensor_name(const MjmlxModel*, int) { return nullptr; }
-MJMLX_API int mjmlx_model_sensor_type(const MjmlxModel*, int) { return -1; }
-MJMLX_API int mjmlx_model_sensor_adr(const MjmlxModel*, int) { return -1; }
-MJMLX_API int mjmlx_model_sensor_dim(const MjmlxModel*, int) { return -1; }
+// sensors of the compiled model (reference src/sensor.cpp:362-377)
+MJMLX_API int mjmlx_model_nsensor(const MjmlxModel* model) { return model ? model->m.nsensor : 0; }
+MJMLX_API int mjmlx_model_nsensordata(const MjmlxModel* model) { return model ? model->m.nsensordata : 0; }
+MJMLX_API const char* mjmlx_model_sensor_name(const MjmlxModel* model, int i) {
+  if (!model || i < 0 || i >= model->m.nsensor) return nullptr;
+  return model->m.sensor_name[i].c_str();
+}
+MJMLX_API int mjmlx_model_sensor_type(const MjmlxModel* model, int i) {
+  return model && i >= 0 && i < model->m.nsensor ? model->m.sensor_type[i] : -1;
+}
+MJMLX_API int mjmlx_model_sensor_adr(const MjmlxModel* model, int i) {
+  return model && i >= 0 && i < model->m.nsensor ? model->m.sensor_adr[i] : -1;
+}
+MJMLX_API int mjmlx_model_sensor_dim(const MjmlxModel* model, int i) {
+  return model && i >= 0 && i < model->m.nsensor ? model->m.sensor_dim[i] : -1;
+}
 MJMLX_API const void* mjmlx_get_mj_model(const MjmlxModel*) { return nullptr; }
 
 MJMLX_API int mjmlx_model_field(const MjmlxModel* model, const char* name, double* out, int cap) {
@@ -642,6 +661,14 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     HIPOK(hipMemset(s->d_status, 0, B * sizeof(int)));
     HIPOK(hipMemset(s->d_ncon, 0, B * sizeof(int)));
     HIPOK(hipMemset(s->d_nefc, 0, B * sizeof(int)));
+    if (m.nsensor > 0) {
+      size_t nsd = (size_t)m.nsensordata, pst = (size_t)s->dm.nsens_acc * 3 + 1;
+      HIPOK(hipMalloc(&s->d_sensordata, B * nsd * sizeof(float)));
+      HIPOK(hipHostMalloc(&s->h_sensordata, B * nsd * sizeof(float), hipHostMallocDefault));
+      HIPOK(hipMalloc(&s->d_sens_prev, B * pst * sizeof(float)));
+      HIPOK(hipMemset(s->d_sensordata, 0, B * nsd * sizeof(float)));
+      HIPOK(hipMemset(s->d_sens_prev, 0, B * pst * sizeof(float)));
+    }
     if (s->nrand > 0) {
       // identity parameters: unit mass, friction and force scales, no COM shift, no drag
       s->h_rand.assign(B * s->nrand, 0.f);
@@ -682,7 +709,8 @@ MJMLX_API void mjmlx_batched_reset(MjmlxBatchedSim* sim, const int* mask) {
     std::memcpy(sim->h_mask, mask, sim->B * sizeof(int));
     HIPOK(hipMemcpyAsync(sim->d_mask, sim->h_mask, sim->B * sizeof(int), hipMemcpyHostToDevice, sim->stream));
     hipLaunchKernelGGL(k_reset, dim3((sim->B + 255) / 256), dim3(256), 0, sim->stream, sim->d_mask, sim->d_qpos,
-                       sim->d_qvel, sim->d_qws, sim->d_qpos0, sim->B, sim->nq, sim->nv);
+                       sim->d_qvel, sim->d_qws, sim->d_qpos0, sim->B, sim->nq, sim->nv, sim->d_sens_prev,
+                       sim->dm.nsens_acc * 3 + 1);
     HIPOK(hipGetLastError());
     HIPOK(hipStreamSynchronize(sim->stream));
   } catch (const std::exception& e) { report("mjmlx_batched_reset", e); }
@@ -748,10 +776,23 @@ MJMLX_API const float* mjmlx_batched_get_cinert(const MjmlxBatchedSim* s, int* n
 MJMLX_API const float* mjmlx_batched_get_cvel(const MjmlxBatchedSim* s, int* n) { return get_aux(s, 4, n); }
 MJMLX_API const float* mjmlx_batched_get_qfrc_actuator(const MjmlxBatchedSim* s, int* n) { return get_aux(s, 5, n); }
 MJMLX_API const float* mjmlx_batched_get_cfrc_ext(const MjmlxBatchedSim* s, int* n) { return get_aux(s, 6, n); }
-MJMLX_API int mjmlx_batched_nsensordata(const MjmlxBatchedSim*) { return 0; }
-MJMLX_API const float* mjmlx_batched_get_sensordata(const MjmlxBatchedSim*, int* n_out) {
+MJMLX_API int mjmlx_batched_nsensordata(const MjmlxBatchedSim* s) { return s ? s->model.nsensordata : 0; }
+
+// Sensor values [num_envs][nsensordata] at the start of the last physics
+// substep of the last step (reference src/batched.cpp:1635-1651, sensor.cpp:
+// the outputs of a step describe the state it started from); NULL without
+// sensors. Valid until the next step, set or free.
+MJMLX_API const float* mjmlx_batched_get_sensordata(const MjmlxBatchedSim* cs, int* n_out) {
   if (n_out) *n_out = 0;
-  return nullptr;
+  if (!cs || cs->model.nsensordata == 0) return nullptr;
+  auto* s = const_cast<MjmlxBatchedSim*>(cs);
+  try {
+    size_t n = (size_t)s->B * s->model.nsensordata;
+    HIPOK(hipMemcpyAsync(s->h_sensordata, s->d_sensordata, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPOK(hipStreamSynchronize(s->stream));
+    if (n_out) *n_out = (int)n;
+    return s->h_sensordata;
+  } catch (const std::exception& e) { report("mjmlx_batched_get_sensordata", e); return nullptr; }
 }
 
 MJMLX_API void mjmlx_batched_set_env_qpos(MjmlxBatchedSim* s, int env, const float* qpos, int nq) {

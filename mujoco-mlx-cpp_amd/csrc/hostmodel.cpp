@@ -238,6 +238,30 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
     }
     for (int g = 0; g < m.ngeom; g++) fclass[g] = limb[m.geom_bodyid[g]];
   }
+  // sensors: frame body and local pose (reference src/sensor.cpp:112-124:
+  // site and geom frames, a body's inertial frame, an xbody's own frame)
+  int ns = m.nsensor, nacc = 0;
+  std::vector<int> s_body(ns, -1), s_qadr(ns, 0), s_dadr(ns, 0), s_acc(ns, -1);
+  std::vector<float> s_pos(3 * ns, 0.f), s_quat(4 * ns, 0.f), s_gear(ns, 1.f);
+  for (int i = 0; i < ns; i++) {
+    int t = m.sensor_type[i], ot = m.sensor_objtype[i], id = m.sensor_objid[i];
+    s_quat[4 * i] = 1.f;
+    auto frame = [&](int body, const double* p, const double* q) {
+      s_body[i] = body;
+      for (int k = 0; k < 3; k++) s_pos[3 * i + k] = p ? (float)p[k] : 0.f;
+      for (int k = 0; k < 4; k++) s_quat[4 * i + k] = q ? (float)q[k] : (k == 0 ? 1.f : 0.f);
+    };
+    if (ot == 6) frame(m.site_bodyid[id], &m.site_pos[3 * id], &m.site_quat[4 * id]);
+    else if (ot == 5) frame(m.geom_bodyid[id], &m.geom_pos[3 * id], &m.geom_quat[4 * id]);
+    else if (ot == 1 && t != SENS_SUBTREECOM) frame(id, &m.body_ipos[3 * id], &m.body_iquat[4 * id]);
+    else if (ot == 1 || ot == 2) frame(id, nullptr, nullptr);
+    else if (ot == 3) { s_qadr[i] = m.jnt_qposadr[id]; s_dadr[i] = m.jnt_dofadr[id]; }
+    else if (ot == 19) {
+      int j = m.actuator_trnid[id];
+      s_qadr[i] = m.jnt_qposadr[j]; s_dadr[i] = m.jnt_dofadr[j]; s_gear[i] = (float)m.actuator_gear[id];
+    }
+    if (t == SENS_ACCELEROMETER) s_acc[i] = nacc++;
+  }
   std::vector<float> gain(3 * m.nu), bias(3 * m.nu);
   for (int i = 0; i < 3 * m.nu; i++) { gain[i] = (float)m.actuator_gainprm[i]; bias[i] = (float)m.actuator_biasprm[i]; }
 
@@ -256,6 +280,8 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   d.tolerance = (float)m.tolerance; d.ls_tolerance = (float)m.ls_tolerance; d.impratio = (float)m.impratio;
   d.meaninertia = (float)m.meaninertia;
   d.nfeat = m.nq + nv + m.nu + FEAT_NCON * FEAT_PER_CON;
+  d.nsensor = ns; d.nsensordata = m.nsensordata; d.nsens_acc = nacc;
+  for (int i = 0; i < 3; i++) d.magnetic[i] = (float)m.magnetic[i];
   std::vector<float> subtreemass = tof(m.body_subtreemass);
 #define ADDP(field, vec) d.field = reinterpret_cast<decltype(d.field)>(B.add(vec))
   ADDP(body_parentid, m.body_parentid); ADDP(body_rootid, m.body_rootid); ADDP(body_jntnum, m.body_jntnum);
@@ -286,6 +312,9 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   ADDP(act_gear, tof(m.actuator_gear)); ADDP(act_gain, gain); ADDP(act_bias, bias);
   ADDP(act_ctrlrange, tof(m.actuator_ctrlrange)); ADDP(act_forcerange, tof(m.actuator_forcerange));
   ADDP(floss_dof, floss_dof); ADDP(lim_jnt, lim_jnt);
+  ADDP(sens_type, m.sensor_type); ADDP(sens_adr, m.sensor_adr); ADDP(sens_dim, m.sensor_dim); ADDP(sens_body, s_body);
+  ADDP(sens_qadr, s_qadr); ADDP(sens_dadr, s_dadr); ADDP(sens_acc, s_acc);
+  ADDP(sens_pos, s_pos); ADDP(sens_quat, s_quat); ADDP(sens_gear, s_gear);
 #undef ADDP
   blob.bytes = std::move(B.bytes);
   if (pairs_out) *pairs_out = pairs;
@@ -359,6 +388,10 @@ int model_field(const Model& m, const char* name, double* out, int cap) {
   if (n == "meaninertia") return scal(m.meaninertia);
   if (n == "npair") return scal((double)m.pair_geom1.size());
   if (n == "nexclude") return scal((double)m.exclude_signature.size());
+  if (n == "nsite") return scal(m.nsite);
+  if (n == "nsensor") return scal(m.nsensor);
+  if (n == "nsensordata") return scal(m.nsensordata);
+  if (n == "magnetic") return put({m.magnetic[0], m.magnetic[1], m.magnetic[2]});
 #define F(x) if (n == #x) return put(m.x);
 #define I(x) if (n == #x) return puti(m.x);
   I(body_parentid) I(body_rootid) I(body_weldid) I(body_jntnum) I(body_jntadr) I(body_dofnum) I(body_dofadr)
@@ -375,6 +408,8 @@ int model_field(const Model& m, const char* name, double* out, int cap) {
   F(actuator_gear) F(actuator_gainprm) F(actuator_biasprm) F(actuator_ctrlrange) F(actuator_forcerange)
   I(pair_geom1) I(pair_geom2) I(pair_dim) F(pair_friction) F(pair_solref) F(pair_solimp) F(pair_margin) F(pair_gap)
   I(exclude_signature) F(qpos0) F(qpos_spring)
+  I(site_bodyid) F(site_pos) F(site_quat)
+  I(sensor_type) I(sensor_objtype) I(sensor_objid) I(sensor_adr) I(sensor_dim)
 #undef F
 #undef I
   if (n.rfind("cpair_", 0) == 0) {

@@ -2606,7 +2606,97 @@ struct StepIO {
   unsigned long long* prof;  // profiling builds: per-env stage cycles [env * KP_N + stage]
   const float* rand;         // domain randomisation: per-env parameter blocks [env * nrand], or null
   int nrand;
+  float* sensordata;         // sensors: [env * nsensordata] of the last substep's start state, or null
+  float* sens_prev;          // accelerometers' previous frame velocities [env * (3 * nsens_acc + 1)], last: valid
 };
+
+// Sensors of the substep's start state (reference src/sensor.cpp:265-359),
+// one lane per sensor, after the smooth forces: frame sensors from the body
+// pose and cvel (point velocity about the root's subtree COM), joint and
+// actuator sensors from qpos / qvel / actuator forces. Accelerometers
+// differentiate their frame's world velocity over one substep (zero until a
+// previous substep exists, gravity removed as in MuJoCo), so their velocity
+// is kept every substep; the row is written on the last substep only.
+KDEV void st_sensors(const DModel& m, const Layout& L, const float* w, const StepIO& io, int env, bool first,
+                     bool last) {
+  const float *xpos = w + L.xpos, *xquat = w + L.xquat, *xmat = w + L.xmat, *cvel = w + L.cvel, *scom = w + L.scom;
+  const float *qpos = w + L.qpos, *qvel = w + L.qvel, *af = w + L.act_force;
+  int pstride = 3 * m.nsens_acc + 1;
+  float* prev = io.sens_prev + (size_t)env * pstride;
+  // a previous velocity exists after the launch's first substep, and at the
+  // first one when an earlier launch left it (a reset clears the flag)
+  bool valid = m.nsens_acc > 0 && (!first || prev[pstride - 1] != 0.f);
+  float* out = io.sensordata + (size_t)env * m.nsensordata;
+  for (int i = LANE; i < m.nsensor; i += WAVE) {
+    int t = m.sens_type[i], acc = m.sens_acc[i], b = m.sens_body[i];
+    if (!last && acc < 0) continue;
+    float y[4] = {0.f, 0.f, 0.f, 0.f};
+    float pos[3] = {0.f, 0.f, 0.f}, R[9], q[4] = {1.f, 0.f, 0.f, 0.f}, ang[3] = {0.f, 0.f, 0.f}, lin[3] = {0.f, 0.f, 0.f};
+    if (b >= 0 && t != SENS_SUBTREECOM) {  // the frame in world coordinates
+      const float *Rb = xmat + 9 * b, *lp = m.sens_pos + 3 * i, *lq = m.sens_quat + 4 * i;
+      float t3[3], Rl[9];
+      mv3(Rb, lp, t3);
+      for (int k = 0; k < 3; k++) pos[k] = xpos[3 * b + k] + t3[k];
+      quat2mat(lq, Rl);
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) R[3 * r + c] = Rb[3 * r] * Rl[c] + Rb[3 * r + 1] * Rl[3 + c] + Rb[3 * r + 2] * Rl[6 + c];
+      mulquat(xquat + 4 * b, lq, q);
+      const float *cv = cvel + 6 * b, *c = scom + 3 * m.body_rootid[b];
+      float off[3] = {pos[0] - c[0], pos[1] - c[1], pos[2] - c[2]}, wx[3];
+      for (int k = 0; k < 3; k++) ang[k] = cv[k];
+      cross3(ang, off, wx);
+      for (int k = 0; k < 3; k++) lin[k] = cv[3 + k] + wx[k];
+    }
+    auto rt = [&](const float* v, float* o) {  // R^T v
+      for (int k = 0; k < 3; k++) o[k] = R[k] * v[0] + R[3 + k] * v[1] + R[6 + k] * v[2];
+    };
+    switch (t) {
+      case SENS_JOINTPOS: y[0] = qpos[m.sens_qadr[i]]; break;
+      case SENS_JOINTVEL: y[0] = qvel[m.sens_dadr[i]]; break;
+      case SENS_ACTUATORPOS: y[0] = m.sens_gear[i] * qpos[m.sens_qadr[i]]; break;
+      case SENS_ACTUATORVEL: y[0] = m.sens_gear[i] * qvel[m.sens_dadr[i]]; break;
+      case SENS_JOINTACTFRC: {
+        int k = m.sens_dadr[i];
+        for (int a = m.dof_act_adr[k]; a < m.dof_act_adr[k] + m.dof_act_num[k]; a++) {
+          int u = m.dof_act_list[a];
+          y[0] += m.act_gear[u] * af[u];
+        }
+        break;
+      }
+      case SENS_BALLQUAT: {
+        const float* qb = qpos + m.sens_qadr[i];
+        float n = sqrtf(qb[0] * qb[0] + qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]);
+        for (int k = 0; k < 4; k++) y[k] = qb[k] / n;
+        break;
+      }
+      case SENS_BALLANGVEL: for (int k = 0; k < 3; k++) y[k] = qvel[m.sens_dadr[i] + k]; break;
+      case SENS_SUBTREECOM: for (int k = 0; k < 3; k++) y[k] = scom[3 * b + k]; break;
+      case SENS_FRAMEPOS: for (int k = 0; k < 3; k++) y[k] = pos[k]; break;
+      case SENS_FRAMEQUAT: { float sg = q[0] < 0.f ? -1.f : 1.f; for (int k = 0; k < 4; k++) y[k] = sg * q[k]; break; }
+      case SENS_FRAMEXAXIS: case SENS_FRAMEYAXIS: case SENS_FRAMEZAXIS: {
+        int k = t - SENS_FRAMEXAXIS;
+        y[0] = R[k]; y[1] = R[3 + k]; y[2] = R[6 + k];
+        break;
+      }
+      case SENS_FRAMELINVEL: for (int k = 0; k < 3; k++) y[k] = lin[k]; break;
+      case SENS_FRAMEANGVEL: for (int k = 0; k < 3; k++) y[k] = ang[k]; break;
+      case SENS_GYRO: rt(ang, y); break;
+      case SENS_VELOCIMETER: rt(lin, y); break;
+      case SENS_MAGNETOMETER: rt(m.magnetic, y); break;
+      case SENS_ACCELEROMETER: {
+        float* pv = prev + 3 * acc;
+        float a[3];
+        for (int k = 0; k < 3; k++) a[k] = (valid ? (lin[k] - pv[k]) / m.timestep : 0.f) - m.gravity[k];
+        rt(a, y);
+        for (int k = 0; k < 3; k++) pv[k] = lin[k];
+        break;
+      }
+    }
+    if (last)
+      for (int k = 0; k < m.sens_dim[i]; k++) out[m.sens_adr[i] + k] = y[k];
+  }
+  if (LANE == 0 && m.nsens_acc > 0 && last) prev[pstride - 1] = 1.f;
+}
 
 KDEV void dump_ws(const Layout& L, const float* w, float* dst, int env) {
   if (!dst) return;
@@ -2747,6 +2837,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     st |= st_smooth(m, L, w);
     KPROF_MARK(&pa, KP_SMOOTH);
     if (last && io.has_aux) write_aux(m, L, w, io.aux, env);
+    if (m.nsensor > 0 && io.sensordata) st_sensors(m, L, w, io, env, s == 0, last);
     st |= st_collision<NPSET>(m, L, w);
     KPROF_MARK(&pa, KP_COLL);
     st |= st_constraints(m, L, w);

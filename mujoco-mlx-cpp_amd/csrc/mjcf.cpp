@@ -399,6 +399,8 @@ class Compiler {
   void add_geom(const XNode& n, int b, const std::string& cls, bool robot);
   void add_joint(const AttrMap& a, int b, bool is_free);
   void actuators(const XNode& n);
+  void add_site(const XNode& n, int b, const std::string& cls);
+  void sensors(const XNode& n);
   void contact(const XNode& n);
   void finish_inertia();
 };
@@ -407,6 +409,7 @@ void Compiler::option(const XNode& n) {
   AttrMap a; for (auto& kv : n.attrs) a[kv.first] = kv.second;
   m_.timestep = num(a, "timestep", m_.timestep);
   if (has(a, "gravity")) { auto g = nums(a, "gravity"); for (int i = 0; i < 3 && i < (int)g.size(); i++) m_.gravity[i] = g[i]; }
+  if (has(a, "magnetic")) { auto g = nums(a, "magnetic"); for (int i = 0; i < 3 && i < (int)g.size(); i++) m_.magnetic[i] = g[i]; }
   m_.iterations = (int)num(a, "iterations", m_.iterations);
   m_.ls_iterations = (int)num(a, "ls_iterations", m_.ls_iterations);
   m_.tolerance = num(a, "tolerance", m_.tolerance);
@@ -652,7 +655,7 @@ void Compiler::body(const XNode& n, int parent, const std::string& childclass, b
       }
       m_.body_mass[b] = mass;
     } else if (k->tag == "site") {
-      m_.nsite++;
+      add_site(*k, b, cls);
     } else if (k->tag == "camera" || k->tag == "light") {
       // ignored: rendering only
     } else if (k->tag == "body" || k->tag == "joint" || k->tag == "freejoint") {
@@ -703,7 +706,7 @@ void Compiler::actuators(const XNode& n) {
   for (auto& k : n.kids) {
     AttrMap a = resolve(cx_, *k, "main");
     if (!has(a, "joint")) throw std::runtime_error("MJCF: only joint transmissions are supported (actuator " + str(a, "name", "") + ")");
-    int j = m_.name2id(2, a.at("joint"));
+    int j = m_.name2id(3, a.at("joint"));
     if (j < 0) throw std::runtime_error("MJCF: actuator joint '" + a.at("joint") + "' not found");
     if (m_.jnt_type[j] == JNT_FREE || m_.jnt_type[j] == JNT_BALL) throw std::runtime_error("MJCF: actuators on free/ball joints are not supported");
     double gain[3] = {0, 0, 0}, bias[3] = {0, 0, 0};
@@ -737,6 +740,65 @@ void Compiler::actuators(const XNode& n) {
     m_.actuator_ctrlrange.push_back(cr[0]); m_.actuator_ctrlrange.push_back(cr[1]);
     m_.actuator_forcerange.push_back(fr[0]); m_.actuator_forcerange.push_back(fr[1]);
     m_.nu++;
+  }
+}
+
+void Compiler::add_site(const XNode& n, int b, const std::string& cls) {
+  AttrMap a = resolve(cx_, n, cls);
+  double pos[3] = {0, 0, 0}, q[4];
+  if (has(a, "pos")) { auto v = nums(a, "pos"); for (int i = 0; i < 3; i++) pos[i] = v.at(i); }
+  orientation(cx_, a, q);
+  if (has(a, "fromto")) throw std::runtime_error("MJCF: site fromto is not supported");
+  m_.site_bodyid.push_back(b);
+  m_.site_name.push_back(str(a, "name", ""));
+  for (int i = 0; i < 3; i++) m_.site_pos.push_back(pos[i]);
+  for (int i = 0; i < 4; i++) m_.site_quat.push_back(q[i]);
+  m_.nsite++;
+}
+
+// <sensor>: the kinds the reference computes on its batched path (reference
+// src/sensor.cpp:126-183), with MuJoCo's mjtSensor ids and sensordata widths
+void Compiler::sensors(const XNode& n) {
+  struct Kind { const char* tag; int type, dim; const char* attr; int objtype; };
+  static const Kind kinds[] = {
+      {"accelerometer", 1, 3, "site", 6}, {"velocimeter", 2, 3, "site", 6}, {"gyro", 3, 3, "site", 6},
+      {"magnetometer", 6, 3, "site", 6}, {"jointpos", 9, 1, "joint", 3}, {"jointvel", 10, 1, "joint", 3},
+      {"actuatorpos", 13, 1, "actuator", 19}, {"actuatorvel", 14, 1, "actuator", 19},
+      {"jointactuatorfrc", 16, 1, "joint", 3}, {"ballquat", 18, 4, "joint", 3}, {"ballangvel", 19, 3, "joint", 3},
+      {"framepos", 26, 3, nullptr, 0}, {"framequat", 27, 4, nullptr, 0}, {"framexaxis", 28, 3, nullptr, 0},
+      {"frameyaxis", 29, 3, nullptr, 0}, {"framezaxis", 30, 3, nullptr, 0}, {"framelinvel", 31, 3, nullptr, 0},
+      {"frameangvel", 32, 3, nullptr, 0}, {"subtreecom", 35, 3, "body", 1},
+  };
+  for (auto& k : n.kids) {
+    AttrMap a = resolve(cx_, *k, "main");
+    const Kind* kd = nullptr;
+    for (auto& c : kinds) if (k->tag == c.tag) kd = &c;
+    if (!kd) throw std::runtime_error("MJCF: unsupported sensor <" + k->tag + ">");
+    int objtype = kd->objtype, objid = -1;
+    if (kd->attr) {
+      objid = m_.name2id(objtype, str(a, kd->attr, ""));
+    } else {  // frame sensors name their object by objtype / objname
+      std::string ot = str(a, "objtype", "");
+      objtype = ot == "body" ? 1 : ot == "xbody" ? 2 : ot == "geom" ? 5 : ot == "site" ? 6 : -1;
+      if (objtype < 0) throw std::runtime_error("MJCF: frame sensor objtype must be body, xbody, geom or site");
+      if (has(a, "reftype") || has(a, "refname")) throw std::runtime_error("MJCF: frame sensors relative to a reference frame are not supported");
+      objid = m_.name2id(objtype, str(a, "objname", ""));
+    }
+    if (objid < 0) throw std::runtime_error("MJCF: sensor <" + k->tag + "> names an unknown object");
+    if (kd->objtype == 3) {
+      int jt = m_.jnt_type[objid];
+      bool ball = kd->type == 18 || kd->type == 19;
+      if (ball ? jt != JNT_BALL : (jt != JNT_HINGE && jt != JNT_SLIDE))
+        throw std::runtime_error("MJCF: sensor <" + k->tag + "> on the wrong joint type");
+    }
+    m_.sensor_type.push_back(kd->type);
+    m_.sensor_objtype.push_back(objtype);
+    m_.sensor_objid.push_back(objid);
+    m_.sensor_adr.push_back(m_.nsensordata);
+    m_.sensor_dim.push_back(kd->dim);
+    m_.sensor_name.push_back(str(a, "name", ""));
+    m_.nsensordata += kd->dim;
+    m_.nsensor++;
   }
 }
 
@@ -802,6 +864,7 @@ Model Compiler::run() {
   if (!wb) throw std::runtime_error("MJCF: missing <worldbody>");
   // world geoms first (MuJoCo numbers world geoms before child bodies' geoms)
   for (auto& k : wb->kids) if (k->tag == "geom") add_geom(*k, 0, "main", false);
+  for (auto& k : wb->kids) if (k->tag == "site") add_site(*k, 0, "main");
   for (auto& k : wb->kids) {
     if (k->tag == "body") body(*k, 0, "main", false);
     else if (k->tag == "geom" || k->tag == "site" || k->tag == "light" || k->tag == "camera") continue;
@@ -811,10 +874,12 @@ Model Compiler::run() {
   for (auto& k : root->kids) {
     if (k->tag == "actuator") actuators(*k);
     else if (k->tag == "contact") contact(*k);
+    else if (k->tag == "sensor") continue;  // after the actuators, whose names sensors use
     else if (k->tag == "compiler" || k->tag == "option" || k->tag == "default" || k->tag == "worldbody" ||
              k->tag == "size" || k->tag == "visual" || k->tag == "statistic" || k->tag == "asset") continue;
     else throw std::runtime_error("MJCF: unsupported top-level element <" + k->tag + ">");
   }
+  for (auto& k : root->kids) if (k->tag == "sensor") sensors(*k);
   // rootid / weldid
   m_.body_rootid.assign(m_.nbody, 0);
   m_.body_weldid.assign(m_.nbody, 0);
@@ -839,10 +904,12 @@ Model Compiler::run() {
 int Model::name2id(int objtype, const std::string& nm) const {
   const std::vector<std::string>* v = nullptr;
   switch (objtype) {
-    case 1: v = &body_name; break;
-    case 2: v = &jnt_name; break;
+    case 1: case 2: v = &body_name; break;
+    case 3: v = &jnt_name; break;
     case 5: v = &geom_name; break;
+    case 6: v = &site_name; break;
     case 19: v = &actuator_name; break;
+    case 20: v = &sensor_name; break;
     default: return -1;
   }
   if (nm.empty()) return -1;

@@ -32,6 +32,7 @@ struct Model {
   // <option>
   double timestep = 0.002;
   double gravity[3] = {0, 0, -9.81};
+  double magnetic[3] = {0, -0.5, 0};
   double tolerance = 1e-8, ls_tolerance = 0.01, impratio = 1.0;
   int iterations = 100, ls_iterations = 50;
   int solver = SOL_NEWTON, cone = CONE_PYRAMIDAL, integrator = INT_EULER;
@@ -70,6 +71,16 @@ struct Model {
       actuator_ctrlrange, actuator_forcerange;
   std::vector<std::string> actuator_name;
 
+  // sites (frames fixed to a body)
+  std::vector<int> site_bodyid;
+  std::vector<double> site_pos, site_quat;
+  std::vector<std::string> site_name;
+
+  // sensors: MuJoCo mjtSensor / mjtObj ids, sensordata address and width
+  int nsensor = 0, nsensordata = 0;
+  std::vector<int> sensor_type, sensor_objtype, sensor_objid, sensor_adr, sensor_dim;
+  std::vector<std::string> sensor_name;
+
   // explicit <pair> and <exclude>
   std::vector<int> pair_geom1, pair_geom2, pair_dim;
   std::vector<double> pair_friction, pair_solref, pair_solimp, pair_margin, pair_gap;
@@ -78,6 +89,7 @@ struct Model {
   std::vector<double> qpos0, qpos_spring;
   double meaninertia = 1.0;
 
+  // MuJoCo mjtObj ids: 1 body (2 xbody), 3 joint, 5 geom, 6 site, 19 actuator, 20 sensor
   int name2id(int objtype, const std::string& nm) const;
 };
 

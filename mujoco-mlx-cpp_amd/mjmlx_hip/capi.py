@@ -80,6 +80,11 @@ def lib() -> C.CDLL:
             "mjmlx_batched_num_free_joints": (C.c_int, [C.c_void_p]),
             "mjmlx_batched_raycast": (C.c_int, [C.c_void_p, _F, _F, C.c_int, C.c_int, C.POINTER(C.c_ubyte), C.c_int,
                                                  _F, _I]),
+            "mjmlx_get_mj_model": (C.c_void_p, [C.c_void_p]),
+            "mjmlx_model_nsensor": (C.c_int, [C.c_void_p]),
+            "mjmlx_model_nsensordata": (C.c_int, [C.c_void_p]),
+            "mjmlx_batched_nsensordata": (C.c_int, [C.c_void_p]),
+            "mjmlx_batched_get_sensordata": (_F, [C.c_void_p, _I]),
             "mjmlx_batched_set_pack_spec": (C.c_int, [C.c_void_p, _I, C.c_int, C.c_int, C.c_int]),
             "mjmlx_batched_get_pack": (C.c_int, [C.c_void_p, _F]),
             "mjmlx_batched_eval_state": (None, [C.c_void_p]),
@@ -227,6 +232,16 @@ class BatchedSim:
                                          int(bool(flg_static)), _fp(dist), _ip(gid))
         assert rc == 0, rc
         return dist, gid
+
+    def sensordata(self) -> np.ndarray:
+        """Sensor values [B, nsensordata] at the start of the last step's last substep."""
+        nsd = lib().mjmlx_batched_nsensordata(self.h)
+        if nsd == 0:
+            return np.zeros((self.B, 0), np.float32)
+        n = C.c_int()
+        p = lib().mjmlx_batched_get_sensordata(self.h, C.byref(n))
+        assert p and n.value == self.B * nsd
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy().reshape(self.B, nsd)
 
     def num_free_joints(self) -> int:
         return lib().mjmlx_batched_num_free_joints(self.h)

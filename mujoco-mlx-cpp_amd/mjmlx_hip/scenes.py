@@ -41,8 +41,10 @@ def _f(x) -> str:
 def quadruped_xml(self_collision: bool = True, friction: float = 0.6,
                   solver: str = "Newton", iterations: int = 16,
                   tolerance: float = 1e-8, timestep: float = 0.005,
-                  cone: str = "pyramidal", integrator: str = "Euler") -> str:
-    """Quadruped of the benchmark configurations: nq 19, nv 18, nu 12, 14 bodies, 14 geoms."""
+                  cone: str = "pyramidal", integrator: str = "Euler", sensors: bool = False) -> str:
+    """Quadruped of the benchmark configurations: nq 19, nv 18, nu 12, 14 bodies, 14 geoms.
+    sensors: an IMU site on the torso and a sensor of each kind the batched
+    path computes (joint, actuator, frame, IMU, subtree)."""
     leg_aff = 2 if self_collision else 0
     hx, hy, hz = QUAD_TORSO
     legs = [("FL", +1, +1), ("FR", +1, -1), ("RL", -1, +1), ("RR", -1, -1)]
@@ -60,6 +62,8 @@ def quadruped_xml(self_collision: bool = True, friction: float = 0.6,
     out.append(f'    <body name="torso" pos="0 0 {nominal_h:.6f}">')
     out.append('      <freejoint name="root"/>')
     out.append(f'      <geom name="torso" type="box" size="{hx} {hy} {hz}" mass="6.9" contype="1" conaffinity="0"/>')
+    if sensors:
+        out.append('      <site name="imu" pos="0.05 0.01 0.02" euler="0.1 -0.2 0.3"/>')
     for name, sx, sy in legs:
         out.append(f'      <body name="{name}_hip" pos="{sx * hx} {sy * hy} 0">')
         out.append('        <inertial pos="0 0 0" mass="0.5" diaginertia="0.0004 0.0004 0.0004"/>')
@@ -86,6 +90,28 @@ def quadruped_xml(self_collision: bool = True, friction: float = 0.6,
             out.append(f'    <position name="{name}_{part}" joint="{name}_{part}_joint" kp="20" kv="0.5" '
                        f'ctrlrange="{rng[0]} {rng[1]}"/>')
     out.append('  </actuator>')
+    if sensors:
+        out += ['  <sensor>',
+                '    <jointpos name="FL_hip_pos" joint="FL_hip_joint"/>',
+                '    <jointvel joint="FR_calf_joint"/>',
+                '    <actuatorpos actuator="RL_thigh"/>',
+                '    <actuatorvel actuator="RR_calf"/>',
+                '    <jointactuatorfrc joint="FL_thigh_joint"/>',
+                '    <framepos objtype="site" objname="imu"/>',
+                '    <framequat objtype="site" objname="imu"/>',
+                '    <framexaxis objtype="site" objname="imu"/>',
+                '    <framezaxis objtype="body" objname="torso"/>',
+                '    <framepos objtype="geom" objname="FL_foot"/>',
+                '    <framelinvel objtype="geom" objname="RR_foot"/>',
+                '    <frameangvel objtype="xbody" objname="FL_calf"/>',
+                '    <frameyaxis objtype="xbody" objname="RL_thigh"/>',
+                '    <gyro name="gyro" site="imu"/>',
+                '    <velocimeter site="imu"/>',
+                '    <accelerometer name="acc" site="imu"/>',
+                '    <magnetometer site="imu"/>',
+                '    <subtreecom body="torso"/>',
+                '    <subtreecom body="FR_thigh"/>',
+                '  </sensor>']
     out.append('</mujoco>')
     return "\n".join(out) + "\n"
 

@@ -43,6 +43,7 @@ def lib():
         L.mjemu_ray_geom.restype = C.c_float
         L.mjemu_ray_geom.argtypes = [C.c_int, _F, _F, _F]
         L.mjemu_rays.argtypes = [C.c_void_p, _F, C.c_int, _I, _F, _I]
+        L.mjemu_sensordata.argtypes = [C.c_void_p, _F]
         L.mjemu_narrow.argtypes = [C.c_int, C.c_int, _F, _F, _F, _F, _F, _F, C.c_float, _F]
         L.mjemu_compile.restype = C.c_void_p
         L.mjemu_compile.argtypes = [C.c_char_p, C.c_int, C.c_int]
@@ -126,6 +127,14 @@ class EmuSim:
         n = lib().mjemu_set_rand(self.h, _fp(self._rand))
         assert n == self._rand.shape[1], (n, self._rand.shape)
         self.total = self.layout("total")[1]
+
+    def sensordata(self) -> np.ndarray:
+        """Sensor values written by the last step, [B, nsensordata]."""
+        nsd = lib().mjemu_sensordata(self.h, None)
+        out = np.zeros((self.B, nsd), np.float32)
+        if nsd:
+            lib().mjemu_sensordata(self.h, _fp(out))
+        return out
 
     def rays(self, rays, gmask):
         """The kernel's per-env rays at the current qpos: rays [B, nray, 6]

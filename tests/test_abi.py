@@ -51,7 +51,8 @@ def test_host_only_calls_without_gpu():
     assert (m.info.nq, m.info.nv, m.info.nu, m.info.nbody, m.info.ngeom) == (19, 18, 12, 14, 14)
     assert L.mjmlx_model_body_mass(m.h, 1) == pytest.approx(6.9)
     assert L.mjmlx_name2id(m.h, 1, b"torso") == 1
-    assert L.mjmlx_name2id(m.h, 2, b"FL_calf_joint") == 3
+    assert L.mjmlx_name2id(m.h, 3, b"FL_calf_joint") == 3  # mjOBJ_JOINT
+    assert L.mjmlx_name2id(m.h, 2, b"torso") == 1  # mjOBJ_XBODY
     assert L.mjmlx_name2id(m.h, 5, b"nope") == -1
     assert L.mjmlx_model_opt_timestep(m.h) == pytest.approx(0.005)
     assert L.mjmlx_load_model_from_string(b"<mujoco><worldbody><geom type='mesh'/></worldbody></mujoco>") is None

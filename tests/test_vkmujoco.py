@@ -47,3 +47,18 @@ def test_no_cpu_fallback(tmp_path):
     if capi.lib().mjmlx_hip_device_count() == 0:
         with pytest.raises(RuntimeError, match="failed to create batched sim"):
             vkmujoco.create_batched(m, 4)
+
+
+def test_sensor_tables(tmp_path):
+    """Model.sensors() lists (name, mjtSensor type, adr, dim) as the
+    reference's binding does (python/rl_bindings.cpp:231-238)."""
+    path = tmp_path / "q.xml"
+    path.write_text(scenes.quadruped_xml(sensors=True))
+    m = vkmujoco.load_model(str(path))
+    s = m.sensors()
+    assert m.nsensordata == 48 and len(s) == 19
+    assert s[0] == ("FL_hip_pos", 9, 0, 1)        # jointpos
+    assert s[6] == ("", 27, 8, 4)                 # framequat
+    assert s[13] == ("gyro", 3, 30, 3)
+    assert s[15] == ("acc", 1, 36, 3)             # accelerometer
+    assert m.name2id(20, "gyro") == 13 and m.name2id(6, "imu") == 0
