@@ -123,7 +123,7 @@ __attribute__((visibility("default"))) int mjemu_step(void* p, const float* ctrl
   int bad = 0;
   for (int e = 0; e < s->B; e++) {
     std::fill(s->ws.begin(), s->ws.end(), 0.f);
-    mjk::run_env<2>(s->dm, s->L, s->ws.data(), io, e, nsub);
+    mjk::run_env<6>(s->dm, s->L, s->ws.data(), io, e, nsub);
     if (s->status[e]) bad++;
   }
   return bad;

@@ -232,23 +232,26 @@ void free_sim(MjmlxBatchedSim* s) {
   delete s;
 }
 
-// one workgroup (one wave) per env, with the narrow-phase set the model needs
-void launch_step(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
+// one workgroup (one wave) per env: the kernel variant for the workspace
+// placement (device memory, LDS at three or two waves per SIMD), compiled
+// for the model's narrow-phase set and, with sensors, the sensor stage
+template <int NP>
+void launch_variant(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
   dim3 g(s->B), b(64);
   size_t sh = s->use_gws ? 0 : s->lds_bytes;
-  float* gw = s->d_gws;
-  if (s->use_gws) {
-    if (s->npset == 2) hipLaunchKernelGGL((k_step<2, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
-    else if (s->npset == 1) hipLaunchKernelGGL((k_step<1, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
-    else hipLaunchKernelGGL((k_step<0, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
-  } else if (s->high_occ) {
-    if (s->npset == 2) hipLaunchKernelGGL((k_step_occ<2>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
-    else if (s->npset == 1) hipLaunchKernelGGL((k_step_occ<1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
-    else hipLaunchKernelGGL((k_step_occ<0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
-  } else {
-    if (s->npset == 2) hipLaunchKernelGGL((k_step<2, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
-    else if (s->npset == 1) hipLaunchKernelGGL((k_step<1, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
-    else hipLaunchKernelGGL((k_step<0, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, gw);
+  if (s->use_gws) hipLaunchKernelGGL((k_step<NP, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
+  else if (s->high_occ) hipLaunchKernelGGL((k_step_occ<NP>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
+  else hipLaunchKernelGGL((k_step<NP, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
+}
+
+void launch_step(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
+  switch (s->npset | (s->model.nsensor > 0 ? 4 : 0)) {
+    case 0: launch_variant<0>(s, io, nsub); break;
+    case 1: launch_variant<1>(s, io, nsub); break;
+    case 2: launch_variant<2>(s, io, nsub); break;
+    case 4: launch_variant<4>(s, io, nsub); break;
+    case 5: launch_variant<5>(s, io, nsub); break;
+    default: launch_variant<6>(s, io, nsub); break;
   }
 }
 

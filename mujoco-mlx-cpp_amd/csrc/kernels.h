@@ -2788,6 +2788,9 @@ KDEV void write_features(const DModel& m, const Layout& L, const float* w, float
 // pairs and box against plane and sphere; 1: also capsule-box and box-box;
 // 2: also ellipsoids and cylinders through GJK / EPA),
 // so models without the heavier pairs do not pay their registers.
+// NPSET: bits 0-1 the narrow-phase set (0 basic pairs, 1 + capsule-box /
+// box-box, 2 + GJK/EPA), bit 2 the sensor stage (models with sensors only, so
+// the plain step carries none of its code)
 template <int NPSET>
 KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, int env, int nsub) {
   int nq = m.nq, nv = m.nv, nu = m.nu;
@@ -2837,8 +2840,9 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     st |= st_smooth(m, L, w);
     KPROF_MARK(&pa, KP_SMOOTH);
     if (last && io.has_aux) write_aux(m, L, w, io.aux, env);
-    if (m.nsensor > 0 && io.sensordata) st_sensors(m, L, w, io, env, s == 0, last);
-    st |= st_collision<NPSET>(m, L, w);
+    if constexpr ((NPSET & 4) != 0)
+      if (io.sensordata) st_sensors(m, L, w, io, env, s == 0, last);
+    st |= st_collision<(NPSET & 3)>(m, L, w);
     KPROF_MARK(&pa, KP_COLL);
     st |= st_constraints(m, L, w);
     KPROF_MARK(&pa, KP_CONSTR);
