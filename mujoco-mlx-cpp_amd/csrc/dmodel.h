@@ -94,10 +94,10 @@ struct Layout {
   // at least every contact the row budget can admit)
   int maxcon, maxefc, efc_cap, con_cap, nv, total;
   int rand;  // per-env physical parameters (rand_size floats), or -1 without domain randomisation
-  int qpos, qvel, ctrl, qacc_ws, qacc_pred, qacc, qM, qLD, qfrc_smooth, qacc_smooth, qfrc_constraint, act_force, misc;
+  int qpos, qvel, ctrl, qacc_ws, qacc, qM, qLD, qfrc_smooth, qacc_smooth, qfrc_constraint, act_force, misc;
   int xpos, xquat, xmat, xipos, xanchor, xaxis, gxpos, gxmat, scom, cinert, cdof, cvel, cdofdot, crb, cacc, cfrc, buf6;
   int H;
-  int con, efc_J, efc_D, efc_R, efc_aref, efc_Jaref, efc_Jv, efc_force, efc_floss, efc_pos, efc_margin,
+  int con, efc_J, efc_D, efc_aref, efc_Jaref, efc_Jv, efc_force, efc_pos, efc_margin,
       efc_type, efc_act, efc_id, efc_Dact;
   int Ma, grad, Mgrad, search, Mv, grad_old, Mgrad_old, tmpv, dvec;
 };
@@ -140,7 +140,6 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.qvel = o; o += al2(nvp);
   L.ctrl = o; o += al2(nup);
   L.qacc_ws = o; o += al2(nvp);
-  L.qacc_pred = o; o += al2(nvp);
   L.qacc = o; o += al2(nvp);
   L.qM = o; o += al2(nvp * nvp);
   if (!qld_shared) { L.qLD = o; o += al2(nvp * nvp); }
@@ -187,12 +186,10 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.con = o; o += CON_STRIDE * ncon;
   L.efc_J = o; o += maxefc * nvp;
   L.efc_D = o; o += maxefc;
-  L.efc_R = o; o += maxefc;
   L.efc_aref = o; o += maxefc;
   L.efc_Jaref = o; o += maxefc;
   L.efc_Jv = o; o += maxefc;
   L.efc_force = o; o += maxefc;
-  L.efc_floss = o; o += maxefc;
   L.efc_type = o; o += maxefc;
   L.efc_act = o; o += maxefc;
   L.efc_id = o; o += maxefc;

@@ -325,7 +325,7 @@ std::vector<LayoutField> layout_fields(const Model& m, const mjk::Layout& L) {
   int nb = m.nbody, nv = m.nv, nj = m.njnt, ng = m.ngeom;
   std::vector<LayoutField> f = {
       {"qpos", L.qpos, m.nq, 0}, {"qvel", L.qvel, nv, 0}, {"ctrl", L.ctrl, m.nu, 0},
-      {"qacc_warmstart", L.qacc_ws, nv, 0}, {"qacc_pred", L.qacc_pred, nv, 0}, {"qacc", L.qacc, nv, 0}, {"qM", L.qM, nv * nv, 0},
+      {"qacc_warmstart", L.qacc_ws, nv, 0}, {"qacc", L.qacc, nv, 0}, {"qM", L.qM, nv * nv, 0},
       {"qLD", L.qLD, nv * nv, 0}, {"qfrc_smooth", L.qfrc_smooth, nv, 0}, {"qacc_smooth", L.qacc_smooth, nv, 0},
       {"qfrc_constraint", L.qfrc_constraint, nv, 0}, {"actuator_force", L.act_force, m.nu, 0},
       {"misc", L.misc, mjk::MISC_SIZE, 1},
@@ -335,10 +335,10 @@ std::vector<LayoutField> layout_fields(const Model& m, const mjk::Layout& L) {
       {"cinert", L.cinert, 10 * nb, 0}, {"cdof", L.cdof, 6 * nv, 0}, {"cvel", L.cvel, 6 * nb, 0},
       {"cdof_dot", L.cdofdot, 6 * nv, 0}, {"crb", L.crb, 10 * nb, 0},
       {"contact", L.con, mjk::CON_STRIDE * L.maxcon, 0},
-      {"efc_J", L.efc_J, L.maxefc * nv, 0}, {"efc_D", L.efc_D, L.maxefc, 0}, {"efc_R", L.efc_R, L.maxefc, 0},
+      {"efc_J", L.efc_J, L.maxefc * nv, 0}, {"efc_D", L.efc_D, L.maxefc, 0},
       {"efc_aref", L.efc_aref, L.maxefc, 0}, {"efc_force", L.efc_force, L.maxefc, 0},
       {"efc_pos", L.efc_pos, L.maxefc, 0}, {"efc_margin", L.efc_margin, L.maxefc, 0},
-      {"efc_type", L.efc_type, L.maxefc, 1}, {"efc_id", L.efc_id, L.maxefc, 1}, {"efc_floss", L.efc_floss, L.maxefc, 0},
+      {"efc_type", L.efc_type, L.maxefc, 1}, {"efc_id", L.efc_id, L.maxefc, 1},
   };
   if (L.rand >= 0) {  // domain randomisation: the env's parameter block
     int nfree = 0;

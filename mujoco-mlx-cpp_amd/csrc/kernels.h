@@ -1501,7 +1501,7 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
   int* misc = iptr(w, L.misc);
   int nv = m.nv;
   int st = 0;
-  float *J = w + L.efc_J, *D = w + L.efc_D, *Rr = w + L.efc_R, *aref = w + L.efc_aref, *floss = w + L.efc_floss;
+  float *J = w + L.efc_J, *D = w + L.efc_D, *aref = w + L.efc_aref;
   float *epos = w + L.efc_pos, *emarg = w + L.efc_margin;
   int *etype = iptr(w, L.efc_type), *eid = iptr(w, L.efc_id);
   const float *qpos = w + L.qpos, *qvel = w + L.qvel;
@@ -1522,7 +1522,7 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
   if (nf > lim_cap) { nf = lim_cap; st |= ST_EFC_OVERFLOW; }
   for (int t = LANE; t < nf; t += WAVE) {
     int k = m.floss_dof[t];
-    etype[t] = EFC_FRICTION_DOF; eid[t] = k; epos[t] = 0.f; emarg[t] = 0.f; floss[t] = m.dof_frictionloss[k];
+    etype[t] = EFC_FRICTION_DOF; eid[t] = k; epos[t] = 0.f; emarg[t] = 0.f;
   }
   nrow = nf;
   // joint limits: 0..2 rows per limited joint
@@ -1543,9 +1543,9 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
       }
       int off = wscan_excl(cnt), tot = wsumi(cnt);
       int r = nrow + off;
-      if (s0 && r < lim_cap) { etype[r] = EFC_LIMIT_JOINT; eid[r] = 2 * j; epos[r] = d0; emarg[r] = m.jnt_margin[j]; floss[r] = 0.f; }
+      if (s0 && r < lim_cap) { etype[r] = EFC_LIMIT_JOINT; eid[r] = 2 * j; epos[r] = d0; emarg[r] = m.jnt_margin[j]; }
       if (s0) r++;
-      if (s1 && r < lim_cap) { etype[r] = EFC_LIMIT_JOINT; eid[r] = 2 * j + 1; epos[r] = d1; emarg[r] = m.jnt_margin[j]; floss[r] = 0.f; }
+      if (s1 && r < lim_cap) { etype[r] = EFC_LIMIT_JOINT; eid[r] = 2 * j + 1; epos[r] = d1; emarg[r] = m.jnt_margin[j]; }
       nrow += tot;
     }
     if (nrow > lim_cap) { nrow = lim_cap; st |= ST_EFC_OVERFLOW; }
@@ -1578,7 +1578,7 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
           etype[r0 + k] = dim == 1 ? EFC_CONTACT_FRICTIONLESS : (m.cone == 0 ? EFC_CONTACT_PYRAMIDAL : EFC_CONTACT_ELLIPTIC);
           bool fric = m.cone != 0 && dim > 1 && k > 0;  // elliptic friction rows carry no position term
           eid[r0 + k] = c; epos[r0 + k] = fric ? 0.f : cr[CON_DIST];
-          emarg[r0 + k] = fric ? 0.f : m.pair_margin[p] - m.pair_gap[p]; floss[r0 + k] = 0.f;
+          emarg[r0 + k] = fric ? 0.f : m.pair_margin[p] - m.pair_gap[p];
         }
       }
       nrow += tot;
@@ -1695,8 +1695,7 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
         R = fmaxf(kr == 1 ? R1 : R1 * f5[0] * f5[0] / (f5[kr - 1] * f5[kr - 1]), KMINVAL);
       }
     }
-    Rr[r] = R;
-    D[r] = 1.f / R;
+    D[r] = 1.f / R;  // friction rows take R back as 1 / D
   }
   if (LANE == 0) misc[MISC_NEFC] = nrow;
   KSYNC();
@@ -1862,9 +1861,10 @@ KDEV int k_ellip_line(const float* x0, const float* v, const float* D, const flo
 
 KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* nchanged) {
   int nv = m.nv;
-  const float *Jaref = w + L.efc_Jaref, *D = w + L.efc_D, *Rr = w + L.efc_R, *floss = w + L.efc_floss;
+  const float *Jaref = w + L.efc_Jaref, *D = w + L.efc_D;
   float *force = w + L.efc_force, *Dact = w + L.efc_Dact;
   int *act = iptr(w, L.efc_act), *etype = iptr(w, L.efc_type);
+  const int* eid = iptr(w, L.efc_id);
   float part = 0.f;
   int nch = 0;
   for (int r = LANE; r < nefc; r += WAVE) {
@@ -1874,7 +1874,7 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
     int old = act[r];
     if (ty == EFC_EQUALITY) { act[r] = 1; force[r] = -d * x; part += 0.5f * d * x * x; }
     else if (ty == EFC_FRICTION_DOF) {
-      float f = floss[r], R = Rr[r];
+      float f = m.dof_frictionloss[eid[r]], R = 1.f / d;
       if (x <= -R * f) { act[r] = 0; force[r] = f; part += -f * x - 0.5f * R * f * f; }
       else if (x >= R * f) { act[r] = 0; force[r] = -f; part += f * x - 0.5f * R * f * f; }
       else { act[r] = 1; force[r] = -d * x; part += 0.5f * d * x * x; }
@@ -2155,12 +2155,12 @@ KDEV void ls_rows(const Layout& L, const float* w, int nefc, LsRows* c) {
 }
 
 // cost terms of one row at alpha (x = Jaref + alpha Jv); zone change against ref_alpha
-KDEV void ls_row(const Layout& L, const float* w, int r, int ty, float jar, float jv, float d, float alpha,
+KDEV void ls_row(const DModel& m, const Layout& L, const float* w, int r, int ty, float jar, float jv, float d, float alpha,
                  float ref_alpha, float& f, float& d1, float& d2, int& nch) {
   if (ty == EFC_CONTACT_ELLIPTIC) return;  // evaluated per contact
   float x = jar + alpha * jv, xr = jar + ref_alpha * jv;
   if (ty == EFC_FRICTION_DOF) {
-    float fl = w[L.efc_floss + r], R = w[L.efc_R + r];
+    float fl = m.dof_frictionloss[iptr(w, L.efc_id)[r]], R = 1.f / d;
     int z = x <= -R * fl ? 0 : (x >= R * fl ? 2 : 1);
     int zr = xr <= -R * fl ? 0 : (xr >= R * fl ? 2 : 1);
     nch += z != zr;
@@ -2203,7 +2203,7 @@ KDEV LsPt ls_eval(const DModel& m, const Layout& L, float* w, int nefc, float al
     for (int q = 0; q < 2; q++)
       if (c.ty[q] >= 0) {
         float t = d1;
-        ls_row(L, w, LANE + q * WAVE, c.ty[q], c.jar[q], c.jv[q], c.d[q], alpha, ref_alpha, f, d1, d2, nch);
+        ls_row(m, L, w, LANE + q * WAVE, c.ty[q], c.jar[q], c.jv[q], c.d[q], alpha, ref_alpha, f, d1, d2, nch);
         d1a += fabsf(d1 - t);
       }
   } else {
@@ -2211,7 +2211,7 @@ KDEV LsPt ls_eval(const DModel& m, const Layout& L, float* w, int nefc, float al
     const int* etype = iptr(w, L.efc_type);
     for (int r = LANE; r < nefc; r += WAVE) {
       float t = d1;
-      ls_row(L, w, r, etype[r], Jaref[r], Jv[r], D[r], alpha, ref_alpha, f, d1, d2, nch);
+      ls_row(m, L, w, r, etype[r], Jaref[r], Jv[r], D[r], alpha, ref_alpha, f, d1, d2, nch);
       d1a += fabsf(d1 - t);
     }
   }
@@ -2288,7 +2288,9 @@ KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int*
 
 // mj_fwdConstraint: warm start choice, Newton or CG iterations with MuJoCo's
 // termination test, qfrc_constraint = J^T force.
-KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KProfAcc* pa, bool use_pred) {
+// pred: the learned warm-start candidate (the env's row of the prediction in
+// device memory), or null
+KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KProfAcc* pa, const float* pred) {
   int* misc = iptr(w, L.misc);
   int nv = m.nv;
   int nefc = misc[MISC_NEFC];
@@ -2310,13 +2312,13 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     cost = cost_grad(m, L, w, nefc, &nch);
     const float* best = cost > cs ? w + L.qacc_smooth : w + L.qacc_ws;
     float cbest = cost > cs ? cs : cost;
-    if (use_pred) {
+    if (pred) {
       // learned candidate: kept only when its cost is strictly the lowest
-      set_qacc(m, L, w, w + L.qacc_pred, nefc);
+      set_qacc(m, L, w, pred, nefc);
       float cp = cost_grad(m, L, w, nefc, &nch);
-      if (cp < cbest) { best = w + L.qacc_pred; cbest = cp; }
+      if (cp < cbest) { best = pred; cbest = cp; }
       cost = cp;
-      if (best != w + L.qacc_pred) {
+      if (best != pred) {
         set_qacc(m, L, w, best, nefc);
         cost = cost_grad(m, L, w, nefc, &nch);
       }
@@ -2798,7 +2800,6 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
   for (int i = LANE; i < nv; i += WAVE) {
     w[L.qvel + i] = io.qvel[(size_t)env * nv + i];
     w[L.qacc_ws + i] = io.qacc_ws[(size_t)env * nv + i];
-    if (io.qacc_pred) w[L.qacc_pred + i] = io.qacc_pred[(size_t)env * nv + i];
   }
   for (int i = LANE; i < nu; i += WAVE) w[L.ctrl + i] = io.ctrl ? io.ctrl[(size_t)env * nu + i] : 0.f;
   if (L.rand >= 0)
@@ -2848,7 +2849,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     KPROF_MARK(&pa, KP_CONSTR);
     if (last) dump_ws(L, w, io.dump1, env);
     int it = 0;
-    st |= st_solve(m, L, w, &it, &pa, io.qacc_pred != nullptr && s == 0);
+    st |= st_solve(m, L, w, &it, &pa, io.qacc_pred && s == 0 ? io.qacc_pred + (size_t)env * nv : nullptr);
     niter += it;
     if (last && io.has_aux) write_cfrc_ext(m, L, w, io.aux, env, true);
     if (last) dump_ws(L, w, io.dump2, env);
