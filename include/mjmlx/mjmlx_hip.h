@@ -29,8 +29,12 @@ MJMLX_API int mjmlx_model_field(const MjmlxModel* model, const char* name, doubl
  *   1  qacc_smooth (warm start disabled)
  *   2  learned: the MLP installed with mjmlx_batched_set_warmstart_mlp
  *      predicts qacc; the solver starts from the cheapest of {prediction,
- *      previous qacc, qacc_smooth} and iterates to the same tolerance.
- * Returns 0, or -1 (bad handle / mode 2 without weights). */
+ *      previous qacc, qacc_smooth} and iterates to the same tolerance; one
+ *      network pass and one launch per physics substep.
+ *   3  learned, first substep only: one network pass per step call (on the
+ *      features of the state the call starts from) and one fused launch of
+ *      all substeps; the other substeps warm-start from the previous qacc.
+ * Returns 0, or -1 (bad handle / modes 2 and 3 without weights). */
 MJMLX_API int mjmlx_batched_set_warmstart_mode(MjmlxBatchedSim* sim, int mode);
 MJMLX_API int mjmlx_batched_get_qacc_warmstart(const MjmlxBatchedSim* sim, float* out);
 MJMLX_API int mjmlx_batched_set_qacc_warmstart(MjmlxBatchedSim* sim, const float* in);

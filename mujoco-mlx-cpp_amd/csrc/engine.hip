@@ -390,6 +390,29 @@ long step_substeps(MjmlxBatchedSim* s, int n, MjmlxMLP* rec) {
   return rows;
 }
 
+// Warm-start mode 3: one network pass per call on the features the previous
+// call left (the state this call starts from) and one fused launch of all
+// substeps, whose first substep takes the prediction as a candidate.
+void step_first_learned(MjmlxBatchedSim* s, int n) {
+  ensure_learned(s);
+  mjk::StepIO io = make_io(s, nullptr, nullptr);
+  HIPOK(hipEventRecord(s->ev0, s->stream));
+  if (s->mlp && s->feat_valid) {
+    const float* base = s->feat_mode ? s->d_feat[s->feat_cur] + s->dm.nfeat : nullptr;
+    mjmlp::pack_features(s->mlp, s->d_feat[s->feat_cur], s->d_ctrl, s->nq + s->nv, s->nu, s->B, s->stream);
+    mjmlp::predict(s->mlp, s->B, s->d_pred, s->stream, base, s->nfeat());
+    io.qacc_pred = s->d_pred;
+  }
+  io.feat = s->d_feat[s->feat_cur ^ 1];
+  io.feat_prev_qacc = s->feat_mode;
+  launch_step(s, io, n);
+  HIPOK(hipGetLastError());
+  HIPOK(hipEventRecord(s->ev1, s->stream));
+  s->timed = true;
+  s->feat_cur ^= 1;
+  s->feat_valid = true;
+}
+
 void ensure_aux(MjmlxBatchedSim* s) {
   if (s->aux_on) return;
   for (int i = 0; i < 7; i++) {
@@ -699,6 +722,7 @@ MJMLX_API void mjmlx_batched_step_n(MjmlxBatchedSim* sim, const float* ctrl, int
   try {
     upload_ctrl(sim, ctrl);
     if (sim->ws_mode == 2) step_substeps(sim, n, nullptr);
+    else if (sim->ws_mode == 3) step_first_learned(sim, n);
     else launch(sim, n, nullptr, nullptr);
   } catch (const std::exception& e) { report("mjmlx_batched_step_n", e); }
 }
@@ -979,9 +1003,9 @@ MJMLX_API int mjmlx_camera_gpu_export_fd(MjmlxCameraRenderer*, int, uint64_t*) {
 
 // ---- additions (mjmlx_hip.h)
 MJMLX_API int mjmlx_batched_set_warmstart_mode(MjmlxBatchedSim* s, int mode) {
-  if (!s || mode < 0 || mode > 2) return -1;
-  if (mode == 2 && !s->mlp) {
-    std::fprintf(stderr, "mjmlx_batched_set_warmstart_mode error: mode 2 needs mjmlx_batched_set_warmstart_mlp first\n");
+  if (!s || mode < 0 || mode > 3) return -1;
+  if (mode >= 2 && !s->mlp) {
+    std::fprintf(stderr, "mjmlx_batched_set_warmstart_mode error: modes 2 and 3 need mjmlx_batched_set_warmstart_mlp first\n");
     return -1;
   }
   s->ws_mode = mode;
@@ -1162,7 +1186,7 @@ MJMLX_API int mjmlx_batched_set_feature_mode(MjmlxBatchedSim* s, int mode) {
     s->feat_valid = false;
     if (s->mlp) {
       s->mlp = nullptr;  // its input width no longer matches
-      if (s->ws_mode == 2) s->ws_mode = 0;
+      if (s->ws_mode >= 2) s->ws_mode = 0;
     }
   }
   return 0;
@@ -1172,7 +1196,7 @@ MJMLX_API int mjmlx_batched_set_warmstart_mlp(MjmlxBatchedSim* s, MjmlxMLP* mlp)
   if (!s) return -1;
   if (!mlp) {
     s->mlp = nullptr;
-    if (s->ws_mode == 2) s->ws_mode = 0;
+    if (s->ws_mode >= 2) s->ws_mode = 0;
     return 0;
   }
   int nin = 0, nout = 0;

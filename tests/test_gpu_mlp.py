@@ -165,6 +165,33 @@ def test_learned_warm_start_keeps_results_and_reports_iterations():
             it_l += sl["niter"].mean()
     print(f"mean solver iterations per env-step: plain {it_p / 3:.3f}, learned {it_l / 3:.3f}")
     assert it_l <= 1.1 * it_p
+    # mode 3: the prediction only at each call's first substep, one fused launch
+    first = capi.BatchedSim(model, B)
+    first.set_warmstart_mlp(net)
+    assert first.set_warmstart_mode(3) == 0
+    ref = capi.BatchedSim(model, B)
+    for s in (first, ref):
+        s.set_state(q, qv)
+        s.set_qacc_warmstart(qa)
+    rng = np.random.default_rng(8)
+    it_r = it_f = 0
+    for k in range(4):
+        c = scenes.quadruped_actions(B, rng).astype(np.float32)
+        first.step(c, 4)
+        ref.step(c, 4)
+        sf, sr = first.stats(), ref.stats()
+        assert not (sf["status"] & ~3).any()
+        if k == 0:  # no features yet: the first call is the plain step, bit for bit
+            for a, b in zip(first.state(), ref.state()):
+                assert np.array_equal(a, b)
+        else:
+            it_f += sf["niter"].mean()
+            it_r += sr["niter"].mean()
+        if k == 1:
+            (qr, vr), (qf, vf) = ref.state(), first.state()
+            assert np.max(np.abs(vr - vf)) / max(1.0, np.max(np.abs(vr))) < 1e-3 and np.max(np.abs(qr - qf)) < 1e-4
+    print(f"first-substep mode: iterations per env-step plain {it_r / 3:.3f}, learned {it_f / 3:.3f}")
+    assert it_f <= 1.1 * it_r
 
 
 def test_graph_replay_matches_eager():
