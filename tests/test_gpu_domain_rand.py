@@ -86,3 +86,43 @@ def test_identity_parameters_and_error_codes():
         del sim
     for a, b in zip(out[0], out[1]):
         assert np.max(np.abs(a - b)) < 1e-3
+
+
+def test_randomised_boxes_device_memory_workspace(monkeypatch):
+    """Randomised masses and friction with the workspace in device memory
+    (box scene, Newton) against the oracle with the same parameters."""
+    monkeypatch.setenv("MJMLX_HIP_GLOBAL_WORKSPACE", "1")
+    model = capi.Model(scenes.boxes_xml(6, 3, solver="Newton", iterations=50))
+    B = 24
+    orc = ref64.Oracle(ref64.fields_from(model.field)).set_caps()
+    q = np.tile(model.field("qpos0"), (B, 1))
+    qv = np.zeros((B, model.nv))
+    qa = np.zeros((B, model.nv))
+    rng = np.random.default_rng(5)
+    lens = rng.integers(20, 80, size=B)
+    for k in range(int(lens.max())):
+        idx = np.nonzero(lens > k)[0]
+        qs_, vs_, as_ = q[idx].copy(), qv[idx].copy(), qa[idx].copy()
+        orc.batch_step(qs_, vs_, as_, np.zeros((len(idx), 0)), 4)
+        q[idx], qv[idx], qa[idx] = qs_, vs_, as_
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    q, qv, qa = f32(q), f32(qv), f32(qa)
+    sim = capi.BatchedSim(model, B, domain_rand=True)
+    nb = orc.nbody
+    ms = rng.uniform(0.6, 1.4, (B, nb)).astype(np.float32)
+    fs = rng.uniform(0.4, 1.6, B).astype(np.float32)
+    assert sim.set_env_params(None, ms, None, fs, None) == 0
+    assert sim.num_free_joints() == 6
+    sim.set_state(q, qv)
+    sim.set_qacc_warmstart(qa)
+    dump = sim.debug_step(None)
+    st = sim.stats()
+    assert not (st["status"] & ~3).any()
+    worst = 0.0
+    for e in range(B):
+        orc.set_env_params(mass_scale=ms[e], friction_scale=[fs[e]])
+        states.oracle_step(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32))
+        assert st["nefc"][e] == orc.nefc()
+        worst = max(worst, _rel(dump.get(2, e, "qacc"), orc.get("qacc")))
+    print(f"randomised boxes (device memory): worst qacc rel {worst:.2e}")
+    assert worst < 1e-3

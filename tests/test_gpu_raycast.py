@@ -75,3 +75,33 @@ def test_raycast_filters():
     # rays pointing up from below the floor: planes are one-sided
     d4, g4 = sim.raycast(pnt * [1, 1, 0] - [0, 0, 1], -vec)
     assert (g4 != 0).all() or (gbody[g4[g4 >= 0]] != 0).all()
+
+
+def test_raycast_device_memory_workspace_boxes(monkeypatch):
+    """Rays through the device-memory workspace variant on a scene of boxes:
+    straight down onto each box's top face, against the oracle."""
+    monkeypatch.setenv("MJMLX_HIP_GLOBAL_WORKSPACE", "1")
+    model = capi.Model(scenes.boxes_xml(6, 3, solver="CG", iterations=16))
+    B, nbox = 32, 6
+    sim = capi.BatchedSim(model, B)
+    assert sim.capacity()[2] > 0
+    q = np.tile(model.field("qpos0"), (B, 1)).astype(np.float32)
+    sim.set_state(q, np.zeros((B, model.nv), np.float32))
+    for _ in range(10):
+        sim.step(None, 4)
+    qs, _ = sim.state()
+    pnt = np.zeros((B, nbox, 3), np.float32)
+    for k in range(nbox):
+        pnt[:, k, :] = qs[:, 7 * k:7 * k + 3] + [0.003, -0.002, 2.0]
+    vec = np.zeros((B, nbox, 3), np.float32)
+    vec[:, :, 2] = -1
+    dist, gid = sim.raycast(pnt, vec)
+    orc = ref64.Oracle(ref64.fields_from(model.field))
+    ok = np.ones(int(model.field("ngeom")[0]), np.int32)
+    for e in range(B):
+        orc.set_state(qs[e].astype(np.float64))
+        orc.kinematics()
+        for r in range(nbox):
+            x, g = orc.ray(pnt[e, r], vec[e, r], ok)
+            assert g == gid[e, r] and abs(x - dist[e, r]) <= 2e-5 * max(1.0, x), (e, r, x, g, dist[e, r], gid[e, r])
+    assert (gid > 0).mean() > 0.9  # mostly boxes, not the floor
