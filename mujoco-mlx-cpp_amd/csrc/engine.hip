@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
   extern __shared__ __align__(16) float lds[];
   int env = blockIdx.x;
   if (env >= B) return;
-  mjk::run_env<NPSET>(m, L, lds, io, env, nsub);
+  mjk::run_env<NPSET | 8>(m, L, lds, io, env, nsub);
 }
 
 __global__ void __launch_bounds__(64) k_step_queue(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub,

@@ -1922,6 +1922,10 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
 // H = M + J^T diag(D act) J (FP32 accumulate), factored in FP64 as
 // H = L D L^T (right-looking, one LDS phase per column, the column left
 // unscaled); the reciprocal pivots go to dvec.
+// TRI_CACHE: keep each lane's triangle coordinates in registers across the
+// elimination steps (off in the register-capped kernel variant, where the
+// extra live registers cost more spills than they save)
+template <bool TRI_CACHE>
 KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KProfAcc* pa) {
   int nv = m.nv;
   double* H = dptr(w, L.H);
@@ -2007,6 +2011,22 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
   KSYNC();
   KPROF_MARK(pa, KP_SPARE);  // profiling builds: the Hessian build is reported apart
   int bad = 0;
+  // A lane's entries of a trailing lower triangle, enumerated row-major, have
+  // the same (row, col) at every elimination step: for up to TRI_R rounds of
+  // lanes they are computed once here (packed row | col << 16) and held in
+  // registers by the unrolled loops below, instead of tri_rc per entry and step.
+  constexpr int TRI_R = TRI_CACHE && WAVE == 64 ? 6 : 1;
+  int tri_ab[TRI_R];
+  bool tri_cached = false;
+  if constexpr (TRI_CACHE && WAVE == 64) {
+    tri_cached = ntri <= TRI_R * WAVE;
+#pragma unroll
+    for (int r = 0; r < TRI_R; r++) {
+      int t = LANE + r * WAVE, a = 0, b = 0;
+      if (tri_cached && t < ntri) tri_rc(t, &a, &b);
+      tri_ab[r] = a | (b << 16);
+    }
+  }
   // Two columns per LDS phase. For the pair (k, k+1), with c = H[k+1][k] / d0
   // and d1 = H[k+1][k+1] - c H[k+1][k], the trailing entries take both rank-1
   // updates at once, computing column k+1's updated entries on the fly:
@@ -2022,6 +2042,20 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     double i1 = 1.0 / d1;
     if (LANE == 0) idg[k] = i0;  // the deferred pass below takes c from it
     int mm = nv - k - 2, nt = mm * (mm + 1) / 2;
+    if constexpr (TRI_CACHE && WAVE == 64) {
+      if (tri_cached) {
+#pragma unroll
+        for (int r = 0; r < TRI_R; r++) {
+          if (LANE + r * WAVE >= nt) continue;
+          int i = k + 2 + (tri_ab[r] & 0xffff), j = k + 2 + (tri_ab[r] >> 16);
+          double ai = H[i * nv + k], aj = H[j * nv + k];
+          double bi = H[i * nv + k + 1] - c * ai, bj = H[j * nv + k + 1] - c * aj;
+          H[i * nv + j] -= ai * aj * i0 + bi * bj * i1;
+        }
+        KSYNC();
+        continue;
+      }
+    }
     for (int t = LANE; t < nt; t += WAVE) {
       int a, b;
       tri_rc(t, &a, &b);
@@ -2037,7 +2071,19 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     if (!(d0 > 0.0)) { bad = 1; d0 = 1e-30; }
     double ik = 1.0 / d0;
     int nt = (nv - k - 1) * (nv - k) / 2;
-    for (int t = LANE; t < nt; t += WAVE) {
+    bool done = false;
+    if constexpr (TRI_CACHE && WAVE == 64) {
+      if (tri_cached) {
+#pragma unroll
+        for (int r = 0; r < TRI_R; r++) {
+          if (LANE + r * WAVE >= nt) continue;
+          int i = k + 1 + (tri_ab[r] & 0xffff), j = k + 1 + (tri_ab[r] >> 16);
+          H[i * nv + j] -= H[i * nv + k] * ik * H[j * nv + k];
+        }
+        done = true;
+      }
+    }
+    for (int t = LANE; !done && t < nt; t += WAVE) {
       int a, b;
       tri_rc(t, &a, &b);
       int i = k + 1 + a, j = k + 1 + b;
@@ -2290,6 +2336,7 @@ KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int*
 // termination test, qfrc_constraint = J^T force.
 // pred: the learned warm-start candidate (the env's row of the prediction in
 // device memory), or null
+template <bool TRI_CACHE>
 KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KProfAcc* pa, const float* pred) {
   int* misc = iptr(w, L.misc);
   int nv = m.nv;
@@ -2334,7 +2381,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
   const float* Jv = w + L.efc_Jv;
   KPROF_MARK(pa, KP_SOLVE_INIT);
   if (newton) {
-    st |= newton_factor(m, L, w, nefc, pa);
+    st |= newton_factor<TRI_CACHE>(m, L, w, nefc, pa);
     newton_solve(m, L, w);
   } else {
     for (int i = LANE; i < nv; i += WAVE) Mg[i] = grad[i];
@@ -2372,7 +2419,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     if (newton && exact) break;
     if (newton) {
       // H depends on qacc only through the active set: refactor only when it changed
-      if (nch) st |= newton_factor(m, L, w, nefc, pa);
+      if (nch) st |= newton_factor<TRI_CACHE>(m, L, w, nefc, pa);
       newton_solve(m, L, w);
     } else {
       for (int i = LANE; i < nv; i += WAVE) Mg[i] = grad[i];
@@ -2792,7 +2839,8 @@ KDEV void write_features(const DModel& m, const Layout& L, const float* w, float
 // so models without the heavier pairs do not pay their registers.
 // NPSET: bits 0-1 the narrow-phase set (0 basic pairs, 1 + capsule-box /
 // box-box, 2 + GJK/EPA), bit 2 the sensor stage (models with sensors only, so
-// the plain step carries none of its code)
+// the plain step carries none of its code), bit 3 set by the register-capped
+// kernel variant
 template <int NPSET>
 KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, int env, int nsub) {
   int nq = m.nq, nv = m.nv, nu = m.nu;
@@ -2849,7 +2897,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     KPROF_MARK(&pa, KP_CONSTR);
     if (last) dump_ws(L, w, io.dump1, env);
     int it = 0;
-    st |= st_solve(m, L, w, &it, &pa, io.qacc_pred && s == 0 ? io.qacc_pred + (size_t)env * nv : nullptr);
+    st |= st_solve<(NPSET & 8) == 0>(m, L, w, &it, &pa, io.qacc_pred && s == 0 ? io.qacc_pred + (size_t)env * nv : nullptr);
     niter += it;
     if (last && io.has_aux) write_cfrc_ext(m, L, w, io.aux, env, true);
     if (last) dump_ws(L, w, io.dump2, env);
