@@ -124,13 +124,14 @@ def run_boxes(args):
         sim.set_state(q.astype(np.float32), np.zeros((nb, model.nv), np.float32))
         groups.append((n, nb, model, sim))
     # the boxes land within the warm-up; the autotuner then picks each group's
-    # workspace placement (LDS or device memory) from its landed state
+    # kernel variant (workspace in LDS or device memory, register budget)
+    # from its landed state
     for k in range(args.warmup):
         for _, _, _, sim in groups:
             sim.step(None, 4)
     placement = {}
     for n, _, _, sim in groups:
-        placement[n] = "device memory" if sim.autotune(2) else "LDS"
+        placement[n] = ("LDS", "device memory", "LDS, 168 VGPRs")[sim.autotune(2)]
     for k in range(2):
         for _, _, _, sim in groups:
             sim.step(None, 4)

@@ -289,8 +289,9 @@ class BatchedSim:
         assert lib().mjmlx_batched_set_graph(self.h, int(on)) == 0
 
     def autotune(self, steps: int = 3) -> int:
-        """Time both workspace placements from the current state and keep the
-        faster; returns 0 (LDS) or 1 (device memory). The state is restored."""
+        """Time the step's kernel variants from the current state and keep the
+        fastest; returns 0 (LDS workspace, 256 VGPRs), 1 (device memory) or
+        2 (LDS workspace, 168 VGPRs). The state is restored."""
         c = C.c_int()
         assert lib().mjmlx_batched_autotune(self.h, steps, C.byref(c)) == 0
         return c.value

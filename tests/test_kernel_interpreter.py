@@ -91,3 +91,28 @@ def test_rollout_divergence_reported():
     print(f"joint-angle divergence after 1 control step {first:.2e} rad, after 100 {last:.2e} rad")
     assert first < 1e-4
     assert np.isfinite(qe).all()
+
+
+@pytest.mark.parametrize("integrator", ["implicitfast", "Euler"])
+def test_quadruped_step_integrators(integrator):
+    """One full physics step (solve and integration) of the quadruped with the
+    servos' velocity derivative folded into the integrator (implicitfast) and
+    without it (Euler): qvel within 1e-4 relative, qpos within 1e-5 of the
+    oracle's step from the same FP32 state."""
+    xml = scenes.quadruped_xml(self_collision=True, integrator=integrator)
+    B = 32
+    sim = emu.EmuSim(xml, B, self_collision=1, maxcon=32, maxefc=96)
+    q, qv, qa, ctrl, orc = states.quadruped_states(ref64.fields_from(sim.field), B, seed=5)
+    sim.set_state(q, qv, qa)
+    sim.step(ctrl, nsub=1)
+    assert (sim.stats()["status"] & ~3 == 0).all()
+    qk, vk, _ = sim.get_state()
+    worst_v = worst_q = 0.0
+    for e in range(B):
+        orc.set_state(q[e].astype(np.float64), qv[e].astype(np.float64), qa[e].astype(np.float64),
+                      ctrl[e].astype(np.float64))
+        orc.step()
+        qo, vo, _ = orc.get_state()
+        worst_v = max(worst_v, np.max(np.abs(vk[e] - vo)) / max(1.0, np.max(np.abs(vo))))
+        worst_q = max(worst_q, np.max(np.abs(qk[e] - qo)))
+    assert worst_v < 1e-4 and worst_q < 1e-5, (worst_v, worst_q)

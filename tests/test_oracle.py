@@ -349,3 +349,38 @@ def test_torque_free_spin_integrates_exactly():
     ang = 3.0 * 100 * 0.002
     assert np.allclose(qp[3:], [math.cos(ang / 2), 0, 0, math.sin(ang / 2)], atol=1e-12)
     assert np.allclose(qvel, qv, atol=1e-12)
+
+
+SLIDER_SERVO = """<mujoco model="slider_servo">
+  <option timestep="0.01" gravity="0 0 0" integrator="{integrator}"/>
+  <worldbody>
+    <body name="b">
+      <joint name="s" type="slide" axis="1 0 0" damping="{damping}"/>
+      <geom type="sphere" size="0.1" mass="2.0" contype="0" conaffinity="0"/>
+    </body>
+  </worldbody>
+  <actuator>
+    <position joint="s" kp="20" kv="{kv}"/>
+  </actuator>
+</mujoco>"""
+
+
+@pytest.mark.parametrize("integrator", ["Euler", "implicitfast", "implicit"])
+@pytest.mark.parametrize("damping,kv", [(0.7, 0.5), (0.0, 3.0), (2.0, 0.0)])
+def test_damped_servo_step_closed_form(integrator, damping, kv):
+    """A 2 kg slider driven by a position servo (kp 20, kv) with joint damping
+    d, no gravity or contacts. One step from (q, v) with ctrl c has the
+    closed form a = (kp (c - q) - (d + kv) v) / (m + h d') with d' = d for
+    Euler (implicit joint damping) and d' = d + kv for implicit/implicitfast
+    (the servo's velocity derivative folded in as well); then v' = v + h a,
+    q' = q + h v'. Dropping either fold, or its sign, fails the check."""
+    orc, _ = oracle_for(SLIDER_SERVO.format(integrator=integrator, damping=damping, kv=kv))
+    m, h, kp = 2.0, 0.01, 20.0
+    q, v, c = 0.3, -1.2, 0.9
+    orc.set_state(np.array([q]), np.array([v]), np.zeros(1), np.array([c]))
+    orc.step()
+    qp, qv, _ = orc.get_state()
+    dd = damping + (kv if integrator != "Euler" else 0.0)
+    a = (kp * (c - q) - (damping + kv) * v) / (m + h * dd)
+    assert qv[0] == pytest.approx(v + h * a, rel=1e-12, abs=1e-14)
+    assert qp[0] == pytest.approx(q + h * (v + h * a), rel=1e-12, abs=1e-14)
