@@ -596,15 +596,15 @@ KDEV void st_com_vel(const DModel& m, const Layout& L, float* w) {
 // mj_rne (zero acceleration) -> qfrc_bias, then passive and actuator forces,
 // qfrc_smooth and qacc_smooth = M^-1 qfrc_smooth.
 // Reference forward.hpp:338-469.
-// Friction coefficients of pair p; under domain randomisation all five are
-// multiplied by the env's friction scale (into tmp) (reference
-// src/compat/mkx_kernels/solver.hpp:146,435-436,530).
-KDEV const float* pair_mu(const DModel& m, const Layout& L, const float* w, int p, float* tmp) {
-  const float* f5 = m.pair_friction + 5 * p;
-  if (L.rand < 0) return f5;
-  float sc = w[L.rand + rand_fric(m.nbody)];
-  for (int i = 0; i < 5; i++) tmp[i] = f5[i] * sc;
-  return tmp;
+// The env's friction scale: under domain randomisation all five friction
+// coefficients of every pair are multiplied by it, otherwise it is exactly 1
+// (reference src/compat/mkx_kernels/solver.hpp:146,435-436,530). Callers
+// read a pair's coefficients from the model (m.pair_friction + 5 p) and scale
+// each value they use, so the reads stay global-memory loads: a pointer that
+// could also point at a private copy made them flat loads with the copy in
+// scratch memory, in the elliptic cone's cost and line search.
+KDEV float fric_scale(const DModel& m, const Layout& L, const float* w) {
+  return L.rand >= 0 ? w[L.rand + rand_fric(m.nbody)] : 1.f;
 }
 
 // Quadratic drag of a free joint's translation under domain randomisation:
@@ -1635,11 +1635,12 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
       for (int q = 1; q < cnt; q++) J[(r0 + q) * nv + k] = Jd[q - 1];
       continue;
     }
-    float mtmp[5];
-    const float* mu = pair_mu(m, L, w, p, mtmp);
+    const float* mu = m.pair_friction + 5 * p;
+    float sc = fric_scale(m, L, w);
     for (int q = 0; q < cnt / 2; q++) {
-      J[(r0 + 2 * q) * nv + k] = Jn + mu[q] * Jd[q];
-      J[(r0 + 2 * q + 1) * nv + k] = Jn - mu[q] * Jd[q];
+      float mq = mu[q] * sc;
+      J[(r0 + 2 * q) * nv + k] = Jn + mq * Jd[q];
+      J[(r0 + 2 * q + 1) * nv + k] = Jn - mq * Jd[q];
     }
   }
   KSYNC();
@@ -1659,8 +1660,7 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
       int p = iptr(con + CON_STRIDE * eid[r], CON_PAIR)[0];
       sr = m.pair_solref + 2 * p; si = m.pair_solimp + 5 * p;
       float tran = m.body_invweight0[2 * m.pair_b1[p]] + m.body_invweight0[2 * m.pair_b2[p]];
-      float mtmp[5];
-      mu0 = pair_mu(m, L, w, p, mtmp)[0];
+      mu0 = m.pair_friction[5 * p] * fric_scale(m, L, w);
       dA = ty == EFC_CONTACT_PYRAMIDAL ? tran * (1.f + mu0 * mu0) : tran;
     }
     float v = kdot(J + r * nv, 1, qvel, 1, nv);
@@ -1690,9 +1690,9 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
         float imp0 = k_impedance(si, cr[CON_DIST] - (m.pair_margin[p] - m.pair_gap[p]));
         float R0 = fmaxf(dA * (1.f - imp0) / imp0, KMINVAL);
         float R1 = R0 / fmaxf(KMINVAL, m.impratio);
-        float mtmp[5];
-        const float* f5 = pair_mu(m, L, w, p, mtmp);
-        R = fmaxf(kr == 1 ? R1 : R1 * f5[0] * f5[0] / (f5[kr - 1] * f5[kr - 1]), KMINVAL);
+        const float* f5 = m.pair_friction + 5 * p;
+        float sc = fric_scale(m, L, w), f0 = f5[0] * sc, fk = f5[kr - 1] * sc;
+        R = fmaxf(kr == 1 ? R1 : R1 * f0 * f0 / (fk * fk), KMINVAL);
       }
     }
     D[r] = 1.f / R;  // friction rows take R back as 1 / D
@@ -1805,9 +1805,9 @@ KDEV void set_qacc(const DModel& m, const Layout& L, float* w, const float* x, i
 // zone 1 (bottom, mu N + T <= 0) every row quadratic; zone 2 (middle) cost
 // 1/2 Dm (N - mu T)^2 with Dm = D_0 / (mu^2 (1 + mu^2)) (MuJoCo's primal
 // elliptic cone). Writes forces when force != null; returns the zone.
-KDEV int k_ellip(const float* x, const float* D, const float* f, int dim, float* cost, float* force) {
-  float mu = f[0], T = 0.f;
-  for (int j = 1; j < dim; j++) { float U = f[j - 1] * x[j]; T += U * U; }
+KDEV int k_ellip(const float* x, const float* D, const float* f, float sc, int dim, float* cost, float* force) {
+  float mu = f[0] * sc, T = 0.f;
+  for (int j = 1; j < dim; j++) { float U = (f[j - 1] * sc) * x[j]; T += U * U; }
   T = sqrtf(T);
   float N = mu * x[0];
   if (N >= mu * T || (T <= 0.f && N >= 0.f)) {
@@ -1829,24 +1829,28 @@ KDEV int k_ellip(const float* x, const float* D, const float* f, int dim, float*
   if (force) {
     float f0 = -Dm * g * mu;
     force[0] = f0;
-    for (int j = 1; j < dim; j++) force[j] = -f0 * f[j - 1] * f[j - 1] * x[j] / T;
+    for (int j = 1; j < dim; j++) {
+      float fj = f[j - 1] * sc;
+      force[j] = -f0 * fj * fj * x[j] / T;
+    }
   }
   return 2;
 }
 
 // elliptic contact cost along x + alpha v: value, slope, curvature; returns the zone
-KDEV int k_ellip_line(const float* x0, const float* v, const float* D, const float* f, int dim, float alpha, float* c,
-                      float* d1, float* d2) {
+KDEV int k_ellip_line(const float* x0, const float* v, const float* D, const float* f, float sc, int dim, float alpha,
+                      float* c, float* d1, float* d2) {
   float x[6];
   for (int j = 0; j < 6; j++) x[j] = j < dim ? x0[j] + alpha * v[j] : 0.f;
-  int z = k_ellip(x, D, f, dim, c, nullptr);
+  int z = k_ellip(x, D, f, sc, dim, c, nullptr);
   *d1 = 0.f; *d2 = 0.f;
   if (z == 1) {
     for (int j = 0; j < dim; j++) { *d1 += D[j] * x[j] * v[j]; *d2 += D[j] * v[j] * v[j]; }
   } else if (z == 2) {
-    float mu = f[0], T = 0.f, TV = 0.f, VV = 0.f;
+    float mu = f[0] * sc, T = 0.f, TV = 0.f, VV = 0.f;
     for (int j = 1; j < dim; j++) {
-      float U = f[j - 1] * x[j], V = f[j - 1] * v[j];
+      float fj = f[j - 1] * sc;
+      float U = fj * x[j], V = fj * v[j];
       T += U * U; TV += U * V; VV += V * V;
     }
     T = sqrtf(T);
@@ -1893,10 +1897,9 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
       const float* cr = con + CON_STRIDE * c;
       int dim = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
       if (dim < 2 || r0 + dim > nefc) continue;
-      float mtmp[5];
-      const float* f5 = pair_mu(m, L, w, reinterpret_cast<const int*>(cr + CON_PAIR)[0], mtmp);
+      const float* f5 = m.pair_friction + 5 * reinterpret_cast<const int*>(cr + CON_PAIR)[0];
       float cst;
-      int z = k_ellip(Jaref + r0, D + r0, f5, dim, &cst, force + r0);
+      int z = k_ellip(Jaref + r0, D + r0, f5, fric_scale(m, L, w), dim, &cst, force + r0);
       part += cst;
       // the middle zone's Hessian moves with qacc: it always asks for a new factor
       nch += (act[r0] != z) || z == 2;
@@ -1986,11 +1989,11 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
       const float* cr = con + CON_STRIDE * c;
       int dim = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
       if (dim < 2 || r0 + dim > nefc || act[r0] != 2) continue;
-      float mtmp[5];
-      const float* f5 = pair_mu(m, L, w, reinterpret_cast<const int*>(cr + CON_PAIR)[0], mtmp);
+      const float* f5 = m.pair_friction + 5 * reinterpret_cast<const int*>(cr + CON_PAIR)[0];
+      float sc = fric_scale(m, L, w);
       const float* x = Jaref + r0;
-      float mu = f5[0], T = 0.f;
-      for (int j = 1; j < dim; j++) { float U = f5[j - 1] * x[j]; T += U * U; }
+      float mu = f5[0] * sc, T = 0.f;
+      for (int j = 1; j < dim; j++) { float U = (f5[j - 1] * sc) * x[j]; T += U * U; }
       T = sqrtf(T);
       float Dm = D[r0] / (mu * mu * (1.f + mu * mu)), g = mu * x[0] - mu * T, cg = g * mu / T;
       const float* Jc = J + r0 * nv;
@@ -1999,7 +2002,7 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
         tri_rc(t, &i, &l);
         float si = 0.f, sl = 0.f, q = 0.f;
         for (int j = 1; j < dim; j++) {
-          float f2 = f5[j - 1] * f5[j - 1], ji = Jc[j * nv + i], jl = Jc[j * nv + l];
+          float fj = f5[j - 1] * sc, f2 = fj * fj, ji = Jc[j * nv + i], jl = Jc[j * nv + l];
           si += f2 * x[j] * ji; sl += f2 * x[j] * jl; q += f2 * ji * jl;
         }
         si /= T; sl /= T;
@@ -2236,11 +2239,10 @@ KDEV LsPt ls_eval(const DModel& m, const Layout& L, float* w, int nefc, float al
       const float* cr = con + CON_STRIDE * q;
       int dim = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
       if (dim < 2 || r0 + dim > nefc) continue;
-      float mtmp[5];
-      const float* f5 = pair_mu(m, L, w, reinterpret_cast<const int*>(cr + CON_PAIR)[0], mtmp);
-      float cv, g1, g2, cr_, h1, h2;
-      int z = k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, dim, alpha, &cv, &g1, &g2);
-      int zr = k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, dim, ref_alpha, &cr_, &h1, &h2);
+      const float* f5 = m.pair_friction + 5 * reinterpret_cast<const int*>(cr + CON_PAIR)[0];
+      float sc = fric_scale(m, L, w), cv, g1, g2, cr_, h1, h2;
+      int z = k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, sc, dim, alpha, &cv, &g1, &g2);
+      int zr = k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, sc, dim, ref_alpha, &cr_, &h1, &h2);
       nch += (z != zr) || z == 2;
       f += cv; d1 += g1; d2 += g2; d1a += fabsf(g1);
     }
