@@ -44,9 +44,13 @@ void report(const char* fn, const std::exception& e) { std::fprintf(stderr, "%s 
 
 // ------------------------------------------------------------------ kernels
 // GWS: the per-env workspace lives in device memory (gws + env * L.total)
-// instead of LDS, for models whose workspace exceeds the LDS of a workgroup
+// instead of LDS, for models whose workspace exceeds the LDS of a workgroup.
+// Capped at 256 registers per lane (VGPRs and AGPRs together) so that two
+// waves fit on a SIMD: without the cap the compiler spills VGPRs into AGPRs
+// beyond 256 and the kernel runs one wave per SIMD, four envs per CU.
 template <int NPSET, int GWS>
-__global__ void __launch_bounds__(64) k_step(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub, float* gws) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_step(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub, float* gws) {
   extern __shared__ __align__(16) float lds[];
   int env = blockIdx.x;
   if (env >= B) return;
@@ -64,6 +68,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
   int env = blockIdx.x;
   if (env >= B) return;
   mjk::run_env<NPSET | 8>(m, L, lds, io, env, nsub);
+}
+
+// The step with the whole register file of a SIMD lane (one wave per SIMD),
+// for LDS workspaces large enough that at most four envs fit on a CU anyway.
+template <int NPSET>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_step_big(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub) {
+  extern __shared__ __align__(16) float lds[];
+  int env = blockIdx.x;
+  if (env >= B) return;
+  mjk::run_env<NPSET>(m, L, lds, io, env, nsub);
 }
 
 __global__ void __launch_bounds__(64) k_step_queue(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub,
@@ -164,6 +179,7 @@ struct MjmlxBatchedSim {
   bool use_gws = false;    // launch the device-memory workspace variant
   bool lds_ok = false;     // the workspace fits the LDS of a workgroup
   bool high_occ = false;   // LDS admits more than 8 envs per CU: the 168-VGPR kernel
+  bool big_regs = false;   // LDS admits at most 4 envs per CU: the 512-register kernel
   int ws_mode = 0;
   // learned control step captured as a HIP graph (mjmlx_batched_set_graph):
   // the captured launch sequence and the inputs it was captured for
@@ -172,7 +188,7 @@ struct MjmlxBatchedSim {
   hipGraphExec_t gexec = nullptr;
   int g_n = -1, g_cur = -1, g_mode = -1;
   const void *g_mlp = nullptr, *g_xb = nullptr;
-  bool g_gws = false, g_occ = false;
+  bool g_gws = false, g_occ = false, g_big = false;
   // packed readback
   int* d_pack_bodies = nullptr;
   float* d_pack = nullptr;
@@ -233,7 +249,7 @@ void free_sim(MjmlxBatchedSim* s) {
 }
 
 // one workgroup (one wave) per env: the kernel variant for the workspace
-// placement (device memory, LDS at three or two waves per SIMD), compiled
+// placement (device memory, LDS at three, two or one waves per SIMD), compiled
 // for the model's narrow-phase set and, with sensors, the sensor stage
 template <int NP>
 void launch_variant(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
@@ -241,6 +257,7 @@ void launch_variant(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
   size_t sh = s->use_gws ? 0 : s->lds_bytes;
   if (s->use_gws) hipLaunchKernelGGL((k_step<NP, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
   else if (s->high_occ) hipLaunchKernelGGL((k_step_occ<NP>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
+  else if (s->big_regs) hipLaunchKernelGGL((k_step_big<NP>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
   else hipLaunchKernelGGL((k_step<NP, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
 }
 
@@ -342,7 +359,7 @@ bool step_graph(MjmlxBatchedSim* s, int n) {
   if (!s->graph_on || s->ws_mode != 2 || !s->mlp || !s->feat_valid) return false;
   s->mlp->ensure(s->B);  // allocations happen before any capture
   if (!s->gexec || s->g_n != n || s->g_cur != s->feat_cur || s->g_mlp != s->mlp || s->g_xb != s->mlp->xb ||
-      s->g_mode != s->feat_mode || s->g_gws != s->use_gws || s->g_occ != s->high_occ) {
+      s->g_mode != s->feat_mode || s->g_gws != s->use_gws || s->g_occ != s->high_occ || s->g_big != s->big_regs) {
     if (s->gexec) { HIPOK(hipGraphExecDestroy(s->gexec)); s->gexec = nullptr; }
     if (s->graph) { HIPOK(hipGraphDestroy(s->graph)); s->graph = nullptr; }
     HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -350,7 +367,7 @@ bool step_graph(MjmlxBatchedSim* s, int n) {
     HIPOK(hipStreamEndCapture(s->stream, &s->graph));
     HIPOK(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
     s->g_n = n; s->g_cur = s->feat_cur; s->g_mlp = s->mlp; s->g_xb = s->mlp->xb;
-    s->g_mode = s->feat_mode; s->g_gws = s->use_gws; s->g_occ = s->high_occ;
+    s->g_mode = s->feat_mode; s->g_gws = s->use_gws; s->g_occ = s->high_occ; s->g_big = s->big_regs;
   }
   HIPOK(hipEventRecord(s->ev0, s->stream));
   HIPOK(hipGraphLaunch(s->gexec, s->stream));
@@ -652,6 +669,8 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)prop.maxSharedMemoryPerMultiProcessor : 163840;
     s->high_occ = !use_gws && lds_cu / std::max<size_t>(s->lds_bytes, 1) > 8;
     if (const char* v = std::getenv("MJMLX_HIP_HIGH_OCC")) s->high_occ = !use_gws && std::atoi(v) != 0;
+    s->big_regs = !use_gws && !s->high_occ && lds_cu / std::max<size_t>(s->lds_bytes, 1) <= 4;
+    if (const char* v = std::getenv("MJMLX_HIP_BIG_REGS")) s->big_regs = !use_gws && !s->high_occ && std::atoi(v) != 0;
     if (use_gws) HIPOK(hipMalloc(&s->d_gws, (size_t)s->B * s->L.total * sizeof(float)));
     HIPOK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIPOK(hipEventCreate(&s->ev0));
@@ -1078,16 +1097,18 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
       hipEvent_t e0, e1;
       HIPOK(hipEventCreate(&e0));
       HIPOK(hipEventCreate(&e1));
-      // candidates: 0 LDS at 256 VGPRs (two waves per SIMD), 1 device memory,
-      // 2 LDS at 168 VGPRs (three waves per SIMD, some spills)
-      float ms[3] = {3.0e38f, 3.0e38f, 3.0e38f};
+      // candidates: 0 LDS at 256 registers (two waves per SIMD), 1 device
+      // memory, 2 LDS at 168 VGPRs (three waves per SIMD, some spills), 3 LDS
+      // at 512 registers (one wave per SIMD)
+      float ms[4] = {3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f};
       mjk::StepIO io = make_io(s, nullptr, nullptr);
       // the device is drained first so that other work does not share the timing
       HIPOK(hipDeviceSynchronize());
       for (int rep = 0; rep < 3; rep++)
-        for (int g = 0; g < 3; g++) {
+        for (int g = 0; g < 4; g++) {
           s->use_gws = g == 1;
           s->high_occ = g == 2;
+          s->big_regs = g == 3;
           copy(s->d_qpos, q, B * nq); copy(s->d_qvel, v, B * nv); copy(s->d_qws, a, B * nv);
           HIPOK(hipEventRecord(e0, s->stream));
           for (int k = 0; k < steps; k++) launch_step(s, io, 4);
@@ -1099,10 +1120,11 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
           if (rep > 0) ms[g] = std::min(ms[g], t);  // the first round warms caches and code
         }
       int best = 0;
-      for (int g = 1; g < 3; g++)
+      for (int g = 1; g < 4; g++)
         if (ms[g] < ms[best]) best = g;
       s->use_gws = best == 1;
       s->high_occ = best == 2;
+      s->big_regs = best == 3;
       copy(s->d_qpos, q, B * nq); copy(s->d_qvel, v, B * nv); copy(s->d_qws, a, B * nv);
       HIPOK(hipStreamSynchronize(s->stream));
       (void)hipEventDestroy(e0);
@@ -1110,7 +1132,7 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
       HIPOK(hipFree(q)); HIPOK(hipFree(v)); HIPOK(hipFree(a));
       if (!s->use_gws) { HIPOK(hipFree(s->d_gws)); s->d_gws = nullptr; }
     }
-    if (choice) *choice = s->use_gws ? 1 : (s->high_occ ? 2 : 0);
+    if (choice) *choice = s->use_gws ? 1 : (s->high_occ ? 2 : (s->big_regs ? 3 : 0));
     return 0;
   } catch (const std::exception& e) { report("mjmlx_batched_autotune", e); return -1; }
 }

@@ -81,7 +81,7 @@ def test_box_scene_single_step_parity(nbox, solver):
 
 def test_device_workspace_matches_lds(monkeypatch):
     """The same kernel with its per-env workspace in device memory instead of
-    LDS, and in LDS at either register budget (the variants the autotuner
+    LDS, and in LDS at each register budget (the variants the autotuner
     picks from), runs the same arithmetic: 20 steps of the quadruped agree bit
     for bit."""
     model = capi.Model(scenes.quadruped_xml())
@@ -90,9 +90,10 @@ def test_device_workspace_matches_lds(monkeypatch):
     rng = np.random.default_rng(3)
     acts = [scenes.quadruped_actions(B, rng).astype(np.float32) for _ in range(20)]
     out = []
-    for gws, occ in (("0", "1"), ("1", "0"), ("0", "0")):
+    for gws, occ, big in (("0", "1", "0"), ("1", "0", "0"), ("0", "0", "0"), ("0", "0", "1")):
         monkeypatch.setenv("MJMLX_HIP_GLOBAL_WORKSPACE", gws)
         monkeypatch.setenv("MJMLX_HIP_HIGH_OCC", occ)
+        monkeypatch.setenv("MJMLX_HIP_BIG_REGS", big)
         sim = capi.BatchedSim(model, B)
         sim.set_state(q0.astype(np.float32), np.zeros((B, model.nv), np.float32))
         for a in acts:
@@ -120,7 +121,7 @@ def test_autotune_keeps_state():
             sim.step(None, 4)
         if tune:
             choice = sim.autotune(2)
-            assert choice in (0, 1, 2)
+            assert choice in (0, 1, 2, 3)
         for _ in range(5):
             sim.step(None, 4)
         out.append(sim.state())
