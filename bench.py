@@ -131,7 +131,8 @@ def run_boxes(args):
             sim.step(None, 4)
     placement = {}
     for n, _, _, sim in groups:
-        placement[n] = ("LDS", "device memory", "LDS, 168 VGPRs", "LDS, 512 registers")[sim.autotune(2)]
+        placement[n] = ("LDS", "device memory", "LDS, 168 VGPRs", "LDS, 512 registers",
+                        "device memory, 168 VGPRs")[sim.autotune(2)]
     for k in range(2):
         for _, _, _, sim in groups:
             sim.step(None, 4)

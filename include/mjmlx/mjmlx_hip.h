@@ -62,12 +62,13 @@ MJMLX_API float mjmlx_batched_last_step_ms(const MjmlxBatchedSim* sim);
 MJMLX_API int mjmlx_batched_capacity(const MjmlxBatchedSim* sim, int* maxcon, int* maxefc, int* lds_bytes);
 
 /* Autotuner: when the workspace fits in LDS, drains the device, times `steps`
- * control steps from the current state with four kernel variants: workspace
+ * control steps from the current state with five kernel variants: workspace
  * in LDS at 256 registers per lane (two waves per SIMD), in device memory, in
- * LDS at 168 VGPRs (three waves per SIMD, with spills) and in LDS at 512
- * registers (one wave per SIMD), best of two rounds after a warm-up round;
- * keeps the fastest (choice: 0 LDS, 1 device memory, 2 LDS at 168 VGPRs,
- * 3 LDS at 512 registers) and restores the state (qpos,
+ * LDS at 168 VGPRs (three waves per SIMD, with spills), in LDS at 512
+ * registers (one wave per SIMD) and in device memory at 168 VGPRs, best of
+ * two rounds after a warm-up round; keeps the fastest (choice: 0 LDS,
+ * 1 device memory, 2 LDS at 168 VGPRs, 3 LDS at 512 registers, 4 device
+ * memory at 168 VGPRs) and restores the state (qpos,
  * qvel, warm start; ctrl of the timed steps is zero, the solver statistics
  * are those of the last timed step). No reference counterpart: the
  * reference's kernels have one workspace placement per size class. */

@@ -60,14 +60,16 @@ k_step(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub, float* gws
 
 // The same step capped at 168 VGPRs (three waves per SIMD, a few hundred
 // bytes of spills per lane), for workspaces small enough that LDS admits more
-// than eight envs per CU.
-template <int NPSET>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_step_occ(mjk::DModel m, mjk::Layout L,
-                                                                                     mjk::StepIO io, int B, int nsub) {
+// than eight envs per CU, and (GWS 1, picked by the autotuner) for device-memory
+// workspaces, whose occupancy only registers limit.
+template <int NPSET, int GWS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+k_step_occ(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub, float* gws) {
   extern __shared__ __align__(16) float lds[];
   int env = blockIdx.x;
   if (env >= B) return;
-  mjk::run_env<NPSET | 8>(m, L, lds, io, env, nsub);
+  float* w = GWS ? gws + (size_t)env * L.total : lds;
+  mjk::run_env<NPSET | 8>(m, L, w, io, env, nsub);
 }
 
 // The step with the whole register file of a SIMD lane (one wave per SIMD),
@@ -255,8 +257,10 @@ template <int NP>
 void launch_variant(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
   dim3 g(s->B), b(64);
   size_t sh = s->use_gws ? 0 : s->lds_bytes;
-  if (s->use_gws) hipLaunchKernelGGL((k_step<NP, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
-  else if (s->high_occ) hipLaunchKernelGGL((k_step_occ<NP>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
+  if (s->use_gws && s->high_occ)
+    hipLaunchKernelGGL((k_step_occ<NP, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
+  else if (s->use_gws) hipLaunchKernelGGL((k_step<NP, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
+  else if (s->high_occ) hipLaunchKernelGGL((k_step_occ<NP, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
   else if (s->big_regs) hipLaunchKernelGGL((k_step_big<NP>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
   else hipLaunchKernelGGL((k_step<NP, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
 }
@@ -668,7 +672,7 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     s->use_gws = use_gws;
     size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)prop.maxSharedMemoryPerMultiProcessor : 163840;
     s->high_occ = !use_gws && lds_cu / std::max<size_t>(s->lds_bytes, 1) > 8;
-    if (const char* v = std::getenv("MJMLX_HIP_HIGH_OCC")) s->high_occ = !use_gws && std::atoi(v) != 0;
+    if (const char* v = std::getenv("MJMLX_HIP_HIGH_OCC")) s->high_occ = std::atoi(v) != 0;
     s->big_regs = !use_gws && !s->high_occ && lds_cu / std::max<size_t>(s->lds_bytes, 1) <= 4;
     if (const char* v = std::getenv("MJMLX_HIP_BIG_REGS")) s->big_regs = !use_gws && !s->high_occ && std::atoi(v) != 0;
     if (use_gws) HIPOK(hipMalloc(&s->d_gws, (size_t)s->B * s->L.total * sizeof(float)));
@@ -1099,15 +1103,15 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
       HIPOK(hipEventCreate(&e1));
       // candidates: 0 LDS at 256 registers (two waves per SIMD), 1 device
       // memory, 2 LDS at 168 VGPRs (three waves per SIMD, some spills), 3 LDS
-      // at 512 registers (one wave per SIMD)
-      float ms[4] = {3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f};
+      // at 512 registers (one wave per SIMD), 4 device memory at 168 VGPRs
+      float ms[5] = {3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f};
       mjk::StepIO io = make_io(s, nullptr, nullptr);
       // the device is drained first so that other work does not share the timing
       HIPOK(hipDeviceSynchronize());
       for (int rep = 0; rep < 3; rep++)
-        for (int g = 0; g < 4; g++) {
-          s->use_gws = g == 1;
-          s->high_occ = g == 2;
+        for (int g = 0; g < 5; g++) {
+          s->use_gws = g == 1 || g == 4;
+          s->high_occ = g == 2 || g == 4;
           s->big_regs = g == 3;
           copy(s->d_qpos, q, B * nq); copy(s->d_qvel, v, B * nv); copy(s->d_qws, a, B * nv);
           HIPOK(hipEventRecord(e0, s->stream));
@@ -1120,10 +1124,10 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
           if (rep > 0) ms[g] = std::min(ms[g], t);  // the first round warms caches and code
         }
       int best = 0;
-      for (int g = 1; g < 4; g++)
+      for (int g = 1; g < 5; g++)
         if (ms[g] < ms[best]) best = g;
-      s->use_gws = best == 1;
-      s->high_occ = best == 2;
+      s->use_gws = best == 1 || best == 4;
+      s->high_occ = best == 2 || best == 4;
       s->big_regs = best == 3;
       copy(s->d_qpos, q, B * nq); copy(s->d_qvel, v, B * nv); copy(s->d_qws, a, B * nv);
       HIPOK(hipStreamSynchronize(s->stream));
@@ -1132,7 +1136,7 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
       HIPOK(hipFree(q)); HIPOK(hipFree(v)); HIPOK(hipFree(a));
       if (!s->use_gws) { HIPOK(hipFree(s->d_gws)); s->d_gws = nullptr; }
     }
-    if (choice) *choice = s->use_gws ? 1 : (s->high_occ ? 2 : (s->big_regs ? 3 : 0));
+    if (choice) *choice = s->use_gws ? (s->high_occ ? 4 : 1) : (s->high_occ ? 2 : (s->big_regs ? 3 : 0));
     return 0;
   } catch (const std::exception& e) { report("mjmlx_batched_autotune", e); return -1; }
 }

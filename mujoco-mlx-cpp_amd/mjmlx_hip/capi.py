@@ -291,8 +291,8 @@ class BatchedSim:
     def autotune(self, steps: int = 3) -> int:
         """Time the step's kernel variants from the current state and keep the
         fastest; returns 0 (LDS workspace, 256 registers), 1 (device memory),
-        2 (LDS workspace, 168 VGPRs) or 3 (LDS workspace, 512 registers). The
-        state is restored."""
+        2 (LDS workspace, 168 VGPRs), 3 (LDS workspace, 512 registers) or
+        4 (device memory, 168 VGPRs). The state is restored."""
         c = C.c_int()
         assert lib().mjmlx_batched_autotune(self.h, steps, C.byref(c)) == 0
         return c.value

@@ -90,7 +90,7 @@ def test_device_workspace_matches_lds(monkeypatch):
     rng = np.random.default_rng(3)
     acts = [scenes.quadruped_actions(B, rng).astype(np.float32) for _ in range(20)]
     out = []
-    for gws, occ, big in (("0", "1", "0"), ("1", "0", "0"), ("0", "0", "0"), ("0", "0", "1")):
+    for gws, occ, big in (("0", "1", "0"), ("1", "0", "0"), ("0", "0", "0"), ("0", "0", "1"), ("1", "1", "0")):
         monkeypatch.setenv("MJMLX_HIP_GLOBAL_WORKSPACE", gws)
         monkeypatch.setenv("MJMLX_HIP_HIGH_OCC", occ)
         monkeypatch.setenv("MJMLX_HIP_BIG_REGS", big)
@@ -121,7 +121,7 @@ def test_autotune_keeps_state():
             sim.step(None, 4)
         if tune:
             choice = sim.autotune(2)
-            assert choice in (0, 1, 2, 3)
+            assert choice in (0, 1, 2, 3, 4)
         for _ in range(5):
             sim.step(None, 4)
         out.append(sim.state())
