@@ -68,9 +68,11 @@ MJMLX_API int mjmlx_batched_capacity(const MjmlxBatchedSim* sim, int* maxcon, in
  * registers (one wave per SIMD) and in device memory at 168 VGPRs, best of
  * two rounds after a warm-up round; keeps the fastest (choice: 0 LDS,
  * 1 device memory, 2 LDS at 168 VGPRs, 3 LDS at 512 registers, 4 device
- * memory at 168 VGPRs) and restores the state (qpos,
- * qvel, warm start; ctrl of the timed steps is zero, the solver statistics
- * are those of the last timed step). No reference counterpart: the
+ * memory at 168 VGPRs) and restores every buffer the timed steps wrote
+ * (qpos, qvel, warm start, ctrl, solver statistics, sensor values and the
+ * accelerometers' history, the aux outputs read by the get_xpos / cvel /
+ * cfrc_ext getters), so the sim continues exactly as without the call; the
+ * timed steps use the current ctrl. No reference counterpart: the
  * reference's kernels have one workspace placement per size class. */
 MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* sim, int steps, int* choice);
 

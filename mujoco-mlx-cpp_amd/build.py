@@ -11,6 +11,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -25,6 +26,9 @@ HIP_FLAGS = ["-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-unused-value"]
 HOST_SRC = ["mjcf.cpp", "hostmodel.cpp"]
+# narrow-phase sets of the step kernel, one translation unit each (step_np.hip)
+NP_SETS = (0, 1, 2, 4, 5, 6)
+JOBS = max(1, min(16, os.cpu_count() or 1))
 
 
 def _run(cmd, cwd=CSRC):
@@ -37,10 +41,31 @@ def _run(cmd, cwd=CSRC):
         raise RuntimeError(f"build failed: {' '.join(cmd)}")
 
 
+def _objects(tag, extra):
+    """Compile the engine's translation units in parallel into _build/<tag>/;
+    returns the object paths."""
+    obj = os.path.join(PKG, "_build", tag)
+    os.makedirs(obj, exist_ok=True)
+    inc = [f"-I{INC}", f"-I{CSRC}"]
+    jobs = [([HIPCC, *HIP_FLAGS, *extra, *inc, "-c", src, "-o", os.path.join(obj, src.rsplit(".", 1)[0] + ".o")],
+             os.path.join(obj, src.rsplit(".", 1)[0] + ".o")) for src in ("engine.hip", "mlp.hip", *HOST_SRC)]
+    jobs += [([HIPCC, *HIP_FLAGS, *extra, *inc, f"-DMJK_NP={k}", "-c", "step_np.hip", "-o",
+               os.path.join(obj, f"step_np{k}.o")], os.path.join(obj, f"step_np{k}.o")) for k in NP_SETS]
+    # the largest translation units first
+    jobs.sort(key=lambda j: 0 if "step_np" in j[1] else 1)
+    with ThreadPoolExecutor(JOBS) as ex:
+        list(ex.map(lambda j: _run(j[0]), jobs))
+    return [o for _, o in jobs]
+
+
+def _link(out, objs):
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs])
+
+
 def build_engine():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libmjmlx.so")
-    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INC}", f"-I{CSRC}", "-o", out, "engine.hip", "mlp.hip", *HOST_SRC])
+    _link(out, _objects("engine", []))
     return out
 
 
@@ -48,8 +73,7 @@ def build_prof():
     """Profiling build of the engine (per-stage cycle counters compiled in)."""
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libmjmlx_prof.so")
-    _run([HIPCC, *HIP_FLAGS, "-DMJK_PROFILE", "-shared", f"-I{INC}", f"-I{CSRC}", "-o", out, "engine.hip", "mlp.hip",
-          *HOST_SRC])
+    _link(out, _objects("prof", ["-DMJK_PROFILE"]))
     return out
 
 

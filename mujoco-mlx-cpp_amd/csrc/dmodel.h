@@ -106,7 +106,7 @@ struct Layout {
 enum { MISC_NCON = 0, MISC_NEFC = 1, MISC_STATUS = 2, MISC_NITER = 3, MISC_NFLOSSROWS = 4, MISC_NLIMROWS = 5,
        MISC_CONSUB = 6, MISC_NCON_ALL = 7, MISC_SIZE = 8 };
 
-#if defined(__HIPCC__) || defined(__CUDACC__)
+#if defined(__HIPCC__)
 #define MJK_HD __host__ __device__
 #else
 #define MJK_HD

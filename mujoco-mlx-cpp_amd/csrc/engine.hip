@@ -3,9 +3,8 @@
 // One 64-lane wavefront steps one env: the env's state and every per-step
 // intermediate live in that workgroup's LDS workspace (kernels.h), state is
 // read from HBM once per call and written back once after all substeps.
-// Two launch shapes: one workgroup per env (the hardware dispatcher refills
-// CUs as envs finish), or a persistent grid whose workgroups pull env indices
-// from a device work queue (an atomic ticket counter).
+// One workgroup per env: the hardware dispatcher refills CUs as envs finish.
+// The step kernels themselves are built per narrow-phase set in step_np.hip.
 //
 // Replaces the reference's per-stage Vulkan dispatches and the host substep
 // loop (reference src/batched.cpp:980-1228,1469-1504).
@@ -26,6 +25,7 @@
 #include "kernels.h"
 #include "mjmlx/mjmlx_hip.h"
 #include "mlp.h"
+#include "step_launch.h"
 
 namespace {
 
@@ -43,59 +43,6 @@ void report(const char* fn, const std::exception& e) { std::fprintf(stderr, "%s 
 }  // namespace
 
 // ------------------------------------------------------------------ kernels
-// GWS: the per-env workspace lives in device memory (gws + env * L.total)
-// instead of LDS, for models whose workspace exceeds the LDS of a workgroup.
-// Capped at 256 registers per lane (VGPRs and AGPRs together) so that two
-// waves fit on a SIMD: without the cap the compiler spills VGPRs into AGPRs
-// beyond 256 and the kernel runs one wave per SIMD, four envs per CU.
-template <int NPSET, int GWS>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-k_step(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub, float* gws) {
-  extern __shared__ __align__(16) float lds[];
-  int env = blockIdx.x;
-  if (env >= B) return;
-  float* w = GWS ? gws + (size_t)env * L.total : lds;
-  mjk::run_env<NPSET>(m, L, w, io, env, nsub);
-}
-
-// The same step capped at 168 VGPRs (three waves per SIMD, a few hundred
-// bytes of spills per lane), for workspaces small enough that LDS admits more
-// than eight envs per CU, and (GWS 1, picked by the autotuner) for device-memory
-// workspaces, whose occupancy only registers limit.
-template <int NPSET, int GWS>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
-k_step_occ(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub, float* gws) {
-  extern __shared__ __align__(16) float lds[];
-  int env = blockIdx.x;
-  if (env >= B) return;
-  float* w = GWS ? gws + (size_t)env * L.total : lds;
-  mjk::run_env<NPSET | 8>(m, L, w, io, env, nsub);
-}
-
-// The step with the whole register file of a SIMD lane (one wave per SIMD),
-// for LDS workspaces large enough that at most four envs fit on a CU anyway.
-template <int NPSET>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-k_step_big(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub) {
-  extern __shared__ __align__(16) float lds[];
-  int env = blockIdx.x;
-  if (env >= B) return;
-  mjk::run_env<NPSET>(m, L, lds, io, env, nsub);
-}
-
-__global__ void __launch_bounds__(64) k_step_queue(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int nsub,
-                                                   int* queue) {
-  extern __shared__ __align__(16) float lds[];
-  // every workgroup leaves after at most B tickets even if the counter is off
-  for (int round = 0; round < B; round++) {
-    int t = 0;
-    if (threadIdx.x == 0) t = atomicAdd(queue, 1);
-    int env = __shfl(t, 0, 64);
-    if (env >= B) break;
-    mjk::run_env<0>(m, L, lds, io, env, nsub);
-  }
-}
-
 // mj_ray for every (env, ray): one wave per env computes the env's geom
 // poses in its workspace, its lanes take the rays (reference
 // src/batched.cpp:1808-1893)
@@ -108,11 +55,6 @@ __global__ void __launch_bounds__(64) k_raycast(mjk::DModel m, mjk::Layout L, co
   if (env >= B) return;
   float* w = GWS ? gws + (size_t)env * L.total : lds;
   mjk::run_rays(m, L, w, qpos, rand, nrand, rays, nray, gmask, dist, gid, env);
-}
-
-__global__ void k_zero_int(int* p, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
 }
 
 // sprev: accelerometer state [B][pstride] whose last entry (previous velocity
@@ -250,30 +192,19 @@ void free_sim(MjmlxBatchedSim* s) {
   delete s;
 }
 
+// kernel variant of the sim's current workspace placement and register budget
+int step_variant(const MjmlxBatchedSim* s) {
+  if (s->use_gws) return s->high_occ ? mjk::VAR_GWS_OCC : mjk::VAR_GWS;
+  if (s->high_occ) return mjk::VAR_LDS_OCC;
+  return s->big_regs ? mjk::VAR_LDS_BIG : mjk::VAR_LDS;
+}
+
 // one workgroup (one wave) per env: the kernel variant for the workspace
 // placement (device memory, LDS at three, two or one waves per SIMD), compiled
 // for the model's narrow-phase set and, with sensors, the sensor stage
-template <int NP>
-void launch_variant(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
-  dim3 g(s->B), b(64);
-  size_t sh = s->use_gws ? 0 : s->lds_bytes;
-  if (s->use_gws && s->high_occ)
-    hipLaunchKernelGGL((k_step_occ<NP, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
-  else if (s->use_gws) hipLaunchKernelGGL((k_step<NP, 1>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
-  else if (s->high_occ) hipLaunchKernelGGL((k_step_occ<NP, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
-  else if (s->big_regs) hipLaunchKernelGGL((k_step_big<NP>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub);
-  else hipLaunchKernelGGL((k_step<NP, 0>), g, b, sh, s->stream, s->dm, s->L, io, s->B, nsub, s->d_gws);
-}
-
 void launch_step(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
-  switch (s->npset | (s->model.nsensor > 0 ? 4 : 0)) {
-    case 0: launch_variant<0>(s, io, nsub); break;
-    case 1: launch_variant<1>(s, io, nsub); break;
-    case 2: launch_variant<2>(s, io, nsub); break;
-    case 4: launch_variant<4>(s, io, nsub); break;
-    case 5: launch_variant<5>(s, io, nsub); break;
-    default: launch_variant<6>(s, io, nsub); break;
-  }
+  mjk::StepLaunch l{s->dm, s->L, s->B, 0, step_variant(s), s->lds_bytes, s->d_gws, s->stream};
+  mjk::launch_step_np(s->npset | (s->model.nsensor > 0 ? 4 : 0), l, io, nsub);
 }
 
 mjk::StepIO make_io(MjmlxBatchedSim* s, float* dump1, float* dump2) {
@@ -307,14 +238,7 @@ void upload_ctrl(MjmlxBatchedSim* s, const float* ctrl) {
 void launch(MjmlxBatchedSim* s, int nsub, float* dump1, float* dump2) {
   mjk::StepIO io = make_io(s, dump1, dump2);
   HIPOK(hipEventRecord(s->ev0, s->stream));
-  if (s->persistent && !s->use_gws) {
-    int grid = s->grid > 0 ? s->grid : std::max(1, s->ncu * 8);
-    grid = std::min(grid, s->B);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s->stream, s->d_queue, 1);
-    hipLaunchKernelGGL(k_step_queue, dim3(grid), dim3(64), s->lds_bytes, s->stream, s->dm, s->L, io, s->B, nsub, s->d_queue);  // LDS workspaces only
-  } else {
-    launch_step(s, io, nsub);
-  }
+  launch_step(s, io, nsub);
   HIPOK(hipGetLastError());
   HIPOK(hipEventRecord(s->ev1, s->stream));
   s->timed = true;
@@ -1087,17 +1011,33 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
   if (!s || steps < 1) return -1;
   try {
     if (s->lds_ok) {
-      size_t B = s->B, nq = std::max(1, s->nq), nv = std::max(1, s->nv);
-      float *q = nullptr, *v = nullptr, *a = nullptr;
-      HIPOK(hipMalloc(&q, B * nq * sizeof(float)));
-      HIPOK(hipMalloc(&v, B * nv * sizeof(float)));
-      HIPOK(hipMalloc(&a, B * nv * sizeof(float)));
-      auto copy = [&](float* dst, const float* src, size_t n) {
-        HIPOK(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+      size_t B = s->B;
+      // every device buffer the timed steps write: the state, the solver
+      // statistics, the sensors (values and the accelerometers' history) and
+      // the aux outputs; restored after the timing so that the sim continues
+      // from the state it was tuned at
+      std::vector<std::pair<void*, size_t>> live = {
+          {s->d_qpos, B * std::max(1, s->nq) * sizeof(float)}, {s->d_qvel, B * std::max(1, s->nv) * sizeof(float)},
+          {s->d_qws, B * std::max(1, s->nv) * sizeof(float)},  {s->d_ctrl, B * std::max(1, s->nu) * sizeof(float)},
+          {s->d_niter, B * sizeof(int)}, {s->d_status, B * sizeof(int)}, {s->d_ncon, B * sizeof(int)},
+          {s->d_nefc, B * sizeof(int)}};
+      if (s->d_sensordata) {
+        live.push_back({s->d_sensordata, B * s->model.nsensordata * sizeof(float)});
+        live.push_back({s->d_sens_prev, B * ((size_t)s->dm.nsens_acc * 3 + 1) * sizeof(float)});
+      }
+      if (s->aux_on)
+        for (int i = 0; i < 7; i++) live.push_back({s->d_aux[i], std::max<size_t>(1, s->aux_len(i)) * sizeof(float)});
+      std::vector<void*> saved(live.size(), nullptr);
+      for (size_t i = 0; i < live.size(); i++) {
+        HIPOK(hipMalloc(&saved[i], live[i].second));
+        HIPOK(hipMemcpyAsync(saved[i], live[i].first, live[i].second, hipMemcpyDeviceToDevice, s->stream));
+      }
+      auto restore = [&]() {
+        for (size_t i = 0; i < live.size(); i++)
+          HIPOK(hipMemcpyAsync(live[i].first, saved[i], live[i].second, hipMemcpyDeviceToDevice, s->stream));
       };
-      copy(q, s->d_qpos, B * nq); copy(v, s->d_qvel, B * nv); copy(a, s->d_qws, B * nv);
+      bool had_gws = s->d_gws != nullptr;
       if (!s->d_gws) HIPOK(hipMalloc(&s->d_gws, B * s->L.total * sizeof(float)));
-      HIPOK(hipMemsetAsync(s->d_ctrl, 0, B * std::max(1, s->nu) * sizeof(float), s->stream));
       hipEvent_t e0, e1;
       HIPOK(hipEventCreate(&e0));
       HIPOK(hipEventCreate(&e1));
@@ -1113,7 +1053,7 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
           s->use_gws = g == 1 || g == 4;
           s->high_occ = g == 2 || g == 4;
           s->big_regs = g == 3;
-          copy(s->d_qpos, q, B * nq); copy(s->d_qvel, v, B * nv); copy(s->d_qws, a, B * nv);
+          restore();
           HIPOK(hipEventRecord(e0, s->stream));
           for (int k = 0; k < steps; k++) launch_step(s, io, 4);
           HIPOK(hipEventRecord(e1, s->stream));
@@ -1129,14 +1069,14 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
       s->use_gws = best == 1 || best == 4;
       s->high_occ = best == 2 || best == 4;
       s->big_regs = best == 3;
-      copy(s->d_qpos, q, B * nq); copy(s->d_qvel, v, B * nv); copy(s->d_qws, a, B * nv);
+      restore();
       HIPOK(hipStreamSynchronize(s->stream));
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
-      HIPOK(hipFree(q)); HIPOK(hipFree(v)); HIPOK(hipFree(a));
-      if (!s->use_gws) { HIPOK(hipFree(s->d_gws)); s->d_gws = nullptr; }
+      for (void* p : saved) HIPOK(hipFree(p));
+      if (!s->use_gws && !had_gws) { HIPOK(hipFree(s->d_gws)); s->d_gws = nullptr; }
     }
-    if (choice) *choice = s->use_gws ? (s->high_occ ? 4 : 1) : (s->high_occ ? 2 : (s->big_regs ? 3 : 0));
+    if (choice) *choice = step_variant(s);
     return 0;
   } catch (const std::exception& e) { report("mjmlx_batched_autotune", e); return -1; }
 }

@@ -1,0 +1,54 @@
+// Launch interface of the fused step kernels.
+//
+// The step kernel is instantiated once per narrow-phase set (NPSET: 0 basic
+// pairs, 1 + capsule-box / box-box, 2 + GJK/EPA, +4 with the sensor stage) and
+// per workspace/register variant. Each narrow-phase set lives in its own
+// translation unit (step_np.hip compiled with -DMJK_NP=<set>), so the sets
+// build in parallel; the engine picks one through launch_step_np().
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace mjk {
+
+// kernel variant: where the per-env workspace lives and the register budget per lane
+enum {
+  VAR_LDS = 0,      // LDS, 256 registers (two waves per SIMD)
+  VAR_GWS = 1,      // device memory, 256 registers
+  VAR_LDS_OCC = 2,  // LDS, 168 VGPRs (three waves per SIMD)
+  VAR_LDS_BIG = 3,  // LDS, the whole register file (one wave per SIMD)
+  VAR_GWS_OCC = 4,  // device memory, 168 VGPRs
+  VAR_N = 5
+};
+
+struct StepLaunch {
+  DModel dm;
+  Layout L;
+  int B;               // envs of this launch
+  int env0;            // index of the launch's first env in the io arrays
+  int variant;         // VAR_*
+  size_t lds_bytes;    // dynamic LDS per workgroup (LDS variants)
+  float* gws;          // device-memory workspaces [B][L.total] (VAR_GWS*)
+  hipStream_t stream;
+};
+
+void launch_step_np0(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np1(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np2(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np4(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np5(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np6(const StepLaunch& l, const StepIO& io, int nsub);
+
+inline void launch_step_np(int npset, const StepLaunch& l, const StepIO& io, int nsub) {
+  switch (npset) {
+    case 0: launch_step_np0(l, io, nsub); break;
+    case 1: launch_step_np1(l, io, nsub); break;
+    case 2: launch_step_np2(l, io, nsub); break;
+    case 4: launch_step_np4(l, io, nsub); break;
+    case 5: launch_step_np5(l, io, nsub); break;
+    default: launch_step_np6(l, io, nsub); break;
+  }
+}
+
+}  // namespace mjk

@@ -1,0 +1,78 @@
+// Fused step kernels of one narrow-phase set (MJK_NP, set by the build):
+// all substeps of a control step, one env per 64-lane workgroup.
+//
+// Replaces the reference's per-stage Vulkan dispatches and the host substep
+// loop (reference src/batched.cpp:980-1228,1469-1504).
+#include "step_launch.h"
+
+#ifndef MJK_NP
+#error "step_np.hip is compiled once per narrow-phase set with -DMJK_NP=<0|1|2|4|5|6>"
+#endif
+
+#define MJK_CAT2(a, b) a##b
+#define MJK_CAT(a, b) MJK_CAT2(a, b)
+
+// GWS: the per-env workspace lives in device memory (gws + blockIdx.x *
+// L.total) instead of LDS, for models whose workspace exceeds the LDS of a
+// workgroup. Capped at 256 registers per lane (VGPRs and AGPRs together) so
+// that two waves fit on a SIMD: without the cap the compiler spills VGPRs
+// into AGPRs beyond 256 and the kernel runs one wave per SIMD, four envs per CU.
+template <int NPSET, int GWS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_step(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int nsub, float* gws) {
+  extern __shared__ __align__(16) float lds[];
+  if ((int)blockIdx.x >= B) return;
+  float* w = GWS ? gws + (size_t)blockIdx.x * L.total : lds;
+  mjk::run_env<NPSET>(m, L, w, io, env0 + (int)blockIdx.x, nsub);
+}
+
+// The same step capped at 168 VGPRs (three waves per SIMD, a few hundred
+// bytes of spills per lane), for workspaces small enough that LDS admits more
+// than eight envs per CU, and (GWS 1, picked by the autotuner) for device-memory
+// workspaces, whose occupancy only registers limit.
+template <int NPSET, int GWS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+k_step_occ(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int nsub, float* gws) {
+  extern __shared__ __align__(16) float lds[];
+  if ((int)blockIdx.x >= B) return;
+  float* w = GWS ? gws + (size_t)blockIdx.x * L.total : lds;
+  mjk::run_env<NPSET | 8>(m, L, w, io, env0 + (int)blockIdx.x, nsub);
+}
+
+// The step with the whole register file of a SIMD lane (one wave per SIMD),
+// for LDS workspaces large enough that at most four envs fit on a CU anyway.
+template <int NPSET>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_step_big(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int nsub) {
+  extern __shared__ __align__(16) float lds[];
+  if ((int)blockIdx.x >= B) return;
+  mjk::run_env<NPSET>(m, L, lds, io, env0 + (int)blockIdx.x, nsub);
+}
+
+namespace mjk {
+
+void MJK_CAT(launch_step_np, MJK_NP)(const StepLaunch& l, const StepIO& io, int nsub) {
+  constexpr int NP = MJK_NP;
+  dim3 g(l.B), b(64);
+  bool gws = l.variant == VAR_GWS || l.variant == VAR_GWS_OCC;
+  size_t sh = gws ? 0 : l.lds_bytes;
+  switch (l.variant) {
+    case VAR_GWS_OCC:
+      hipLaunchKernelGGL((k_step_occ<NP, 1>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub, l.gws);
+      break;
+    case VAR_GWS:
+      hipLaunchKernelGGL((k_step<NP, 1>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub, l.gws);
+      break;
+    case VAR_LDS_OCC:
+      hipLaunchKernelGGL((k_step_occ<NP, 0>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub, l.gws);
+      break;
+    case VAR_LDS_BIG:
+      hipLaunchKernelGGL((k_step_big<NP>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub);
+      break;
+    default:
+      hipLaunchKernelGGL((k_step<NP, 0>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub, l.gws);
+      break;
+  }
+}
+
+}  // namespace mjk

@@ -362,6 +362,9 @@ class BatchedSim:
     def qfrc_actuator(self):
         return self._getter("mjmlx_batched_get_qfrc_actuator", (self.B, -1))
 
+    def cfrc_ext(self):
+        return self._getter("mjmlx_batched_get_cfrc_ext", (self.B, -1, 6))
+
 
 class DebugDump:
     def __init__(self, sim: BatchedSim, d1, d2, total):

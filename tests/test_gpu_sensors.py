@@ -70,3 +70,31 @@ def test_accelerometer_after_reset():
     want, _ = sm.sensordata(orc)
     assert np.allclose(y[0, a:a + 3], want[a:a + 3], atol=1e-4)
     assert not np.allclose(y[1:, a:a + 3], want[a:a + 3], atol=1e-2)  # falling envs difference their velocity
+
+
+def test_autotune_keeps_sensor_history():
+    """The autotuner restores what its timed steps wrote, the accelerometers'
+    previous velocities and the aux outputs included: a tuned sim's sensors
+    and cfrc_ext after further steps are bitwise those of an untuned one."""
+    model = capi.Model(scenes.quadruped_xml(sensors=True))
+    B = 64
+    q = scenes.quadruped_initial_qpos(B, 83).astype(np.float32)
+    ctrl = scenes.quadruped_actions(B, np.random.default_rng(83)).astype(np.float32)
+    out = []
+    for tune in (False, True):
+        sim = capi.BatchedSim(model, B)
+        sim.set_state(q, np.zeros((B, model.nv), np.float32))
+        sim.cvel()  # aux outputs on
+        for _ in range(3):
+            sim.step(ctrl, 4)
+        if tune:
+            assert sim.autotune(2) in (0, 1, 2, 3, 4)
+        before = (sim.sensordata().copy(), sim.cfrc_ext())
+        sim.step(ctrl, 1)
+        out.append((before, sim.sensordata().copy(), sim.state()))
+        del sim
+    (b0, s0, st0), (b1, s1, st1) = out
+    assert np.array_equal(b0[0], b1[0]) and np.array_equal(b0[1], b1[1])
+    assert np.array_equal(s0, s1)
+    for a, b in zip(st0, st1):
+        assert np.array_equal(a, b)
