@@ -78,6 +78,10 @@ struct orc_data {
    * per-free-joint quadratic drag; off until orc_set_env_params is called */
   int has_rand, nfree;
   double *rand_mass, *rand_com, *rand_gain, *rand_drag, rand_fric;
+  /* injected contacts (orc_inject_contacts): when ninject >= 0, collision
+   * uses these instead of the narrow phase */
+  int ninject;
+  orc_contact_t* inject;
 };
 
 /* ------------------------------------------------------------------ math */
@@ -372,12 +376,14 @@ double orc_meaninertia(const orc_prep_t* p) { return p->meaninertia; }
 
 orc_data_t* orc_data_new(const orc_prep_t* p) {
   orc_data_t* d = (orc_data_t*)calloc(1, sizeof(orc_data_t));
+  d->ninject = -1;
   data_alloc(d, &p->m);
   memcpy(d->qpos, p->m.qpos0, sizeof(double) * p->m.nq);
   return d;
 }
 void orc_data_free(orc_data_t* d) {
   if (!d) return;
+  free(d->inject);
   free_data_arrays(d);
   free(d);
 }
@@ -1379,6 +1385,32 @@ static int narrowphase(int t1, int t2, const double* s1, const double* s2, const
   return -1;
 }
 
+/* Replace the narrow phase of the following forwards by a given contact list
+ * (pair index, distance, position, normal from geom1 to geom2 per contact; the
+ * frame and the pair's friction, solref, solimp, condim and margin as the
+ * narrow phase would set them); n < 0 restores the narrow phase. Used to
+ * solve another implementation's contact set in FP64. */
+void orc_inject_contacts(const orc_prep_t* p, orc_data_t* d, int n, const int* pair, const double* dist,
+                         const double* pos, const double* normal) {
+  free(d->inject);
+  d->inject = NULL;
+  d->ninject = n < 0 ? -1 : n;
+  if (n <= 0) return;
+  d->inject = (orc_contact_t*)calloc((size_t)n, sizeof(orc_contact_t));
+  for (int k = 0; k < n; k++) {
+    const orc_pair_t* pr = &p->pairs[pair[k]];
+    orc_contact_t* c = &d->inject[k];
+    c->dist = dist[k];
+    memcpy(c->pos, pos + 3 * k, 3 * sizeof(double));
+    make_frame(normal + 3 * k, c->frame);
+    c->geom1 = pr->g1; c->geom2 = pr->g2; c->pair = pair[k]; c->dim = pr->condim;
+    memcpy(c->friction, pr->friction, 5 * sizeof(double));
+    memcpy(c->solref, pr->solref, 2 * sizeof(double));
+    memcpy(c->solimp, pr->solimp, 5 * sizeof(double));
+    c->includemargin = pr->margin - pr->gap;
+  }
+}
+
 int orc_collide_pair(const orc_prep_t* p, int pi, const double* xpos1, const double* xmat1, const double* xpos2,
                      const double* xmat2, orc_contact_t* out, int maxc) {
   const orc_model_t* m = &p->m;
@@ -1411,6 +1443,15 @@ static int collision(const orc_prep_t* p, orc_data_t* d) {
   d->ncon = 0;
   int status = 0;
   if (m->disableflags & (DS_CONTACT | DS_CONSTRAINT)) return 0;
+  if (d->ninject >= 0) {
+    for (int k = 0; k < d->ninject && k < ORC_MAXCON; k++) {
+      d->con[d->ncon] = d->inject[k];
+      if (d->has_rand)
+        for (int i = 0; i < 5; i++) d->con[d->ncon].friction[i] *= d->rand_fric;
+      d->ncon++;
+    }
+    return d->ninject > ORC_MAXCON ? 1 : 0;
+  }
   for (int pi = 0; pi < p->npair; pi++) {
     const orc_pair_t* pr = &p->pairs[pi];
     const double *x1 = d->geom_xpos + 3 * pr->g1, *x2 = d->geom_xpos + 3 * pr->g2;

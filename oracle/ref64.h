@@ -111,6 +111,8 @@ double orc_ray(const orc_prep_t* p, const orc_data_t* d, const double* pnt, cons
 /* One physics step (mj_step): forward pipeline, solver, integrator.
  * Returns 0, or a negative status (>0 capacity overflow bits). */
 int orc_step(const orc_prep_t* p, orc_data_t* d);
+void orc_inject_contacts(const orc_prep_t* p, orc_data_t* d, int n, const int* pair, const double* dist,
+                         const double* pos, const double* normal);
 /* Forward pipeline only (no integration), leaving every field readable. */
 int orc_forward(const orc_prep_t* p, orc_data_t* d);
 /* Kinematics-only pieces (for unit checks). */

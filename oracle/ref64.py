@@ -104,6 +104,8 @@ def lib() -> C.CDLL:
         L.orc_get_state.argtypes = [C.c_void_p, _D, _D, _D]
         L.orc_set_env_params.argtypes = [C.c_void_p, _D, _D, _D, _D, _D]
         L.orc_nfree.argtypes = [C.c_void_p]
+        L.orc_inject_contacts.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _I, _D, _D, _D]
+        L.orc_inject_contacts.restype = None
         L.orc_ray_geom.restype = C.c_double
         L.orc_ray_geom.argtypes = [C.c_int, _D, _D, _D]
         L.orc_ray.restype = C.c_double
@@ -240,6 +242,21 @@ class Oracle:
         g = C.c_int()
         x = lib().orc_ray(self.p, self.d, _dp(p), _dp(v), _ip(ok), C.byref(g))
         return x, g.value
+
+    def inject_contacts(self, pairs=None, dist=None, pos=None, normal=None):
+        """Following forwards use this contact list (pair index, distance,
+        position [n,3], normal [n,3] from geom1 to geom2) instead of the
+        narrow phase; no arguments restores the narrow phase."""
+        L = lib()
+        if pairs is None:
+            L.orc_inject_contacts(self.p, self.d, -1, None, None, None, None)
+            return
+        pr = np.ascontiguousarray(np.asarray(pairs, np.int32).ravel())
+        f = lambda a: np.ascontiguousarray(np.asarray(a, np.float64).ravel())
+        dd, pp, nn = f(dist), f(pos), f(normal)
+        assert dd.size == pr.size and pp.size == 3 * pr.size and nn.size == 3 * pr.size
+        self._inj = (pr, dd, pp, nn)
+        L.orc_inject_contacts(self.p, self.d, pr.size, _ip(pr), _dp(dd), _dp(pp), _dp(nn))
 
     def nfree(self) -> int:
         return lib().orc_nfree(self.d)

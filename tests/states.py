@@ -94,3 +94,64 @@ def cg_progress_ok(orc_cg16, orc_opt, q, qv, qa, ctrl, qk, slack: float = 1.25):
     gk, g16 = c(qk) - c_opt, c(a16) - c_opt
     oracle_step(orc_cg16, q, qv, qa, ctrl)
     return gk <= slack * max(g16, 0.0) + 1e-6 * sc, gk, g16, sc
+
+
+def accel_scale(orc) -> float:
+    """Scale of the accelerations that balance in the oracle's last step: the
+    larger of 1, |qacc| and |qacc_smooth| (max norms). A resting body has
+    qacc ~ 0 while its contact forces cancel gravity's g, so qacc's error is
+    measured against the accelerations being balanced."""
+    return max(1.0, float(np.max(np.abs(orc.get("qacc")))), float(np.max(np.abs(orc.get("qacc_smooth")))))
+
+
+def qacc_force_errors(orc, qacc_k, force_k=None, nefc_k=None):
+    """(qacc error relative to accel_scale, efc_force error relative to the
+    oracle's largest row force floored at 1, or None when the row counts
+    differ or there are no rows) of the kernel's qacc / efc_force against the
+    oracle's last step. Rows are in the same order on both sides (contacts in
+    pair order, each contact's rows in the same order)."""
+    b = orc.get("qacc")
+    eq = float(np.max(np.abs(np.asarray(qacc_k, np.float64) - b))) / accel_scale(orc)
+    ef = None
+    n = orc.nefc()
+    if force_k is not None and n and (nefc_k is None or int(nefc_k) == n):
+        f = orc.get("efc_force")
+        ef = float(np.max(np.abs(np.asarray(force_k[:n], np.float64) - f))) / max(1.0, float(np.max(np.abs(f))))
+    return eq, ef
+
+
+def input_sensitivity(orc, q, qv, qa, ctrl, seed: int = 0, rel: float = 6e-8) -> float:
+    """The FP64 oracle's own qacc change, relative to accel_scale, when qpos is
+    perturbed by FP32 rounding-level noise (rel): the error floor that FP32
+    state arithmetic cannot go below. Leaves the oracle at the perturbed step."""
+    oracle_step(orc, q, qv, qa, ctrl)
+    b, sc = orc.get("qacc").copy(), accel_scale(orc)
+    rng = np.random.default_rng(seed)
+    qp = q.astype(np.float64) * (1 + rng.uniform(-rel, rel, q.size))
+    orc.set_state(qp, qv.astype(np.float64), qa.astype(np.float64), ctrl.astype(np.float64))
+    orc.forward()
+    return float(np.max(np.abs(orc.get("qacc") - b))) / sc
+
+
+def oracle_cfrc_ext(orc, pair_b1, pair_b2, nbody):
+    """Per-body contact force of the oracle's last step in the engine's
+    cfrc_ext convention (reference src/compat/mkx_kernels/solver.hpp:906-929):
+    each contact's normal force along its normal, + on body2 and - on body1,
+    zero torque. The normal force is the sum of a pyramidal contact's edge
+    forces, the normal row's force otherwise."""
+    out = np.zeros((nbody, 6))
+    ty, ids, f = orc.get("efc_type").astype(int), orc.get("efc_id").astype(int), orc.get("efc_force")
+    cons = orc.contacts()
+    fn = np.zeros(len(cons))
+    seen = set()
+    for r in range(orc.nefc()):
+        if ty[r] == 6:          # pyramidal edge
+            fn[ids[r]] += f[r]
+        elif ty[r] in (5, 7) and ids[r] not in seen:   # frictionless / elliptic normal row
+            fn[ids[r]] = f[r]
+            seen.add(ids[r])
+    for i, c in enumerate(cons):
+        n = c["frame"][:3]
+        out[pair_b2[c["pair"]], 3:] += fn[i] * n
+        out[pair_b1[c["pair"]], 3:] -= fn[i] * n
+    return out

@@ -54,29 +54,35 @@ def test_box_scene_single_step_parity(nbox, solver):
     st = sim.stats()
     assert not (st["status"] & ~3).any(), st["status"]
     ncon = 0
-    worst = 0.0
+    worst = worstf = worsts = 0.0
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32))
         con = dump.get(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
         ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4, ncon=st["ncon"][e])
         assert ok, f"env {e}: {msg}"
         ncon += len(con)
-        b = orc.get("qacc")
-        worst = max(worst, np.max(np.abs(dump.get(2, e, "qacc") - b)) / max(1.0, np.max(np.abs(b))))
         assert st["nefc"][e] == orc.nefc(), (e, st["nefc"][e], orc.nefc())
+        eq, ef = states.qacc_force_errors(orc, dump.get(2, e, "qacc"), dump.get(2, e, "efc_force"), st["nefc"][e])
+        worst, worstf = max(worst, eq), max(worstf, ef or 0.0)
+        if e % 8 == 0:
+            worsts = max(worsts, states.input_sensitivity(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32), seed=e))
         if solver == "CG":
             ok, gk, g16, sc = states.cg_progress_ok(orc, orc_opt, q[e], qv[e], qa[e], np.zeros(0, np.float32),
                                                     dump.get(2, e, "qacc").astype(np.float64))
             worstp = max(worstp, gk / max(g16, 1e-6 * sc))
             assert ok, (e, gk, g16, sc)
-    print(f"nbox {nbox} {solver}: contacts {ncon}, worst qacc rel err {worst:.2e}, "
+    print(f"nbox {nbox} {solver}: contacts {ncon}, worst qacc rel err {worst:.2e}, efc_force {worstf:.2e}, "
+          f"oracle's own FP32-input sensitivity {worsts:.2e}, "
           f"mean iterations {st['niter'].mean():.2f}, lds {sim.capacity()}"
           + (f", worst remaining-gap ratio to FP64 CG {worstp:.2f}" if solver == "CG" else ""))
     assert ncon > B
-    # Newton converges to the 1e-8 scaled residual. CG at its 16-iteration
-    # budget is unconverged: FP32 and FP64 CG end at different iterates of
-    # about equal cost (checked above), up to a few 1e-2 apart in qacc
-    assert worst < (1e-3 if solver == "Newton" else 5e-2)
+    # Newton converges to the 1e-8 scaled residual: qacc within 1e-4 of the
+    # accelerations that balance (resting boxes have qacc ~ 0 while contact
+    # forces cancel g), row forces within 1e-4 of the largest. CG at its
+    # 16-iteration budget is unconverged: FP32 and FP64 CG end at different
+    # iterates of about equal cost, judged by the remaining-gap ratio above
+    if solver == "Newton":
+        assert worst < 1e-4 and worstf < 1e-4
 
 
 def test_device_workspace_matches_lds(monkeypatch):

@@ -25,7 +25,7 @@ def test_convex_drop_single_step_parity():
     st = sim.stats()
     assert not (st["status"] & ~3).any(), st["status"]
     ncon = 0
-    worst = worstn = worstc = worstp = 0.0
+    worst = worstn = worstc = worstp = worsti = worstf = 0.0
     cg1, cg2 = model.field("cpair_g1").astype(int), model.field("cpair_g2").astype(int)
     gtype = model.field("geom_type").astype(int)
     size = model.field("geom_size").reshape(-1, 3)
@@ -55,17 +55,28 @@ def test_convex_drop_single_step_parity():
             eb = abs(n @ pb - n @ support(KIND[gtype[g2]], size[g2], gx[g2], gm[g2], -n))
             worstp = max(worstp, ea, eb)
         ncon += len(con)
-        b = orc.get("qacc")
         qk = dump.get(2, e, "qacc")
-        worst = max(worst, np.max(np.abs(qk - b)) / max(1.0, np.max(np.abs(b))))
-        c_opt = orc.cost(b)[0]
+        worst = max(worst, states.qacc_force_errors(orc, qk)[0])
+        # the constraint problem of the kernel's own (geometrically checked)
+        # contact set, solved by the oracle in FP64
+        c64 = con.astype(np.float64)
+        orc.inject_contacts(con[:, 13].view(np.int32), c64[:, 12], c64[:, 0:3], c64[:, 3:6])
+        states.oracle_step(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32))
+        assert orc.nefc() == st["nefc"][e]
+        ei, ef = states.qacc_force_errors(orc, qk, dump.get(2, e, "efc_force"), st["nefc"][e])
+        worsti, worstf = max(worsti, ei), max(worstf, ef or 0.0)
+        c_opt = orc.cost(orc.get("qacc"))[0]
         worstc = max(worstc, (orc.cost(qk.astype(np.float64))[0] - c_opt) / states.cost_scale(orc, c_opt))
-    print(f"convex drop: contacts {ncon}, worst qacc rel err {worst:.2e}, worst normal {worstn:.3f} deg, "
-          f"worst support-plane offset {worstp:.1e} m, worst cost gap {worstc:.2e}")
+        orc.inject_contacts()
+    print(f"convex drop: contacts {ncon}, worst normal {worstn:.3f} deg, worst support-plane offset {worstp:.1e} m; "
+          f"on the kernel's contact set: qacc rel err {worsti:.2e}, efc_force {worstf:.2e}, cost gap {worstc:.2e}; "
+          f"qacc against the oracle's own contact set {worst:.2e}")
     assert ncon > B
     assert worstn < 1.5 and worstp < 1e-4
-    # FP32 EPA normals differ from FP64 ones by up to ~1 degree and, on flat
-    # patches (cylinder caps, box faces), the contact point is any point of the
-    # patch: the kernel solves a nearby problem, its optimum a few 1e-2 of qacc
-    # away on centimetre-deep stiff contacts
-    assert worst < 5e-2 and worstc < 1e-2
+    # Contacts: same pairs, depths within 1e-5 m, normals within 1.5 degrees,
+    # points on both shapes' supporting planes. On flat patches (cylinder
+    # caps, box faces) any point of the patch is a valid contact point, and
+    # FP32 and FP64 EPA pick different ones: the two contact sets pose nearby
+    # but different constraint problems (qacc a few 1e-3 apart on stiff
+    # contacts). The solve itself is held to the 1e-4 bar on the kernel's set.
+    assert worsti < 1e-4 and worstf < 1e-4 and worstc < 1e-8

@@ -39,7 +39,7 @@ def test_humanoid_single_step_parity():
     dump = sim.debug_step(ctrl)
     st = sim.stats()
     assert not (st["status"] & ~3).any(), st["status"]
-    worst = worstc = 0.0
+    worst = worstc = worstf = worsts = 0.0
     ncon = 0
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], ctrl[e])
@@ -49,14 +49,19 @@ def test_humanoid_single_step_parity():
         ncon += len(con)
         b = orc.get("qacc")
         qk = dump.get(2, e, "qacc")
-        worst = max(worst, np.max(np.abs(qk - b)) / max(1.0, np.max(np.abs(b))))
+        eq, ef = states.qacc_force_errors(orc, qk, dump.get(2, e, "efc_force"), st["nefc"][e])
+        assert st["nefc"][e] == orc.nefc(), (e, st["nefc"][e], orc.nefc())
+        worst, worstf = max(worst, eq), max(worstf, ef or 0.0)
         if orc.nefc():
             c_opt = orc.cost(b)[0]
             worstc = max(worstc, (orc.cost(qk.astype(np.float64))[0] - c_opt) / states.cost_scale(orc, c_opt))
-    print(f"humanoid: contacts {ncon}, worst qacc rel err {worst:.2e}, cost gap {worstc:.2e}, "
+    for e in range(0, B, 8):
+        worsts = max(worsts, states.input_sensitivity(orc, q[e], qv[e], qa[e], ctrl[e], seed=e))
+    print(f"humanoid: contacts {ncon}, worst qacc rel err {worst:.2e}, worst efc_force rel err {worstf:.2e}, "
+          f"cost gap {worstc:.2e}, oracle's own FP32-input sensitivity {worsts:.2e}, "
           f"mean iterations {st['niter'].mean():.2f}")
     assert ncon > B // 2
-    assert worst < 3e-4 and worstc < 1e-8
+    assert worst < 1e-4 and worstf < 1e-4 and worstc < 1e-8
 
 
 def test_humanoid_rollout_stays_finite():

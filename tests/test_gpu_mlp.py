@@ -164,7 +164,9 @@ def test_learned_warm_start_keeps_results_and_reports_iterations():
             it_p += sp["niter"].mean()
             it_l += sl["niter"].mean()
     print(f"mean solver iterations per env-step: plain {it_p / 3:.3f}, learned {it_l / 3:.3f}")
-    assert it_l <= 1.1 * it_p
+    # held-out states (later in the rollout, actions from another stream): the
+    # trained network saves iterations
+    assert it_l < it_p
     # mode 3: the prediction only at each call's first substep, one fused launch
     first = capi.BatchedSim(model, B)
     first.set_warmstart_mlp(net)
@@ -191,7 +193,7 @@ def test_learned_warm_start_keeps_results_and_reports_iterations():
             (qr, vr), (qf, vf) = ref.state(), first.state()
             assert np.max(np.abs(vr - vf)) / max(1.0, np.max(np.abs(vr))) < 1e-3 and np.max(np.abs(qr - qf)) < 1e-4
     print(f"first-substep mode: iterations per env-step plain {it_r / 3:.3f}, learned {it_f / 3:.3f}")
-    assert it_f <= 1.1 * it_r
+    assert it_f < it_r
 
 
 def test_graph_replay_matches_eager():
@@ -224,3 +226,52 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(qe, qg) and np.array_equal(ve, vg), k
         se, sg = sims[0].stats(), sims[1].stats()
         assert np.array_equal(se["niter"], sg["niter"]), k
+
+
+def test_humanoid_learned_warm_start():
+    """The humanoid's leg of the learned warm start: a network trained on the
+    humanoid's own solver states (features 204, output 27) keeps the results
+    of the plain warm start and its iteration counts are reported."""
+    B, nsteps = 1024, 12
+    model = capi.Model(scenes.humanoid_xml())
+    sim = capi.BatchedSim(model, B)
+    rng = np.random.default_rng(3)
+    q0 = np.tile(model.field("qpos0"), (B, 1)).astype(np.float32)
+    sim.set_state(q0, rng.normal(0, 0.3, (B, model.nv)).astype(np.float32))
+    nf = sim.feature_dim()
+    assert nf == model.nq + model.nv + model.nu + 128
+    net = capi.MLP(nf, NH, model.nv, seed=1)
+    net.reserve(B * 4 * nsteps)
+    rows = 0
+    for _ in range(nsteps):
+        rows += sim.record(net, rng.uniform(-0.4, 0.4, (B, model.nu)).astype(np.float32), 4)
+    net.fit_normalization()
+    loss = net.fit(6, 1024, lr=1e-3, seed=0)
+    print(f"humanoid: trained on {rows} pairs, loss {loss:.4f} (normalised units)")
+    assert 0 <= loss < 1.0
+    q, qv = sim.state()
+    qa = sim.qacc_warmstart()
+    plain = capi.BatchedSim(model, B)
+    learned = capi.BatchedSim(model, B)
+    learned.set_warmstart_mlp(net)
+    assert learned.set_warmstart_mode(2) == 0
+    for s in (plain, learned):
+        s.set_state(q, qv)
+        s.set_qacc_warmstart(qa)
+    it_p = it_l = 0.0
+    for k in range(4):
+        c = rng.uniform(-0.4, 0.4, (B, model.nu)).astype(np.float32)
+        plain.step(c, 4)
+        learned.step(c, 4)
+        sp, sl = plain.stats(), learned.stats()
+        assert not (sp["status"] & ~3).any() and not (sl["status"] & ~3).any()
+        if k == 0:
+            (qp, vp), (ql, vl) = plain.state(), learned.state()
+            dv = np.max(np.abs(vp - vl)) / max(1.0, np.max(np.abs(vp)))
+            dq = np.max(np.abs(qp - ql))
+            print(f"humanoid after one control step: max |dqvel| rel {dv:.2e}, max |dqpos| {dq:.2e}")
+            assert dv < 1e-3 and dq < 1e-4
+        else:
+            it_p += sp["niter"].mean()
+            it_l += sl["niter"].mean()
+    print(f"humanoid mean solver iterations per env-step: plain {it_p / 3:.3f}, learned {it_l / 3:.3f}")

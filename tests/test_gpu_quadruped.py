@@ -27,9 +27,14 @@ def model():
     return capi.Model(scenes.quadruped_xml(self_collision=True))
 
 
-def test_single_step_parity(model):
+@pytest.mark.parametrize("self_collision", [True, False])
+def test_single_step_parity(model, self_collision):
+    """With self-collision (the GPU workload) and without it (the CPU
+    baseline's quadruped: floor contacts only)."""
     B = 96
     import ref64
+    if not self_collision:
+        model = capi.Model(scenes.quadruped_xml(self_collision=False))
     q, qv, qa, ctrl, orc = states.quadruped_states(ref64.fields_from(model.field), B, seed=3)
     sim = capi.BatchedSim(model, B)
     sim.set_state(q, qv)
@@ -202,50 +207,65 @@ def test_duplicate_envs_bitwise(model):
 
 
 def test_cfrc_ext_from_contact_forces(model):
-    """cfrc_ext after a step: each contact's normal force (sum of its rows'
-    forces) along the contact normal, +body2 / -body1, zero torque."""
+    """cfrc_ext after a one-substep step against the FP64 oracle's contacts and
+    row forces at the same state: each contact's normal force (sum of its
+    rows' forces) along the contact normal, +body2 / -body1, zero torque."""
     import ref64
     B = 64
-    q, qv, qa, ctrl, _ = states.quadruped_states(ref64.fields_from(model.field), B, seed=17)
+    q, qv, qa, ctrl, orc = states.quadruped_states(ref64.fields_from(model.field), B, seed=17)
     sim = capi.BatchedSim(model, B)
     sim.set_state(q, qv)
     sim.set_qacc_warmstart(qa)
     sim.xpos()  # turns the per-body outputs on
-    dump = sim.debug_step(ctrl)
-    st = sim.stats()
+    sim.step(ctrl, 1)
     got = sim._getter("mjmlx_batched_get_cfrc_ext", (B, model.info.nbody, 6))
     b1, b2 = model.field("cpair_b1").astype(int), model.field("cpair_b2").astype(int)
     nonzero = 0
+    worst = 0.0
     for e in range(B):
-        want = np.zeros((model.info.nbody, 6))
-        con = dump.get(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
-        force = dump.get(2, e, "efc_force")
-        for c in con:
-            p, nrow, r0 = (int(v) for v in c[13:16].view(np.int32))
-            fn = force[r0:r0 + nrow].sum() if r0 + nrow <= st["nefc"][e] else 0.0
-            want[b2[p], 3:] += fn * c[3:6]
-            want[b1[p], 3:] -= fn * c[3:6]
+        states.oracle_step(orc, q[e], qv[e], qa[e], ctrl[e])
+        want = states.oracle_cfrc_ext(orc, b1, b2, model.info.nbody)
         nonzero += int(np.abs(want).sum() > 0)
-        assert np.allclose(got[e], want, rtol=1e-5, atol=1e-4 * max(1.0, np.abs(want).max())), e
+        err = np.max(np.abs(got[e] - want)) / max(1.0, np.abs(want).max())
+        worst = max(worst, err)
+        assert err < 1e-4, (e, err)
+    print(f"cfrc_ext: worst error {worst:.2e} of the largest body force")
     assert nonzero > B // 2
 
 
 def test_cg_solver_parity():
-    """The CG solver (MuJoCo's preconditioned nonlinear CG) against the oracle's."""
+    """The CG solver (MuJoCo's preconditioned nonlinear CG) against the FP64
+    optimum of the same constraint problem. CG's exit is not a fixed point:
+    MuJoCo's FP64 CG itself stops about 2e-4 (relative to the balanced
+    accelerations) short of the optimum at tolerance 1e-8, and FP32 and FP64
+    CG stop at different iterates. The bar is the cost: the kernel's qacc
+    reaches the optimum's cost to 2e-6 of the cost scale (FP32 resolution of
+    the summed cost), and its qacc is reported against the optimum's."""
     import ref64
     m = capi.Model(scenes.quadruped_xml(self_collision=True, solver="CG", iterations=50))
     B = 64
     q, qv, qa, ctrl, orc = states.quadruped_states(ref64.fields_from(m.field), B, seed=23)
+    f = ref64.fields_from(m.field)
+    f.update(solver=2, iterations=200, tolerance=1e-14)
+    opt = ref64.Oracle(f)
     sim = capi.BatchedSim(m, B)
     sim.set_state(q, qv)
     sim.set_qacc_warmstart(qa)
     dump = sim.debug_step(ctrl)
     st = sim.stats()
     assert (st["status"] & ~3 == 0).all(), st["status"]
-    worst = 0.0
+    worst = worst64 = worstc = 0.0
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], ctrl[e])
-        b = orc.get("qacc")
-        worst = max(worst, np.max(np.abs(dump.get(2, e, "qacc") - b)) / max(1.0, np.max(np.abs(b))))
-    print(f"CG worst qacc rel err {worst:.2e}, mean iterations {st['niter'].mean():.2f}")
-    assert worst < 5e-3  # CG's improvement test stops FP32 and FP64 at different iterates
+        a_cg64 = orc.get("qacc").copy()
+        states.oracle_step(opt, q[e], qv[e], qa[e], ctrl[e])
+        a_opt = opt.get("qacc")
+        qk = dump.get(2, e, "qacc").astype(np.float64)
+        sc = states.accel_scale(opt)
+        worst = max(worst, np.max(np.abs(qk - a_opt)) / sc)
+        worst64 = max(worst64, np.max(np.abs(a_cg64 - a_opt)) / sc)
+        c_opt = opt.cost(a_opt)[0]
+        worstc = max(worstc, (opt.cost(qk)[0] - c_opt) / states.cost_scale(opt, c_opt))
+    print(f"CG: worst qacc rel err to the optimum {worst:.2e} (FP64 CG's own {worst64:.2e}), "
+          f"cost gap {worstc:.2e}, mean iterations {st['niter'].mean():.2f}")
+    assert worstc < 2e-6

@@ -95,3 +95,49 @@ def test_argument_checks_and_domain_rand(tmp_path):
     plain = vkmujoco.create_batched(model, B)
     with pytest.raises(RuntimeError, match="domain_rand=True"):
         plain.set_env_params(friction_scale=np.ones(B, np.float32))
+
+
+def test_pack_columns_match_oracle(tmp_path):
+    """Every column of the packed readback after a one-substep step against the
+    FP64 oracle at the same state: qpos / qvel (the step's result), |contact
+    force| of the chosen bodies (the oracle's contacts and row forces, in the
+    cfrc_ext convention) and the sum of squares / of absolute values of
+    qfrc_actuator over the chosen dofs (the oracle's actuator forces)."""
+    import ref64
+    import states
+    path = tmp_path / "quadruped.xml"
+    path.write_text(scenes.quadruped_xml())
+    B = 64
+    model = vkmujoco.load_model(str(path))
+    sim = vkmujoco.create_batched(model, B)
+    cm = capi.Model(scenes.quadruped_xml())
+    q, qv, qa, ctrl, orc = states.quadruped_states(ref64.fields_from(cm.field), B, seed=29)
+    sim.set_state(q, qv)
+    assert capi.lib().mjmlx_batched_set_qacc_warmstart(sim.handle, capi._fp(qa)) == 0
+    feet = [model.name2id(1, f"{leg}_calf") for leg in ("FL", "FR", "RL", "RR")]
+    torso = model.name2id(1, "torso")
+    bodies = np.array([torso, *feet], np.int32)
+    lo, nd = 6, 12
+    stride = sim.set_pack_spec(bodies, lo, nd)
+    assert stride == model.nq + model.nv + len(bodies) + 2
+    sim.step(ctrl, 1)
+    buf = sim.read_pack()
+    qp, qvv = sim.state()
+    assert np.array_equal(buf[:, :model.nq], qp) and np.array_equal(buf[:, model.nq:model.nq + model.nv], qvv)
+    b1, b2 = cm.field("cpair_b1").astype(int), cm.field("cpair_b2").astype(int)
+    worst_f = worst_a = 0.0
+    touching = 0
+    for e in range(B):
+        states.oracle_step(orc, q[e], qv[e], qa[e], ctrl[e])
+        cf = states.oracle_cfrc_ext(orc, b1, b2, model.nbody)
+        fn = np.linalg.norm(cf[bodies, 3:], axis=1)
+        touching += int((fn > 0).any())
+        got = buf[e, model.nq + model.nv: model.nq + model.nv + len(bodies)]
+        worst_f = max(worst_f, np.max(np.abs(got - fn)) / max(1.0, np.abs(cf).max()))
+        qfa = orc.get("qfrc_actuator")[lo:lo + nd]
+        sq, ab = float(qfa @ qfa), float(np.abs(qfa).sum())
+        g_sq, g_ab = buf[e, -2], buf[e, -1]
+        worst_a = max(worst_a, abs(g_sq - sq) / max(1.0, sq), abs(g_ab - ab) / max(1.0, ab))
+    print(f"pack: |contact force| worst {worst_f:.2e}, actuator sums worst {worst_a:.2e}, envs touching {touching}")
+    assert touching > B // 2
+    assert worst_f < 1e-4 and worst_a < 1e-4
