@@ -8,6 +8,8 @@
 #ifndef MJMLX_HIP_H
 #define MJMLX_HIP_H
 
+#include <stddef.h>
+
 #include "mjmlx.h"
 
 #ifdef __cplusplus
@@ -157,6 +159,21 @@ MJMLX_API int mjmlx_batched_set_warmstart_mlp(MjmlxBatchedSim* sim, MjmlxMLP* ml
 MJMLX_API long mjmlx_batched_record(MjmlxBatchedSim* sim, MjmlxMLP* mlp, const float* ctrl, int nsub);
 /* raw features after the last substep [num_envs x feature_dim] (ctrl columns zero); returns the width or -1 */
 MJMLX_API int mjmlx_batched_get_features(const MjmlxBatchedSim* sim, float* out);
+
+/* Allocation-free step. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Steps like mjmlx_batched_step_n
+ * with the plain warm start (modes 0 and 1; the learned modes return -1), with
+ * the per-env workspaces in a fixed device array of the library (one per GPU,
+ * held until the process ends) instead of memory the library allocates, and
+ * calls no allocating routine. Limit: num_envs x workspace bytes (see
+ * mjmlx_batched_capacity) at most mjmlx_batched_noalloc_scratch_bytes()
+ * (64 MiB), else returns -1 with a message. Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (sims on different streams or
+ * host threads) race. The array needs no clearing: each call initialises the
+ * workspaces it uses from the sim's state. Returns 0, or -1 on error. */
+MJMLX_API int mjmlx_batched_step_n_noalloc(MjmlxBatchedSim* sim, const float* ctrl, int n);
+MJMLX_API size_t mjmlx_batched_noalloc_scratch_bytes(void);
 
 #ifdef __cplusplus
 }

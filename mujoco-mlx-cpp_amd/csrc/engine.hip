@@ -16,6 +16,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -675,6 +676,60 @@ MJMLX_API void mjmlx_batched_step_n(MjmlxBatchedSim* sim, const float* ctrl, int
 }
 
 MJMLX_API void mjmlx_batched_step(MjmlxBatchedSim* sim, const float* ctrl) { mjmlx_batched_step_n(sim, ctrl, 1); }
+
+// ------------------------------------------------------- allocation-free step
+// The per-env workspaces of mjmlx_batched_step_n_noalloc live in this
+// file-scope device array (one copy per GPU, held until the process ends)
+// instead of memory the library allocates. Host code never uses the array's
+// name as an address: each device's address comes from hipGetSymbolAddress,
+// taken once per device with that device current.
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) float scratch[kNoAllocScratchBytes / sizeof(float)];
+float* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+float* scratch_ptr() {
+  int dev = 0;
+  HIPOK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+MJMLX_API size_t mjmlx_batched_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+MJMLX_API int mjmlx_batched_step_n_noalloc(MjmlxBatchedSim* sim, const float* ctrl, int n) {
+  if (!sim || n <= 0) return -1;
+  try {
+    if (sim->ws_mode == 2 || sim->ws_mode == 3)
+      throw std::runtime_error("the learned warm-start modes are not available without allocation");
+    size_t need = (size_t)sim->B * sim->L.total * sizeof(float);
+    if (need > kNoAllocScratchBytes)
+      throw std::runtime_error("the workspaces of " + std::to_string(sim->B) + " envs need " + std::to_string(need) +
+                               " bytes, beyond the " + std::to_string(kNoAllocScratchBytes) + "-byte scratch");
+    float* ws = scratch_ptr();
+    upload_ctrl(sim, ctrl);
+    mjk::StepIO io = make_io(sim, nullptr, nullptr);
+    int var = sim->use_gws ? step_variant(sim) : mjk::VAR_GWS;
+    mjk::StepLaunch l{sim->dm, sim->L, sim->B, 0, var, 0, ws, sim->stream};
+    HIPOK(hipEventRecord(sim->ev0, sim->stream));
+    mjk::launch_step_np(sim->npset | (sim->model.nsensor > 0 ? 4 : 0), l, io, n);
+    HIPOK(hipGetLastError());
+    HIPOK(hipEventRecord(sim->ev1, sim->stream));
+    sim->timed = true;
+    return 0;
+  } catch (const std::exception& e) { report("mjmlx_batched_step_n_noalloc", e); return -1; }
+}
 
 MJMLX_API void mjmlx_batched_reset(MjmlxBatchedSim* sim, const int* mask) {
   if (!sim || !mask) return;
