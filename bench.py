@@ -43,34 +43,72 @@ def bytes_per_env_step(nq: int, nv: int, nu: int) -> int:
     return 4 * (2 * nq + 4 * nv + nu)
 
 
-def cpu_baseline(xml_fields, nenv=4096, warm=5, target_s=12.0):
-    """FP64 oracle on the host cores: env-steps/s on a bounded sample of about
-    target_s seconds (the step count is sized from a 5-step pilot)."""
+def cpu_baseline_leg(model_nosc_fields, model_fields, check):
+    """The CPU leg, where the FP64 oracle (oracle/ref64.c) runs, as baseline
+    and as checker:
+
+    * baseline: the CPU workload of the benchmark configurations as written
+      (the quadruped without self-collision, 64 envs, seed 0: nominal pose +
+      U(+-0.05), actions nominal + U(+-0.5); 1000 control steps of 4 substeps
+      after 25 warm-up steps; min(threads, 64 envs) threads spawned per call,
+      as reference src/mjb.cpp:826-838 does; median of per-step wall times, as
+      reference scripts/bench_physics_vs_others.py:55-65). Threads are capped
+      at 16, the host share of one GPU on the measurement machine.
+    * check: the FP64 oracle's qacc / efc_force against the kernel's on envs
+      sampled from the timed state (one substep from that state, outside the
+      timed region); relative to the balanced accelerations max(1, |qacc|,
+      |qacc_smooth|) and to the largest row force."""
     import ref64
     from mjmlx_hip import scenes
 
-    nthreads = max(1, min(16, os.cpu_count() or 1))
-    orc = ref64.Oracle(xml_fields)
+    nproc = os.cpu_count() or 1
+    B = 64
+    nthreads = max(1, min(16, nproc, B))
+    orc = ref64.Oracle(model_nosc_fields)
     rng = np.random.default_rng(0)
-    q = scenes.quadruped_initial_qpos(nenv, 0)
-    qv = np.zeros((nenv, orc.nv))
-    qa = np.zeros((nenv, orc.nv))
-    for _ in range(warm):
-        orc.batch_step(q, qv, qa, scenes.quadruped_actions(nenv, rng), 4, nthreads)
-    t0 = time.perf_counter()
-    for _ in range(5):
-        orc.batch_step(q, qv, qa, scenes.quadruped_actions(nenv, rng), 4, nthreads)
-    pilot = (time.perf_counter() - t0) / 5
-    steps = int(min(2000, max(20, target_s / max(pilot, 1e-6))))
-    warm += 5
-    ctrls = [np.ascontiguousarray(scenes.quadruped_actions(nenv, rng)) for _ in range(16)]
-    t0 = time.perf_counter()
-    for k in range(steps):
-        orc.batch_step(q, qv, qa, ctrls[k % 16], 4, nthreads)
-    dt = time.perf_counter() - t0
-    return {"value": nenv * steps / dt, "unit": "env-steps/s", "cores": nthreads, "kind": "port",
-            "sample": f"own FP64 restatement (oracle/ref64.c, not MuJoCo C): {nenv} envs x {steps} control steps "
-                      f"(4 substeps each) after {warm} warm-up steps, {nthreads} threads, {dt:.2f} s"}
+    q = scenes.quadruped_initial_qpos(B, 0)
+    qv = np.zeros((B, orc.nv))
+    qa = np.zeros((B, orc.nv))
+    for _ in range(25):
+        orc.batch_step(q, qv, qa, scenes.quadruped_actions(B, rng), 4, nthreads)
+    ctrls = [np.ascontiguousarray(scenes.quadruped_actions(B, rng)) for _ in range(1000)]
+    per = np.zeros(1000)
+    t_all = time.perf_counter()
+    for k in range(1000):
+        t0 = time.perf_counter()
+        orc.batch_step(q, qv, qa, ctrls[k], 4, nthreads)
+        per[k] = time.perf_counter() - t0
+    t_all = time.perf_counter() - t_all
+    med = float(np.median(per))
+    base = {"value": B / med, "unit": "env-steps/s", "cores": nthreads, "nproc": nproc, "kind": "port",
+            "mean_value": B * 1000 / t_all,
+            "sample": f"own FP64 restatement (oracle/ref64.c; MuJoCo C is not available): quadruped without "
+                      f"self-collision, {B} envs x 1000 control steps (4 substeps each, Newton 16 it, tol 1e-8) "
+                      f"after 25 warm-up steps, {nthreads} threads per call, median step {med * 1e3:.3f} ms, "
+                      f"{t_all:.2f} s in all"}
+    chk = None
+    if check is not None:
+        sim, q32, v32, a32, c32 = check
+        dump = sim.debug_step(c32)
+        st = sim.stats()
+        o = ref64.Oracle(model_fields)
+        wq = wf = 0.0
+        ncon = 0
+        for e in range(q32.shape[0]):
+            o.set_state(q32[e].astype(np.float64), v32[e].astype(np.float64), a32[e].astype(np.float64),
+                        c32[e].astype(np.float64))
+            o.forward()
+            b = o.get("qacc")
+            sc = max(1.0, float(np.max(np.abs(b))), float(np.max(np.abs(o.get("qacc_smooth")))))
+            wq = max(wq, float(np.max(np.abs(dump.get(2, e, "qacc") - b))) / sc)
+            n = o.nefc()
+            if n and int(st["nefc"][e]) == n:
+                f = o.get("efc_force")
+                wf = max(wf, float(np.max(np.abs(dump.get(2, e, "efc_force")[:n] - f))) / max(1.0, float(np.max(np.abs(f)))))
+            ncon += len(o.contacts())
+        chk = {"envs": int(q32.shape[0]), "contacts": ncon, "max_qacc_rel_err": wq, "max_efc_force_rel_err": wf,
+               "against": "FP64 oracle (oracle/ref64.c), one substep from the timed state"}
+    return base, chk
 
 
 def timed_run(sim, pool, warmup, steps, offset=0):
@@ -90,7 +128,7 @@ def timed_run(sim, pool, warmup, steps, offset=0):
     return float(np.mean(kernel_ms)), dt, float(np.mean(iters))
 
 
-def run_boxes(args):
+def run_boxes(B, warmup, steps):
     """Uneven-contact batching: every env is a plane plus N in U{4..32} free
     boxes (half-sizes U(0.02, 0.1), densities U(500, 2000), friction
     U(0.3, 1.0), drop heights U(0.1, 1.0) above the box's bounding radius,
@@ -102,7 +140,6 @@ def run_boxes(args):
 
     if capi.lib().mjmlx_hip_device_count() < 1:
         raise SystemExit("bench: no HIP device visible")
-    B = args.envs
     rng = np.random.default_rng(3)
     ns = rng.integers(4, 33, size=B)
     groups = []
@@ -126,7 +163,7 @@ def run_boxes(args):
     # the boxes land within the warm-up; the autotuner then picks each group's
     # kernel variant (workspace in LDS or device memory, register budget)
     # from its landed state
-    for k in range(args.warmup):
+    for k in range(warmup):
         for _, _, _, sim in groups:
             sim.step(None, 4)
     placement = {}
@@ -139,7 +176,7 @@ def run_boxes(args):
     for _, _, _, sim in groups:
         sim.sync()
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         for _, _, _, sim in groups:
             sim.step(None, 4)
     for _, _, _, sim in groups:
@@ -158,25 +195,31 @@ def run_boxes(args):
                     "lds_bytes_per_env": sim.capacity()[2], "mean_ncon": float(st["ncon"].mean()),
                     "mean_nefc": float(st["nefc"].mean()), "row_budget_overflow_envs": int(((st["status"] & 2) > 0).sum()),
                     "mean_iters": float(st["niter"].mean())})
-    value = B * args.steps / wall
+    value = B * steps / wall
     bpe = float(np.mean([bytes_per_env_step(7 * n, 6 * n, 0) for n in ns]))
+    sum_ms = float(sum(g["ms_per_step_alone"] for g in per))
+    traffic = _traffic("boxes", B)
     out = {
         "metric": "physics env-steps/s @16384 envs, plane + U{4..32} boxes",
-        "value": value, "unit": "env-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": value, "unit": "env-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "plane + N in U{4..32} free boxes, condim 3 pyramidal, CG 16 it, tol 1e-8, "
-                               "dt 0.005 x 4 substeps; reference capacities (128 contacts, 72/256 rows)",
+                               "dt 0.005 x 4 substeps; no row budget (up to 512 contacts per env, all with rows)",
                    "num_envs": B, "substeps_per_env_step": 4, "groups": len(groups),
                    "parallelism": "single GPU, 1 wavefront per env, one sim (stream) per N",
                    "workspace": "LDS or device memory per group, picked by the autotuner (mjmlx_batched_autotune)"},
-        "roofline": {"bound": "hbm", "achieved": B * bpe / (wall / args.steps) / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": B * bpe / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_step", "bytes_per_env_step": bpe},
+        # the groups' launches overlap on their streams: the control step's
+        # time is the wall time per step of the queued groups
+        "roofline": {"bound": "hbm", "achieved": B * bpe / (wall / steps) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": B * bpe / (wall / steps) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": traffic, "traffic_unit": "HBM bytes per control step, all groups",
+                     "kernel": "k_step (one launch per group)", "kernel_ms_sum_alone": sum_ms,
+                     "bytes_per_env_step": bpe},
         "groups": per,
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 STAGES = [("kin", "kinematics"), ("compos", "subtree com"), ("crb", "CRB"), ("factor", "LDL of qM"),
@@ -262,6 +305,40 @@ def scene_setup(name, B, rng):
     return model, pool, q0
 
 
+def _traffic(scene, B):
+    """Calibrated HBM bytes per launch (per control step for the boxes'
+    groups) from the committed PMC summary of that workload, or None."""
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{scene}{B}.txt")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def _roofline(B, bpe, ms, traffic, kernel):
+    achieved = B * bpe / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel, "kernel_ms_avg": ms,
+            "bytes_per_env_step": bpe}
+
+
+def run_plain_scene(scene, B, warmup, steps):
+    """Plain warm start on one scene: HIP-event kernel time per control step
+    after `warmup` steps, with its roofline (sub-result of the default line)."""
+    from mjmlx_hip import capi
+    rng = np.random.default_rng(1)
+    model, pool, q0 = scene_setup(scene, B, rng)
+    sim = capi.BatchedSim(model, B)
+    sim.set_state(q0, np.zeros((B, model.nv), np.float32))
+    ms, wall, it = timed_run(sim, pool, warmup, steps)
+    st = sim.stats()
+    bpe = bytes_per_env_step(model.nq, model.nv, model.nu)
+    return {"workload": SCENES[scene], "num_envs": B, "warmup": warmup, "steps": steps,
+            "env_steps_per_s": B / ms * 1e3, "ms_per_step": ms, "wall_env_steps_per_s": B * steps / wall,
+            "mean_solver_iters_per_env_step": it, "overflow_envs": int(((st["status"] & 3) > 0).sum()),
+            "roofline": _roofline(B, bpe, ms, _traffic(scene, B), "k_step (fused 4-substep step)")}
+
+
 def run(args):
     from mjmlx_hip import capi
 
@@ -296,6 +373,8 @@ def run(args):
     q, qv = src.state()
     qa = src.qacc_warmstart()
     results = {}
+    check = None
+    preroll = None
     for mode in ("plain", "learned", "learned_first"):
         if mode != "plain" and learned is None:
             continue
@@ -311,11 +390,35 @@ def run(args):
         sim.set_state(q, qv)
         sim.set_qacc_warmstart(qa)
         ms, wall, it = timed_run(sim, pool, 5, args.steps, offset=3)
+        st = sim.stats()
         results[mode] = {"env_steps_per_s": B / ms * 1e3, "ms_per_step": ms,
                          "wall_env_steps_per_s": B * args.steps / wall,
-                         "mean_solver_iters_per_env_step": it}
+                         "mean_solver_iters_per_env_step": it,
+                         "overflow_envs": int(((st["status"] & 3) > 0).sum())}
+        if mode == "plain":
+            # envs sampled from the timed state for the oracle check (run in the CPU leg)
+            idx = np.linspace(0, B - 1, 64).astype(int)
+            qs, vs = sim.state()
+            as_ = sim.qacc_warmstart()
+            cs = pool[(3 + 5 + args.steps) % len(pool)]
+            csim = capi.BatchedSim(model, len(idx))
+            q32, v32, a32, c32 = (np.ascontiguousarray(x[idx]) for x in (qs, vs, as_, cs))
+            csim.set_state(q32, v32)
+            csim.set_qacc_warmstart(a32)
+            check = (csim, q32, v32, a32, c32)
+            # a second window after a 200-step untimed pre-roll (the robots fall and settle)
+            nroll = 200
+            for k in range(nroll):
+                sim.step(pool[k % len(pool)], 4)
+            ms2, wall2, it2 = timed_run(sim, pool, 0, args.steps, offset=7)
+            st2 = sim.stats()
+            preroll = {"preroll_steps": 5 + args.steps + nroll, "env_steps_per_s": B / ms2 * 1e3,
+                       "ms_per_step": ms2, "wall_env_steps_per_s": B * args.steps / wall2,
+                       "mean_solver_iters_per_env_step": it2,
+                       "overflow_envs": int(((st2["status"] & 3) > 0).sum())}
         del sim
     plain = results["plain"]
+    plain["after_preroll"] = preroll
     if learned is not None:
         learned.update(results["learned"])
         learned["speedup_vs_plain"] = learned["env_steps_per_s"] / plain["env_steps_per_s"]
@@ -329,13 +432,6 @@ def run(args):
     best = max(results, key=lambda k: results[k]["env_steps_per_s"])
     head = results[best]
     bpe = bytes_per_env_step(model.nq, model.nv, model.nu)
-    # roofline of the fused step kernel (plain path: one k_step launch per control step)
-    achieved = B * bpe / (plain["ms_per_step"] / 1e3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.scene}{B}.txt")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
     out = {
         "metric": "physics env-steps/s @16384 envs, plain vs learned warm-start at 1e-8 residual",
         "value": head["env_steps_per_s"],
@@ -354,16 +450,21 @@ def run(args):
                    "warm_start": best, "launch": "one workgroup per env"},
         "plain": plain,
         "learned": learned,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_step (fused 4-substep step; k_step_occ, its 168-VGPR variant, for small workspaces)",
-                     "kernel_ms_avg": plain["ms_per_step"],
-                     "bytes_per_env_step": bpe},
+        # roofline of the fused step kernel (plain path: one k_step launch per control step)
+        "roofline": _roofline(B, bpe, plain["ms_per_step"], _traffic(args.scene, B),
+                              "k_step (fused 4-substep step; k_step_occ, its 168-VGPR variant, for small workspaces)"),
         "cpu_baseline": None,
+        "oracle_check": None,
     }
     if not args.no_cpu and args.scene == "quadruped":
         import ref64
-        out["cpu_baseline"] = cpu_baseline(ref64.fields_from(model.field))
+        from mjmlx_hip import scenes
+        nosc = capi.Model(scenes.quadruped_xml(self_collision=False))
+        out["cpu_baseline"], out["oracle_check"] = cpu_baseline_leg(ref64.fields_from(nosc.field),
+                                                                    ref64.fields_from(model.field), check)
+    if args.scene == "quadruped" and not args.headline_only:
+        out["humanoid"] = run_plain_scene("humanoid", B, 25, 20)
+        out["boxes"] = run_boxes(B, 30, 10)
     print(json.dumps(out), flush=True)
 
 
@@ -378,6 +479,8 @@ def main():
                     help="quadruped is the headline workload; humanoid is the high-DoF elliptic-cone one; "
                          "boxes the uneven-contact one")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the humanoid and boxes sub-results of the default line")
     ap.add_argument("--pairs", type=int, default=2_000_000, help="training pairs for the learned warm start")
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--stages", action="store_true", help="per-stage cycle table from the profiling build")
@@ -394,7 +497,7 @@ def main():
     if args.stages:
         run_stages(args)
     elif args.scene == "boxes":
-        run_boxes(args)
+        print(json.dumps(run_boxes(args.envs, args.warmup, args.steps)), flush=True)
     else:
         run(args)
 

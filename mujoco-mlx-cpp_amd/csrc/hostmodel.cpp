@@ -85,6 +85,37 @@ int max_rows(const Model& m, const std::vector<CollisionPair>& pairs) {
 Caps default_caps(const Model& m, const std::vector<CollisionPair>& pairs, int maxcon_override,
                   int efc_cap_override) {
   Caps c;
+  bool budget_env = false;
+  if (const char* v = std::getenv("MJMLX_MAX_EFC")) budget_env = std::atoi(v) > 0;
+  bool ref_caps = false;
+  if (const char* v = std::getenv("MJMLX_HIP_REF_CAPS")) ref_caps = std::atoi(v) != 0;
+  int nfree = 0;
+  for (int j = 0; j < m.njnt; j++) nfree += m.jnt_type[j] == mjk::JFREE;
+  if ((m.nv > 32 || nfree > 1) && efc_cap_override <= 0 && !budget_env && !ref_caps) {
+    // Models of more than 32 dofs or of several free bodies (contacts grow
+    // with the bodies): MuJoCo's semantics, no row budget. Every stored
+    // contact gets its rows; up to min(512, the pairs' contact maxima)
+    // contacts are stored and counted. (The reference caps these at 128
+    // contacts and a 72 / 256-row budget, reference src/batched.cpp:290,
+    // src/compat/mkx_kernels/solver.hpp:29-33, which leaves contacts of
+    // landed box piles without force.)
+    int mc = max_contacts(m, pairs);
+    c.con_cap = maxcon_override > 0 ? maxcon_override : std::max(1, std::min(mc, 512));
+    c.maxcon = std::max(1, std::min(mc, c.con_cap));
+    int rmax = 1, fixed = 0;
+    for (auto& p : pairs) {
+      int per = p.condim == 1 ? 1 : (m.cone == CONE_PYRAMIDAL ? 2 * (p.condim - 1) : p.condim);
+      rmax = std::max(rmax, per);
+    }
+    for (int k = 0; k < m.nv; k++) if (m.dof_frictionloss[k] > 0) fixed++;
+    for (int j = 0; j < m.njnt; j++) if (m.jnt_limited[j]) fixed += 2;
+    c.maxefc = std::max(1, std::min(max_rows(m, pairs), fixed + rmax * c.maxcon));
+    c.efc_cap = c.maxefc + 10;  // no admission test binds before the buffer ends
+    return c;
+  }
+  // The reference's capacities (reference src/batched.cpp:290 and
+  // src/compat/mkx_kernels/solver.hpp:29-33,299,349,403): 128 contacts counted,
+  // a row budget of 72 rows for nv <= 32 and 256 beyond
   int con_cap = maxcon_override > 0 ? maxcon_override : 128;
   c.efc_cap = efc_cap_override > 0 ? efc_cap_override : (m.nv <= 32 ? 72 : 256);
   if (const char* v = std::getenv("MJMLX_MAX_EFC")) if (efc_cap_override <= 0 && std::atoi(v) > 0) c.efc_cap = std::atoi(v);

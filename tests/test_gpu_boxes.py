@@ -12,47 +12,54 @@ from mjmlx_hip import capi, scenes
 pytestmark = pytest.mark.gpu
 
 
+def _caps(model):
+    """The engine's capacities for a scene of several free bodies: no row
+    budget, min(512, the pairs' contact maxima) contacts."""
+    sim = capi.BatchedSim(model, 1)
+    maxcon = sim.capacity()[0]
+    del sim
+    return dict(con_cap=maxcon, efc_cap=0)
+
+
 def _settled_states(model, B, seed, nbox):
-    orc = ref64.Oracle(ref64.fields_from(model.field)).set_caps()
+    orc = ref64.Oracle(ref64.fields_from(model.field)).set_caps(**_caps(model))
     rng = np.random.default_rng(seed)
     q = np.tile(model.field("qpos0"), (B, 1))
     qv = np.zeros((B, model.nv))
     qa = np.zeros((B, model.nv))
-    # 32 boxes are taken around the step where the landed boxes' contacts
-    # outgrow the row budget (later, boxes whose contacts get no rows sink
-    # through the floor)
-    lens, nsub = (rng.integers(20, 120, size=B), 4) if nbox <= 14 else (rng.integers(95, 125, size=B), 1)
+    lens, nsub = (rng.integers(20, 120, size=B), 4) if nbox <= 14 else (rng.integers(30, 50, size=B), 4)
     for k in range(int(lens.max())):
         idx = np.nonzero(lens > k)[0]
         qs, vs, as_ = q[idx].copy(), qv[idx].copy(), qa[idx].copy()
-        orc.batch_step(qs, vs, as_, np.zeros((len(idx), 0)), nsub)
+        orc.batch_step(qs, vs, as_, np.zeros((len(idx), 0)), nsub, 8)
         q[idx], qv[idx], qa[idx] = qs, vs, as_
     assert np.isfinite(qv).all()
     f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
     return f32(q), f32(qv), f32(qa), orc
 
 
-def _newton_optimum(getter):
-    f = ref64.fields_from(getter)
+def _newton_optimum(model):
+    f = ref64.fields_from(model.field)
     f.update(solver=2, iterations=200, tolerance=1e-14)
-    return ref64.Oracle(f).set_caps()
+    return ref64.Oracle(f).set_caps(**_caps(model))
 
 
 @pytest.mark.parametrize("nbox,solver", [(6, "Newton"), (6, "CG"), (14, "Newton"), (32, "CG")])
 def test_box_scene_single_step_parity(nbox, solver):
-    """6 boxes run from LDS; 14 (nv 84) and 32 (nv 192, row budget overflowing)
-    from the device-memory workspace. The oracle applies the same capacities."""
+    """6 boxes run from LDS; 14 (nv 84) and 32 (nv 192, landed: ~470 rows,
+    beyond the reference's 256-row budget) from the device-memory workspace.
+    No row budget: every contact gets its rows, here and in the oracle."""
     model = capi.Model(scenes.boxes_xml(nbox, 3, solver=solver, iterations=50 if solver == "Newton" else 16))
     B = 32 if nbox <= 14 else 16
     q, qv, qa, orc = _settled_states(model, B, 4, nbox)
-    orc_opt = _newton_optimum(model.field)
+    orc_opt = _newton_optimum(model)
     worstp = 0.0
     sim = capi.BatchedSim(model, B)
     sim.set_state(q, qv)
     sim.set_qacc_warmstart(qa)
     dump = sim.debug_step(None)
     st = sim.stats()
-    assert not (st["status"] & ~3).any(), st["status"]
+    assert not st["status"].any(), st["status"]  # no overflow either
     ncon = 0
     worst = worstf = worsts = 0.0
     for e in range(B):
