@@ -29,6 +29,8 @@ struct EmuSim {
   mjh::Caps caps;
   int nrand = 0;
   std::vector<int> niter, status, ncon, nefc;
+  int poison_mode = 0;
+  unsigned poison = 0;
 };
 }  // namespace
 
@@ -122,11 +124,28 @@ __attribute__((visibility("default"))) int mjemu_step(void* p, const float* ctrl
   }
   int bad = 0;
   for (int e = 0; e < s->B; e++) {
-    std::fill(s->ws.begin(), s->ws.end(), 0.f);
+    // poison 0: zeroed workspace per env; 1: the bit pattern s->poison per
+    // env; 2: the previous env's workspace left as it is (one workspace
+    // serving successive envs, as a persistent launch does)
+    if (s->poison_mode == 0) std::fill(s->ws.begin(), s->ws.end(), 0.f);
+    else if (s->poison_mode == 1 || e == 0) {
+      float f;
+      std::memcpy(&f, &s->poison, sizeof(f));
+      std::fill(s->ws.begin(), s->ws.end(), f);
+    }
     mjk::run_env<6>(s->dm, s->L, s->ws.data(), io, e, nsub);
     if (s->status[e]) bad++;
   }
   return bad;
+}
+
+// workspace initialisation before each env (0 zeros, 1 the given bit
+// pattern, 2 the pattern once, then each env reuses the previous env's
+// workspace): results must not depend on it
+__attribute__((visibility("default"))) void mjemu_set_poison(void* p, int mode, unsigned pattern) {
+  auto* s = static_cast<EmuSim*>(p);
+  s->poison_mode = mode;
+  s->poison = pattern;
 }
 
 // learned warm-start features written by the last step, float[B * nfeat]; returns nfeat

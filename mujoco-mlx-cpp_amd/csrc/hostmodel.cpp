@@ -140,7 +140,7 @@ Caps default_caps(const Model& m, const std::vector<CollisionPair>& pairs, int m
 
 Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   using namespace mjk;
-  if (m.nv > 256) throw std::runtime_error("HIP engine: models with nv > 256 are not supported by the dense-row kernel");
+  if (m.nv > 512) throw std::runtime_error("HIP engine: models with nv > 512 are not supported by the dense-row kernel");
   if (m.solver == SOL_PGS) throw std::runtime_error("HIP engine: the PGS solver is not implemented (use Newton or CG)");
   if (m.integrator == INT_RK4) throw std::runtime_error("HIP engine: the RK4 integrator is not implemented");
   std::vector<CollisionPair> pairs = build_pairs(m);

@@ -1744,41 +1744,44 @@ KDEV float jrow_dot(const DModel& m, const Layout& L, const float* w, int r, con
   return acc;
 }
 
-// y = J^T f. GPU beyond 64 dofs: lane l keeps dofs l, l + 64, .. in registers
-// and streams the rows with nonzero force, coalesced across lanes.
+// y = J^T f. GPU beyond 64 dofs: lane l keeps dofs base + l, base + l + 64,
+// .. (four per lane, 256-dof chunks) in registers and streams the rows,
+// coalesced across lanes.
 KDEV void mul_jt(const DModel& m, const Layout& L, const float* w, const float* f, int nefc, float* y) {
   int nv = m.nv;
   const float* J = w + L.efc_J;
 #if WAVE == 64
   if (nv > WAVE) {
-    // eight rows per step: up to 32 independent loads in flight per lane
-    float g[4] = {0.f, 0.f, 0.f, 0.f};
-    int r = 0;
-    for (; r + 8 <= nefc; r += 8) {
-      float fr[8];
+    for (int base = 0; base < nv; base += 4 * WAVE) {
+      // eight rows per step: up to 32 independent loads in flight per lane
+      float g[4] = {0.f, 0.f, 0.f, 0.f};
+      int r = 0;
+      for (; r + 8 <= nefc; r += 8) {
+        float fr[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) fr[u] = f[r + u];
+        for (int u = 0; u < 8; u++) fr[u] = f[r + u];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        int i = LANE + q * WAVE;
-        if (i < nv) {
-          float a[8];
+        for (int q = 0; q < 4; q++) {
+          int i = base + LANE + q * WAVE;
+          if (i < nv) {
+            float a[8];
 #pragma unroll
-          for (int u = 0; u < 8; u++) a[u] = J[(r + u) * nv + i];
+            for (int u = 0; u < 8; u++) a[u] = J[(r + u) * nv + i];
 #pragma unroll
-          for (int u = 0; u < 8; u++) g[q] += a[u] * fr[u];
+            for (int u = 0; u < 8; u++) g[q] += a[u] * fr[u];
+          }
         }
       }
-    }
-    for (; r < nefc; r++) {
-      float fr = f[r];
+      for (; r < nefc; r++) {
+        float fr = f[r];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (base + LANE + q * WAVE < nv) g[q] += J[r * nv + base + LANE + q * WAVE] * fr;
+      }
 #pragma unroll
       for (int q = 0; q < 4; q++)
-        if (LANE + q * WAVE < nv) g[q] += J[r * nv + LANE + q * WAVE] * fr;
+        if (base + LANE + q * WAVE < nv) y[base + LANE + q * WAVE] = g[q];
     }
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-      if (LANE + q * WAVE < nv) y[LANE + q * WAVE] = g[q];
     return;
   }
 #endif

@@ -45,6 +45,7 @@ def lib():
         L.mjemu_rays.argtypes = [C.c_void_p, _F, C.c_int, _I, _F, _I]
         L.mjemu_sensordata.argtypes = [C.c_void_p, _F]
         L.mjemu_narrow.argtypes = [C.c_int, C.c_int, _F, _F, _F, _F, _F, _F, C.c_float, _F]
+        L.mjemu_set_poison.argtypes = [C.c_void_p, C.c_int, C.c_uint]
         L.mjemu_compile.restype = C.c_void_p
         L.mjemu_compile.argtypes = [C.c_char_p, C.c_int, C.c_int]
         _lib = L
@@ -99,6 +100,12 @@ class EmuSim:
     def step(self, ctrl=None, nsub: int = 1, dump: bool = False) -> int:
         c = None if ctrl is None else np.ascontiguousarray(ctrl, dtype=np.float32)
         return lib().mjemu_step(self.h, _fp(c), nsub, int(dump))
+
+    def set_poison(self, mode: int, pattern: int = 0):
+        """Workspace before each env: 0 zeroed, 1 filled with the 32-bit
+        pattern, 2 the pattern once and then each env reuses the previous
+        env's workspace (as one persistent workgroup serving successive envs)."""
+        lib().mjemu_set_poison(self.h, int(mode), int(pattern) & 0xFFFFFFFF)
 
     def stats(self):
         out = [np.zeros(self.B, np.int32) for _ in range(4)]

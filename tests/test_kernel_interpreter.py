@@ -116,3 +116,41 @@ def test_quadruped_step_integrators(integrator):
         worst_v = max(worst_v, np.max(np.abs(vk[e] - vo)) / max(1.0, np.max(np.abs(vo))))
         worst_q = max(worst_q, np.max(np.abs(qk[e] - qo)))
     assert worst_v < 1e-4 and worst_q < 1e-5, (worst_v, worst_q)
+
+
+@pytest.mark.parametrize("scene", ["quadruped", "humanoid", "boxes6", "boxes14_cg", "convex"])
+def test_results_independent_of_workspace_contents(scene):
+    """No workspace word is read before the step writes it: the kernel source
+    gives bit-identical states, statistics and sensors whether each env's
+    workspace starts zeroed, filled with NaN bit patterns (positive and
+    negative), with 1.0f, or left as the previous env wrote it (one workspace
+    serving successive envs, as a persistent launch would)."""
+    from test_convex_collision import DROP
+    xml, nu = {"quadruped": (scenes.quadruped_xml(sensors=True), 12), "humanoid": (scenes.humanoid_xml(), 21),
+               "boxes6": (scenes.boxes_xml(6, 3, solver="Newton"), 0), "boxes14_cg": (scenes.boxes_xml(14, 3), 0),
+               "convex": (DROP, 0)}[scene]
+    res = []
+    for mode, pat in [(0, 0), (1, 0x7FC00000), (1, 0xFFC00000), (1, 0x3F800000), (2, 0x7FC00000)]:
+        B = 6
+        sim = emu.EmuSim(xml, B)
+        rng = np.random.default_rng(0)
+        if scene == "quadruped":
+            q = scenes.quadruped_initial_qpos(B, 0).astype(np.float32)
+            q[:, 2] -= 0.03
+        else:
+            q = np.tile(sim.field("qpos0"), (B, 1)).astype(np.float32)
+        sim.set_state(q, np.zeros((B, sim.nv), np.float32), np.zeros((B, sim.nv), np.float32))
+        sim.set_poison(mode, pat)
+        outs = []
+        for k in range(25):
+            c = rng.uniform(-0.4, 0.4, (B, nu)).astype(np.float32) if nu else None
+            sim.step(c, 2)
+            outs.append((*sim.get_state(), sim.sensordata()))
+        res.append((outs, sim.stats()))
+    for outs, st in res[1:]:
+        for a, b in zip(res[0][0], outs):
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+        for k in st:
+            assert np.array_equal(st[k], res[0][1][k])
+    assert not res[0][1]["status"].any()
