@@ -121,6 +121,7 @@ struct MjmlxBatchedSim {
   int persistent = 0, grid = 0, ncu = 0;
   int npset = 0;  // narrow-phase set compiled into the step kernel
   float* d_gws = nullptr;  // per-env workspaces in device memory when they exceed the LDS of a workgroup
+  bool ctrl_zero = true;   // d_ctrl holds zeros (set at creation; cleared by an upload)
   bool use_gws = false;    // launch the device-memory workspace variant
   bool lds_ok = false;     // the workspace fits the LDS of a workgroup
   bool high_occ = false;   // LDS admits more than 8 envs per CU: the 168-VGPR kernel
@@ -231,9 +232,11 @@ void upload_ctrl(MjmlxBatchedSim* s, const float* ctrl) {
     HIPOK(hipMemcpyAsync(s->d_ctrl, s->h_ctrl, (size_t)s->B * s->nu * sizeof(float), hipMemcpyHostToDevice, s->stream));
     HIPOK(hipEventRecord(s->ev_ctrl, s->stream));
     s->ctrl_pending = true;
-  } else {
+  } else if (s->nu > 0 && !s->ctrl_zero) {
+    // no ctrl: zero it once; later NULL calls find it zero already
     HIPOK(hipMemsetAsync(s->d_ctrl, 0, n * sizeof(float), s->stream));
   }
+  s->ctrl_zero = !(ctrl && s->nu > 0);
 }
 
 void launch(MjmlxBatchedSim* s, int nsub, float* dump1, float* dump2) {

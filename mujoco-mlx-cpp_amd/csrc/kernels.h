@@ -2280,11 +2280,71 @@ KDEV LsPt ls_eval(const DModel& m, const Layout& L, float* w, int nefc, float al
 // its terms' magnitudes, below which further steps only chase rounding)
 KDEV bool ls_flat(const LsPt& p, float gtol) { return fabsf(p.d1) <= fmaxf(gtol, 1e-6f * p.d1n); }
 
+// Decrease of the constraint problem's cost from alpha = 0 to alpha along the
+// search direction, summed from per-row changes (a quadratic row active at
+// both points changes by 1/2 D alpha Jv (x0 + x)) and the Gauss term's
+// alpha g1 + 1/2 alpha^2 g2: MuJoCo's "improvement" (old cost - new cost)
+// without the cancellation of two large FP32 sums, which near the optimum is
+// the size of the improvement itself.
+KDEV float ls_decrease(const DModel& m, const Layout& L, const float* w, int nefc, float alpha, const float* gq) {
+  const float *Jaref = w + L.efc_Jaref, *Jv = w + L.efc_Jv, *D = w + L.efc_D;
+  const int* etype = iptr(w, L.efc_type);
+  float df = 0.f;
+  for (int r = LANE; r < nefc; r += WAVE) {
+    int ty = etype[r];
+    if (ty == EFC_CONTACT_ELLIPTIC) continue;
+    float x0 = Jaref[r], jv = Jv[r], d = D[r], x = x0 + alpha * jv;
+    if (ty == EFC_FRICTION_DOF) {
+      float fl = m.dof_frictionloss[iptr(w, L.efc_id)[r]], R = 1.f / d;
+      int z = x <= -R * fl ? 0 : (x >= R * fl ? 2 : 1);
+      int z0 = x0 <= -R * fl ? 0 : (x0 >= R * fl ? 2 : 1);
+      if (z == z0) df += z == 1 ? 0.5f * d * alpha * jv * (x0 + x) : (z == 0 ? -fl : fl) * alpha * jv;
+      else {
+        float c = z == 1 ? 0.5f * d * x * x : (z == 0 ? -fl * x : fl * x) - 0.5f * R * fl * fl;
+        float c0 = z0 == 1 ? 0.5f * d * x0 * x0 : (z0 == 0 ? -fl * x0 : fl * x0) - 0.5f * R * fl * fl;
+        df += c - c0;
+      }
+    } else if (ty == EFC_EQUALITY || (x < 0.f && x0 < 0.f)) {
+      df += 0.5f * d * alpha * jv * (x0 + x);
+    } else if (x < 0.f) {
+      df += 0.5f * d * x * x;
+    } else if (x0 < 0.f) {
+      df -= 0.5f * d * x0 * x0;
+    }
+  }
+  if (m.cone != 0) {
+    const int* misc = reinterpret_cast<const int*>(w + L.misc);
+    int ncon = misc[MISC_NCON];
+    const float* con = w + L.con;
+    for (int q = LANE; q < ncon; q += WAVE) {
+      const float* cr = con + CON_STRIDE * q;
+      int dim = reinterpret_cast<const int*>(cr + CON_NROW)[0], r0 = reinterpret_cast<const int*>(cr + CON_ROW0)[0];
+      if (dim < 2 || r0 + dim > nefc) continue;
+      const float* f5 = m.pair_friction + 5 * reinterpret_cast<const int*>(cr + CON_PAIR)[0];
+      float sc = fric_scale(m, L, w), c, c0, g1, g2;
+      k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, sc, dim, alpha, &c, &g1, &g2);
+      k_ellip_line(Jaref + r0, Jv + r0, D + r0, f5, sc, dim, 0.f, &c0, &g1, &g2);
+      df += c - c0;
+    }
+  }
+  return -(wsum(df) + alpha * gq[1] + 0.5f * alpha * alpha * gq[2]);
+}
+
 // Exact line search on the convex piecewise-quadratic cost (same algorithm as
 // the oracle): Newton steps until the derivative changes sign, then Newton
 // from the left end / secant on the bracket. A Newton step whose end point is
 // in the same quadratic piece as its start is the exact minimiser.
-KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int* exact) {
+KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* gq_out);
+
+// dec (when not null): the cost decrease at the returned step, by ls_decrease
+KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* dec) {
+  float gq[3];
+  float a = linesearch_(m, L, w, nefc, exact, gq);
+  if (dec) *dec = a > 0.f ? ls_decrease(m, L, w, nefc, a, gq) : 0.f;
+  return a;
+}
+
+KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* gq_out) {
   int nv = m.nv;
   const float *s = w + L.search, *M = w + L.qM;
   float *Mv = w + L.Mv, *Jv = w + L.efc_Jv;
@@ -2305,6 +2365,7 @@ KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int*
     g2 += s[i] * Mv[i];
   }
   float gq[3] = {wsum(g0), wsum(g1), wsum(g2)};
+  gq_out[0] = gq[0]; gq_out[1] = gq[1]; gq_out[2] = gq[2];
   *exact = 0;
   LsRows rows;
   ls_rows(L, w, nefc, &rows);
@@ -2399,7 +2460,8 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
   for (it = 0; it < m.iterations; it++) {
     int exact = 0;
     KPROF_MARK(pa, KP_DIR);
-    float alpha = linesearch(m, L, w, nefc, &exact);
+    float dec = 0.f;
+    float alpha = linesearch(m, L, w, nefc, &exact, newton ? nullptr : &dec);
     KPROF_MARK(pa, KP_LS);
     *niter_out = it + 1;
     if (alpha == 0.f) break;
@@ -2417,7 +2479,9 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     KPROF_MARK(pa, KP_COSTGRAD);
     float gp = 0.f;
     for (int i = LANE; i < nv; i += WAVE) gp += grad[i] * grad[i];
-    float improvement = scale * (oldcost - cost), gradient = scale * sqrtf(wsum(gp));
+    // CG: the decrease summed per row (ls_decrease); Newton ends on an exact
+    // quadratic piece or on the difference of the two costs
+    float improvement = scale * (newton ? oldcost - cost : dec), gradient = scale * sqrtf(wsum(gp));
     if (improvement < m.tolerance || gradient < m.tolerance) break;
     // a Newton step that stayed inside one quadratic piece (no row changed
     // zone) landed on that piece's stationary point: the exact minimiser

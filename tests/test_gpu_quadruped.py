@@ -233,16 +233,19 @@ def test_cfrc_ext_from_contact_forces(model):
     assert nonzero > B // 2
 
 
-def test_cg_solver_parity():
+@pytest.mark.parametrize("iterations,tol", [(100, 1e-10), (50, 1e-8)])
+def test_cg_solver_parity(iterations, tol):
     """The CG solver (MuJoCo's preconditioned nonlinear CG) against the FP64
-    optimum of the same constraint problem. CG's exit is not a fixed point:
-    MuJoCo's FP64 CG itself stops about 2e-4 (relative to the balanced
-    accelerations) short of the optimum at tolerance 1e-8, and FP32 and FP64
-    CG stop at different iterates. The bar is the cost: the kernel's qacc
-    reaches the optimum's cost to 2e-6 of the cost scale (FP32 resolution of
-    the summed cost), and its qacc is reported against the optimum's."""
+    oracle's CG and the FP64 optimum. CG's exit is not a fixed point: at
+    MuJoCo's default tolerance 1e-8 the FP64 CG itself stops about 2e-4
+    (relative to the balanced accelerations) short of the optimum. Run to
+    tolerance 1e-10 both CGs reach the optimum, and the kernel's qacc and
+    efc_force must match the oracle's CG within 1e-4. At 1e-8 the two CGs stop
+    at different points of the approach (qacc ~1e-4 apart): the bar is qacc
+    within 1e-3 of the oracle's CG and the kernel's remaining cost gap to the
+    optimum at most 1e-6 of the cost scale (the FP64 CG's own is ~1e-7)."""
     import ref64
-    m = capi.Model(scenes.quadruped_xml(self_collision=True, solver="CG", iterations=50))
+    m = capi.Model(scenes.quadruped_xml(self_collision=True, solver="CG", iterations=iterations, tolerance=tol))
     B = 64
     q, qv, qa, ctrl, orc = states.quadruped_states(ref64.fields_from(m.field), B, seed=23)
     f = ref64.fields_from(m.field)
@@ -254,18 +257,26 @@ def test_cg_solver_parity():
     dump = sim.debug_step(ctrl)
     st = sim.stats()
     assert (st["status"] & ~3 == 0).all(), st["status"]
-    worst = worst64 = worstc = 0.0
+    worst = worstf = worst_opt = worst64 = 0.0
+    worstc = 0.0
     for e in range(B):
         states.oracle_step(orc, q[e], qv[e], qa[e], ctrl[e])
         a_cg64 = orc.get("qacc").copy()
+        qk = dump.get(2, e, "qacc").astype(np.float64)
+        eq, ef = states.qacc_force_errors(orc, qk, dump.get(2, e, "efc_force"), st["nefc"][e])
+        worst, worstf = max(worst, eq), max(worstf, ef or 0.0)
         states.oracle_step(opt, q[e], qv[e], qa[e], ctrl[e])
         a_opt = opt.get("qacc")
-        qk = dump.get(2, e, "qacc").astype(np.float64)
         sc = states.accel_scale(opt)
-        worst = max(worst, np.max(np.abs(qk - a_opt)) / sc)
+        worst_opt = max(worst_opt, np.max(np.abs(qk - a_opt)) / sc)
         worst64 = max(worst64, np.max(np.abs(a_cg64 - a_opt)) / sc)
         c_opt = opt.cost(a_opt)[0]
-        worstc = max(worstc, (opt.cost(qk)[0] - c_opt) / states.cost_scale(opt, c_opt))
-    print(f"CG: worst qacc rel err to the optimum {worst:.2e} (FP64 CG's own {worst64:.2e}), "
-          f"cost gap {worstc:.2e}, mean iterations {st['niter'].mean():.2f}")
-    assert worstc < 2e-6
+        cs = states.cost_scale(opt, c_opt)
+        worstc = max(worstc, (opt.cost(qk)[0] - c_opt) / cs)
+    print(f"CG it {iterations} tol {tol:g}: qacc rel err {worst:.2e}, efc_force {worstf:.2e} against the oracle's CG; "
+          f"to the optimum {worst_opt:.2e} (FP64 CG's own {worst64:.2e}); cost gap {worstc:.2e}; "
+          f"mean iterations {st['niter'].mean():.2f}")
+    if tol <= 1e-10:
+        assert worst < 1e-4 and worstf < 1e-4
+    else:
+        assert worstc < 1e-6 and worst < 1e-3
