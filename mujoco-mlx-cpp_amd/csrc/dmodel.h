@@ -31,6 +31,11 @@ enum { FEAT_NCON = 16, FEAT_PER_CON = 8, FEAT_NCLASS = 4 };
 // max contacts one narrow-phase call emits
 enum { MAXC_PAIR = 4 };
 enum { MAX_DOF_DEPTH = 64 };
+// beyond this many dofs the constraint rows are stored compressed: each row
+// keeps only the dofs of its source (a friction / limit row its dof, a
+// contact row the union of its two bodies' dof chains), jw values and dof
+// indices per row
+enum { SPARSE_ROWS_NV = 64 };
 
 struct DModel {
   int nq, nv, nu, nbody, njnt, ngeom, npair;
@@ -41,6 +46,7 @@ struct DModel {
   float timestep, gravity[3], tolerance, ls_tolerance, impratio, meaninertia;
   int nfeat;  // learned warm-start input width: nq + nv + nu + FEAT_NCON * FEAT_PER_CON
   int nsensor, nsensordata, nsens_acc;  // sensors, sensordata width, accelerometers
+  int jw;  // entries per compressed constraint row (0: dense rows of nv)
   float magnetic[3];
   // pointers only from here on: Blob::bind relocates every word after body_parentid
   // bodies
@@ -99,6 +105,9 @@ struct Layout {
   int H;
   int con, efc_J, efc_D, efc_aref, efc_Jaref, efc_Jv, efc_force, efc_pos, efc_margin,
       efc_type, efc_act, efc_id, efc_Dact;
+  int jw;        // compressed rows: entries per row (0: dense rows of nv entries)
+  int efc_Jidx;  // compressed rows: the dof of each entry (-1 past the row's end), ints
+  int wrench;    // compressed rows: per-body contact wrench scratch of J^T f [nbody][6]
   int Ma, grad, Mgrad, search, Mv, grad_old, Mgrad_old, tmpv, dvec;
 };
 
@@ -128,10 +137,14 @@ MJK_HD inline int rand_size(int nbody, int nu, int nfree) { return 4 * nbody + 1
 // factor lives in the efc Jacobian's space: it is dead once qacc_smooth is
 // solved, before the Jacobian is built, and the integrator reuses the space
 // as scratch after the solve.
+// jw > 0: compressed constraint rows of jw entries (models beyond
+// SPARSE_ROWS_NV dofs); subtree com and cdof then stay live through the solve
+// (J^T f is accumulated from per-body contact wrenches).
 MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, int ngeom, int maxcon, int maxefc,
-                                 bool qld_shared = false, int nrand = 0) {
+                                 bool qld_shared = false, int nrand = 0, int jw = 0) {
   Layout L;
   L.maxcon = maxcon; L.maxefc = maxefc; L.efc_cap = maxefc; L.con_cap = maxcon; L.nv = nv;
+  L.jw = jw;
   int nvp = nv > 0 ? nv : 1, nup = nu > 0 ? nu : 1, njp = njnt > 0 ? njnt : 1, ngp = ngeom > 0 ? ngeom : 1;
   int ncon = maxcon > 0 ? maxcon : 1;
   int o = 0;
@@ -184,7 +197,9 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   int kc_end = al2(o);
   o = kc0;
   L.con = o; o += CON_STRIDE * ncon;
-  L.efc_J = o; o += maxefc * nvp;
+  L.efc_J = o; o += maxefc * (jw > 0 ? jw : nvp);
+  L.efc_Jidx = -1;
+  if (jw > 0) { L.efc_Jidx = o; o += maxefc * jw; }
   L.efc_D = o; o += maxefc;
   L.efc_aref = o; o += maxefc;
   L.efc_Jaref = o; o += maxefc;
@@ -203,7 +218,7 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.H = o; o += al2(2 * nvp * nvp);
   if (qld_shared) L.qLD = L.H;
   int nvec = al2(8 * nvp) + 4 * nvp;  // eight FP32 vectors, then two FP64 ones
-  int vbase = nvec <= kc0 - r0 ? r0 : o;
+  int vbase = jw == 0 && nvec <= kc0 - r0 ? r0 : o;
   if (vbase == o) o += nvec;
   int v = vbase;
   L.Ma = v; v += nvp;
@@ -216,6 +231,8 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.tmpv = v; v += nvp;
   v = al2(v);
   L.dvec = v;  // two FP64 vectors: inverse pivots and a solve vector
+  L.wrench = -1;
+  if (jw > 0) { L.wrench = o; o += 6 * nbody; }
   L.total = al2(o);
   return L;
 }

@@ -206,7 +206,7 @@ int step_variant(const MjmlxBatchedSim* s) {
 // for the model's narrow-phase set and, with sensors, the sensor stage
 void launch_step(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
   mjk::StepLaunch l{s->dm, s->L, s->B, 0, step_variant(s), s->lds_bytes, s->d_gws, s->stream};
-  mjk::launch_step_np(s->npset | (s->model.nsensor > 0 ? 4 : 0), l, io, nsub);
+  mjk::launch_step_np(s->npset | (s->model.nsensor > 0 ? 4 : 0), s->nv > mjk::SPARSE_ROWS_NV, l, io, nsub);
 }
 
 mjk::StepIO make_io(MjmlxBatchedSim* s, float* dump1, float* dump2) {
@@ -578,7 +578,7 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     for (int j = 0; j < m.njnt; j++) s->nfree += m.jnt_type[j] == mjk::JFREE;
     if (config->domain_rand) s->nrand = mjk::rand_size(m.nbody, m.nu, s->nfree);
     s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, caps.maxcon, caps.maxefc,
-                            m.solver == mjh::SOL_NEWTON, s->nrand);
+                            m.solver == mjh::SOL_NEWTON, s->nrand, s->blob.proto.jw);
     s->L.efc_cap = caps.efc_cap;
     s->L.con_cap = caps.con_cap;
     s->fields = mjh::layout_fields(m, s->L);
@@ -726,7 +726,7 @@ MJMLX_API int mjmlx_batched_step_n_noalloc(MjmlxBatchedSim* sim, const float* ct
     int var = sim->use_gws ? step_variant(sim) : mjk::VAR_GWS;
     mjk::StepLaunch l{sim->dm, sim->L, sim->B, 0, var, 0, ws, sim->stream};
     HIPOK(hipEventRecord(sim->ev0, sim->stream));
-    mjk::launch_step_np(sim->npset | (sim->model.nsensor > 0 ? 4 : 0), l, io, n);
+    mjk::launch_step_np(sim->npset | (sim->model.nsensor > 0 ? 4 : 0), sim->nv > mjk::SPARSE_ROWS_NV, l, io, n);
     HIPOK(hipGetLastError());
     HIPOK(hipEventRecord(sim->ev1, sim->stream));
     sim->timed = true;

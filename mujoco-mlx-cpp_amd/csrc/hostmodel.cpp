@@ -197,6 +197,21 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
       for (int a = q; a >= 0; a = m.dof_parentid[a]) if (a == k) { desc = true; break; }
       if (!desc) throw std::runtime_error("HIP engine: dof subtree is not contiguous");
     }
+  // compressed constraint rows beyond SPARSE_ROWS_NV dofs: a contact row's
+  // nonzeros are body2's dof chain then body1's up to the first shared dof
+  int jw = 0;
+  bool dense_rows = false;
+  if (const char* v = std::getenv("MJMLX_HIP_DENSE_ROWS")) dense_rows = std::atoi(v) != 0;
+  if (nv > SPARSE_ROWS_NV && !dense_rows) {
+    jw = 1;
+    auto on_chain = [&](int d, int last) { return last >= d && last < dsend[d]; };
+    for (auto& p : pairs) {
+      int l2 = lastdof[p.b2], l1 = lastdof[p.b1];
+      int n = l2 >= 0 ? nchain[l2] : 0;
+      for (int d = l1; d >= 0 && !(l2 >= 0 && on_chain(d, l2)); d = m.dof_parentid[d]) n++;
+      jw = std::max(jw, n);
+    }
+  }
   int ndl = nv > 0 ? maxdd + 1 : 0;
   // level-parallel LDL of qM: for each dof level (deepest first, the root
   // level has no ancestors to update) the entries (i, j), i and j strict
@@ -312,6 +327,7 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   d.meaninertia = (float)m.meaninertia;
   d.nfeat = m.nq + nv + m.nu + FEAT_NCON * FEAT_PER_CON;
   d.nsensor = ns; d.nsensordata = m.nsensordata; d.nsens_acc = nacc;
+  d.jw = jw;
   for (int i = 0; i < 3; i++) d.magnetic[i] = (float)m.magnetic[i];
   std::vector<float> subtreemass = tof(m.body_subtreemass);
 #define ADDP(field, vec) d.field = reinterpret_cast<decltype(d.field)>(B.add(vec))
@@ -366,7 +382,7 @@ std::vector<LayoutField> layout_fields(const Model& m, const mjk::Layout& L) {
       {"cinert", L.cinert, 10 * nb, 0}, {"cdof", L.cdof, 6 * nv, 0}, {"cvel", L.cvel, 6 * nb, 0},
       {"cdof_dot", L.cdofdot, 6 * nv, 0}, {"crb", L.crb, 10 * nb, 0},
       {"contact", L.con, mjk::CON_STRIDE * L.maxcon, 0},
-      {"efc_J", L.efc_J, L.maxefc * nv, 0}, {"efc_D", L.efc_D, L.maxefc, 0},
+      {"efc_J", L.efc_J, L.maxefc * (L.jw > 0 ? L.jw : nv), 0}, {"efc_D", L.efc_D, L.maxefc, 0},
       {"efc_aref", L.efc_aref, L.maxefc, 0}, {"efc_force", L.efc_force, L.maxefc, 0},
       {"efc_pos", L.efc_pos, L.maxefc, 0}, {"efc_margin", L.efc_margin, L.maxefc, 0},
       {"efc_type", L.efc_type, L.maxefc, 1}, {"efc_id", L.efc_id, L.maxefc, 1},

@@ -16,6 +16,14 @@
 
 #include "dmodel.h"
 
+// MJK_BIG 0 compiles the step for models of at most SPARSE_ROWS_NV dofs only:
+// the paths beyond (tree-sparse products, compressed constraint rows,
+// streamed J^T f, memory-resident triangular solves) drop out of the kernel,
+// and with them their registers. The default builds every path.
+#ifndef MJK_BIG
+#define MJK_BIG 1
+#endif
+
 #if defined(MJ_EMU)
 #define KDEV static inline
 #define WAVE 1
@@ -453,7 +461,7 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
   // beyond 64 dofs only the tree's entries are copied and scaled (row k:
   // its diagonal and its ancestors), all its readers touch; up to 64 the
   // dense matrix, whose zeros the register solve reads
-  bool sparse = nv > 64;
+  bool sparse = MJK_BIG && nv > 64;
   if (sparse) {
     for (int k = LANE; k < nv; k += WAVE)
       for (int i = k; i >= 0; i = i == k ? m.dof_parentid[k] : m.dof_parentid[i]) LD[k * nv + i] = A[k * nv + i];
@@ -510,7 +518,7 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
 KDEV void solve_ld(const DModel& m, const float* LD, float* x) {
   int nv = m.nv;
 #if WAVE == 64
-  if (nv <= WAVE) {
+  if (!MJK_BIG || nv <= WAVE) {
     // the next step's factor entry is loaded ahead of the current step's
     // readlane -> FMA chain, so the LDS latency overlaps it
     int ln = LANE < nv ? LANE : 0;
@@ -1589,27 +1597,61 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
     if (nrow > L.maxefc) { nrow = L.maxefc; st |= ST_EFC_OVERFLOW; }
   }
   KSYNC();
-  // Jacobian rows
-  for (int t = LANE; t < nrow * nv; t += WAVE) J[t] = 0.f;
+  // Jacobian rows: dense (nv entries per row) or, beyond SPARSE_ROWS_NV dofs,
+  // compressed (jw entries per row with their dof indices)
+  const int jw = MJK_BIG ? L.jw : 0, rw = jw > 0 ? jw : nv;
+  int* Jidx = jw > 0 ? iptr(w, L.efc_Jidx) : nullptr;
+  for (int t = LANE; t < nrow * rw; t += WAVE) {
+    J[t] = 0.f;
+    if (jw > 0) Jidx[t] = -1;
+  }
   KSYNC();
   int nlr = nrow;
   for (int r = LANE; r < nlr; r += WAVE) {
     int ty = etype[r];
-    if (ty == EFC_FRICTION_DOF) J[r * nv + eid[r]] = 1.f;
+    int k = -1;
+    float v = 0.f;
+    if (ty == EFC_FRICTION_DOF) { k = eid[r]; v = 1.f; }
     else if (ty == EFC_LIMIT_JOINT) {
       int j = eid[r] >> 1;
-      J[r * nv + m.jnt_dofadr[j]] = (eid[r] & 1) ? -1.f : 1.f;
+      k = m.jnt_dofadr[j];
+      v = (eid[r] & 1) ? -1.f : 1.f;
+    }
+    if (k >= 0) {
+      if (jw > 0) { J[r * jw] = v; Jidx[r * jw] = k; }
+      else J[r * nv + k] = v;
     }
   }
   const float *cdof = w + L.cdof, *scom = w + L.scom;
-  for (int t = LANE; t < ncon * nv; t += WAVE) {
-    int c = t / nv, k = t - c * nv;
+  // contacts x entries: entry k is dof k (dense) or the contact's k-th chain
+  // dof (compressed: body2's chain from its last dof up, then body1's up to
+  // the first dof on body2's chain)
+  for (int t = LANE; t < ncon * rw; t += WAVE) {
+    int c = t / rw, k = t - c * rw;
     float* cr = con + CON_STRIDE * c;
     int cnt = iptr(cr, CON_NROW)[0];
     if (cnt == 0) continue;
     int r0 = iptr(cr, CON_ROW0)[0];
     int p = iptr(cr, CON_PAIR)[0];
     int b1 = m.pair_b1[p], b2 = m.pair_b2[p];
+    int slot = k;
+    if (jw > 0) {
+      int l2 = m.body_lastdof[b2], l1 = m.body_lastdof[b1];
+      int n2 = l2 >= 0 ? m.dof_nchain[l2] : 0;
+      if (slot < n2) k = m.dof_chain[l2 * MAX_DOF_DEPTH + slot];
+      else {
+        // the q-th dof of body1's chain off body2's chain (a prefix of it)
+        int q = slot - n2, d = l1;
+        while (d >= 0) {
+          if (l2 >= d && l2 < m.dof_subtree_end[d]) { d = -1; break; }  // the shared part
+          if (q == 0) break;
+          q--;
+          d = m.dof_parentid[d];
+        }
+        if (d < 0) continue;  // past the row's entries
+        k = d;
+      }
+    }
     float jp[3] = {0.f, 0.f, 0.f}, jr[3] = {0.f, 0.f, 0.f};
     const float* cd = cdof + 6 * k;
     // dof k moves body b iff it is in the chain of b's last dof: k <= d < subtree_end(k)
@@ -1628,19 +1670,23 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
     }
     const float* fr = cr + CON_FRAME;
     float Jn = dot3(fr, jp);
-    if (cnt == 1) { J[r0 * nv + k] = Jn; continue; }
+    // entry (row r, slot) of the dense or compressed layout
+    auto at = [&](int r) -> float& { return J[r * rw + slot]; };
+    if (jw > 0)
+      for (int q = 0; q < cnt; q++) Jidx[(r0 + q) * jw + slot] = k;
+    if (cnt == 1) { at(r0) = Jn; continue; }
     float Jd[5] = {dot3(fr + 3, jp), dot3(fr + 6, jp), dot3(fr, jr), dot3(fr + 3, jr), dot3(fr + 6, jr)};
     if (m.cone != 0) {  // elliptic: normal, then t1, t2, spin, roll1, roll2
-      J[r0 * nv + k] = Jn;
-      for (int q = 1; q < cnt; q++) J[(r0 + q) * nv + k] = Jd[q - 1];
+      at(r0) = Jn;
+      for (int q = 1; q < cnt; q++) at(r0 + q) = Jd[q - 1];
       continue;
     }
     const float* mu = m.pair_friction + 5 * p;
     float sc = fric_scale(m, L, w);
     for (int q = 0; q < cnt / 2; q++) {
       float mq = mu[q] * sc;
-      J[(r0 + 2 * q) * nv + k] = Jn + mq * Jd[q];
-      J[(r0 + 2 * q + 1) * nv + k] = Jn - mq * Jd[q];
+      at(r0 + 2 * q) = Jn + mq * Jd[q];
+      at(r0 + 2 * q + 1) = Jn - mq * Jd[q];
     }
   }
   KSYNC();
@@ -1663,7 +1709,16 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
       mu0 = m.pair_friction[5 * p] * fric_scale(m, L, w);
       dA = ty == EFC_CONTACT_PYRAMIDAL ? tran * (1.f + mu0 * mu0) : tran;
     }
-    float v = kdot(J + r * nv, 1, qvel, 1, nv);
+    float v = 0.f;
+    if (jw > 0) {
+      for (int q = 0; q < jw; q++) {
+        int k = Jidx[r * jw + q];
+        if (k < 0) break;
+        v += J[r * jw + q] * qvel[k];
+      }
+    } else {
+      v = kdot(J + r * nv, 1, qvel, 1, nv);
+    }
     float tc = sr[0], dr = sr[1];
     float dmax = clampf(si[1], KMINIMP, KMAXIMP);
     float kk, bb;
@@ -1710,7 +1765,7 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
 KDEV float mulM_row(const DModel& m, const float* M, const float* x, int i) {
   int nv = m.nv;
   const float* Mi = M + i * nv;
-  if (nv <= 64) return kdot(Mi, 1, x, 1, nv);
+  if (!MJK_BIG || nv <= 64) return kdot(Mi, 1, x, 1, nv);
   float acc = 0.f;
   for (int j = m.dof_parentid[i]; j >= 0; j = m.dof_parentid[j]) acc += Mi[j] * x[j];
   for (int k = i; k < m.dof_subtree_end[i]; k++) acc += Mi[k] * x[k];
@@ -1726,8 +1781,19 @@ KDEV void mulM(const DModel& m, const float* M, const float* x, float* y) {
 // up to the first dof body2's chain shares); rows of other types are dense.
 KDEV float jrow_dot(const DModel& m, const Layout& L, const float* w, int r, const float* x) {
   int nv = m.nv;
+  if (MJK_BIG && L.jw > 0) {  // compressed row: its entries and their dofs
+    const float* Jr = w + L.efc_J + r * L.jw;
+    const int* ix = iptr(w, L.efc_Jidx) + r * L.jw;
+    float acc = 0.f;
+    for (int q = 0; q < L.jw; q++) {
+      int k = ix[q];
+      if (k < 0) break;
+      acc += Jr[q] * x[k];
+    }
+    return acc;
+  }
   const float* Jr = w + L.efc_J + r * nv;
-  if (nv <= 64) return kdot(Jr, 1, x, 1, nv);
+  if (!MJK_BIG || nv <= 64) return kdot(Jr, 1, x, 1, nv);
   int ty = reinterpret_cast<const int*>(w + L.efc_type)[r], id = reinterpret_cast<const int*>(w + L.efc_id)[r];
   if (ty == EFC_FRICTION_DOF) return Jr[id] * x[id];
   if (ty == EFC_LIMIT_JOINT) { int d = m.jnt_dofadr[id >> 1]; return Jr[d] * x[d]; }
@@ -1744,14 +1810,90 @@ KDEV float jrow_dot(const DModel& m, const Layout& L, const float* w, int r, con
   return acc;
 }
 
+// y = J^T f for compressed rows, without the rows: each contact's rows add
+// up to a force F and torque T at the contact point (normal n, tangents t1, t2
+// of its frame; pyramidal edge pairs J_n +- mu_q J_q, elliptic rows n, t1, t2,
+// then spin and rolls as torques), applied +/- to body2 / body1 as a wrench
+// about the tree root's subtree COM (the point cdof refers to); a dof then
+// takes cdof . (sum of the wrenches on its body's subtree). Friction and
+// limit rows (the leading rows) add J f to their dof. One lane per body over
+// the contacts in order, then one lane per dof: deterministic sums, work
+// nbody x ncon + nv x subtree size instead of nefc x nv.
+KDEV void mul_jt_sparse(const DModel& m, const Layout& L, const float* w, const float* f, int nefc, float* y) {
+  int nv = m.nv, nb = m.nbody;
+  float* wr = const_cast<float*>(w) + L.wrench;
+  const int* misc = iptr(w, L.misc);
+  int ncon = misc[MISC_NCON];
+  const float *con = w + L.con, *scom = w + L.scom, *cdof = w + L.cdof;
+  float sc = fric_scale(m, L, w);
+  for (int b = LANE; b < nb; b += WAVE) {
+    float W[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < ncon && b > 0; c++) {
+      const float* cr = con + CON_STRIDE * c;
+      int cnt = iptr(cr, CON_NROW)[0];
+      if (cnt == 0) continue;
+      int p = iptr(cr, CON_PAIR)[0];
+      int b1 = m.pair_b1[p], b2 = m.pair_b2[p];
+      if (b != b1 && b != b2) continue;
+      const float* fr = f + iptr(cr, CON_ROW0)[0];
+      const float *n = cr + CON_FRAME, *t1 = n + 3, *t2 = n + 6;
+      float F[3], T[3] = {0.f, 0.f, 0.f};
+      if (cnt == 1) {
+        for (int i = 0; i < 3; i++) F[i] = fr[0] * n[i];
+      } else if (m.cone != 0) {
+        for (int i = 0; i < 3; i++) F[i] = fr[0] * n[i] + fr[1] * t1[i] + fr[2] * t2[i];
+        if (cnt > 3) for (int i = 0; i < 3; i++) T[i] = fr[3] * n[i];
+        if (cnt > 4) for (int i = 0; i < 3; i++) T[i] += fr[4] * t1[i] + fr[5] * t2[i];
+      } else {
+        const float* mu = m.pair_friction + 5 * p;
+        for (int i = 0; i < 3; i++) F[i] = 0.f;
+        for (int q = 0; q < cnt / 2; q++) {
+          float fp = fr[2 * q], fm = fr[2 * q + 1], dd = mu[q] * sc * (fp - fm);
+          const float* dir = q == 0 ? t1 : (q == 1 ? t2 : (q == 2 ? n : (q == 3 ? t1 : t2)));
+          float* acc = q < 2 ? F : T;
+          for (int i = 0; i < 3; i++) { F[i] += (fp + fm) * n[i]; acc[i] += dd * dir[i]; }
+        }
+      }
+      float sg = b == b2 ? 1.f : -1.f;
+      const float* cc = scom + 3 * m.body_rootid[b];
+      float off[3] = {cr[0] - cc[0], cr[1] - cc[1], cr[2] - cc[2]}, t[3];
+      cross3(off, F, t);
+      for (int i = 0; i < 3; i++) { W[i] += sg * (t[i] + T[i]); W[3 + i] += sg * F[i]; }
+    }
+    for (int i = 0; i < 6; i++) wr[6 * b + i] = W[i];
+  }
+  KSYNC();
+  const float* J = w + L.efc_J;
+  const int *ix = iptr(w, L.efc_Jidx), *etype = iptr(w, L.efc_type);
+  for (int d = LANE; d < nv; d += WAVE) {
+    const float* cd = cdof + 6 * d;
+    int bd = m.dof_bodyid[d];
+    float acc = 0.f;
+    for (int bb = bd; bb < m.body_subtree_end[bd]; bb++) {
+      const float* W = wr + 6 * bb;
+      acc += cd[0] * W[0] + cd[1] * W[1] + cd[2] * W[2] + cd[3] * W[3] + cd[4] * W[4] + cd[5] * W[5];
+    }
+    for (int r = 0; r < nefc; r++) {
+      int ty = etype[r];
+      if (ty != EFC_FRICTION_DOF && ty != EFC_LIMIT_JOINT) break;
+      if (ix[r * L.jw] == d) acc += J[r * L.jw] * f[r];
+    }
+    y[d] = acc;
+  }
+}
+
 // y = J^T f. GPU beyond 64 dofs: lane l keeps dofs base + l, base + l + 64,
 // .. (four per lane, 256-dof chunks) in registers and streams the rows,
 // coalesced across lanes.
 KDEV void mul_jt(const DModel& m, const Layout& L, const float* w, const float* f, int nefc, float* y) {
   int nv = m.nv;
+  if (MJK_BIG && L.jw > 0) {
+    mul_jt_sparse(m, L, w, f, nefc, y);
+    return;
+  }
   const float* J = w + L.efc_J;
 #if WAVE == 64
-  if (nv > WAVE) {
+  if (MJK_BIG && nv > WAVE) {
     for (int base = 0; base < nv; base += 4 * WAVE) {
       // eight rows per step: up to 32 independent loads in flight per lane
       float g[4] = {0.f, 0.f, 0.f, 0.f};
@@ -1970,7 +2112,50 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     KSYNC();
   } else
 #endif
-  {
+  if (MJK_BIG && L.jw > 0) {
+    // compressed rows: H = M, then each row group sharing its entries' dofs
+    // (a friction or limit row, a contact's rows) adds D_r J_r^T J_r into
+    // its dofs' entries; the groups go one after the other (they share H
+    // entries), the entry pairs of one group in parallel (distinct entries)
+    const int jw = L.jw;
+    const int* ix = iptr(w, L.efc_Jidx);
+    const int* etype = iptr(w, L.efc_type);
+    for (int t = LANE; t < ntri; t += WAVE) {
+      int i, j;
+      tri_rc(t, &i, &j);
+      H[i * nv + j] = (double)M[i * nv + j];
+    }
+    KSYNC();
+    const int* misc = iptr(w, L.misc);
+    const float* con = w + L.con;
+    int ncon = misc[MISC_NCON], npr = jw * (jw + 1) / 2;
+    int nfront = 0;
+    while (nfront < nefc && (etype[nfront] == EFC_FRICTION_DOF || etype[nfront] == EFC_LIMIT_JOINT)) nfront++;
+    for (int g = 0; g < nfront + ncon; g++) {
+      int ra = g, rb = g + 1;
+      if (g >= nfront) {
+        const float* cr = con + CON_STRIDE * (g - nfront);
+        int cnt = iptr(cr, CON_NROW)[0];
+        if (cnt == 0) continue;
+        ra = iptr(cr, CON_ROW0)[0];
+        rb = ra + cnt;
+      }
+      bool any = false;
+      for (int r = ra; r < rb; r++) any = any || Dact[r] != 0.f;
+      if (!any) continue;
+      for (int t = LANE; t < npr; t += WAVE) {
+        int a, b;
+        tri_rc(t, &a, &b);
+        int ia = ix[ra * jw + a], ib = ix[ra * jw + b];
+        if (ia < 0 || ib < 0) continue;
+        float sacc = 0.f;
+        for (int r = ra; r < rb; r++) sacc += Dact[r] * J[r * jw + a] * J[r * jw + b];
+        int i = ia > ib ? ia : ib, j = ia > ib ? ib : ia;
+        H[i * nv + j] += (double)sacc;
+      }
+      KSYNC();
+    }
+  } else {
     for (int t = LANE; t < ntri; t += WAVE) {
       int i, j;
       tri_rc(t, &i, &j);
@@ -2121,7 +2306,7 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
   const float* grad = w + L.grad;
   float* Mg = w + L.Mgrad;
 #if WAVE == 64
-  if (nv > WAVE) {
+  if (MJK_BIG && nv > WAVE) {
     double* y = dptr(w, L.dvec) + nv;
     for (int i = LANE; i < nv; i += WAVE) y[i] = grad[i];
     KSYNC();

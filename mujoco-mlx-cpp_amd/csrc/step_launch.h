@@ -3,8 +3,9 @@
 // The step kernel is instantiated once per narrow-phase set (NPSET: 0 basic
 // pairs, 1 + capsule-box / box-box, 2 + GJK/EPA, +4 with the sensor stage) and
 // per workspace/register variant. Each narrow-phase set lives in its own
-// translation unit (step_np.hip compiled with -DMJK_NP=<set>), so the sets
-// build in parallel; the engine picks one through launch_step_np().
+// translation units (step_np.hip compiled with -DMJK_NP=<set>, once for models
+// of at most SPARSE_ROWS_NV dofs and once with every path), so the sets build
+// in parallel; the engine picks one through launch_step_np().
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -33,14 +34,35 @@ struct StepLaunch {
   hipStream_t stream;
 };
 
+// kernels built for models of at most SPARSE_ROWS_NV dofs (MJK_BIG 0) ...
 void launch_step_np0(const StepLaunch& l, const StepIO& io, int nsub);
 void launch_step_np1(const StepLaunch& l, const StepIO& io, int nsub);
 void launch_step_np2(const StepLaunch& l, const StepIO& io, int nsub);
 void launch_step_np4(const StepLaunch& l, const StepIO& io, int nsub);
 void launch_step_np5(const StepLaunch& l, const StepIO& io, int nsub);
 void launch_step_np6(const StepLaunch& l, const StepIO& io, int nsub);
+// ... and with every path, for any model size (MJK_BIG 1)
+void launch_step_np0_big(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np1_big(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np2_big(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np4_big(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np5_big(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np6_big(const StepLaunch& l, const StepIO& io, int nsub);
 
-inline void launch_step_np(int npset, const StepLaunch& l, const StepIO& io, int nsub) {
+// npset: narrow-phase set (+4 with sensors); big: the model has more than
+// SPARSE_ROWS_NV dofs
+inline void launch_step_np(int npset, bool big, const StepLaunch& l, const StepIO& io, int nsub) {
+  if (big) {
+    switch (npset) {
+      case 0: launch_step_np0_big(l, io, nsub); break;
+      case 1: launch_step_np1_big(l, io, nsub); break;
+      case 2: launch_step_np2_big(l, io, nsub); break;
+      case 4: launch_step_np4_big(l, io, nsub); break;
+      case 5: launch_step_np5_big(l, io, nsub); break;
+      default: launch_step_np6_big(l, io, nsub); break;
+    }
+    return;
+  }
   switch (npset) {
     case 0: launch_step_np0(l, io, nsub); break;
     case 1: launch_step_np1(l, io, nsub); break;

@@ -8,9 +8,17 @@
 #ifndef MJK_NP
 #error "step_np.hip is compiled once per narrow-phase set with -DMJK_NP=<0|1|2|4|5|6>"
 #endif
+#ifndef MJK_BIG
+#error "step_np.hip is compiled with -DMJK_BIG=0 (models of at most 64 dofs) and -DMJK_BIG=1 (any size)"
+#endif
 
 #define MJK_CAT2(a, b) a##b
 #define MJK_CAT(a, b) MJK_CAT2(a, b)
+#if MJK_BIG
+#define MJK_LAUNCH MJK_CAT(MJK_CAT(launch_step_np, MJK_NP), _big)
+#else
+#define MJK_LAUNCH MJK_CAT(launch_step_np, MJK_NP)
+#endif
 
 // GWS: the per-env workspace lives in device memory (gws + blockIdx.x *
 // L.total) instead of LDS, for models whose workspace exceeds the LDS of a
@@ -51,8 +59,9 @@ k_step_big(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int ns
 
 namespace mjk {
 
-void MJK_CAT(launch_step_np, MJK_NP)(const StepLaunch& l, const StepIO& io, int nsub) {
-  constexpr int NP = MJK_NP;
+void MJK_LAUNCH(const StepLaunch& l, const StepIO& io, int nsub) {
+  // bit 16 of the set names the kernels built with every path (MJK_BIG 1)
+  constexpr int NP = MJK_NP | (MJK_BIG ? 16 : 0);
   dim3 g(l.B), b(64);
   bool gws = l.variant == VAR_GWS || l.variant == VAR_GWS_OCC;
   size_t sh = gws ? 0 : l.lds_bytes;

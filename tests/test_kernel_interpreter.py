@@ -154,3 +154,57 @@ def test_results_independent_of_workspace_contents(scene):
         for k in st:
             assert np.array_equal(st[k], res[0][1][k])
     assert not res[0][1]["status"].any()
+
+
+@pytest.mark.parametrize("scene", ["boxes14_newton", "boxes32_cg", "multi_geom_scene", "contact_stress"])
+def test_compressed_rows_match_dense_rows(scene, monkeypatch):
+    """Beyond 64 dofs the constraint rows are stored compressed (each row's
+    chain dofs only) and J^T f is summed from per-body contact wrenches; the
+    same step with dense rows (MJMLX_HIP_DENSE_ROWS=1) gives the same qacc and
+    qfrc_constraint up to FP32 summation order, and both match the oracle."""
+    from test_reference_models import model_xml
+    xml = {"boxes14_newton": scenes.boxes_xml(14, 3, solver="Newton", iterations=50),
+           "boxes32_cg": scenes.boxes_xml(32, 132), "multi_geom_scene": model_xml("multi_geom_scene"),
+           "contact_stress": model_xml("contact_stress")}[scene]
+    cm = emu.CompiledModel(xml)
+    nfree = int(np.sum(cm.field("jnt_type") == 0))
+    orc = ref64.Oracle(ref64.fields_from(cm.field))
+    B = 3
+    rng = np.random.default_rng(4)
+    q = np.tile(cm.field("qpos0"), (B, 1))
+    qv = rng.uniform(-0.1, 0.1, (B, orc.nv))
+    qa = np.zeros((B, orc.nv))
+    orc.batch_step(q, qv, qa, rng.uniform(-0.5, 0.5, (B, orc.nu)) if orc.nu else None,
+                   {"boxes14_newton": 160, "boxes32_cg": 160, "multi_geom_scene": 160, "contact_stress": 360}[scene], 3)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    q, qv, qa = f32(q), f32(qv), f32(qa)
+    ctrl = f32(rng.uniform(-0.5, 0.5, (B, orc.nu))) if orc.nu else None
+    out = []
+    for dense in ("1", "0"):
+        monkeypatch.setenv("MJMLX_HIP_DENSE_ROWS", dense)
+        sim = emu.EmuSim(xml, B)
+        sim.set_state(q, qv, qa)
+        sim.step(ctrl, 1, dump=True)
+        st = sim.stats()
+        assert not (st["status"] & ~3).any()
+        out.append((sim, st))
+    (sd, std), (ss, sts) = out
+    assert ss.total < sd.total  # compressed rows take less workspace
+    maxcon = ss.layout("contact")[1] // 16
+    if cm.field("nv")[0] > 32 or nfree > 1:
+        orc.set_caps(con_cap=maxcon, efc_cap=0)
+    ncon = 0
+    for e in range(B):
+        assert std["nefc"][e] == sts["nefc"][e] and std["ncon"][e] == sts["ncon"][e]
+        ncon += sts["ncon"][e]
+        states.oracle_step(orc, q[e], qv[e], qa[e], ctrl[e] if ctrl is not None else np.zeros(0, np.float32))
+        sc = states.accel_scale(orc)
+        a_d, a_s = sd.dump(2, e, "qacc"), ss.dump(2, e, "qacc")
+        f_d, f_s = sd.dump(2, e, "qfrc_constraint"), ss.dump(2, e, "qfrc_constraint")
+        tol = 1e-4 if scene != "boxes32_cg" else 2e-3  # CG at its 16-iteration budget amplifies rounding
+        assert np.max(np.abs(a_d - a_s)) / sc < tol
+        assert np.max(np.abs(f_d - f_s)) / max(1.0, np.max(np.abs(f_d))) < tol
+        if scene in ("boxes14_newton", "contact_stress"):  # (flat GJK patches: see test_reference_models)
+            eq, ef = states.qacc_force_errors(orc, a_s, ss.dump(2, e, "efc_force"), sts["nefc"][e])
+            assert eq < 1e-4 and ef < 1e-4
+    assert ncon > 0
