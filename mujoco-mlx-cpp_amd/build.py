@@ -49,7 +49,7 @@ def _objects(tag, extra):
     inc = [f"-I{INC}", f"-I{CSRC}"]
     jobs = [([HIPCC, *HIP_FLAGS, *extra, *inc, "-c", src, "-o", os.path.join(obj, src.rsplit(".", 1)[0] + ".o")],
              os.path.join(obj, src.rsplit(".", 1)[0] + ".o")) for src in ("engine.hip", "mlp.hip", *HOST_SRC)]
-    # each set twice: for models of at most 64 dofs (MJK_BIG 0) and with every path
+    # each set twice: for models of at most 32 dofs (MJK_BIG 0) and with every path
     jobs += [([HIPCC, *HIP_FLAGS, *extra, *inc, f"-DMJK_NP={k}", f"-DMJK_BIG={big}", "-c", "step_np.hip", "-o",
                os.path.join(obj, f"step_np{k}{sfx}.o")], os.path.join(obj, f"step_np{k}{sfx}.o"))
              for k in NP_SETS for big, sfx in ((0, ""), (1, "_big"))]

@@ -34,8 +34,9 @@ enum { MAX_DOF_DEPTH = 64 };
 // beyond this many dofs the constraint rows are stored compressed: each row
 // keeps only the dofs of its source (a friction / limit row its dof, a
 // contact row the union of its two bodies' dof chains), jw values and dof
-// indices per row
-enum { SPARSE_ROWS_NV = 64 };
+// indices per row. Up to it the rows stay dense for the Hessian's matrix-core
+// tiles (32 x 32).
+enum { SPARSE_ROWS_NV = 32 };
 
 struct DModel {
   int nq, nv, nu, nbody, njnt, ngeom, npair;
@@ -108,6 +109,9 @@ struct Layout {
   int jw;        // compressed rows: entries per row (0: dense rows of nv entries)
   int efc_Jidx;  // compressed rows: the dof of each entry (-1 past the row's end), ints
   int wrench;    // compressed rows: per-body contact wrench scratch of J^T f [nbody][6]
+  int cwrench;   // compressed rows: per-contact wrenches on body2 and body1 [maxcon][12]
+  int bcon_adr;  // compressed rows: per-body contact lists, CSR offsets [nbody + 1] (ints)
+  int bcon;      // compressed rows: per-body contact lists, entries 2 c + side [2 maxcon] (ints)
   int Ma, grad, Mgrad, search, Mv, grad_old, Mgrad_old, tmpv, dvec;
 };
 
@@ -231,8 +235,13 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.tmpv = v; v += nvp;
   v = al2(v);
   L.dvec = v;  // two FP64 vectors: inverse pivots and a solve vector
-  L.wrench = -1;
-  if (jw > 0) { L.wrench = o; o += 6 * nbody; }
+  L.wrench = L.cwrench = L.bcon_adr = L.bcon = -1;
+  if (jw > 0) {
+    L.wrench = o; o += 6 * nbody;
+    L.cwrench = o; o += 12 * ncon;
+    L.bcon_adr = o; o += nbody + 1;
+    L.bcon = o; o += 2 * ncon;
+  }
   L.total = al2(o);
   return L;
 }

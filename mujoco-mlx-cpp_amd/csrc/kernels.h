@@ -16,10 +16,11 @@
 
 #include "dmodel.h"
 
-// MJK_BIG 0 compiles the step for models of at most SPARSE_ROWS_NV dofs only:
-// the paths beyond (tree-sparse products, compressed constraint rows,
-// streamed J^T f, memory-resident triangular solves) drop out of the kernel,
-// and with them their registers. The default builds every path.
+// MJK_BIG 0 compiles the step for models of at most SPARSE_ROWS_NV (32) dofs
+// only: the paths of larger models (compressed constraint rows, and beyond 64
+// dofs tree-sparse products, streamed J^T f and memory-resident triangular
+// solves) drop out of the kernel, and with them their registers. The default
+// builds every path.
 #ifndef MJK_BIG
 #define MJK_BIG 1
 #endif
@@ -1505,6 +1506,8 @@ KDEV float k_impedance(const float* si, float x) {
 // joint-limit rows (one per violated side), contact rows (frictionless or
 // pyramidal). Row layout and formulas match the FP64 oracle; the
 // per-row quantities follow reference solver.hpp:120-139,299-371,398-587.
+KDEV void build_body_contacts(const DModel& m, const Layout& L, float* w);
+
 KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
   int* misc = iptr(w, L.misc);
   int nv = m.nv;
@@ -1754,6 +1757,7 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
   }
   if (LANE == 0) misc[MISC_NEFC] = nrow;
   KSYNC();
+  if (jw > 0) build_body_contacts(m, L, w);
   return st;
 }
 
@@ -1810,55 +1814,100 @@ KDEV float jrow_dot(const DModel& m, const Layout& L, const float* w, int r, con
   return acc;
 }
 
+// Per-body contact lists for mul_jt_sparse, built once per substep after the
+// rows: body b's entries are 2 c + side (side 0: b is the contact's body2,
+// 1: its body1) in contact order, CSR in bcon_adr / bcon.
+KDEV void build_body_contacts(const DModel& m, const Layout& L, float* w) {
+  int nb = m.nbody;
+  int ncon = iptr(w, L.misc)[MISC_NCON];
+  const float* con = w + L.con;
+  int *adr = iptr(w, L.bcon_adr), *lst = iptr(w, L.bcon);
+  int* cw = iptr(w, L.cwrench);  // body pairs of the contacts, staged in the wrench scratch
+  for (int c = LANE; c < ncon; c += WAVE) {
+    const float* cr = con + CON_STRIDE * c;
+    int p = iptr(cr, CON_PAIR)[0];
+    bool rows = iptr(cr, CON_NROW)[0] > 0;
+    cw[2 * c] = rows ? m.pair_b2[p] : -1;
+    cw[2 * c + 1] = rows ? m.pair_b1[p] : -1;
+  }
+  KSYNC();
+  int run = 0;
+  for (int b0 = 0; b0 < nb; b0 += WAVE) {
+    int b = b0 + LANE, cnt = 0;
+    if (b < nb && b > 0)
+      for (int t = 0; t < 2 * ncon; t++) cnt += cw[t] == b;
+    int off = run + wscan_excl(cnt);
+    if (b < nb) adr[b] = off;
+    run += wsumi(cnt);
+    if (b < nb && b > 0) {
+      int k = off;
+      for (int t = 0; t < 2 * ncon; t++)
+        if (cw[t] == b) lst[k++] = t;
+    }
+  }
+  if (LANE == 0) adr[nb] = run;
+  KSYNC();
+}
+
 // y = J^T f for compressed rows, without the rows: each contact's rows add
 // up to a force F and torque T at the contact point (normal n, tangents t1, t2
 // of its frame; pyramidal edge pairs J_n +- mu_q J_q, elliptic rows n, t1, t2,
 // then spin and rolls as torques), applied +/- to body2 / body1 as a wrench
 // about the tree root's subtree COM (the point cdof refers to); a dof then
 // takes cdof . (sum of the wrenches on its body's subtree). Friction and
-// limit rows (the leading rows) add J f to their dof. One lane per body over
-// the contacts in order, then one lane per dof: deterministic sums, work
-// nbody x ncon + nv x subtree size instead of nefc x nv.
+// limit rows (the leading rows) add J f to their dof. One lane per contact
+// for its two wrenches, one lane per body over its contact list (in contact
+// order), one lane per dof: deterministic sums, work ncon + list lengths +
+// nv x subtree size instead of nefc x nv.
 KDEV void mul_jt_sparse(const DModel& m, const Layout& L, const float* w, const float* f, int nefc, float* y) {
   int nv = m.nv, nb = m.nbody;
   float* wr = const_cast<float*>(w) + L.wrench;
+  float* cw = const_cast<float*>(w) + L.cwrench;
   const int* misc = iptr(w, L.misc);
   int ncon = misc[MISC_NCON];
   const float *con = w + L.con, *scom = w + L.scom, *cdof = w + L.cdof;
   float sc = fric_scale(m, L, w);
-  for (int b = LANE; b < nb; b += WAVE) {
-    float W[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int c = 0; c < ncon && b > 0; c++) {
-      const float* cr = con + CON_STRIDE * c;
-      int cnt = iptr(cr, CON_NROW)[0];
-      if (cnt == 0) continue;
-      int p = iptr(cr, CON_PAIR)[0];
-      int b1 = m.pair_b1[p], b2 = m.pair_b2[p];
-      if (b != b1 && b != b2) continue;
-      const float* fr = f + iptr(cr, CON_ROW0)[0];
-      const float *n = cr + CON_FRAME, *t1 = n + 3, *t2 = n + 6;
-      float F[3], T[3] = {0.f, 0.f, 0.f};
-      if (cnt == 1) {
-        for (int i = 0; i < 3; i++) F[i] = fr[0] * n[i];
-      } else if (m.cone != 0) {
-        for (int i = 0; i < 3; i++) F[i] = fr[0] * n[i] + fr[1] * t1[i] + fr[2] * t2[i];
-        if (cnt > 3) for (int i = 0; i < 3; i++) T[i] = fr[3] * n[i];
-        if (cnt > 4) for (int i = 0; i < 3; i++) T[i] += fr[4] * t1[i] + fr[5] * t2[i];
-      } else {
-        const float* mu = m.pair_friction + 5 * p;
-        for (int i = 0; i < 3; i++) F[i] = 0.f;
-        for (int q = 0; q < cnt / 2; q++) {
-          float fp = fr[2 * q], fm = fr[2 * q + 1], dd = mu[q] * sc * (fp - fm);
-          const float* dir = q == 0 ? t1 : (q == 1 ? t2 : (q == 2 ? n : (q == 3 ? t1 : t2)));
-          float* acc = q < 2 ? F : T;
-          for (int i = 0; i < 3; i++) { F[i] += (fp + fm) * n[i]; acc[i] += dd * dir[i]; }
-        }
+  for (int c = LANE; c < ncon; c += WAVE) {
+    const float* cr = con + CON_STRIDE * c;
+    int cnt = iptr(cr, CON_NROW)[0];
+    if (cnt == 0) continue;
+    int p = iptr(cr, CON_PAIR)[0];
+    const float* fr = f + iptr(cr, CON_ROW0)[0];
+    const float *n = cr + CON_FRAME, *t1 = n + 3, *t2 = n + 6;
+    float F[3], T[3] = {0.f, 0.f, 0.f};
+    if (cnt == 1) {
+      for (int i = 0; i < 3; i++) F[i] = fr[0] * n[i];
+    } else if (m.cone != 0) {
+      for (int i = 0; i < 3; i++) F[i] = fr[0] * n[i] + fr[1] * t1[i] + fr[2] * t2[i];
+      if (cnt > 3) for (int i = 0; i < 3; i++) T[i] = fr[3] * n[i];
+      if (cnt > 4) for (int i = 0; i < 3; i++) T[i] += fr[4] * t1[i] + fr[5] * t2[i];
+    } else {
+      const float* mu = m.pair_friction + 5 * p;
+      for (int i = 0; i < 3; i++) F[i] = 0.f;
+      for (int q = 0; q < cnt / 2; q++) {
+        float fp = fr[2 * q], fm = fr[2 * q + 1], dd = mu[q] * sc * (fp - fm);
+        const float* dir = q == 0 ? t1 : (q == 1 ? t2 : (q == 2 ? n : (q == 3 ? t1 : t2)));
+        float* acc = q < 2 ? F : T;
+        for (int i = 0; i < 3; i++) { F[i] += (fp + fm) * n[i]; acc[i] += dd * dir[i]; }
       }
-      float sg = b == b2 ? 1.f : -1.f;
+    }
+    for (int side = 0; side < 2; side++) {
+      int b = side == 0 ? m.pair_b2[p] : m.pair_b1[p];
+      float sg = side == 0 ? 1.f : -1.f;
       const float* cc = scom + 3 * m.body_rootid[b];
       float off[3] = {cr[0] - cc[0], cr[1] - cc[1], cr[2] - cc[2]}, t[3];
       cross3(off, F, t);
-      for (int i = 0; i < 3; i++) { W[i] += sg * (t[i] + T[i]); W[3 + i] += sg * F[i]; }
+      float* o = cw + 12 * c + 6 * side;
+      for (int i = 0; i < 3; i++) { o[i] = sg * (t[i] + T[i]); o[3 + i] = sg * F[i]; }
+    }
+  }
+  KSYNC();
+  const int *adr = iptr(w, L.bcon_adr), *lst = iptr(w, L.bcon);
+  for (int b = LANE; b < nb; b += WAVE) {
+    float W[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int e = adr[b]; e < adr[b + 1]; e++) {
+      const float* o = cw + 6 * lst[e];  // 12 c + 6 side
+      for (int i = 0; i < 6; i++) W[i] += o[i];
     }
     for (int i = 0; i < 6; i++) wr[6 * b + i] = W[i];
   }

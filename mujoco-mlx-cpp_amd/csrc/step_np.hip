@@ -9,7 +9,7 @@
 #error "step_np.hip is compiled once per narrow-phase set with -DMJK_NP=<0|1|2|4|5|6>"
 #endif
 #ifndef MJK_BIG
-#error "step_np.hip is compiled with -DMJK_BIG=0 (models of at most 64 dofs) and -DMJK_BIG=1 (any size)"
+#error "step_np.hip is compiled with -DMJK_BIG=0 (models of at most SPARSE_ROWS_NV = 32 dofs) and -DMJK_BIG=1 (any size)"
 #endif
 
 #define MJK_CAT2(a, b) a##b
