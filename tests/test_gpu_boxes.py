@@ -141,3 +141,61 @@ def test_autotune_keeps_state():
         del sim
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def test_mixed_box_counts_batch():
+    """The uneven-contact workload as the bench runs it, at a size the oracle
+    covers: 96 envs with N in U{4..32} boxes, grouped one sim per N, queued
+    together, 30 control steps of landing, then one substep of every group
+    compared env by env with the FP64 oracle (same capacities: no row
+    budget): identical contact pairs and row counts, no overflow, and CG's
+    remaining cost gap to the optimum at most 1.25x the FP64 CG's at the same
+    16-iteration budget."""
+    rng = np.random.default_rng(9)
+    ns = rng.integers(4, 33, size=96)
+    groups = []
+    for n in sorted(set(ns.tolist())):
+        nb = int((ns == n).sum())
+        model = capi.Model(scenes.boxes_xml(n, 100 + n, solver="CG", iterations=16))
+        q = np.tile(model.field("qpos0"), (nb, 1))
+        hs = model.field("geom_size").reshape(-1, 3)[1:]
+        for b in range(n):
+            a = 7 * b
+            q[:, a + 2] = rng.uniform(0.1, 1.0, nb) + np.linalg.norm(hs[b])
+            qq = rng.normal(size=(nb, 4))
+            q[:, a + 3:a + 7] = qq / np.linalg.norm(qq, axis=1, keepdims=True)
+        sim = capi.BatchedSim(model, nb)
+        sim.set_state(q.astype(np.float32), np.zeros((nb, model.nv), np.float32))
+        groups.append((n, nb, model, sim))
+    for _ in range(30):
+        for _, _, _, sim in groups:
+            sim.step(None, 4)
+    for _, _, _, sim in groups:
+        sim.sync()
+    ncon = nenv = 0
+    worst_ratio = 0.0
+    for n, nb, model, sim in groups:
+        q, qv = sim.state()
+        qa = sim.qacc_warmstart()
+        caps = _caps(model)
+        orc = ref64.Oracle(ref64.fields_from(model.field)).set_caps(**caps)
+        opt = _newton_optimum(model)
+        dump = sim.debug_step(None)
+        st = sim.stats()
+        assert not st["status"].any(), (n, st["status"])
+        for e in range(nb):
+            states.oracle_step(orc, q[e], qv[e], qa[e], np.zeros(0, np.float32))
+            con = dump.get(1, e, "contact").reshape(-1, 16)[: st["ncon"][e]]
+            ok, msg = states.compare_contacts(con[:, 13].view(np.int32), con[:, 12], orc.contacts(), margin_tol=1e-4,
+                                              ncon=st["ncon"][e])
+            assert ok, f"N {n} env {e}: {msg}"
+            assert st["nefc"][e] == orc.nefc(), (n, e, st["nefc"][e], orc.nefc())
+            ok, gk, g16, sc = states.cg_progress_ok(orc, opt, q[e], qv[e], qa[e], np.zeros(0, np.float32),
+                                                    dump.get(2, e, "qacc").astype(np.float64))
+            assert ok, (n, e, gk, g16, sc)
+            worst_ratio = max(worst_ratio, gk / max(g16, 1e-6 * sc))
+            ncon += len(con)
+            nenv += 1
+    print(f"mixed boxes: {len(groups)} groups, {nenv} envs, {ncon} contacts, worst remaining-gap ratio to FP64 CG "
+          f"{worst_ratio:.2f}")
+    assert nenv == 96 and ncon > 96
