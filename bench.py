@@ -169,7 +169,8 @@ def run_boxes(B, warmup, steps):
     placement = {}
     for n, _, _, sim in groups:
         placement[n] = ("LDS", "device memory", "LDS, 168 VGPRs", "LDS, 512 registers",
-                        "device memory, 168 VGPRs")[sim.autotune(2)]
+                        "device memory, 168 VGPRs", "LDS, 32 lanes per env",
+                        "LDS, 32 lanes per env, 168 VGPRs")[sim.autotune(2)]
     for k in range(2):
         for _, _, _, sim in groups:
             sim.step(None, 4)

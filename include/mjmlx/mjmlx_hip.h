@@ -67,16 +67,26 @@ MJMLX_API int mjmlx_batched_capacity(const MjmlxBatchedSim* sim, int* maxcon, in
  * control steps from the current state with five kernel variants: workspace
  * in LDS at 256 registers per lane (two waves per SIMD), in device memory, in
  * LDS at 168 VGPRs (three waves per SIMD, with spills), in LDS at 512
- * registers (one wave per SIMD) and in device memory at 168 VGPRs, best of
- * two rounds after a warm-up round; keeps the fastest (choice: 0 LDS,
- * 1 device memory, 2 LDS at 168 VGPRs, 3 LDS at 512 registers, 4 device
- * memory at 168 VGPRs) and restores every buffer the timed steps wrote
+ * registers (one wave per SIMD) and in device memory at 168 VGPRs, and, for
+ * models of at most 32 dofs whose two workspaces fit a workgroup's LDS, two
+ * more with 32 lanes per env (two envs per wave) at 256 registers and at 168
+ * VGPRs; best of two rounds after a warm-up round; keeps the fastest (choice:
+ * 0 LDS, 1 device memory, 2 LDS at 168 VGPRs, 3 LDS at 512 registers,
+ * 4 device memory at 168 VGPRs, 5 LDS with 32 lanes per env, 6 the same at
+ * 168 VGPRs) and restores every buffer the timed steps wrote
  * (qpos, qvel, warm start, ctrl, solver statistics, sensor values and the
  * accelerometers' history, the aux outputs read by the get_xpos / cvel /
  * cfrc_ext getters), so the sim continues exactly as without the call; the
  * timed steps use the current ctrl. No reference counterpart: the
  * reference's kernels have one workspace placement per size class. */
 MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* sim, int steps, int* choice);
+
+/* Selects the step's kernel variant by the autotuner's numbering (0-6).
+ * Returns -1 for a variant the sim cannot run: an LDS variant when the
+ * workspace exceeds a workgroup's LDS, a 32-lane variant for a model of more
+ * than 32 dofs or whose two workspaces exceed it. A device-memory variant
+ * allocates the workspaces on first selection. No reference counterpart. */
+MJMLX_API int mjmlx_batched_set_variant(MjmlxBatchedSim* sim, int choice);
 
 /* Learned warm start (mode 2): replay each control step's launches (features,
  * network, one step launch per substep) as a captured HIP graph instead of

@@ -53,6 +53,9 @@ def _objects(tag, extra):
     jobs += [([HIPCC, *HIP_FLAGS, *extra, *inc, f"-DMJK_NP={k}", f"-DMJK_BIG={big}", "-c", "step_np.hip", "-o",
                os.path.join(obj, f"step_np{k}{sfx}.o")], os.path.join(obj, f"step_np{k}{sfx}.o"))
              for k in NP_SETS for big, sfx in ((0, ""), (1, "_big"))]
+    # and with 32 lanes per env (two envs per wave), small models only
+    jobs += [([HIPCC, *HIP_FLAGS, *extra, *inc, f"-DMJK_NP={k}", "-DMJK_BIG=0", "-DMJK_HALF=1", "-c", "step_np.hip",
+               "-o", os.path.join(obj, f"step_np{k}_half.o")], os.path.join(obj, f"step_np{k}_half.o")) for k in NP_SETS]
     # the largest translation units first
     jobs.sort(key=lambda j: 0 if "step_np" in j[1] else 1)
     with ThreadPoolExecutor(JOBS) as ex:

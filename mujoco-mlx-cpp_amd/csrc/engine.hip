@@ -126,6 +126,8 @@ struct MjmlxBatchedSim {
   bool lds_ok = false;     // the workspace fits the LDS of a workgroup
   bool high_occ = false;   // LDS admits more than 8 envs per CU: the 168-VGPR kernel
   bool big_regs = false;   // LDS admits at most 4 envs per CU: the 512-register kernel
+  bool half = false;       // 32 lanes per env, two envs per wave (LDS workspaces only)
+  bool half_ok = false;    // the model fits the 32-lane kernels (at most 32 dofs, two workspaces per workgroup)
   int ws_mode = 0;
   // learned control step captured as a HIP graph (mjmlx_batched_set_graph):
   // the captured launch sequence and the inputs it was captured for
@@ -134,7 +136,7 @@ struct MjmlxBatchedSim {
   hipGraphExec_t gexec = nullptr;
   int g_n = -1, g_cur = -1, g_mode = -1;
   const void *g_mlp = nullptr, *g_xb = nullptr;
-  bool g_gws = false, g_occ = false, g_big = false;
+  bool g_gws = false, g_occ = false, g_big = false, g_half = false;
   // packed readback
   int* d_pack_bodies = nullptr;
   float* d_pack = nullptr;
@@ -197,6 +199,7 @@ void free_sim(MjmlxBatchedSim* s) {
 // kernel variant of the sim's current workspace placement and register budget
 int step_variant(const MjmlxBatchedSim* s) {
   if (s->use_gws) return s->high_occ ? mjk::VAR_GWS_OCC : mjk::VAR_GWS;
+  if (s->half) return s->high_occ ? mjk::VAR_LDS_OCC_HALF : mjk::VAR_LDS_HALF;
   if (s->high_occ) return mjk::VAR_LDS_OCC;
   return s->big_regs ? mjk::VAR_LDS_BIG : mjk::VAR_LDS;
 }
@@ -291,7 +294,8 @@ bool step_graph(MjmlxBatchedSim* s, int n) {
   if (!s->graph_on || s->ws_mode != 2 || !s->mlp || !s->feat_valid) return false;
   s->mlp->ensure(s->B);  // allocations happen before any capture
   if (!s->gexec || s->g_n != n || s->g_cur != s->feat_cur || s->g_mlp != s->mlp || s->g_xb != s->mlp->xb ||
-      s->g_mode != s->feat_mode || s->g_gws != s->use_gws || s->g_occ != s->high_occ || s->g_big != s->big_regs) {
+      s->g_mode != s->feat_mode || s->g_gws != s->use_gws || s->g_occ != s->high_occ || s->g_big != s->big_regs ||
+      s->g_half != s->half) {
     if (s->gexec) { HIPOK(hipGraphExecDestroy(s->gexec)); s->gexec = nullptr; }
     if (s->graph) { HIPOK(hipGraphDestroy(s->graph)); s->graph = nullptr; }
     HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -300,6 +304,7 @@ bool step_graph(MjmlxBatchedSim* s, int n) {
     HIPOK(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
     s->g_n = n; s->g_cur = s->feat_cur; s->g_mlp = s->mlp; s->g_xb = s->mlp->xb;
     s->g_mode = s->feat_mode; s->g_gws = s->use_gws; s->g_occ = s->high_occ; s->g_big = s->big_regs;
+    s->g_half = s->half;
   }
   HIPOK(hipEventRecord(s->ev0, s->stream));
   HIPOK(hipGraphLaunch(s->gexec, s->stream));
@@ -603,6 +608,21 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     if (const char* v = std::getenv("MJMLX_HIP_HIGH_OCC")) s->high_occ = std::atoi(v) != 0;
     s->big_regs = !use_gws && !s->high_occ && lds_cu / std::max<size_t>(s->lds_bytes, 1) <= 4;
     if (const char* v = std::getenv("MJMLX_HIP_BIG_REGS")) s->big_regs = !use_gws && !s->high_occ && std::atoi(v) != 0;
+    // 32 lanes per env (two envs' workspaces per workgroup) by default for
+    // models of at most 8 dofs, at 256 registers: measured faster than every
+    // 64-lane variant on hinge chains of 2-8 dofs, slower from 16 dofs on
+    // (quadruped 0.6x, humanoid 0.45x: half the waves per CU at the same LDS
+    // occupancy). MJMLX_HIP_LANES=32 or 64 forces either; the autotuner
+    // times both.
+    s->half_ok = s->nv <= mjk::SPARSE_ROWS_NV && 2 * s->lds_bytes <= (size_t)prop.sharedMemPerBlock &&
+                 2 * s->lds_bytes <= lds_max;
+    s->half = !use_gws && s->half_ok && s->nv <= 8;
+    if (const char* v = std::getenv("MJMLX_HIP_LANES")) s->half = !use_gws && s->half_ok && std::atoi(v) == 32;
+    if (s->half) {
+      s->big_regs = false;
+      s->high_occ = false;
+      if (const char* v = std::getenv("MJMLX_HIP_HIGH_OCC")) s->high_occ = std::atoi(v) != 0;
+    }
     if (use_gws) HIPOK(hipMalloc(&s->d_gws, (size_t)s->B * s->L.total * sizeof(float)));
     HIPOK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIPOK(hipEventCreate(&s->ev0));
@@ -1101,16 +1121,23 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
       HIPOK(hipEventCreate(&e1));
       // candidates: 0 LDS at 256 registers (two waves per SIMD), 1 device
       // memory, 2 LDS at 168 VGPRs (three waves per SIMD, some spills), 3 LDS
-      // at 512 registers (one wave per SIMD), 4 device memory at 168 VGPRs
-      float ms[5] = {3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f};
+      // at 512 registers (one wave per SIMD), 4 device memory at 168 VGPRs,
+      // and for models that fit them, 32 lanes per env: 5 at 256 registers, 6
+      // at 168 VGPRs
+      const int ncand = s->half_ok ? 7 : 5;
+      float ms[7] = {3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f};
+      auto pick = [&](int g) {
+        s->use_gws = g == 1 || g == 4;
+        s->high_occ = g == 2 || g == 4 || g == 6;
+        s->big_regs = g == 3;
+        s->half = g == 5 || g == 6;
+      };
       mjk::StepIO io = make_io(s, nullptr, nullptr);
       // the device is drained first so that other work does not share the timing
       HIPOK(hipDeviceSynchronize());
       for (int rep = 0; rep < 3; rep++)
-        for (int g = 0; g < 5; g++) {
-          s->use_gws = g == 1 || g == 4;
-          s->high_occ = g == 2 || g == 4;
-          s->big_regs = g == 3;
+        for (int g = 0; g < ncand; g++) {
+          pick(g);
           restore();
           HIPOK(hipEventRecord(e0, s->stream));
           for (int k = 0; k < steps; k++) launch_step(s, io, 4);
@@ -1122,11 +1149,9 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
           if (rep > 0) ms[g] = std::min(ms[g], t);  // the first round warms caches and code
         }
       int best = 0;
-      for (int g = 1; g < 5; g++)
+      for (int g = 1; g < ncand; g++)
         if (ms[g] < ms[best]) best = g;
-      s->use_gws = best == 1 || best == 4;
-      s->high_occ = best == 2 || best == 4;
-      s->big_regs = best == 3;
+      pick(best);
       restore();
       HIPOK(hipStreamSynchronize(s->stream));
       (void)hipEventDestroy(e0);
@@ -1137,6 +1162,21 @@ MJMLX_API int mjmlx_batched_autotune(MjmlxBatchedSim* s, int steps, int* choice)
     if (choice) *choice = step_variant(s);
     return 0;
   } catch (const std::exception& e) { report("mjmlx_batched_autotune", e); return -1; }
+}
+
+MJMLX_API int mjmlx_batched_set_variant(MjmlxBatchedSim* s, int choice) {
+  if (!s || choice < 0 || choice >= mjk::VAR_N) return -1;
+  bool gws = choice == 1 || choice == 4, half = choice == 5 || choice == 6;
+  if (!gws && !s->lds_ok) return -1;  // LDS variants: the workspace must fit a workgroup's LDS
+  if (half && !s->half_ok) return -1;
+  try {
+    if (gws && !s->d_gws) HIPOK(hipMalloc(&s->d_gws, (size_t)s->B * s->L.total * sizeof(float)));
+    s->use_gws = gws;
+    s->high_occ = choice == 2 || choice == 4 || choice == 6;
+    s->big_regs = choice == 3;
+    s->half = half;
+    return 0;
+  } catch (const std::exception& e) { report("mjmlx_batched_set_variant", e); return -1; }
 }
 
 MJMLX_API int mjmlx_batched_set_graph(MjmlxBatchedSim* s, int on) {

@@ -25,11 +25,22 @@
 #define MJK_BIG 1
 #endif
 
+// MJK_HALF 1 compiles the step with 32 lanes per env: each 64-lane wave
+// steps two envs, one per half (lanes 0-31 and 32-63), each with its own
+// workspace; only for models of at most 32 dofs (MJK_BIG 0).
 #if defined(MJ_EMU)
 #define KDEV static inline
 #define WAVE 1
 #define LANE 0
 #define KSYNC() ((void)0)
+#elif defined(MJK_HALF) && MJK_HALF
+#if MJK_BIG
+#error "the 32-lane step is built for models of at most 32 dofs only (MJK_BIG 0)"
+#endif
+#define KDEV __device__ __forceinline__
+#define WAVE 32
+#define LANE ((int)(threadIdx.x & 31))
+#define KSYNC() __syncthreads()
 #else
 #define KDEV __device__ __forceinline__
 #define WAVE 64
@@ -66,10 +77,12 @@ enum { KP_KIN = 0, KP_COMPOS, KP_CRB, KP_FACTOR, KP_COMVEL, KP_SMOOTH, KP_COLL, 
 
 // ------------------------------------------------------------------ wave helpers
 // Wave-wide reductions: DPP exchanges inside each 16-lane row (quad
-// permutations, half-row and row mirrors), then the four row totals are read
-// into scalar registers and added in a fixed order, so every lane gets the
-// same bit pattern and the result can drive uniform control flow.
-#if WAVE == 64
+// permutations, half-row and row mirrors), then the row totals are read into
+// scalar registers and added in a fixed order, so every lane gets the same bit
+// pattern and the result can drive uniform control flow. With 32 lanes per env
+// each half of the wave reduces its own two rows (rows 0-1, rows 2-3) and a
+// lane takes its half's total.
+#if WAVE >= 32
 KDEV int dpp_i(int v, int ctrl_sel) {
   switch (ctrl_sel) {
     case 0: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
@@ -79,73 +92,103 @@ KDEV int dpp_i(int v, int ctrl_sel) {
   }
 }
 KDEV float dpp_f(float v, int c) { return __int_as_float(dpp_i(__float_as_int(v), c)); }
-KDEV float rl_f(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
-KDEV double rl_d(double v, int lane) {
+KDEV int rl_abs(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+KDEV float rl_absf(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
+KDEV double rl_absd(double v, int lane) {
   long long b = __double_as_longlong(v);
   int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
   int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+#if WAVE == 64
+// value of lane `lane` (uniform) of this env's lanes
+KDEV float rl_f(float v, int lane) { return rl_absf(v, lane); }
+KDEV double rl_d(double v, int lane) { return rl_absd(v, lane); }
+#define WHALF_SEL(a, b) (a)
+#else
+KDEV bool whi() { return (threadIdx.x & 32) != 0; }
+KDEV float rl_f(float v, int lane) { float a = rl_absf(v, lane), b = rl_absf(v, lane + 32); return whi() ? b : a; }
+KDEV double rl_d(double v, int lane) { double a = rl_absd(v, lane), b = rl_absd(v, lane + 32); return whi() ? b : a; }
+#define WHALF_SEL(a, b) (whi() ? (b) : (a))
+#endif
 #endif
 KDEV float wsum(float v) {
-#if WAVE == 64
+#if WAVE >= 32
   v += dpp_f(v, 0);
   v += dpp_f(v, 1);
   v += dpp_f(v, 2);
   v += dpp_f(v, 3);
-  return (rl_f(v, 0) + rl_f(v, 16)) + (rl_f(v, 32) + rl_f(v, 48));
+#if WAVE == 64
+  return (rl_absf(v, 0) + rl_absf(v, 16)) + (rl_absf(v, 32) + rl_absf(v, 48));
+#else
+  return WHALF_SEL(rl_absf(v, 0) + rl_absf(v, 16), rl_absf(v, 32) + rl_absf(v, 48));
+#endif
 #else
   return v;
 #endif
 }
 KDEV double wsumd(double v) {
-#if WAVE == 64
+#if WAVE >= 32
   for (int c = 0; c < 4; c++) {
     long long b = __double_as_longlong(v);
     int lo = dpp_i((int)(b & 0xffffffffll), c), hi = dpp_i((int)(b >> 32), c);
     v += __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
   }
-  return (rl_d(v, 0) + rl_d(v, 16)) + (rl_d(v, 32) + rl_d(v, 48));
+#if WAVE == 64
+  return (rl_absd(v, 0) + rl_absd(v, 16)) + (rl_absd(v, 32) + rl_absd(v, 48));
+#else
+  return WHALF_SEL(rl_absd(v, 0) + rl_absd(v, 16), rl_absd(v, 32) + rl_absd(v, 48));
+#endif
 #else
   return v;
 #endif
 }
 KDEV int wsumi(int v) {
-#if WAVE == 64
+#if WAVE >= 32
   for (int c = 0; c < 4; c++) v += dpp_i(v, c);
-  return (__builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16)) +
-         (__builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48));
+#if WAVE == 64
+  return (rl_abs(v, 0) + rl_abs(v, 16)) + (rl_abs(v, 32) + rl_abs(v, 48));
+#else
+  return WHALF_SEL(rl_abs(v, 0) + rl_abs(v, 16), rl_abs(v, 32) + rl_abs(v, 48));
+#endif
 #else
   return v;
 #endif
 }
 KDEV int wori(int v) {
-#if WAVE == 64
+#if WAVE >= 32
   for (int c = 0; c < 4; c++) v |= dpp_i(v, c);
-  return __builtin_amdgcn_readlane(v, 0) | __builtin_amdgcn_readlane(v, 16) | __builtin_amdgcn_readlane(v, 32) |
-         __builtin_amdgcn_readlane(v, 48);
+#if WAVE == 64
+  return rl_abs(v, 0) | rl_abs(v, 16) | rl_abs(v, 32) | rl_abs(v, 48);
+#else
+  return WHALF_SEL(rl_abs(v, 0) | rl_abs(v, 16), rl_abs(v, 32) | rl_abs(v, 48));
+#endif
 #else
   return v;
 #endif
 }
 KDEV int wmini(int v) {
-#if WAVE == 64
+#if WAVE >= 32
   for (int c = 0; c < 4; c++) { int t = dpp_i(v, c); v = t < v ? t : v; }
-  int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
-  int c2 = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+  int a = rl_abs(v, 0), b = rl_abs(v, 16);
+  int c2 = rl_abs(v, 32), d = rl_abs(v, 48);
   a = a < b ? a : b;
   c2 = c2 < d ? c2 : d;
+#if WAVE == 64
   return a < c2 ? a : c2;
+#else
+  return WHALF_SEL(a, c2);
+#endif
 #else
   return v;
 #endif
 }
-// exclusive prefix sum over the wave
+// exclusive prefix sum over this env's lanes
 KDEV int wscan_excl(int v) {
-#if WAVE == 64
+#if WAVE >= 32
   int x = v;
-  for (int o = 1; o < 64; o <<= 1) {
-    int t = __shfl_up(x, o, 64);
+  for (int o = 1; o < WAVE; o <<= 1) {
+    int t = __shfl_up(x, o, WAVE);
     if (LANE >= o) x += t;
   }
   return x - v;
@@ -518,7 +561,7 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
 // L z = y from the roots, each dof gathering from its (final) ancestors.
 KDEV void solve_ld(const DModel& m, const float* LD, float* x) {
   int nv = m.nv;
-#if WAVE == 64
+#if WAVE >= 32
   if (!MJK_BIG || nv <= WAVE) {
     // the next step's factor entry is loaded ahead of the current step's
     // readlane -> FMA chain, so the LDS latency overlaps it
@@ -2354,7 +2397,7 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
   const double* idg = dptr(w, L.dvec);
   const float* grad = w + L.grad;
   float* Mg = w + L.Mgrad;
-#if WAVE == 64
+#if WAVE >= 32
   if (MJK_BIG && nv > WAVE) {
     double* y = dptr(w, L.dvec) + nv;
     for (int i = LANE; i < nv; i += WAVE) y[i] = grad[i];
@@ -2421,7 +2464,7 @@ struct LsRows { float jar[2], jv[2], d[2]; int ty[2]; int on; };
 
 KDEV void ls_rows(const Layout& L, const float* w, int nefc, LsRows* c) {
   c->on = 0;
-#if WAVE == 64
+#if WAVE >= 32
   if (nefc <= 2 * WAVE) {
     c->on = 1;
     const float *Jaref = w + L.efc_Jaref, *Jv = w + L.efc_Jv, *D = w + L.efc_D;

@@ -20,7 +20,9 @@ enum {
   VAR_LDS_OCC = 2,  // LDS, 168 VGPRs (three waves per SIMD)
   VAR_LDS_BIG = 3,  // LDS, the whole register file (one wave per SIMD)
   VAR_GWS_OCC = 4,  // device memory, 168 VGPRs
-  VAR_N = 5
+  VAR_LDS_HALF = 5,      // LDS, 32 lanes per env (two envs per wave), 256 registers
+  VAR_LDS_OCC_HALF = 6,  // LDS, 32 lanes per env, 168 VGPRs
+  VAR_N = 7
 };
 
 struct StepLaunch {
@@ -49,9 +51,28 @@ void launch_step_np4_big(const StepLaunch& l, const StepIO& io, int nsub);
 void launch_step_np5_big(const StepLaunch& l, const StepIO& io, int nsub);
 void launch_step_np6_big(const StepLaunch& l, const StepIO& io, int nsub);
 
+// ... and with 32 lanes per env (models of at most SPARSE_ROWS_NV dofs)
+void launch_step_np0_half(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np1_half(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np2_half(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np4_half(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np5_half(const StepLaunch& l, const StepIO& io, int nsub);
+void launch_step_np6_half(const StepLaunch& l, const StepIO& io, int nsub);
+
 // npset: narrow-phase set (+4 with sensors); big: the model has more than
-// SPARSE_ROWS_NV dofs
+// SPARSE_ROWS_NV dofs; the 32-lane variants (VAR_*_HALF) need a small model
 inline void launch_step_np(int npset, bool big, const StepLaunch& l, const StepIO& io, int nsub) {
+  if (l.variant == VAR_LDS_HALF || l.variant == VAR_LDS_OCC_HALF) {
+    switch (npset) {
+      case 0: launch_step_np0_half(l, io, nsub); break;
+      case 1: launch_step_np1_half(l, io, nsub); break;
+      case 2: launch_step_np2_half(l, io, nsub); break;
+      case 4: launch_step_np4_half(l, io, nsub); break;
+      case 5: launch_step_np5_half(l, io, nsub); break;
+      default: launch_step_np6_half(l, io, nsub); break;
+    }
+    return;
+  }
   if (big) {
     switch (npset) {
       case 0: launch_step_np0_big(l, io, nsub); break;

@@ -14,7 +14,9 @@
 
 #define MJK_CAT2(a, b) a##b
 #define MJK_CAT(a, b) MJK_CAT2(a, b)
-#if MJK_BIG
+#if defined(MJK_HALF) && MJK_HALF
+#define MJK_LAUNCH MJK_CAT(MJK_CAT(launch_step_np, MJK_NP), _half)
+#elif MJK_BIG
 #define MJK_LAUNCH MJK_CAT(MJK_CAT(launch_step_np, MJK_NP), _big)
 #else
 #define MJK_LAUNCH MJK_CAT(launch_step_np, MJK_NP)
@@ -57,6 +59,44 @@ k_step_big(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int ns
   mjk::run_env<NPSET>(m, L, lds, io, env0 + (int)blockIdx.x, nsub);
 }
 
+#if defined(MJK_HALF) && MJK_HALF
+// 32 lanes per env: each 64-lane workgroup steps envs 2b (lanes 0-31) and
+// 2b + 1 (lanes 32-63), each half with its own LDS workspace; the second half
+// of the last workgroup is idle when B is odd. Register budgets as k_step and
+// k_step_occ.
+template <int NPSET>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_step_half(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int nsub) {
+  extern __shared__ __align__(16) float lds[];
+  int h = (int)(threadIdx.x >> 5), e = 2 * (int)blockIdx.x + h;
+  if (e >= B) return;
+  mjk::run_env<NPSET>(m, L, lds + h * L.total, io, env0 + e, nsub);
+}
+
+template <int NPSET>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+k_step_half_occ(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int nsub) {
+  extern __shared__ __align__(16) float lds[];
+  int h = (int)(threadIdx.x >> 5), e = 2 * (int)blockIdx.x + h;
+  if (e >= B) return;
+  mjk::run_env<NPSET | 8>(m, L, lds + h * L.total, io, env0 + e, nsub);
+}
+
+namespace mjk {
+
+void MJK_LAUNCH(const StepLaunch& l, const StepIO& io, int nsub) {
+  // bit 32 of the set names the 32-lane kernels
+  constexpr int NP = MJK_NP | 32;
+  dim3 g((l.B + 1) / 2), b(64);
+  size_t sh = 2 * l.lds_bytes;
+  if (l.variant == VAR_LDS_OCC_HALF)
+    hipLaunchKernelGGL((k_step_half_occ<NP>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub);
+  else
+    hipLaunchKernelGGL((k_step_half<NP>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub);
+}
+
+}  // namespace mjk
+#else
 namespace mjk {
 
 void MJK_LAUNCH(const StepLaunch& l, const StepIO& io, int nsub) {
@@ -85,3 +125,4 @@ void MJK_LAUNCH(const StepLaunch& l, const StepIO& io, int nsub) {
 }
 
 }  // namespace mjk
+#endif

@@ -96,6 +96,7 @@ def lib() -> C.CDLL:
             "mjmlx_batched_last_step_ms": (C.c_float, [C.c_void_p]),
             "mjmlx_batched_capacity": (C.c_int, [C.c_void_p, _I, _I, _I]),
             "mjmlx_batched_autotune": (C.c_int, [C.c_void_p, C.c_int, _I]),
+            "mjmlx_batched_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
             "mjmlx_batched_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
             "mjmlx_batched_debug_step": (C.c_int, [C.c_void_p, _F, _F, _F]),
             "mjmlx_batched_layout": (C.c_int, [C.c_void_p, C.c_char_p, _I, _I, _I]),
@@ -291,11 +292,17 @@ class BatchedSim:
     def autotune(self, steps: int = 3) -> int:
         """Time the step's kernel variants from the current state and keep the
         fastest; returns 0 (LDS workspace, 256 registers), 1 (device memory),
-        2 (LDS workspace, 168 VGPRs), 3 (LDS workspace, 512 registers) or
-        4 (device memory, 168 VGPRs). The state is restored."""
+        2 (LDS workspace, 168 VGPRs), 3 (LDS workspace, 512 registers),
+        4 (device memory, 168 VGPRs), 5 (LDS workspace, 32 lanes per env) or
+        6 (the same at 168 VGPRs). The state is restored."""
         c = C.c_int()
         assert lib().mjmlx_batched_autotune(self.h, steps, C.byref(c)) == 0
         return c.value
+
+    def set_variant(self, choice: int) -> bool:
+        """Select the kernel variant by the autotuner's numbering; False when
+        the sim cannot run it."""
+        return lib().mjmlx_batched_set_variant(self.h, int(choice)) == 0
 
     def capacity(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

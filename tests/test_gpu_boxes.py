@@ -134,7 +134,7 @@ def test_autotune_keeps_state():
             sim.step(None, 4)
         if tune:
             choice = sim.autotune(2)
-            assert choice in (0, 1, 2, 3, 4)
+            assert choice in (0, 1, 2, 3, 4, 5, 6)
         for _ in range(5):
             sim.step(None, 4)
         out.append(sim.state())
