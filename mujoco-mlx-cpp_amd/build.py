@@ -103,6 +103,8 @@ def build_all():
     if os.path.exists(os.path.join(CSRC, "mjb.cpp")):
         build_mjb()
     build_emu()
+    # the stage profiles (bench.py --stages) always describe the current kernels
+    build_prof()
 
 
 if __name__ == "__main__":
