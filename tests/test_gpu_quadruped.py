@@ -4,7 +4,8 @@ Bars: contact pair lists and geom ids identical;
 single-step qacc and efc_force within 1e-4 relative (relative to the largest
 magnitude of the oracle's vector, floored at 1); determinism of duplicate envs
 and of the two launch shapes; a 1000-step rollout stays finite, with its
-divergence from the oracle reported.
+divergence from the oracle reported; the 4096-env workload's 1000-step
+rollout checked against the oracle at every step on 64 sampled envs.
 """
 from __future__ import annotations
 
@@ -169,6 +170,64 @@ def test_rollout_1000_steps(model):
     print("qpos divergence from the FP64 oracle (median, max over 64 envs): "
           + ", ".join(f"step {k}: {m:.1e} / {x:.1e}" for k, (m, x) in report.items()))
     assert report[1][1] < 1e-4
+
+
+def test_rollout_checked_every_step(model):
+    """The FP32 workload's rollout (self-collision, 4096 envs, actions nominal
+    + U(+-0.5) from seed 1, 1000 control steps of 4 substeps) with 64 sampled
+    envs (every 64th) checked against the FP64 oracle at every step: one oracle
+    control step from the engine's own FP32 state, warm start and actions of
+    the step before. Bar: qpos within 1e-5 and qvel within 1e-4 (relative to
+    max(1, |qvel|)) of the oracle. A sample beyond it must lie within the bar
+    of the oracle's step from one of 16 copies of its input state perturbed at
+    FP32 rounding level (qpos x (1 + U(+-6e-8))): a contact whose distance
+    crosses zero within FP32 position resolution (about 3e-8 m at the torso's
+    height) switches on in one branch and off in the other, and FP32 state
+    arithmetic lands on either."""
+    import ref64
+    B, NS, STEPS = 4096, 64, 1000
+    sample = np.arange(0, B, B // NS)
+    sim = capi.BatchedSim(model, B)
+    orc = ref64.Oracle(ref64.fields_from(model.field))
+    rng = np.random.default_rng(1)
+    sim.set_state(scenes.quadruped_initial_qpos(B, 0).astype(np.float32), np.zeros((B, model.nv), np.float32))
+    q, v = sim.state()
+    qa = sim.qacc_warmstart()
+
+    def errors(qk, vk, qo, vo):
+        return (np.abs(qk - qo).max(axis=-1),
+                np.abs(vk - vo).max(axis=-1) / np.maximum(1.0, np.abs(vo).max(axis=-1)))
+
+    eq_all, ev_all, branched = [], [], []
+    for k in range(STEPS):
+        a = scenes.quadruped_actions(B, rng).astype(np.float32)
+        qs, vs, as_ = (x[sample].astype(np.float64) for x in (q, v, qa))
+        cs = a[sample].astype(np.float64)
+        sim.step(a, 4)
+        q, v = sim.state()
+        qa = sim.qacc_warmstart()
+        qo, vo, ao = qs.copy(), vs.copy(), as_.copy()
+        orc.batch_step(qo, vo, ao, cs, 4, 8)
+        eq, ev = errors(q[sample], v[sample], qo, vo)
+        eq_all.append(eq)
+        ev_all.append(ev)
+        for i in np.nonzero((eq > 1e-5) | (ev > 1e-4))[0]:
+            rs = np.random.default_rng(k * NS + i)
+            qp = qs[i] * (1 + rs.uniform(-6e-8, 6e-8, (16, qs.shape[1])))
+            vp, ap = np.tile(vs[i], (16, 1)), np.tile(as_[i], (16, 1))
+            orc.batch_step(qp, vp, ap, np.tile(cs[i], (16, 1)), 4, 8)
+            pq, pv = errors(q[sample[i]], v[sample[i]], qp, vp)
+            hit = (pq <= 1e-5) & (pv <= 1e-4)
+            assert hit.any(), (k, int(sample[i]), float(eq[i]), float(ev[i]), float(pq.min()), float(pv.min()))
+            branched.append((k, int(sample[i]), float(ev[i]), int(hit.sum())))
+    st = sim.stats()
+    assert np.isfinite(q).all() and np.isfinite(v).all() and not (st["status"] & ~3).any()
+    eq_all, ev_all = np.concatenate(eq_all), np.concatenate(ev_all)
+    print(f"{STEPS} steps x {NS} sampled envs: qpos error median {np.median(eq_all):.1e}, p99 "
+          f"{np.quantile(eq_all, 0.99):.1e}, max {eq_all.max():.1e}; qvel error median {np.median(ev_all):.1e}, "
+          f"p99 {np.quantile(ev_all, 0.99):.1e}, max {ev_all.max():.1e}; beyond the bar on the oracle's own "
+          f"FP32-perturbed branch (step, env, qvel error, matching draws of 16): {branched}")
+    assert len(branched) <= STEPS * NS // 1000
 
 
 def test_state_roundtrip_and_reset(model):
