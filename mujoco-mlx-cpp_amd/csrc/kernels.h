@@ -2163,8 +2163,9 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
 // H = L D L^T (right-looking, one LDS phase per column, the column left
 // unscaled); the reciprocal pivots go to dvec.
 // TRI_CACHE: keep each lane's triangle coordinates in registers across the
-// elimination steps (off in the register-capped kernel variant, where the
-// extra live registers cost more spills than they save)
+// elimination steps for six rounds of lanes (nv <= 27 on 64 lanes); three in
+// the register-capped kernel variant (nv <= 20), where more live registers
+// cost more spills than they save
 template <bool TRI_CACHE>
 KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KProfAcc* pa) {
   int nv = m.nv;
@@ -2298,10 +2299,10 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
   // the same (row, col) at every elimination step: for up to TRI_R rounds of
   // lanes they are computed once here (packed row | col << 16) and held in
   // registers by the unrolled loops below, instead of tri_rc per entry and step.
-  constexpr int TRI_R = TRI_CACHE && WAVE == 64 ? 6 : 1;
+  constexpr int TRI_R = WAVE >= 32 ? (TRI_CACHE ? 6 : 3) * (64 / WAVE) : 1;
   int tri_ab[TRI_R];
   bool tri_cached = false;
-  if constexpr (TRI_CACHE && WAVE == 64) {
+  if constexpr (WAVE >= 32) {
     tri_cached = ntri <= TRI_R * WAVE;
 #pragma unroll
     for (int r = 0; r < TRI_R; r++) {
@@ -2325,7 +2326,7 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     double i1 = 1.0 / d1;
     if (LANE == 0) idg[k] = i0;  // the deferred pass below takes c from it
     int mm = nv - k - 2, nt = mm * (mm + 1) / 2;
-    if constexpr (TRI_CACHE && WAVE == 64) {
+    if constexpr (WAVE >= 32) {
       if (tri_cached) {
 #pragma unroll
         for (int r = 0; r < TRI_R; r++) {
@@ -2355,7 +2356,7 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     double ik = 1.0 / d0;
     int nt = (nv - k - 1) * (nv - k) / 2;
     bool done = false;
-    if constexpr (TRI_CACHE && WAVE == 64) {
+    if constexpr (WAVE >= 32) {
       if (tri_cached) {
 #pragma unroll
         for (int r = 0; r < TRI_R; r++) {
