@@ -38,6 +38,16 @@ enum { MAX_DOF_DEPTH = 64 };
 // tiles (32 x 32).
 enum { SPARSE_ROWS_NV = 32 };
 
+// Device passes see the model's arrays as global memory (address space 1), so
+// loads through them stay global loads wherever the struct itself is read
+// from (a kernel argument or a device-memory copy); host code and the CPU
+// interpreter see plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MJK_G __attribute__((address_space(1)))
+#else
+#define MJK_G
+#endif
+
 struct DModel {
   int nq, nv, nu, nbody, njnt, ngeom, npair;
   int nlevel, ndoflevel, maxdepth, nfloss, nlim;
@@ -51,38 +61,38 @@ struct DModel {
   float magnetic[3];
   // pointers only from here on: Blob::bind relocates every word after body_parentid
   // bodies
-  const int *body_parentid, *body_rootid, *body_jntnum, *body_jntadr, *body_dofnum, *body_dofadr,
+  const MJK_G int *body_parentid, *body_rootid, *body_jntnum, *body_jntadr, *body_dofnum, *body_dofadr,
       *body_subtree_end, *level_adr, *level_body, *body_lastdof;
-  const float *body_pos, *body_quat, *body_ipos, *body_iquat, *body_mass, *body_inertia,
+  const MJK_G float *body_pos, *body_quat, *body_ipos, *body_iquat, *body_mass, *body_inertia,
       *body_subtreemass, *body_invweight0;
   // joints
-  const int *jnt_type, *jnt_qposadr, *jnt_dofadr, *jnt_bodyid;
-  const float *jnt_pos, *jnt_axis, *jnt_range, *jnt_margin, *jnt_solref, *jnt_solimp, *jnt_stiffness;
-  const float *qpos0, *qpos_spring;
+  const MJK_G int *jnt_type, *jnt_qposadr, *jnt_dofadr, *jnt_bodyid;
+  const MJK_G float *jnt_pos, *jnt_axis, *jnt_range, *jnt_margin, *jnt_solref, *jnt_solimp, *jnt_stiffness;
+  const MJK_G float *qpos0, *qpos_spring;
   // dofs
-  const int *dof_bodyid, *dof_jntid, *dof_parentid, *dof_depth, *dof_subtree_end, *doflevel_adr,
+  const MJK_G int *dof_bodyid, *dof_jntid, *dof_parentid, *dof_depth, *dof_subtree_end, *doflevel_adr,
       *doflevel_dof, *dof_chain, *dof_nchain, *dof_act_adr, *dof_act_num, *dof_act_list;
   // level-parallel LDL of qM: level l updates entries [flev_adr[l], flev_adr[l+1]),
   // entry e is (flev_i[e], flev_j[e]) and sums over dofs flev_k[flev_kadr[e] .. + flev_kcnt[e])
-  const int *flev_adr, *flev_i, *flev_j, *flev_kadr, *flev_kcnt, *flev_k;
-  const float *dof_armature, *dof_damping, *dof_frictionloss, *dof_invweight0, *dof_solref, *dof_solimp;
+  const MJK_G int *flev_adr, *flev_i, *flev_j, *flev_kadr, *flev_kcnt, *flev_k;
+  const MJK_G float *dof_armature, *dof_damping, *dof_frictionloss, *dof_invweight0, *dof_solref, *dof_solimp;
   // geoms
-  const int *geom_type, *geom_bodyid, *geom_fclass;
-  const float *geom_size, *geom_pos, *geom_quat, *geom_rbound;
+  const MJK_G int *geom_type, *geom_bodyid, *geom_fclass;
+  const MJK_G float *geom_size, *geom_pos, *geom_quat, *geom_rbound;
   // static pair table (geom1 has the lower type)
-  const int *pair_g1, *pair_g2, *pair_b1, *pair_b2, *pair_condim;
-  const float *pair_friction, *pair_solref, *pair_solimp, *pair_margin, *pair_gap;
+  const MJK_G int *pair_g1, *pair_g2, *pair_b1, *pair_b2, *pair_condim;
+  const MJK_G float *pair_friction, *pair_solref, *pair_solimp, *pair_margin, *pair_gap;
   // actuators
-  const int *act_dof, *act_qadr, *act_ctrllimited, *act_forcelimited;
-  const float *act_gear, *act_gain, *act_bias, *act_ctrlrange, *act_forcerange;
+  const MJK_G int *act_dof, *act_qadr, *act_ctrllimited, *act_forcelimited;
+  const MJK_G float *act_gear, *act_gain, *act_bias, *act_ctrlrange, *act_forcerange;
   // constraint source tables
-  const int *floss_dof, *lim_jnt;
+  const MJK_G int *floss_dof, *lim_jnt;
   // sensors (MuJoCo mjtSensor ids): sensordata address and width, frame or
   // subtree body (-1 for joint sensors) with the frame's pose in that body,
   // qpos / dof address and gear of joint and actuator sensors, accelerometer
   // slot (-1 for other sensors)
-  const int *sens_type, *sens_adr, *sens_dim, *sens_body, *sens_qadr, *sens_dadr, *sens_acc;
-  const float *sens_pos, *sens_quat, *sens_gear;
+  const MJK_G int *sens_type, *sens_adr, *sens_dim, *sens_body, *sens_qadr, *sens_dadr, *sens_acc;
+  const MJK_G float *sens_pos, *sens_quat, *sens_gear;
 };
 // sensor kinds (MuJoCo mjtSensor)
 enum { SENS_ACCELEROMETER = 1, SENS_VELOCIMETER = 2, SENS_GYRO = 3, SENS_MAGNETOMETER = 6, SENS_JOINTPOS = 9,

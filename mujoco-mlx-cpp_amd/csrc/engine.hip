@@ -121,6 +121,7 @@ struct MjmlxBatchedSim {
   int persistent = 0, grid = 0, ncu = 0;
   int npset = 0;  // narrow-phase set compiled into the step kernel
   float* d_gws = nullptr;  // per-env workspaces in device memory when they exceed the LDS of a workgroup
+  mjk::KParams* d_kp = nullptr;  // device copy of dm and L, which the step kernels read
   bool ctrl_zero = true;   // d_ctrl holds zeros (set at creation; cleared by an upload)
   bool use_gws = false;    // launch the device-memory workspace variant
   bool lds_ok = false;     // the workspace fits the LDS of a workgroup
@@ -179,7 +180,7 @@ void free_sim(MjmlxBatchedSim* s) {
   void* dev[] = {s->d_blob, s->d_qpos, s->d_qvel, s->d_qws, s->d_ctrl, s->d_qpos0, s->d_niter, s->d_status,
                  s->d_ncon, s->d_nefc, s->d_queue, s->d_mask, s->d_pack_bodies, s->d_pack, s->d_dump1, s->d_dump2,
                  s->d_feat[0], s->d_feat[1], s->d_pred, s->d_gws, s->d_rand, s->d_rays, s->d_rdist,
-                 s->d_rgid, s->d_gmask, s->d_sensordata, s->d_sens_prev};
+                 s->d_rgid, s->d_gmask, s->d_sensordata, s->d_sens_prev, s->d_kp};
   for (void* p : dev) if (p) hipFree(p);
   for (int i = 0; i < 7; i++) { if (s->d_aux[i]) hipFree(s->d_aux[i]); if (s->h_aux[i]) hipHostFree(s->h_aux[i]); }
   if (s->h_qpos) hipHostFree(s->h_qpos);
@@ -196,6 +197,14 @@ void free_sim(MjmlxBatchedSim* s) {
   delete s;
 }
 
+// dm and L to the device copy the step kernels read, once the steps queued
+// before have read the previous contents
+void upload_kparams(MjmlxBatchedSim* s) {
+  mjk::KParams kp{s->dm, s->L};
+  HIPOK(hipStreamSynchronize(s->stream));
+  HIPOK(hipMemcpy(s->d_kp, &kp, sizeof kp, hipMemcpyHostToDevice));
+}
+
 // kernel variant of the sim's current workspace placement and register budget
 int step_variant(const MjmlxBatchedSim* s) {
   if (s->use_gws) return s->high_occ ? mjk::VAR_GWS_OCC : mjk::VAR_GWS;
@@ -208,7 +217,7 @@ int step_variant(const MjmlxBatchedSim* s) {
 // placement (device memory, LDS at three, two or one waves per SIMD), compiled
 // for the model's narrow-phase set and, with sensors, the sensor stage
 void launch_step(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
-  mjk::StepLaunch l{s->dm, s->L, s->B, 0, step_variant(s), s->lds_bytes, s->d_gws, s->stream};
+  mjk::StepLaunch l{s->d_kp, s->B, 0, step_variant(s), s->lds_bytes, s->d_gws, s->stream};
   mjk::launch_step_np(s->npset | (s->model.nsensor > 0 ? 4 : 0), s->nv > mjk::SPARSE_ROWS_NV, l, io, nsub);
 }
 
@@ -631,6 +640,8 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     HIPOK(hipMalloc(&s->d_blob, s->blob.bytes.size()));
     HIPOK(hipMemcpy(s->d_blob, s->blob.bytes.data(), s->blob.bytes.size(), hipMemcpyHostToDevice));
     s->dm = s->blob.bind(s->d_blob);
+    HIPOK(hipMalloc(&s->d_kp, sizeof(mjk::KParams)));
+    upload_kparams(s);
     size_t B = s->B;
     HIPOK(hipMalloc(&s->d_qpos, B * std::max(1, m.nq) * sizeof(float)));
     HIPOK(hipMalloc(&s->d_qvel, B * std::max(1, m.nv) * sizeof(float)));
@@ -744,7 +755,7 @@ MJMLX_API int mjmlx_batched_step_n_noalloc(MjmlxBatchedSim* sim, const float* ct
     upload_ctrl(sim, ctrl);
     mjk::StepIO io = make_io(sim, nullptr, nullptr);
     int var = sim->use_gws ? step_variant(sim) : mjk::VAR_GWS;
-    mjk::StepLaunch l{sim->dm, sim->L, sim->B, 0, var, 0, ws, sim->stream};
+    mjk::StepLaunch l{sim->d_kp, sim->B, 0, var, 0, ws, sim->stream};
     HIPOK(hipEventRecord(sim->ev0, sim->stream));
     mjk::launch_step_np(sim->npset | (sim->model.nsensor > 0 ? 4 : 0), sim->nv > mjk::SPARSE_ROWS_NV, l, io, n);
     HIPOK(hipGetLastError());
@@ -1033,10 +1044,13 @@ MJMLX_API int mjmlx_batched_set_warmstart_mode(MjmlxBatchedSim* s, int mode) {
     std::fprintf(stderr, "mjmlx_batched_set_warmstart_mode error: modes 2 and 3 need mjmlx_batched_set_warmstart_mlp first\n");
     return -1;
   }
-  s->ws_mode = mode;
-  if (mode == 1) s->dm.disableflags |= mjk::DS_WARMSTART;
-  else s->dm.disableflags &= ~mjk::DS_WARMSTART;
-  return 0;
+  try {
+    s->ws_mode = mode;
+    if (mode == 1) s->dm.disableflags |= mjk::DS_WARMSTART;
+    else s->dm.disableflags &= ~mjk::DS_WARMSTART;
+    upload_kparams(s);
+    return 0;
+  } catch (const std::exception& e) { report("mjmlx_batched_set_warmstart_mode", e); return -1; }
 }
 
 MJMLX_API int mjmlx_batched_get_qacc_warmstart(const MjmlxBatchedSim* cs, float* out) {

@@ -75,6 +75,39 @@ enum { KP_KIN = 0, KP_COMPOS, KP_CRB, KP_FACTOR, KP_COMVEL, KP_SMOOTH, KP_COLL, 
 #define KMINIMP 0.0001f
 #define KMAXIMP 0.9999f
 
+// The model and the workspace layout of one sim, as one device-memory copy
+// the step kernels read through a pointer.
+struct KParams {
+  DModel m;
+  Layout L;
+};
+
+// kparams(): the kernels' view of that copy as constant memory (address
+// space 4: loads through it are scalar loads the compiler may repeat).
+// kfresh(x): the same object, for x inside such a copy, through a pointer the
+// compiler cannot prove equal to the one before: the fields a stage reads are
+// loaded again at the stage instead of staying live in scalar registers
+// across the whole step, where they spill into vector-register lanes and
+// every use reloads one with a v_readlane. x must live in device memory (a
+// kernel's by-value argument does not: its address is a private copy).
+#if defined(MJ_EMU)
+static inline const KParams& kparams(const KParams* p) { return *p; }
+template <class T>
+static inline const T& kfresh(const T& x) { return x; }
+#else
+__device__ __forceinline__ const KParams& kparams(const KParams* p) {
+  typedef const __attribute__((address_space(4))) KParams* cptr;
+  return *(const KParams*)(cptr)p;
+}
+template <class T>
+__device__ __forceinline__ const T& kfresh(const T& x) {
+  typedef const __attribute__((address_space(4))) T* cptr;
+  cptr p = (cptr)&x;
+  asm volatile("" : "+s"(p));
+  return *(const T*)p;
+}
+#endif
+
 // ------------------------------------------------------------------ wave helpers
 // Wave-wide reductions: DPP exchanges inside each 16-lane row (quad
 // permutations, half-row and row mirrors), then the row totals are read into
@@ -3188,6 +3221,7 @@ KDEV void write_features(const DModel& m, const Layout& L, const float* w, float
 // box-box, 2 + GJK/EPA), bit 2 the sensor stage (models with sensors only, so
 // the plain step carries none of its code), bit 3 set by the register-capped
 // kernel variant
+// m and L: the sim's device-memory copy (KParams), read through kfresh().
 template <int NPSET>
 KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, int env, int nsub) {
   int nq = m.nq, nv = m.nv, nu = m.nu;
@@ -3223,32 +3257,33 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
   KPROF_START(&pa);
   for (int s = 0; s < nsub; s++) {
     bool last = s == nsub - 1;
-    st_kinematics(m, L, w);
+    st_kinematics(kfresh(m), kfresh(L), w);
     KPROF_MARK(&pa, KP_KIN);
-    st_com_pos(m, L, w);
+    st_com_pos(kfresh(m), kfresh(L), w);
     KPROF_MARK(&pa, KP_COMPOS);
-    st_crb(m, L, w);
+    st_crb(kfresh(m), kfresh(L), w);
     KPROF_MARK(&pa, KP_CRB);
-    if (nv > 0) st |= factor_ld(m, w + L.qM, w + L.qLD);
+    if (nv > 0) st |= factor_ld(kfresh(m), w + L.qM, w + L.qLD);
     KPROF_MARK(&pa, KP_FACTOR);
-    st_com_vel(m, L, w);
+    st_com_vel(kfresh(m), kfresh(L), w);
     KPROF_MARK(&pa, KP_COMVEL);
-    st |= st_smooth(m, L, w);
+    st |= st_smooth(kfresh(m), kfresh(L), w);
     KPROF_MARK(&pa, KP_SMOOTH);
     if (last && io.has_aux) write_aux(m, L, w, io.aux, env);
     if constexpr ((NPSET & 4) != 0)
-      if (io.sensordata) st_sensors(m, L, w, io, env, s == 0, last);
-    st |= st_collision<(NPSET & 3)>(m, L, w);
+      if (io.sensordata) st_sensors(kfresh(m), kfresh(L), w, io, env, s == 0, last);
+    st |= st_collision<(NPSET & 3)>(kfresh(m), kfresh(L), w);
     KPROF_MARK(&pa, KP_COLL);
-    st |= st_constraints(m, L, w);
+    st |= st_constraints(kfresh(m), kfresh(L), w);
     KPROF_MARK(&pa, KP_CONSTR);
     if (last) dump_ws(L, w, io.dump1, env);
     int it = 0;
-    st |= st_solve<(NPSET & 8) == 0>(m, L, w, &it, &pa, io.qacc_pred && s == 0 ? io.qacc_pred + (size_t)env * nv : nullptr);
+    st |= st_solve<(NPSET & 8) == 0>(kfresh(m), kfresh(L), w, &it, &pa,
+                                     io.qacc_pred && s == 0 ? io.qacc_pred + (size_t)env * nv : nullptr);
     niter += it;
     if (last && io.has_aux) write_cfrc_ext(m, L, w, io.aux, env, true);
     if (last) dump_ws(L, w, io.dump2, env);
-    st |= st_integrate(m, L, w);
+    st |= st_integrate(kfresh(m), kfresh(L), w);
     KPROF_MARK(&pa, KP_INTEGRATE);
   }
   int bad = 0;

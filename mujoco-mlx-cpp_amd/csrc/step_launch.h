@@ -26,8 +26,7 @@ enum {
 };
 
 struct StepLaunch {
-  DModel dm;
-  Layout L;
+  const KParams* kp;   // the sim's model and layout in device memory
   int B;               // envs of this launch
   int env0;            // index of the launch's first env in the io arrays
   int variant;         // VAR_*

@@ -29,11 +29,12 @@
 // into AGPRs beyond 256 and the kernel runs one wave per SIMD, four envs per CU.
 template <int NPSET, int GWS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-k_step(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int nsub, float* gws) {
+k_step(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0, int nsub, float* gws) {
   extern __shared__ __align__(16) float lds[];
   if ((int)blockIdx.x >= B) return;
-  float* w = GWS ? gws + (size_t)blockIdx.x * L.total : lds;
-  mjk::run_env<NPSET>(m, L, w, io, env0 + (int)blockIdx.x, nsub);
+  const mjk::KParams& P = mjk::kparams(kp);
+  float* w = GWS ? gws + (size_t)blockIdx.x * P.L.total : lds;
+  mjk::run_env<NPSET>(P.m, P.L, w, io, env0 + (int)blockIdx.x, nsub);
 }
 
 // The same step capped at 168 VGPRs (three waves per SIMD, a few hundred
@@ -42,21 +43,23 @@ k_step(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int nsub, 
 // workspaces, whose occupancy only registers limit.
 template <int NPSET, int GWS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
-k_step_occ(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int nsub, float* gws) {
+k_step_occ(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0, int nsub, float* gws) {
   extern __shared__ __align__(16) float lds[];
   if ((int)blockIdx.x >= B) return;
-  float* w = GWS ? gws + (size_t)blockIdx.x * L.total : lds;
-  mjk::run_env<NPSET | 8>(m, L, w, io, env0 + (int)blockIdx.x, nsub);
+  const mjk::KParams& P = mjk::kparams(kp);
+  float* w = GWS ? gws + (size_t)blockIdx.x * P.L.total : lds;
+  mjk::run_env<NPSET | 8>(P.m, P.L, w, io, env0 + (int)blockIdx.x, nsub);
 }
 
 // The step with the whole register file of a SIMD lane (one wave per SIMD),
 // for LDS workspaces large enough that at most four envs fit on a CU anyway.
 template <int NPSET>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-k_step_big(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int nsub) {
+k_step_big(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0, int nsub) {
   extern __shared__ __align__(16) float lds[];
   if ((int)blockIdx.x >= B) return;
-  mjk::run_env<NPSET>(m, L, lds, io, env0 + (int)blockIdx.x, nsub);
+  const mjk::KParams& P = mjk::kparams(kp);
+  mjk::run_env<NPSET>(P.m, P.L, lds, io, env0 + (int)blockIdx.x, nsub);
 }
 
 #if defined(MJK_HALF) && MJK_HALF
@@ -66,20 +69,22 @@ k_step_big(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int ns
 // k_step_occ.
 template <int NPSET>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-k_step_half(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int nsub) {
+k_step_half(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0, int nsub) {
   extern __shared__ __align__(16) float lds[];
   int h = (int)(threadIdx.x >> 5), e = 2 * (int)blockIdx.x + h;
   if (e >= B) return;
-  mjk::run_env<NPSET>(m, L, lds + h * L.total, io, env0 + e, nsub);
+  const mjk::KParams& P = mjk::kparams(kp);
+  mjk::run_env<NPSET>(P.m, P.L, lds + h * P.L.total, io, env0 + e, nsub);
 }
 
 template <int NPSET>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
-k_step_half_occ(mjk::DModel m, mjk::Layout L, mjk::StepIO io, int B, int env0, int nsub) {
+k_step_half_occ(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0, int nsub) {
   extern __shared__ __align__(16) float lds[];
   int h = (int)(threadIdx.x >> 5), e = 2 * (int)blockIdx.x + h;
   if (e >= B) return;
-  mjk::run_env<NPSET | 8>(m, L, lds + h * L.total, io, env0 + e, nsub);
+  const mjk::KParams& P = mjk::kparams(kp);
+  mjk::run_env<NPSET | 8>(P.m, P.L, lds + h * P.L.total, io, env0 + e, nsub);
 }
 
 namespace mjk {
@@ -90,9 +95,9 @@ void MJK_LAUNCH(const StepLaunch& l, const StepIO& io, int nsub) {
   dim3 g((l.B + 1) / 2), b(64);
   size_t sh = 2 * l.lds_bytes;
   if (l.variant == VAR_LDS_OCC_HALF)
-    hipLaunchKernelGGL((k_step_half_occ<NP>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub);
+    hipLaunchKernelGGL((k_step_half_occ<NP>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub);
   else
-    hipLaunchKernelGGL((k_step_half<NP>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub);
+    hipLaunchKernelGGL((k_step_half<NP>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub);
 }
 
 }  // namespace mjk
@@ -107,19 +112,19 @@ void MJK_LAUNCH(const StepLaunch& l, const StepIO& io, int nsub) {
   size_t sh = gws ? 0 : l.lds_bytes;
   switch (l.variant) {
     case VAR_GWS_OCC:
-      hipLaunchKernelGGL((k_step_occ<NP, 1>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub, l.gws);
+      hipLaunchKernelGGL((k_step_occ<NP, 1>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws);
       break;
     case VAR_GWS:
-      hipLaunchKernelGGL((k_step<NP, 1>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub, l.gws);
+      hipLaunchKernelGGL((k_step<NP, 1>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws);
       break;
     case VAR_LDS_OCC:
-      hipLaunchKernelGGL((k_step_occ<NP, 0>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub, l.gws);
+      hipLaunchKernelGGL((k_step_occ<NP, 0>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws);
       break;
     case VAR_LDS_BIG:
-      hipLaunchKernelGGL((k_step_big<NP>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub);
+      hipLaunchKernelGGL((k_step_big<NP>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub);
       break;
     default:
-      hipLaunchKernelGGL((k_step<NP, 0>), g, b, sh, l.stream, l.dm, l.L, io, l.B, l.env0, nsub, l.gws);
+      hipLaunchKernelGGL((k_step<NP, 0>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws);
       break;
   }
 }
