@@ -2772,7 +2772,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     int exact = 0;
     KPROF_MARK(pa, KP_DIR);
     float dec = 0.f;
-    float alpha = linesearch(m, L, w, nefc, &exact, newton ? nullptr : &dec);
+    float alpha = linesearch(kfresh(m), kfresh(L), w, nefc, &exact, newton ? nullptr : &dec);
     KPROF_MARK(pa, KP_LS);
     *niter_out = it + 1;
     if (alpha == 0.f) break;
@@ -2786,7 +2786,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     KSYNC();
     float oldcost = cost;
     KPROF_MARK(pa, KP_SOLVE_REST);
-    cost = cost_grad(m, L, w, nefc, &nch);
+    cost = cost_grad(kfresh(m), kfresh(L), w, nefc, &nch);
     KPROF_MARK(pa, KP_COSTGRAD);
     float gp = 0.f;
     for (int i = LANE; i < nv; i += WAVE) gp += grad[i] * grad[i];
@@ -2799,8 +2799,8 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     if (newton && exact) break;
     if (newton) {
       // H depends on qacc only through the active set: refactor only when it changed
-      if (nch) st |= newton_factor<TRI_CACHE>(m, L, w, nefc, pa);
-      newton_solve(m, L, w);
+      if (nch) st |= newton_factor<TRI_CACHE>(kfresh(m), kfresh(L), w, nefc, pa);
+      newton_solve(kfresh(m), kfresh(L), w);
     } else {
       for (int i = LANE; i < nv; i += WAVE) Mg[i] = grad[i];
       KSYNC();
