@@ -22,8 +22,14 @@ ARCH = os.environ.get("MJMLX_HIP_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 instructions are disabled for all device code of this project
+# FP32 division and square root by the hardware's reciprocal / square-root
+# instructions (1-2 ulp, not correctly rounded) and FP32 denormals flushed to
+# zero: fewer instructions per division, square root and reciprocal in the
+# step kernels (quadruped -1.3 %, humanoid -2.7 % per control step); FP64 is
+# unaffected (the Newton factor and the quaternion integration keep it).
 HIP_FLAGS = ["-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero",
              "-Wno-unused-result", "-Wno-unused-value"]
 HOST_SRC = ["mjcf.cpp", "hostmodel.cpp"]
 # narrow-phase sets of the step kernel, one translation unit each (step_np.hip)
