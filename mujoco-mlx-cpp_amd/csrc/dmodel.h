@@ -184,8 +184,9 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   //   KC the other kinematics and dynamics intermediates, kinematics -> smooth
   //   con, rows              collision / constraint rows -> end of the substep
   //   V  solver vectors      solve (tmpv also integration)
-  //   H  FP64 Hessian        solve; under Newton the LDL factor of qM (factor
-  //                          -> smooth, and the integrator's factor) uses it
+  //   H  FP64 Hessian        solve (Newton only, lower triangle packed); the
+  //                          LDL factor of qM (factor -> smooth, and the
+  //                          integrator's factor) uses its space
   // con starts where KC starts (KC is dead once collision runs); the rows
   // follow con; V takes S and G's space when it fits (both dead in the solve).
   int r0 = o;
@@ -229,8 +230,10 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.efc_margin = L.efc_Dact;
   int rows_end = al2(o);
   o = al2(kc_end > rows_end ? kc_end : rows_end);
-  L.H = o; o += al2(2 * nvp * nvp);
-  if (qld_shared) L.qLD = L.H;
+  // the FP64 Newton Hessian, its lower triangle packed row by row (Newton
+  // only; its nv (nv + 1) floats also hold the nv x nv LDL factor of qM)
+  L.H = o;
+  if (qld_shared) { o += al2(nvp * (nvp + 1)); L.qLD = L.H; }
   int nvec = al2(8 * nvp) + 4 * nvp;  // eight FP32 vectors, then two FP64 ones
   int vbase = jw == 0 && nvec <= kc0 - r0 ? r0 : o;
   if (vbase == o) o += nvec;

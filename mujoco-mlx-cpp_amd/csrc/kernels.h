@@ -236,6 +236,10 @@ KDEV const int* iptr(const float* w, int off) { return reinterpret_cast<const in
 KDEV double* dptr(float* w, int off) { return reinterpret_cast<double*>(w + off); }
 
 // (row, col) with col <= row of lower-triangular entry t (row-major packing)
+// entry (i, j), j <= i, of a lower triangle packed row by row (the Newton
+// Hessian and its factor)
+KDEV int hx(int i, int j) { return ((i * (i + 1)) >> 1) + j; }
+
 KDEV void tri_rc(int t, int* r, int* c) {
   int row = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
   while ((row + 1) * (row + 2) / 2 <= t) row++;
@@ -2231,9 +2235,9 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     int col = LANE & 15, row0 = 4 * (LANE >> 4);
     for (int q = 0; q < 4; q++) {
       int i = row0 + q;
-      if (i < nv && col <= i) H[i * nv + col] = (double)(M[i * nv + col] + t00[q]);
-      if (16 + i < nv && col < nv) H[(16 + i) * nv + col] = (double)(M[(16 + i) * nv + col] + t10[q]);
-      if (16 + i < nv && 16 + col <= 16 + i) H[(16 + i) * nv + 16 + col] = (double)(M[(16 + i) * nv + 16 + col] + t11[q]);
+      if (i < nv && col <= i) H[hx(i, col)] = (double)(M[i * nv + col] + t00[q]);
+      if (16 + i < nv && col < nv) H[hx(16 + i, col)] = (double)(M[(16 + i) * nv + col] + t10[q]);
+      if (16 + i < nv && 16 + col <= 16 + i) H[hx(16 + i, 16 + col)] = (double)(M[(16 + i) * nv + 16 + col] + t11[q]);
     }
     KSYNC();
   } else
@@ -2249,7 +2253,7 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     for (int t = LANE; t < ntri; t += WAVE) {
       int i, j;
       tri_rc(t, &i, &j);
-      H[i * nv + j] = (double)M[i * nv + j];
+      H[hx(i, j)] = (double)M[i * nv + j];
     }
     KSYNC();
     const int* misc = iptr(w, L.misc);
@@ -2277,7 +2281,7 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
         float sacc = 0.f;
         for (int r = ra; r < rb; r++) sacc += Dact[r] * J[r * jw + a] * J[r * jw + b];
         int i = ia > ib ? ia : ib, j = ia > ib ? ib : ia;
-        H[i * nv + j] += (double)sacc;
+        H[hx(i, j)] += (double)sacc;
       }
       KSYNC();
     }
@@ -2285,7 +2289,7 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     for (int t = LANE; t < ntri; t += WAVE) {
       int i, j;
       tri_rc(t, &i, &j);
-      H[i * nv + j] = (double)(M[i * nv + j] + kdot3(Dact, J + i, nv, J + j, nv, nefc));
+      H[hx(i, j)] = (double)(M[i * nv + j] + kdot3(Dact, J + i, nv, J + j, nv, nefc));
     }
     KSYNC();
   }
@@ -2321,7 +2325,7 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
         }
         si /= T; sl /= T;
         float hv = Dm * (mu * mu * (Jc[i] - si) * (Jc[l] - sl) - cg * (q - si * sl));
-        H[i * nv + l] += (double)hv;
+        H[hx(i, l)] += (double)hv;
       }
     }
   }
@@ -2351,10 +2355,10 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
   // Column k+1 itself (its diagonal included) is updated in one pass at the end.
   int k = 0;
   for (; k + 1 < nv - 1; k += 2) {
-    double d0 = H[k * nv + k];
+    double d0 = H[hx(k, k)];
     if (!(d0 > 0.0)) { bad = 1; d0 = 1e-30; }
-    double i0 = 1.0 / d0, h10 = H[(k + 1) * nv + k], c = h10 * i0;
-    double d1 = H[(k + 1) * nv + k + 1] - c * h10;
+    double i0 = 1.0 / d0, h10 = H[hx(k + 1, k)], c = h10 * i0;
+    double d1 = H[hx(k + 1, k + 1)] - c * h10;
     if (!(d1 > 0.0)) { bad = 1; d1 = 1e-30; }
     double i1 = 1.0 / d1;
     if (LANE == 0) idg[k] = i0;  // the deferred pass below takes c from it
@@ -2364,10 +2368,10 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
 #pragma unroll
         for (int r = 0; r < TRI_R; r++) {
           if (LANE + r * WAVE >= nt) continue;
-          int i = k + 2 + (tri_ab[r] & 0xffff), j = k + 2 + (tri_ab[r] >> 16);
-          double ai = H[i * nv + k], aj = H[j * nv + k];
-          double bi = H[i * nv + k + 1] - c * ai, bj = H[j * nv + k + 1] - c * aj;
-          H[i * nv + j] -= ai * aj * i0 + bi * bj * i1;
+          int i = k + 2 + (tri_ab[r] & 0xffff), j = k + 2 + (tri_ab[r] >> 16), ri = hx(i, 0), rj = hx(j, 0);
+          double ai = H[ri + k], aj = H[rj + k];
+          double bi = H[ri + k + 1] - c * ai, bj = H[rj + k + 1] - c * aj;
+          H[ri + j] -= ai * aj * i0 + bi * bj * i1;
         }
         KSYNC();
         continue;
@@ -2376,15 +2380,15 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     for (int t = LANE; t < nt; t += WAVE) {
       int a, b;
       tri_rc(t, &a, &b);
-      int i = k + 2 + a, j = k + 2 + b;
-      double ai = H[i * nv + k], aj = H[j * nv + k];
-      double bi = H[i * nv + k + 1] - c * ai, bj = H[j * nv + k + 1] - c * aj;
-      H[i * nv + j] -= ai * aj * i0 + bi * bj * i1;
+      int i = k + 2 + a, j = k + 2 + b, ri = hx(i, 0), rj = hx(j, 0);
+      double ai = H[ri + k], aj = H[rj + k];
+      double bi = H[ri + k + 1] - c * ai, bj = H[rj + k + 1] - c * aj;
+      H[ri + j] -= ai * aj * i0 + bi * bj * i1;
     }
     KSYNC();
   }
   if (k < nv - 1) {  // one column left with trailing rows
-    double d0 = H[k * nv + k];
+    double d0 = H[hx(k, k)];
     if (!(d0 > 0.0)) { bad = 1; d0 = 1e-30; }
     double ik = 1.0 / d0;
     int nt = (nv - k - 1) * (nv - k) / 2;
@@ -2394,8 +2398,8 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
 #pragma unroll
         for (int r = 0; r < TRI_R; r++) {
           if (LANE + r * WAVE >= nt) continue;
-          int i = k + 1 + (tri_ab[r] & 0xffff), j = k + 1 + (tri_ab[r] >> 16);
-          H[i * nv + j] -= H[i * nv + k] * ik * H[j * nv + k];
+          int i = k + 1 + (tri_ab[r] & 0xffff), j = k + 1 + (tri_ab[r] >> 16), ri = hx(i, 0);
+          H[ri + j] -= H[ri + k] * ik * H[hx(j, k)];
         }
         done = true;
       }
@@ -2403,19 +2407,19 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     for (int t = LANE; !done && t < nt; t += WAVE) {
       int a, b;
       tri_rc(t, &a, &b);
-      int i = k + 1 + a, j = k + 1 + b;
-      H[i * nv + j] -= H[i * nv + k] * ik * H[j * nv + k];
+      int i = k + 1 + a, j = k + 1 + b, ri = hx(i, 0);
+      H[ri + j] -= H[ri + k] * ik * H[hx(j, k)];
     }
     KSYNC();
   }
   // deferred updates of the second column of each pair: H[i][k+1] -= c H[i][k], i >= k+1
   for (int kk = 0; kk + 1 < nv - 1; kk += 2) {
-    double c = H[(kk + 1) * nv + kk] * idg[kk];
-    for (int i = kk + 1 + LANE; i < nv; i += WAVE) H[i * nv + kk + 1] -= c * H[i * nv + kk];
+    double c = H[hx(kk + 1, kk)] * idg[kk];
+    for (int i = kk + 1 + LANE; i < nv; i += WAVE) H[hx(i, kk + 1)] -= c * H[hx(i, kk)];
   }
   KSYNC();
   for (int i = LANE; i < nv; i += WAVE) {
-    double d = H[i * nv + i];
+    double d = H[hx(i, i)];
     if (!(d > 0.0)) { bad = 1; d = 1e-30; }
     idg[i] = 1.0 / d;
   }
@@ -2438,14 +2442,14 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
     KSYNC();
     for (int k = 0; k < nv - 1; k++) {
       double yk = y[k] * idg[k];
-      for (int i = k + 1 + LANE; i < nv; i += WAVE) y[i] -= H[i * nv + k] * yk;
+      for (int i = k + 1 + LANE; i < nv; i += WAVE) y[i] -= H[hx(i, k)] * yk;
       KSYNC();
     }
     for (int i = LANE; i < nv; i += WAVE) y[i] *= idg[i];
     KSYNC();
     for (int k = nv - 1; k >= 1; k--) {
       double xk = y[k];
-      for (int i = LANE; i < k; i += WAVE) y[i] -= H[k * nv + i] * idg[i] * xk;
+      for (int i = LANE; i < k; i += WAVE) y[i] -= H[hx(k, i)] * idg[i] * xk;
       KSYNC();
     }
     for (int i = LANE; i < nv; i += WAVE) Mg[i] = (float)y[i];
@@ -2456,7 +2460,9 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
   int ln = LANE < nv ? LANE : 0;
   double v = LANE < nv ? (double)grad[LANE] : 0.0;
   double id = LANE < nv ? idg[LANE] : 1.0;
-  const double* Hr = H + ln * nv;
+  // rows of the packed triangle: reads past a row's end (masked lanes) stay
+  // inside the triangle for every ln < nv
+  const double* Hr = H + hx(ln, 0);
   double hn = Hr[0];
   for (int k = 0; k < nv - 1; k++) {          // L y = g, L_ik = H_ik / H_kk
     double h = hn;
@@ -2465,10 +2471,10 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
     if (LANE > k && LANE < nv) v -= h * yk;
   }
   v *= id;                                      // D z = y
-  hn = nv > 1 ? H[(nv - 1) * nv + ln] : 0.0;
+  hn = nv > 1 ? H[hx(nv - 1, ln)] : 0.0;
   for (int k = nv - 1; k >= 1; k--) {          // L^T x = z
     double h = hn;
-    if (k > 1) hn = H[(k - 1) * nv + ln];
+    if (k > 1) hn = H[hx(k - 1, ln)];
     double xk = rl_d(v, k);
     if (LANE < k) v -= h * id * xk;
   }
@@ -2478,10 +2484,10 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
   double* y = dptr(w, L.dvec) + nv;
   for (int i = 0; i < nv; i++) y[i] = grad[i];
   for (int k = 0; k < nv - 1; k++)
-    for (int i = k + 1; i < nv; i++) y[i] -= H[i * nv + k] * idg[k] * y[k];
+    for (int i = k + 1; i < nv; i++) y[i] -= H[hx(i, k)] * idg[k] * y[k];
   for (int i = 0; i < nv; i++) y[i] *= idg[i];
   for (int k = nv - 1; k >= 1; k--)
-    for (int i = 0; i < k; i++) y[i] -= H[k * nv + i] * idg[i] * y[k];
+    for (int i = 0; i < k; i++) y[i] -= H[hx(k, i)] * idg[i] * y[k];
   for (int i = 0; i < nv; i++) Mg[i] = (float)y[i];
 #endif
   KSYNC();
