@@ -110,11 +110,13 @@ __device__ __forceinline__ const T& kfresh(const T& x) {
 
 // ------------------------------------------------------------------ wave helpers
 // Wave-wide reductions: DPP exchanges inside each 16-lane row (quad
-// permutations, half-row and row mirrors), then the row totals are read into
-// scalar registers and added in a fixed order, so every lane gets the same bit
-// pattern and the result can drive uniform control flow. With 32 lanes per env
-// each half of the wave reduces its own two rows (rows 0-1, rows 2-3) and a
-// lane takes its half's total.
+// permutations, half-row and row mirrors) leave every lane of a row holding
+// the row's total r_k; row broadcasts then add row 0's total into row 1 and
+// row 2's into row 3 (row_bcast:15), and row 1's into row 3 (row_bcast:31),
+// and lane 63's (r2 + r3) + (r0 + r1) is read into a scalar register: every
+// lane gets the same bit pattern and the result can drive uniform control
+// flow. With 32 lanes per env each half of the wave stops after the first
+// broadcast and a lane takes its half's total (lane 31 or lane 63).
 #if WAVE >= 32
 KDEV int dpp_i(int v, int ctrl_sel) {
   switch (ctrl_sel) {
@@ -124,6 +126,9 @@ KDEV int dpp_i(int v, int ctrl_sel) {
     default: return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false); // row_mirror
   }
 }
+// row broadcasts into the rows of row_mask; the other rows return `old`
+KDEV int dpp_bc15(int old, int v) { return __builtin_amdgcn_update_dpp(old, v, 0x142, 0xA, 0xF, false); }
+KDEV int dpp_bc31(int old, int v) { return __builtin_amdgcn_update_dpp(old, v, 0x143, 0xC, 0xF, false); }
 KDEV float dpp_f(float v, int c) { return __int_as_float(dpp_i(__float_as_int(v), c)); }
 KDEV int rl_abs(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 KDEV float rl_absf(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
@@ -131,6 +136,13 @@ KDEV double rl_absd(double v, int lane) {
   long long b = __double_as_longlong(v);
   int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
   int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+KDEV double dpp_bcd(double v, bool b31) {
+  long long b = __double_as_longlong(v);
+  int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+  lo = b31 ? dpp_bc31(0, lo) : dpp_bc15(0, lo);
+  hi = b31 ? dpp_bc31(0, hi) : dpp_bc15(0, hi);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 #if WAVE == 64
@@ -151,10 +163,12 @@ KDEV float wsum(float v) {
   v += dpp_f(v, 1);
   v += dpp_f(v, 2);
   v += dpp_f(v, 3);
+  v += __int_as_float(dpp_bc15(0, __float_as_int(v)));
 #if WAVE == 64
-  return (rl_absf(v, 0) + rl_absf(v, 16)) + (rl_absf(v, 32) + rl_absf(v, 48));
+  v += __int_as_float(dpp_bc31(0, __float_as_int(v)));
+  return rl_absf(v, 63);
 #else
-  return WHALF_SEL(rl_absf(v, 0) + rl_absf(v, 16), rl_absf(v, 32) + rl_absf(v, 48));
+  return WHALF_SEL(rl_absf(v, 31), rl_absf(v, 63));
 #endif
 #else
   return v;
@@ -167,10 +181,12 @@ KDEV double wsumd(double v) {
     int lo = dpp_i((int)(b & 0xffffffffll), c), hi = dpp_i((int)(b >> 32), c);
     v += __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
   }
+  v += dpp_bcd(v, false);
 #if WAVE == 64
-  return (rl_absd(v, 0) + rl_absd(v, 16)) + (rl_absd(v, 32) + rl_absd(v, 48));
+  v += dpp_bcd(v, true);
+  return rl_absd(v, 63);
 #else
-  return WHALF_SEL(rl_absd(v, 0) + rl_absd(v, 16), rl_absd(v, 32) + rl_absd(v, 48));
+  return WHALF_SEL(rl_absd(v, 31), rl_absd(v, 63));
 #endif
 #else
   return v;
@@ -179,10 +195,12 @@ KDEV double wsumd(double v) {
 KDEV int wsumi(int v) {
 #if WAVE >= 32
   for (int c = 0; c < 4; c++) v += dpp_i(v, c);
+  v += dpp_bc15(0, v);
 #if WAVE == 64
-  return (rl_abs(v, 0) + rl_abs(v, 16)) + (rl_abs(v, 32) + rl_abs(v, 48));
+  v += dpp_bc31(0, v);
+  return rl_abs(v, 63);
 #else
-  return WHALF_SEL(rl_abs(v, 0) + rl_abs(v, 16), rl_abs(v, 32) + rl_abs(v, 48));
+  return WHALF_SEL(rl_abs(v, 31), rl_abs(v, 63));
 #endif
 #else
   return v;
@@ -191,10 +209,12 @@ KDEV int wsumi(int v) {
 KDEV int wori(int v) {
 #if WAVE >= 32
   for (int c = 0; c < 4; c++) v |= dpp_i(v, c);
+  v |= dpp_bc15(0, v);
 #if WAVE == 64
-  return rl_abs(v, 0) | rl_abs(v, 16) | rl_abs(v, 32) | rl_abs(v, 48);
+  v |= dpp_bc31(0, v);
+  return rl_abs(v, 63);
 #else
-  return WHALF_SEL(rl_abs(v, 0) | rl_abs(v, 16), rl_abs(v, 32) | rl_abs(v, 48));
+  return WHALF_SEL(rl_abs(v, 31), rl_abs(v, 63));
 #endif
 #else
   return v;
@@ -203,14 +223,12 @@ KDEV int wori(int v) {
 KDEV int wmini(int v) {
 #if WAVE >= 32
   for (int c = 0; c < 4; c++) { int t = dpp_i(v, c); v = t < v ? t : v; }
-  int a = rl_abs(v, 0), b = rl_abs(v, 16);
-  int c2 = rl_abs(v, 32), d = rl_abs(v, 48);
-  a = a < b ? a : b;
-  c2 = c2 < d ? c2 : d;
+  { int t = dpp_bc15(v, v); v = t < v ? t : v; }  // rows outside the mask compare with themselves
 #if WAVE == 64
-  return a < c2 ? a : c2;
+  { int t = dpp_bc31(v, v); v = t < v ? t : v; }
+  return rl_abs(v, 63);
 #else
-  return WHALF_SEL(a, c2);
+  return WHALF_SEL(rl_abs(v, 31), rl_abs(v, 63));
 #endif
 #else
   return v;
