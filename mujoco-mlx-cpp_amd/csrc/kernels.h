@@ -254,6 +254,19 @@ KDEV const int* iptr(const float* w, int off) { return reinterpret_cast<const in
 KDEV double* dptr(float* w, int off) { return reinterpret_cast<double*>(w + off); }
 
 // (row, col) with col <= row of lower-triangular entry t (row-major packing)
+// 1 / x in FP64 for x > 0 (not denormal): the hardware reciprocal refined by
+// two Newton-Raphson steps (within an ulp), without the scaling and fix-up
+// steps of an IEEE division
+KDEV double krcp(double x) {
+#if defined(MJ_EMU)
+  return 1.0 / x;
+#else
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+#endif
+}
+
 // entry (i, j), j <= i, of a lower triangle packed row by row (the Newton
 // Hessian and its factor)
 KDEV int hx(int i, int j) { return ((i * (i + 1)) >> 1) + j; }
@@ -2373,12 +2386,14 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
   // Column k+1 itself (its diagonal included) is updated in one pass at the end.
   int k = 0;
   for (; k + 1 < nv - 1; k += 2) {
-    double d0 = H[hx(k, k)];
+    double d0 = H[hx(k, k)], h10 = H[hx(k + 1, k)], h11 = H[hx(k + 1, k + 1)];
     if (!(d0 > 0.0)) { bad = 1; d0 = 1e-30; }
-    double i0 = 1.0 / d0, h10 = H[hx(k + 1, k)], c = h10 * i0;
-    double d1 = H[hx(k + 1, k + 1)] - c * h10;
-    if (!(d1 > 0.0)) { bad = 1; d1 = 1e-30; }
-    double i1 = 1.0 / d1;
+    // the pair's second pivot d1 = h11 - h10^2 / d0 = det / d0: 1 / d0 and
+    // 1 / det do not wait for each other
+    double det = d0 * h11 - h10 * h10;
+    if (!(det > 0.0)) { bad = 1; det = 1e-30 * d0; }
+    double i0 = krcp(d0), idet = krcp(det);
+    double c = h10 * i0, i1 = d0 * idet;
     if (LANE == 0) idg[k] = i0;  // the deferred pass below takes c from it
     int mm = nv - k - 2, nt = mm * (mm + 1) / 2;
     if constexpr (WAVE >= 32) {
@@ -2408,7 +2423,7 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
   if (k < nv - 1) {  // one column left with trailing rows
     double d0 = H[hx(k, k)];
     if (!(d0 > 0.0)) { bad = 1; d0 = 1e-30; }
-    double ik = 1.0 / d0;
+    double ik = krcp(d0);
     int nt = (nv - k - 1) * (nv - k) / 2;
     bool done = false;
     if constexpr (WAVE >= 32) {
@@ -2439,7 +2454,7 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
   for (int i = LANE; i < nv; i += WAVE) {
     double d = H[hx(i, i)];
     if (!(d > 0.0)) { bad = 1; d = 1e-30; }
-    idg[i] = 1.0 / d;
+    idg[i] = krcp(d);
   }
   KSYNC();
   return wori(bad) ? ST_NOT_PD : 0;
