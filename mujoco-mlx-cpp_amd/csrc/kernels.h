@@ -538,20 +538,24 @@ KDEV void st_crb(const DModel& m, const Layout& L, float* w) {
   const float *cinert = w + L.cinert, *cdof = w + L.cdof;
   float *crb = w + L.crb, *buf = w + L.buf6, *M = w + L.qM;
   int nv = m.nv;
-  for (int b = LANE; b < m.nbody; b += WAVE) {
-    float s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // one lane per (body, component): the subtree sums run side by side
+  for (int t = LANE; t < 10 * m.nbody; t += WAVE) {
+    int b = t / 10, q = t - 10 * b;
+    float s = 0.f;
     if (b > 0)
-      for (int k = b; k < m.body_subtree_end[b]; k++)
-        for (int q = 0; q < 10; q++) s[q] += cinert[10 * k + q];
-    for (int q = 0; q < 10; q++) crb[10 * b + q] = s[q];
+      for (int k = b; k < m.body_subtree_end[b]; k++) s += cinert[10 * k + q];
+    crb[t] = s;
   }
   for (int t = LANE; t < nv * nv; t += WAVE) M[t] = 0.f;
   KSYNC();
   for (int i = LANE; i < nv; i += WAVE) mul_inert(crb + 10 * m.dof_bodyid[i], cdof + 6 * i, buf + 6 * i);
   KSYNC();
+  // each dof's ancestors from its chain table (no dependent parent loads)
   for (int i = LANE; i < nv; i += WAVE) {
     const float* bi = buf + 6 * i;
-    for (int j = i; j >= 0; j = m.dof_parentid[j]) {
+    const MJK_G int* ch = m.dof_chain + i * MAX_DOF_DEPTH;
+    for (int a = 0, n = m.dof_nchain[i]; a < n; a++) {
+      int j = ch[a];
       const float* cj = cdof + 6 * j;
       float s = cj[0] * bi[0] + cj[1] * bi[1] + cj[2] * bi[2] + cj[3] * bi[3] + cj[4] * bi[4] + cj[5] * bi[5];
       M[i * nv + j] = s;
@@ -768,11 +772,13 @@ KDEV int st_smooth(const DModel& m, const Layout& L, float* w) {
     }
     KSYNC();
   }
-  // subtree sums of body forces into cacc (no longer needed)
-  for (int b = LANE; b < m.nbody; b += WAVE) {
-    float s[6] = {0, 0, 0, 0, 0, 0};
-    for (int k = b; k < m.body_subtree_end[b]; k++) for (int c = 0; c < 6; c++) s[c] += cfrc[6 * k + c];
-    for (int c = 0; c < 6; c++) cacc[6 * b + c] = s[c];
+  // subtree sums of body forces into cacc (no longer needed), one lane per
+  // (body, component)
+  for (int t = LANE; t < 6 * m.nbody; t += WAVE) {
+    int b = t / 6, c = t - 6 * b;
+    float s = 0.f;
+    for (int k = b; k < m.body_subtree_end[b]; k++) s += cfrc[6 * k + c];
+    cacc[t] = s;
   }
   KSYNC();
   float *qfs = w + L.qfrc_smooth, *qas = w + L.qacc_smooth, *af = w + L.act_force;
