@@ -217,7 +217,8 @@ int step_variant(const MjmlxBatchedSim* s) {
 // placement (device memory, LDS at three, two or one waves per SIMD), compiled
 // for the model's narrow-phase set and, with sensors, the sensor stage
 void launch_step(MjmlxBatchedSim* s, const mjk::StepIO& io, int nsub) {
-  mjk::StepLaunch l{s->d_kp, s->B, 0, step_variant(s), s->lds_bytes, s->d_gws, s->stream};
+  mjk::StepLaunch l{s->d_kp, s->B, 0, step_variant(s), s->lds_bytes, s->d_gws, s->stream,
+                    mjk::ls_cache_rows(s->L.maxefc)};
   mjk::launch_step_np(s->npset | (s->model.nsensor > 0 ? 4 : 0), s->nv > mjk::SPARSE_ROWS_NV, l, io, nsub);
 }
 
@@ -755,7 +756,7 @@ MJMLX_API int mjmlx_batched_step_n_noalloc(MjmlxBatchedSim* sim, const float* ct
     upload_ctrl(sim, ctrl);
     mjk::StepIO io = make_io(sim, nullptr, nullptr);
     int var = sim->use_gws ? step_variant(sim) : mjk::VAR_GWS;
-    mjk::StepLaunch l{sim->d_kp, sim->B, 0, var, 0, ws, sim->stream};
+    mjk::StepLaunch l{sim->d_kp, sim->B, 0, var, 0, ws, sim->stream, mjk::ls_cache_rows(sim->L.maxefc)};
     HIPOK(hipEventRecord(sim->ev0, sim->stream));
     mjk::launch_step_np(sim->npset | (sim->model.nsensor > 0 ? 4 : 0), sim->nv > mjk::SPARSE_ROWS_NV, l, io, n);
     HIPOK(hipGetLastError());

@@ -108,6 +108,10 @@ __device__ __forceinline__ const T& kfresh(const T& x) {
 }
 #endif
 
+// bit casts between int and float (row type bits kept in float slots)
+KDEV float ki2f(int v) { return __builtin_bit_cast(float, v); }
+KDEV int kf2i(float v) { return __builtin_bit_cast(int, v); }
+
 // ------------------------------------------------------------------ wave helpers
 // Wave-wide reductions: DPP exchanges inside each 16-lane row (quad
 // permutations, half-row and row mirrors) leave every lane of a row holding
@@ -2536,13 +2540,36 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
 // rounding
 struct LsPt { float alpha, f, d1, d2, d1n; int nchange; };
 
+// One row's line-search operands (Jaref, Jv, D, type bits), one 16-byte entry.
+struct alignas(16) LsRow4 { float jar, jv, d, ty; };
+
+// Free LDS beside a device-memory workspace: up to cap rows of the line
+// search are staged there (the kernels whose workspace is in device memory
+// hold their occupancy by registers, so a few KB of LDS per workgroup are free).
+struct LsCache { LsRow4* p; int cap; };
+
 // The rows a lane owns (r = LANE + q WAVE) held in registers for the whole
 // line search, when there are at most two per lane: every evaluation is then
-// register arithmetic plus the wave reductions, with no LDS traffic.
-struct LsRows { float jar[2], jv[2], d[2]; int ty[2]; int on; };
+// register arithmetic plus the wave reductions, with no LDS traffic. More
+// rows: staged once in the LDS cache when there is one (every evaluation then
+// reads one 16-byte LDS entry per row instead of four workspace words).
+struct LsRows { float jar[2], jv[2], d[2]; int ty[2]; int on; const LsRow4* staged; };
 
-KDEV void ls_rows(const Layout& L, const float* w, int nefc, LsRows* c) {
+KDEV void ls_rows(const Layout& L, const float* w, int nefc, LsRows* c, LsCache lc) {
   c->on = 0;
+  c->staged = nullptr;
+  if (nefc > 2 * WAVE && nefc <= lc.cap) {
+    const float *Jaref = w + L.efc_Jaref, *Jv = w + L.efc_Jv, *D = w + L.efc_D;
+    const int* etype = reinterpret_cast<const int*>(w + L.efc_type);
+    for (int r = LANE; r < nefc; r += WAVE) {
+      LsRow4 e;
+      e.jar = Jaref[r]; e.jv = Jv[r]; e.d = D[r]; e.ty = ki2f(etype[r]);
+      lc.p[r] = e;
+    }
+    KSYNC();
+    c->staged = lc.p;
+    return;
+  }
 #if WAVE >= 32
   if (nefc <= 2 * WAVE) {
     c->on = 1;
@@ -2613,6 +2640,13 @@ KDEV LsPt ls_eval(const DModel& m, const Layout& L, float* w, int nefc, float al
         ls_row(m, L, w, LANE + q * WAVE, c.ty[q], c.jar[q], c.jv[q], c.d[q], alpha, ref_alpha, f, d1, d2, nch);
         d1a += fabsf(d1 - t);
       }
+  } else if (c.staged) {
+    for (int r = LANE; r < nefc; r += WAVE) {
+      LsRow4 e = c.staged[r];
+      float t = d1;
+      ls_row(m, L, w, r, kf2i(e.ty), e.jar, e.jv, e.d, alpha, ref_alpha, f, d1, d2, nch);
+      d1a += fabsf(d1 - t);
+    }
   } else {
     const float *Jaref = w + L.efc_Jaref, *Jv = w + L.efc_Jv, *D = w + L.efc_D;
     const int* etype = iptr(w, L.efc_type);
@@ -2642,14 +2676,18 @@ KDEV bool ls_flat(const LsPt& p, float gtol) { return fabsf(p.d1) <= fmaxf(gtol,
 // alpha g1 + 1/2 alpha^2 g2: MuJoCo's "improvement" (old cost - new cost)
 // without the cancellation of two large FP32 sums, which near the optimum is
 // the size of the improvement itself.
-KDEV float ls_decrease(const DModel& m, const Layout& L, const float* w, int nefc, float alpha, const float* gq) {
+KDEV float ls_decrease(const DModel& m, const Layout& L, const float* w, int nefc, float alpha, const float* gq,
+                       const LsRow4* staged) {
   const float *Jaref = w + L.efc_Jaref, *Jv = w + L.efc_Jv, *D = w + L.efc_D;
   const int* etype = iptr(w, L.efc_type);
   float df = 0.f;
   for (int r = LANE; r < nefc; r += WAVE) {
-    int ty = etype[r];
+    LsRow4 e;
+    if (staged) e = staged[r];
+    else { e.jar = Jaref[r]; e.jv = Jv[r]; e.d = D[r]; e.ty = ki2f(etype[r]); }
+    int ty = kf2i(e.ty);
     if (ty == EFC_CONTACT_ELLIPTIC) continue;
-    float x0 = Jaref[r], jv = Jv[r], d = D[r], x = x0 + alpha * jv;
+    float x0 = e.jar, jv = e.jv, d = e.d, x = x0 + alpha * jv;
     if (ty == EFC_FRICTION_DOF) {
       float fl = m.dof_frictionloss[iptr(w, L.efc_id)[r]], R = 1.f / d;
       int z = x <= -R * fl ? 0 : (x >= R * fl ? 2 : 1);
@@ -2690,17 +2728,20 @@ KDEV float ls_decrease(const DModel& m, const Layout& L, const float* w, int nef
 // the oracle): Newton steps until the derivative changes sign, then Newton
 // from the left end / secant on the bracket. A Newton step whose end point is
 // in the same quadratic piece as its start is the exact minimiser.
-KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* gq_out);
+KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* gq_out, LsCache lc,
+                       const LsRow4** staged);
 
 // dec (when not null): the cost decrease at the returned step, by ls_decrease
-KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* dec) {
+KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* dec, LsCache lc) {
   float gq[3];
-  float a = linesearch_(m, L, w, nefc, exact, gq);
-  if (dec) *dec = a > 0.f ? ls_decrease(m, L, w, nefc, a, gq) : 0.f;
+  const LsRow4* staged = nullptr;
+  float a = linesearch_(m, L, w, nefc, exact, gq, lc, &staged);
+  if (dec) *dec = a > 0.f ? ls_decrease(m, L, w, nefc, a, gq, staged) : 0.f;
   return a;
 }
 
-KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* gq_out) {
+KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* gq_out, LsCache lc,
+                       const LsRow4** staged) {
   int nv = m.nv;
   const float *s = w + L.search, *M = w + L.qM;
   float *Mv = w + L.Mv, *Jv = w + L.efc_Jv;
@@ -2724,7 +2765,8 @@ KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int
   gq_out[0] = gq[0]; gq_out[1] = gq[1]; gq_out[2] = gq[2];
   *exact = 0;
   LsRows rows;
-  ls_rows(L, w, nefc, &rows);
+  ls_rows(L, w, nefc, &rows, lc);
+  *staged = rows.staged;
   LsPt p0 = ls_eval(m, L, w, nefc, 0.f, 0.f, gq, rows);
   if (!(p0.d1 < 0.f)) return 0.f;
   int it = 1, maxit = m.ls_iterations;
@@ -2759,7 +2801,8 @@ KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int
 // pred: the learned warm-start candidate (the env's row of the prediction in
 // device memory), or null
 template <bool TRI_CACHE>
-KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KProfAcc* pa, const float* pred) {
+KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KProfAcc* pa, const float* pred,
+                  LsCache lc) {
   int* misc = iptr(w, L.misc);
   int nv = m.nv;
   int nefc = misc[MISC_NEFC];
@@ -2817,7 +2860,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     int exact = 0;
     KPROF_MARK(pa, KP_DIR);
     float dec = 0.f;
-    float alpha = linesearch(kfresh(m), kfresh(L), w, nefc, &exact, newton ? nullptr : &dec);
+    float alpha = linesearch(kfresh(m), kfresh(L), w, nefc, &exact, newton ? nullptr : &dec, lc);
     KPROF_MARK(pa, KP_LS);
     *niter_out = it + 1;
     if (alpha == 0.f) break;
@@ -3267,8 +3310,10 @@ KDEV void write_features(const DModel& m, const Layout& L, const float* w, float
 // the plain step carries none of its code), bit 3 set by the register-capped
 // kernel variant
 // m and L: the sim's device-memory copy (KParams), read through kfresh().
+// lc: LDS for the line search's rows beside a device-memory workspace (or none)
 template <int NPSET>
-KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, int env, int nsub) {
+KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, int env, int nsub,
+                  LsCache lc = LsCache{nullptr, 0}) {
   int nq = m.nq, nv = m.nv, nu = m.nu;
   for (int i = LANE; i < nq; i += WAVE) w[L.qpos + i] = io.qpos[(size_t)env * nq + i];
   for (int i = LANE; i < nv; i += WAVE) {
@@ -3324,7 +3369,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     if (last) dump_ws(L, w, io.dump1, env);
     int it = 0;
     st |= st_solve<(NPSET & 8) == 0>(kfresh(m), kfresh(L), w, &it, &pa,
-                                     io.qacc_pred && s == 0 ? io.qacc_pred + (size_t)env * nv : nullptr);
+                                     io.qacc_pred && s == 0 ? io.qacc_pred + (size_t)env * nv : nullptr, lc);
     niter += it;
     if (last && io.has_aux) write_cfrc_ext(m, L, w, io.aux, env, true);
     if (last) dump_ws(L, w, io.dump2, env);

@@ -33,7 +33,14 @@ struct StepLaunch {
   size_t lds_bytes;    // dynamic LDS per workgroup (LDS variants)
   float* gws;          // device-memory workspaces [B][L.total] (VAR_GWS*)
   hipStream_t stream;
+  int ls_cap;          // VAR_GWS*: line-search rows staged in LDS (0: none)
 };
+
+// rows of the line-search cache beside a device-memory workspace: only for
+// models whose rows exceed the two per lane held in registers, at most 800
+// (12.8 KB of LDS per workgroup, within the share of a CU's LDS that the
+// register-limited occupancy of those kernels leaves to each workgroup)
+inline int ls_cache_rows(int maxefc) { return maxefc > 128 ? (maxefc < 800 ? maxefc : 800) : 0; }
 
 // kernels built for models of at most SPARSE_ROWS_NV dofs (MJK_BIG 0) ...
 void launch_step_np0(const StepLaunch& l, const StepIO& io, int nsub);

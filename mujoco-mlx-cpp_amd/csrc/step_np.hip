@@ -29,12 +29,13 @@
 // into AGPRs beyond 256 and the kernel runs one wave per SIMD, four envs per CU.
 template <int NPSET, int GWS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-k_step(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0, int nsub, float* gws) {
+k_step(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0, int nsub, float* gws, int ls_cap) {
   extern __shared__ __align__(16) float lds[];
   if ((int)blockIdx.x >= B) return;
   const mjk::KParams& P = mjk::kparams(kp);
   float* w = GWS ? gws + (size_t)blockIdx.x * P.L.total : lds;
-  mjk::run_env<NPSET>(P.m, P.L, w, io, env0 + (int)blockIdx.x, nsub);
+  mjk::LsCache lc{GWS ? reinterpret_cast<mjk::LsRow4*>(lds) : nullptr, GWS ? ls_cap : 0};
+  mjk::run_env<NPSET>(P.m, P.L, w, io, env0 + (int)blockIdx.x, nsub, lc);
 }
 
 // The same step capped at 168 VGPRs (three waves per SIMD, a few hundred
@@ -43,12 +44,13 @@ k_step(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0, int
 // workspaces, whose occupancy only registers limit.
 template <int NPSET, int GWS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
-k_step_occ(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0, int nsub, float* gws) {
+k_step_occ(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0, int nsub, float* gws, int ls_cap) {
   extern __shared__ __align__(16) float lds[];
   if ((int)blockIdx.x >= B) return;
   const mjk::KParams& P = mjk::kparams(kp);
   float* w = GWS ? gws + (size_t)blockIdx.x * P.L.total : lds;
-  mjk::run_env<NPSET | 8>(P.m, P.L, w, io, env0 + (int)blockIdx.x, nsub);
+  mjk::LsCache lc{GWS ? reinterpret_cast<mjk::LsRow4*>(lds) : nullptr, GWS ? ls_cap : 0};
+  mjk::run_env<NPSET | 8>(P.m, P.L, w, io, env0 + (int)blockIdx.x, nsub, lc);
 }
 
 // The step with the whole register file of a SIMD lane (one wave per SIMD),
@@ -109,22 +111,23 @@ void MJK_LAUNCH(const StepLaunch& l, const StepIO& io, int nsub) {
   constexpr int NP = MJK_NP | (MJK_BIG ? 16 : 0);
   dim3 g(l.B), b(64);
   bool gws = l.variant == VAR_GWS || l.variant == VAR_GWS_OCC;
-  size_t sh = gws ? 0 : l.lds_bytes;
+  // device-memory workspaces: the dynamic LDS holds the line search's row cache
+  size_t sh = gws ? (size_t)l.ls_cap * sizeof(LsRow4) : l.lds_bytes;
   switch (l.variant) {
     case VAR_GWS_OCC:
-      hipLaunchKernelGGL((k_step_occ<NP, 1>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws);
+      hipLaunchKernelGGL((k_step_occ<NP, 1>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws, l.ls_cap);
       break;
     case VAR_GWS:
-      hipLaunchKernelGGL((k_step<NP, 1>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws);
+      hipLaunchKernelGGL((k_step<NP, 1>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws, l.ls_cap);
       break;
     case VAR_LDS_OCC:
-      hipLaunchKernelGGL((k_step_occ<NP, 0>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws);
+      hipLaunchKernelGGL((k_step_occ<NP, 0>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws, 0);
       break;
     case VAR_LDS_BIG:
       hipLaunchKernelGGL((k_step_big<NP>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub);
       break;
     default:
-      hipLaunchKernelGGL((k_step<NP, 0>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws);
+      hipLaunchKernelGGL((k_step<NP, 0>), g, b, sh, l.stream, l.kp, io, l.B, l.env0, nsub, l.gws, 0);
       break;
   }
 }
