@@ -199,3 +199,27 @@ def test_mixed_box_counts_batch():
     print(f"mixed boxes: {len(groups)} groups, {nenv} envs, {ncon} contacts, worst remaining-gap ratio to FP64 CG "
           f"{worst_ratio:.2f}")
     assert nenv == 96 and ncon > 96
+
+
+@pytest.mark.parametrize("nbox", [8, 24])
+def test_boxes_500_step_rollout(nbox):
+    """The boxes workload's rollout length (500 control steps, CG 16): 64 envs
+    dropped from rest stay finite, never overflow a capacity, and settle on
+    the floor (every box centre above the floor by at least its smallest
+    half-size minus 1 cm, and the boxes' mean speed below 0.05 m/s at the end)."""
+    model = capi.Model(scenes.boxes_xml(nbox, 7))
+    B = 64
+    sim = capi.BatchedSim(model, B)
+    for _ in range(500):
+        sim.step(None, 4)
+    q, v = sim.state()
+    st = sim.stats()
+    assert np.isfinite(q).all() and np.isfinite(v).all()
+    assert not (st["status"] & ~0).any(), st["status"]
+    size = model.field("geom_size").reshape(-1, 3)[1:]
+    z = q.reshape(B, nbox, 7)[:, :, 2]
+    assert (z > size.min(axis=1)[None, :] - 0.01).all(), z.min()
+    speed = np.linalg.norm(v.reshape(B, nbox, 6)[:, :, :3], axis=2)
+    print(f"{nbox} boxes, 500 steps: lowest centre {z.min():.3f} m, mean speed {speed.mean():.4f} m/s, "
+          f"mean contacts {st['ncon'].mean():.1f}, iterations {st['niter'].mean():.1f}")
+    assert speed.mean() < 0.05
