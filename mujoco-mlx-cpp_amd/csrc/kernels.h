@@ -2729,19 +2729,22 @@ KDEV float ls_decrease(const DModel& m, const Layout& L, const float* w, int nef
 // from the left end / secant on the bracket. A Newton step whose end point is
 // in the same quadratic piece as its start is the exact minimiser.
 KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* gq_out, LsCache lc,
-                       const LsRow4** staged);
+                       const LsRow4** staged, const float* newton_cost);
 
-// dec (when not null): the cost decrease at the returned step, by ls_decrease
-KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* dec, LsCache lc) {
+// dec (when not null): the cost decrease at the returned step, by ls_decrease.
+// newton_cost (when not null): the search direction is the Newton direction
+// of the current point, whose cost it holds
+KDEV float linesearch(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* dec, LsCache lc,
+                      const float* newton_cost) {
   float gq[3];
   const LsRow4* staged = nullptr;
-  float a = linesearch_(m, L, w, nefc, exact, gq, lc, &staged);
+  float a = linesearch_(m, L, w, nefc, exact, gq, lc, &staged, newton_cost);
   if (dec) *dec = a > 0.f ? ls_decrease(m, L, w, nefc, a, gq, staged) : 0.f;
   return a;
 }
 
 KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int* exact, float* gq_out, LsCache lc,
-                       const LsRow4** staged) {
+                       const LsRow4** staged, const float* newton_cost) {
   int nv = m.nv;
   const float *s = w + L.search, *M = w + L.qM;
   float *Mv = w + L.Mv, *Jv = w + L.efc_Jv;
@@ -2767,7 +2770,19 @@ KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int
   LsRows rows;
   ls_rows(L, w, nefc, &rows, lc);
   *staged = rows.staged;
-  LsPt p0 = ls_eval(m, L, w, nefc, 0.f, 0.f, gq, rows);
+  LsPt p0;
+  if (newton_cost) {
+    // s = -H^-1 g: the slope along s at alpha = 0 is g.s and the curvature
+    // s'Hs = -g.s (H the Hessian s was solved with, that of the current
+    // active set), so the start point needs no pass over the rows
+    const float* g = w + L.grad;
+    float gs = 0.f;
+    for (int i = LANE; i < nv; i += WAVE) gs += g[i] * s[i];
+    gs = wsum(gs);
+    p0.alpha = 0.f; p0.f = *newton_cost; p0.d1 = gs; p0.d2 = -gs; p0.d1n = fabsf(gs); p0.nchange = 0;
+  } else {
+    p0 = ls_eval(m, L, w, nefc, 0.f, 0.f, gq, rows);
+  }
   if (!(p0.d1 < 0.f)) return 0.f;
   int it = 1, maxit = m.ls_iterations;
   LsPt lo = p0;
@@ -2860,7 +2875,8 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     int exact = 0;
     KPROF_MARK(pa, KP_DIR);
     float dec = 0.f;
-    float alpha = linesearch(kfresh(m), kfresh(L), w, nefc, &exact, newton ? nullptr : &dec, lc);
+    float alpha = linesearch(kfresh(m), kfresh(L), w, nefc, &exact, newton ? nullptr : &dec, lc,
+                             newton ? &cost : nullptr);
     KPROF_MARK(pa, KP_LS);
     *niter_out = it + 1;
     if (alpha == 0.f) break;
