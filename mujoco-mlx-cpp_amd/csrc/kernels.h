@@ -2923,7 +2923,12 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     KSYNC();
   }
   KPROF_MARK(pa, KP_DIR);
-  mul_jt(m, L, w, w + L.efc_force, nefc, qfc);
+  // qfrc_constraint = J^T force at the final point, which the last
+  // cost_grad left in grad = Ma - qfrc_smooth - J^T force
+  {
+    const float* qfs = w + L.qfrc_smooth;
+    for (int i = LANE; i < nv; i += WAVE) qfc[i] = (Ma[i] - qfs[i]) - grad[i];
+  }
   KSYNC();
   KPROF_MARK(pa, KP_SOLVE_REST);
   return st;
