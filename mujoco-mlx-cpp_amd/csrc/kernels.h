@@ -2101,11 +2101,12 @@ KDEV void mul_jt(const DModel& m, const Layout& L, const float* w, const float* 
 }
 
 // qacc <- x; Ma = M x; Jaref = J x - aref
-KDEV void set_qacc(const DModel& m, const Layout& L, float* w, const float* x, int nefc) {
+// Mx (when not null): M x already known (qfrc_smooth for x = qacc_smooth)
+KDEV void set_qacc(const DModel& m, const Layout& L, float* w, const float* x, int nefc, const float* Mx = nullptr) {
   int nv = m.nv;
   float *qacc = w + L.qacc, *Ma = w + L.Ma, *Jaref = w + L.efc_Jaref;
   const float *aref = w + L.efc_aref, *M = w + L.qM;
-  for (int i = LANE; i < nv; i += WAVE) Ma[i] = mulM_row(m, M, x, i);
+  for (int i = LANE; i < nv; i += WAVE) Ma[i] = Mx ? Mx[i] : mulM_row(m, M, x, i);
   for (int r = LANE; r < nefc; r += WAVE) Jaref[r] = jrow_dot(m, L, w, r, x) - aref[r];
   KSYNC();
   for (int i = LANE; i < nv; i += WAVE) qacc[i] = x[i];
@@ -2178,7 +2179,20 @@ KDEV int k_ellip_line(const float* x0, const float* v, const float* D, const flo
   return z;
 }
 
-KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* nchanged) {
+// grad = Ma - qfrc_smooth - J^T force at the current forces
+KDEV void grad_only(const DModel& m, const Layout& L, float* w, int nefc) {
+  int nv = m.nv;
+  const float *Ma = w + L.Ma, *qfs = w + L.qfrc_smooth;
+  float* grad = w + L.grad;
+  mul_jt(m, L, w, w + L.efc_force, nefc, grad);
+  KSYNC();
+  for (int i = LANE; i < nv; i += WAVE) grad[i] = Ma[i] - qfs[i] - grad[i];
+  KSYNC();
+}
+
+// with_grad false: forces, active flags and cost only (comparing warm-start
+// candidates), no gradient
+KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* nchanged, bool with_grad = true) {
   int nv = m.nv;
   const float *Jaref = w + L.efc_Jaref, *D = w + L.efc_D;
   float *force = w + L.efc_force, *Dact = w + L.efc_Dact;
@@ -2229,11 +2243,7 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
   float cost = wsum(part);
   *nchanged = wsumi(nch);
   KSYNC();
-  float* grad = w + L.grad;
-  mul_jt(m, L, w, force, nefc, grad);
-  KSYNC();
-  for (int i = LANE; i < nv; i += WAVE) grad[i] = Ma[i] - qfs[i] - grad[i];
-  KSYNC();
+  if (with_grad) grad_only(m, L, w, nefc);
   return cost;
 }
 
@@ -2831,27 +2841,32 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     return 0;
   }
   int nch = 0;
-  set_qacc(m, L, w, w + L.qacc_smooth, nefc);
-  float cost = cost_grad(m, L, w, nefc, &nch);
-  if (!(m.disableflags & DS_WARMSTART)) {
+  // warm start: the candidates compared by cost alone (M qacc_smooth =
+  // qfrc_smooth needs no product), the gradient only at the one kept
+  const float *qas = w + L.qacc_smooth, *qws = w + L.qacc_ws, *qfs0 = w + L.qfrc_smooth;
+  bool ws = !(m.disableflags & DS_WARMSTART);
+  set_qacc(m, L, w, qas, nefc, qfs0);
+  float cost = cost_grad(m, L, w, nefc, &nch, !ws);
+  if (ws) {
     float cs = cost;
-    set_qacc(m, L, w, w + L.qacc_ws, nefc);
-    cost = cost_grad(m, L, w, nefc, &nch);
-    const float* best = cost > cs ? w + L.qacc_smooth : w + L.qacc_ws;
+    set_qacc(m, L, w, qws, nefc);
+    cost = cost_grad(m, L, w, nefc, &nch, false);
+    const float* best = cost > cs ? qas : qws;
     float cbest = cost > cs ? cs : cost;
+    const float* cur = qws;  // the candidate the workspace holds
     if (pred) {
       // learned candidate: kept only when its cost is strictly the lowest
       set_qacc(m, L, w, pred, nefc);
-      float cp = cost_grad(m, L, w, nefc, &nch);
+      float cp = cost_grad(m, L, w, nefc, &nch, false);
       if (cp < cbest) { best = pred; cbest = cp; }
       cost = cp;
-      if (best != pred) {
-        set_qacc(m, L, w, best, nefc);
-        cost = cost_grad(m, L, w, nefc, &nch);
-      }
-    } else if (cost > cs) {
-      set_qacc(m, L, w, w + L.qacc_smooth, nefc);
+      cur = pred;
+    }
+    if (best != cur) {
+      set_qacc(m, L, w, best, nefc, best == qas ? qfs0 : nullptr);
       cost = cost_grad(m, L, w, nefc, &nch);
+    } else {
+      grad_only(m, L, w, nefc);
     }
   }
   float scale = 1.f / (m.meaninertia * (float)(nv > 1 ? nv : 1));
