@@ -576,6 +576,9 @@ KDEV void st_crb(const DModel& m, const Layout& L, float* w) {
 // final once step k starts: its entries are only touched by steps of its
 // descendants, which have larger indices). Rows are scaled by 1/D in one
 // final pass. Returns ST_M_NOT_PD on a non-positive pivot.
+// DENSE: the dense elimination below is compiled in (not in the 168-VGPR
+// kernels, whose register allocation it upsets: quadruped +0.7 %)
+template <bool DENSE = true>
 KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
   int nv = m.nv;
   // beyond 64 dofs only the tree's entries are copied and scaled (row k:
@@ -590,11 +593,76 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
   }
   KSYNC();
   int bad = 0;
+  // trees of at most 32 dofs with more levels than half their dofs (long
+  // chains: the humanoid's 14 levels for 27 dofs): dense elimination, whose
+  // nv / 2 phases need no table loads (humanoid -2.1 % per control step);
+  // the others (the quadruped's 8 levels for 18 dofs, free bodies' 6-dof
+  // chains): the level tables
+  bool dense = DENSE && !sparse && nv <= SPARSE_ROWS_NV && 2 * m.nflev > nv;
+  if (dense) {
+    // Dense elimination from the last dof to the first, two dofs (k, k - 1)
+    // per LDS phase: with c = A[k][k-1] / A[k][k] and the pair's second pivot
+    // d1 = A[k-1][k-1] - c A[k][k-1], the leading entries (i, j),
+    // j <= i < k - 1, take both rank-1 updates at once,
+    //   A[i][j] -= A[k][i] A[k][j] / A[k][k] + b_i b_j / d1,  b_x = A[k-1][x] - c A[k][x];
+    // row k - 1 takes its own update in one pass at the end. Entries outside
+    // the dof tree stay zero (A[k][i] = 0 unless i is an ancestor of k), so
+    // this is the tree-sparse factor, without the level tables' loads. A
+    // lane's entries of a leading triangle, enumerated row-major, are the
+    // same at every step: for up to TR rounds of lanes they are computed once.
+    constexpr int TR = WAVE >= 32 ? 3 * (64 / WAVE) : 1;
+    int tri_ij[TR];
+    int ntri0 = nv >= 2 ? (nv - 2) * (nv - 1) / 2 : 0;
+    bool cached = ntri0 <= TR * WAVE;
+#pragma unroll
+    for (int r = 0; r < TR; r++) {
+      int t = LANE + r * WAVE, a = 0, b = 0;
+      if (cached && t < ntri0) tri_rc(t, &a, &b);
+      tri_ij[r] = a | (b << 16);
+    }
+    for (int k = nv - 1; k >= 1; k -= 2) {
+      float d0 = LD[k * nv + k], h = LD[k * nv + k - 1], a11 = LD[(k - 1) * nv + k - 1];
+      if (!(d0 > KMINVAL)) { bad = 1; d0 = KMINVAL; }
+      float i0 = 1.f / d0, c = h * i0, d1 = a11 - c * h;
+      if (!(d1 > KMINVAL)) { bad = 1; d1 = KMINVAL; }
+      float i1 = 1.f / d1;
+      int nt = (k - 1) * k / 2;  // entries (i, j), j <= i < k - 1
+      const float *rk = LD + k * nv, *rk1 = LD + (k - 1) * nv;
+      if (cached) {
+#pragma unroll
+        for (int r = 0; r < TR; r++) {
+          if (LANE + r * WAVE >= nt) continue;
+          int i = tri_ij[r] & 0xffff, j = tri_ij[r] >> 16;
+          float ai = rk[i], aj = rk[j];
+          float bi = rk1[i] - c * ai, bj = rk1[j] - c * aj;
+          LD[i * nv + j] -= ai * aj * i0 + bi * bj * i1;
+        }
+      } else {
+        for (int t = LANE; t < nt; t += WAVE) {
+          int i, j;
+          tri_rc(t, &i, &j);
+          float ai = rk[i], aj = rk[j];
+          float bi = rk1[i] - c * ai, bj = rk1[j] - c * aj;
+          LD[i * nv + j] -= ai * aj * i0 + bi * bj * i1;
+        }
+      }
+      KSYNC();
+    }
+    // the pairs' second rows (their diagonal included), each from its first
+    // row, which is final: rows of different pairs do not read each other
+    for (int k = nv - 1; k >= 1; k -= 2) {
+      float d0 = LD[k * nv + k];
+      if (!(d0 > KMINVAL)) d0 = KMINVAL;
+      float c = LD[k * nv + k - 1] * (1.f / d0);
+      for (int x = LANE; x <= k - 1; x += WAVE) LD[(k - 1) * nv + x] -= c * LD[k * nv + x];
+    }
+    KSYNC();
+  }
   // dofs of one tree level are neither ancestors nor descendants of each
   // other, so their steps run together: each entry (i, j) of the level's list
   // sums the contributions of the level's dofs that have both i and j as
   // strict ancestors. Levels go deepest first, so a dof's row is final when read.
-  for (int l = 0; l < m.nflev; l++) {
+  for (int l = 0; !dense && l < m.nflev; l++) {
     for (int e = m.flev_adr[l] + LANE; e < m.flev_adr[l + 1]; e += WAVE) {
       int i = m.flev_i[e], j = m.flev_j[e];
       float acc = 0.f;
@@ -2953,6 +3021,7 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
 // Euler (implicit dof damping when present) / implicitfast, then the position
 // update; free and ball quaternions are integrated in FP64
 // (reference euler.hpp:26-110 restated with MuJoCo's mj_integratePos).
+template <bool DENSE>
 KDEV int st_integrate(const DModel& m, const Layout& L, float* w) {
   int nv = m.nv;
   float *qvel = w + L.qvel, *qpos = w + L.qpos, *qacc = w + L.qacc, *ws = w + L.qacc_ws;
@@ -2981,7 +3050,7 @@ KDEV int st_integrate(const DModel& m, const Layout& L, float* w) {
       }
       KSYNC();
     }
-    st |= factor_ld(m, Bm, A);
+    st |= factor_ld<DENSE>(m, Bm, A);
     const float *qfs = w + L.qfrc_smooth, *qfc = w + L.qfrc_constraint;
     for (int k = LANE; k < nv; k += WAVE) acc[k] = qfs[k] + qfc[k];
     KSYNC();
@@ -3365,7 +3434,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     st_kinematics(m, L, w);
     st_com_pos(m, L, w);
     st_crb(m, L, w);
-    if (nv > 0) factor_ld(m, w + L.qM, w + L.qLD);
+    if (nv > 0) factor_ld<(NPSET & 8) == 0>(m, w + L.qM, w + L.qLD);
     st_com_vel(m, L, w);
     st_smooth(m, L, w);
     // the kinematics and dynamics intermediates are intact only here: the
@@ -3389,7 +3458,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     KPROF_MARK(&pa, KP_COMPOS);
     st_crb(kfresh(m), kfresh(L), w);
     KPROF_MARK(&pa, KP_CRB);
-    if (nv > 0) st |= factor_ld(kfresh(m), w + L.qM, w + L.qLD);
+    if (nv > 0) st |= factor_ld<(NPSET & 8) == 0>(kfresh(m), w + L.qM, w + L.qLD);
     KPROF_MARK(&pa, KP_FACTOR);
     st_com_vel(kfresh(m), kfresh(L), w);
     KPROF_MARK(&pa, KP_COMVEL);
@@ -3409,7 +3478,7 @@ KDEV void run_env(const DModel& m, const Layout& L, float* w, const StepIO& io, 
     niter += it;
     if (last && io.has_aux) write_cfrc_ext(m, L, w, io.aux, env, true);
     if (last) dump_ws(L, w, io.dump2, env);
-    st |= st_integrate(kfresh(m), kfresh(L), w);
+    st |= st_integrate<(NPSET & 8) == 0>(kfresh(m), kfresh(L), w);
     KPROF_MARK(&pa, KP_INTEGRATE);
   }
   int bad = 0;
