@@ -385,6 +385,58 @@ KDEV void st_kinematics(const DModel& m, const Layout& L, float* w) {
     xipos[0] = xipos[1] = xipos[2] = 0.f;
   }
   KSYNC();
+  // pass 1, all bodies at once: each body's frame relative to its parent's
+  // (free joints: in the world), kept in its own xpos / xquat, and its
+  // joints' anchors and axes in the parent's frame (world for free joints),
+  // kept in xanchor / xaxis; the model reads and the joints' trigonometry
+  // leave the level-by-level chain
+  for (int b = 1 + LANE; b < m.nbody; b += WAVE) {
+    int ja = m.body_jntadr[b], jn = m.body_jntnum[b];
+    float pos[3], q[4];
+    if (jn > 0 && m.jnt_type[ja] == JFREE) {
+      int qa = m.jnt_qposadr[ja];
+      pos[0] = qpos[qa]; pos[1] = qpos[qa + 1]; pos[2] = qpos[qa + 2];
+      q[0] = qpos[qa + 3]; q[1] = qpos[qa + 4]; q[2] = qpos[qa + 5]; q[3] = qpos[qa + 6];
+      normq(q);
+      for (int k = 0; k < 3; k++) { xanc[3 * ja + k] = pos[k]; xax[3 * ja + k] = m.jnt_axis[3 * ja + k]; }
+    } else {
+      for (int k = 0; k < 3; k++) pos[k] = m.body_pos[3 * b + k];
+      for (int k = 0; k < 4; k++) q[k] = m.body_quat[4 * b + k];
+      for (int j = ja; j < ja + jn; j++) {
+        int qa = m.jnt_qposadr[j];
+        float R[9], anc[3], ax[3];
+        quat2mat(q, R);
+        mv3(R, m.jnt_pos + 3 * j, anc);
+        anc[0] += pos[0]; anc[1] += pos[1]; anc[2] += pos[2];
+        mv3(R, m.jnt_axis + 3 * j, ax);
+        for (int k = 0; k < 3; k++) { xanc[3 * j + k] = anc[k]; xax[3 * j + k] = ax[k]; }
+        int jt = m.jnt_type[j];
+        if (jt == JSLIDE) {
+          float sl = qpos[qa] - m.qpos0[qa];
+          for (int k = 0; k < 3; k++) pos[k] += ax[k] * sl;
+        } else {
+          float ql[4];
+          if (jt == JHINGE) {
+            float ang = 0.5f * (qpos[qa] - m.qpos0[qa]);
+            float sn = sinf(ang), cs = cosf(ang);
+            ql[0] = cs; ql[1] = m.jnt_axis[3 * j] * sn; ql[2] = m.jnt_axis[3 * j + 1] * sn; ql[3] = m.jnt_axis[3 * j + 2] * sn;
+          } else {
+            ql[0] = qpos[qa]; ql[1] = qpos[qa + 1]; ql[2] = qpos[qa + 2]; ql[3] = qpos[qa + 3];
+            normq(ql);
+          }
+          mulquat(q, ql, q);
+          float R2[9], r[3];
+          quat2mat(q, R2);
+          mv3(R2, m.jnt_pos + 3 * j, r);
+          for (int k = 0; k < 3; k++) pos[k] = anc[k] - r[k];
+        }
+      }
+    }
+    for (int k = 0; k < 3; k++) xpos[3 * b + k] = pos[k];
+    for (int k = 0; k < 4; k++) xquat[4 * b + k] = q[k];
+  }
+  KSYNC();
+  // pass 2, level by level: compose with the parent's world frame
   for (int lv = 0; lv < m.nlevel; lv++) {
     int a = m.level_adr[lv], n = m.level_adr[lv + 1] - a;
     for (int i = LANE; i < n; i += WAVE) {
@@ -392,45 +444,19 @@ KDEV void st_kinematics(const DModel& m, const Layout& L, float* w) {
       int par = m.body_parentid[b];
       int ja = m.body_jntadr[b], jn = m.body_jntnum[b];
       float pos[3], q[4];
-      if (jn > 0 && m.jnt_type[ja] == JFREE) {
-        int qa = m.jnt_qposadr[ja];
-        pos[0] = qpos[qa]; pos[1] = qpos[qa + 1]; pos[2] = qpos[qa + 2];
-        q[0] = qpos[qa + 3]; q[1] = qpos[qa + 4]; q[2] = qpos[qa + 5]; q[3] = qpos[qa + 6];
-        normq(q);
-        for (int k = 0; k < 3; k++) { xanc[3 * ja + k] = pos[k]; xax[3 * ja + k] = m.jnt_axis[3 * ja + k]; }
-      } else {
+      for (int k = 0; k < 3; k++) pos[k] = xpos[3 * b + k];
+      for (int k = 0; k < 4; k++) q[k] = xquat[4 * b + k];
+      if (!(jn > 0 && m.jnt_type[ja] == JFREE)) {
+        const float *Rp = xmat + 9 * par, *pp = xpos + 3 * par;
         float t[3];
-        mv3(xmat + 9 * par, m.body_pos + 3 * b, t);
-        pos[0] = xpos[3 * par] + t[0]; pos[1] = xpos[3 * par + 1] + t[1]; pos[2] = xpos[3 * par + 2] + t[2];
-        mulquat(xquat + 4 * par, m.body_quat + 4 * b, q);
+        mv3(Rp, pos, t);
+        pos[0] = pp[0] + t[0]; pos[1] = pp[1] + t[1]; pos[2] = pp[2] + t[2];
+        mulquat(xquat + 4 * par, q, q);
         for (int j = ja; j < ja + jn; j++) {
-          int qa = m.jnt_qposadr[j];
-          float R[9], anc[3], ax[3];
-          quat2mat(q, R);
-          mv3(R, m.jnt_pos + 3 * j, anc);
-          anc[0] += pos[0]; anc[1] += pos[1]; anc[2] += pos[2];
-          mv3(R, m.jnt_axis + 3 * j, ax);
-          for (int k = 0; k < 3; k++) { xanc[3 * j + k] = anc[k]; xax[3 * j + k] = ax[k]; }
-          int jt = m.jnt_type[j];
-          if (jt == JSLIDE) {
-            float s = qpos[qa] - m.qpos0[qa];
-            for (int k = 0; k < 3; k++) pos[k] += ax[k] * s;
-          } else {
-            float ql[4];
-            if (jt == JHINGE) {
-              float ang = 0.5f * (qpos[qa] - m.qpos0[qa]);
-              float s = sinf(ang), c = cosf(ang);
-              ql[0] = c; ql[1] = m.jnt_axis[3 * j] * s; ql[2] = m.jnt_axis[3 * j + 1] * s; ql[3] = m.jnt_axis[3 * j + 2] * s;
-            } else {
-              ql[0] = qpos[qa]; ql[1] = qpos[qa + 1]; ql[2] = qpos[qa + 2]; ql[3] = qpos[qa + 3];
-              normq(ql);
-            }
-            mulquat(q, ql, q);
-            float R2[9], r[3];
-            quat2mat(q, R2);
-            mv3(R2, m.jnt_pos + 3 * j, r);
-            for (int k = 0; k < 3; k++) pos[k] = anc[k] - r[k];
-          }
+          float an[3], ax[3];
+          mv3(Rp, xanc + 3 * j, an);
+          mv3(Rp, xax + 3 * j, ax);
+          for (int k = 0; k < 3; k++) { xanc[3 * j + k] = pp[k] + an[k]; xax[3 * j + k] = ax[k]; }
         }
         normq(q);
       }
