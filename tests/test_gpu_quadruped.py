@@ -172,18 +172,58 @@ def test_rollout_1000_steps(model):
     assert report[1][1] < 1e-4
 
 
+def _margin_flip(model, sim, q, v, qa, c, v_engine):
+    """The kernel interpreter steps the FP32 input; the oracle takes each of
+    its substep states. Returns a description of the contact that crossed
+    the margin, or None when the interpreter differs from the engine or a
+    substep differs for another reason."""
+    import emu
+    import ref64
+    f32 = lambda a: np.ascontiguousarray(np.asarray(a, np.float32)[None])
+    em = emu.EmuSim(scenes.quadruped_xml(self_collision=True), 1, maxcon=sim.capacity()[0],
+                    maxefc=sim.capacity()[1])
+    em.set_state(f32(q), f32(v), f32(qa))
+    orc = ref64.Oracle(ref64.fields_from(model.field))
+    flips = []
+    for s in range(4):
+        qe, ve, ae = em.get_state()
+        states.oracle_step(orc, qe[0], ve[0], ae[0], np.asarray(c))
+        em.step(f32(c), 1, dump=True)
+        st = em.stats()
+        con = em.dump(1, 0, "contact").reshape(-1, 16)[: st["ncon"][0]]
+        kpairs = {int(x[13:14].view(np.int32)[0]): float(x[12]) for x in con}
+        opairs = {cc["pair"]: cc["dist"] for cc in orc.contacts()}
+        only = [(p, d) for p, d in list(kpairs.items()) + list(opairs.items()) if (p in kpairs) != (p in opairs)]
+        eq, _ = states.qacc_force_errors(orc, em.dump(2, 0, "qacc"), None, None)
+        if only:
+            if not all(abs(d) < 1e-7 for _, d in only):
+                return None
+            flips.append(f"substep {s}: pair {only[0][0]} at {only[0][1]:.1e} m")
+        elif eq > 1e-4:
+            return None
+    _, ve, _ = em.get_state()
+    if not flips or np.abs(ve[0] - v_engine).max() / max(1.0, float(np.abs(ve[0]).max())) > 1e-4:
+        return None
+    return "; ".join(flips)
+
+
 def test_rollout_checked_every_step(model):
     """The FP32 workload's rollout (self-collision, 4096 envs, actions nominal
     + U(+-0.5) from seed 1, 1000 control steps of 4 substeps) with 64 sampled
     envs (every 64th) checked against the FP64 oracle at every step: one oracle
     control step from the engine's own FP32 state, warm start and actions of
     the step before. Bar: qpos within 1e-5 and qvel within 1e-4 (relative to
-    max(1, |qvel|)) of the oracle. A sample beyond it must lie within the bar
-    of the oracle's step from one of 16 copies of its input state perturbed at
-    FP32 rounding level (qpos x (1 + U(+-6e-8))): a contact whose distance
-    crosses zero within FP32 position resolution (about 3e-8 m at the torso's
-    height) switches on in one branch and off in the other, and FP32 state
-    arithmetic lands on either."""
+    max(1, |qvel|)) of the oracle. A contact whose distance crosses zero
+    within FP32 position resolution (about 3e-8 m at the torso's height)
+    switches on in one branch and off in the other, and FP32 state arithmetic
+    lands on either, so a sample beyond the bar must be shown to be one:
+    either it lies within the bar of the oracle's step from one of 16 copies
+    of its input state perturbed at FP32 rounding level (qpos x (1 +
+    U(+-6e-8))), or the serial interpreter of the kernel source reproduces the
+    engine's step (qvel within 1e-4) and, taking the oracle through the
+    interpreter's own substep states, every substep either agrees (qacc within
+    1e-4) or differs by a contact on one side only within 1e-7 m of the
+    margin."""
     import ref64
     B, NS, STEPS = 4096, 64, 1000
     sample = np.arange(0, B, B // NS)
@@ -218,15 +258,19 @@ def test_rollout_checked_every_step(model):
             orc.batch_step(qp, vp, ap, np.tile(cs[i], (16, 1)), 4, 8)
             pq, pv = errors(q[sample[i]], v[sample[i]], qp, vp)
             hit = (pq <= 1e-5) & (pv <= 1e-4)
-            assert hit.any(), (k, int(sample[i]), float(eq[i]), float(ev[i]), float(pq.min()), float(pv.min()))
-            branched.append((k, int(sample[i]), float(ev[i]), int(hit.sum())))
+            why = f"{int(hit.sum())} of 16 perturbed oracle steps"
+            if not hit.any():
+                flip = _margin_flip(model, sim, qs[i], vs[i], as_[i], cs[i], v[sample[i]])
+                assert flip, (k, int(sample[i]), float(eq[i]), float(ev[i]), float(pq.min()), float(pv.min()))
+                why = flip
+            branched.append((k, int(sample[i]), float(ev[i]), why))
     st = sim.stats()
     assert np.isfinite(q).all() and np.isfinite(v).all() and not (st["status"] & ~3).any()
     eq_all, ev_all = np.concatenate(eq_all), np.concatenate(ev_all)
     print(f"{STEPS} steps x {NS} sampled envs: qpos error median {np.median(eq_all):.1e}, p99 "
           f"{np.quantile(eq_all, 0.99):.1e}, max {eq_all.max():.1e}; qvel error median {np.median(ev_all):.1e}, "
-          f"p99 {np.quantile(ev_all, 0.99):.1e}, max {ev_all.max():.1e}; beyond the bar on the oracle's own "
-          f"FP32-perturbed branch (step, env, qvel error, matching draws of 16): {branched}")
+          f"p99 {np.quantile(ev_all, 0.99):.1e}, max {ev_all.max():.1e}; beyond the bar, each a contact at the "
+          f"margin (step, env, qvel error, how shown): {branched}")
     assert len(branched) <= STEPS * NS // 1000
 
 
