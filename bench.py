@@ -226,9 +226,9 @@ def run_boxes(B, warmup, steps):
 STAGES = [("kin", "kinematics"), ("compos", "subtree com"), ("crb", "CRB"), ("factor", "LDL of qM"),
           ("comvel", "com velocities"), ("smooth", "smooth forces (RNE, passive, actuation)"),
           ("coll", "collision"), ("constr", "constraint rows"), ("solve_init", "solver set-up"),
-          ("ls", "line search"), ("costgrad", "cost / gradient"), ("dir", "search direction (factor + solve)"),
+          ("ls", "line search"), ("costgrad", "cost / gradient"), ("dir", "search direction (LDL^T of H; CG: M^-1 solve)"),
           ("solve_rest", "other solver work"), ("integrate", "integration"), ("io", "state in/out"),
-          ("hbuild", "Newton Hessian build")]
+          ("hbuild", "Newton Hessian build"), ("nsolve", "Newton direction substitution")]
 
 
 def run_stages(args):

@@ -101,10 +101,11 @@ MJMLX_API int mjmlx_batched_set_graph(MjmlxBatchedSim* sim, int on);
  * float[num_envs * total] each (total from mjmlx_batched_layout("total")). */
 MJMLX_API int mjmlx_batched_debug_step(MjmlxBatchedSim* sim, const float* ctrl, float* dump1, float* dump2);
 /* Profiling builds (libmjmlx_prof.so) only: one call like step_n that also
- * returns per-env cycle counts of 16 pipeline stages (kinematics, com,
+ * returns per-env cycle counts of 17 pipeline stages (kinematics, com,
  * CRB, factorisation, velocities, smooth forces, collision, constraints,
- * solver set-up, line search, cost/gradient, Newton direction, other solver
- * work, integration, state I/O), unsigned long long[num_envs * 16].
+ * solver set-up, line search, cost/gradient, search direction, other solver
+ * work, integration, state I/O, Newton Hessian build, Newton substitution),
+ * unsigned long long[num_envs * 17].
  * Returns -1 in the product build. */
 MJMLX_API int mjmlx_batched_profile_step(MjmlxBatchedSim* sim, const float* ctrl, int nsub, unsigned long long* out);
 /* offset / length (floats) / kind (0 float, 1 int) of a workspace field */

@@ -1238,7 +1238,7 @@ MJMLX_API int mjmlx_batched_debug_step(MjmlxBatchedSim* s, const float* ctrl, fl
 }
 
 // Per-stage cycle counts of one call (profiling builds only: compiled with
-// MJK_PROFILE); out is unsigned long long[num_envs * 16]. Returns 0, or -1
+// MJK_PROFILE); out is unsigned long long[num_envs * 17]. Returns 0, or -1
 // in builds without the instrumentation.
 MJMLX_API int mjmlx_batched_profile_step(MjmlxBatchedSim* s, const float* ctrl, int nsub, unsigned long long* out) {
 #if defined(MJK_PROFILE)

@@ -52,12 +52,12 @@ namespace mjk {
 
 // Per-stage cycle accounting, compiled in only for profiling builds
 // (MJK_PROFILE); the product build has empty marks.
+enum { KP_KIN = 0, KP_COMPOS, KP_CRB, KP_FACTOR, KP_COMVEL, KP_SMOOTH, KP_COLL, KP_CONSTR, KP_SOLVE_INIT, KP_LS,
+       KP_COSTGRAD, KP_DIR, KP_SOLVE_REST, KP_INTEGRATE, KP_IO, KP_SPARE, KP_NSOLVE, KP_N };
 struct KProfAcc {
-  unsigned long long t[16];
+  unsigned long long t[KP_N];
   unsigned long long last;
 };
-enum { KP_KIN = 0, KP_COMPOS, KP_CRB, KP_FACTOR, KP_COMVEL, KP_SMOOTH, KP_COLL, KP_CONSTR, KP_SOLVE_INIT, KP_LS,
-       KP_COSTGRAD, KP_DIR, KP_SOLVE_REST, KP_INTEGRATE, KP_IO, KP_SPARE, KP_N };
 #if defined(MJK_PROFILE) && !defined(MJ_EMU)
 #define KPROF_START(pa) ((pa)->last = clock64())
 #define KPROF_MARK(pa, i)                          \
@@ -2971,7 +2971,9 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
   KPROF_MARK(pa, KP_SOLVE_INIT);
   if (newton) {
     st |= newton_factor<TRI_CACHE>(m, L, w, nefc, pa);
+    KPROF_MARK(pa, KP_DIR);
     newton_solve(m, L, w);
+    KPROF_MARK(pa, KP_NSOLVE);
   } else {
     for (int i = LANE; i < nv; i += WAVE) Mg[i] = grad[i];
     KSYNC();
@@ -3013,7 +3015,9 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     if (newton) {
       // H depends on qacc only through the active set: refactor only when it changed
       if (nch) st |= newton_factor<TRI_CACHE>(kfresh(m), kfresh(L), w, nefc, pa);
+      KPROF_MARK(pa, KP_DIR);
       newton_solve(kfresh(m), kfresh(L), w);
+      KPROF_MARK(pa, KP_NSOLVE);
     } else {
       for (int i = LANE; i < nv; i += WAVE) Mg[i] = grad[i];
       KSYNC();
