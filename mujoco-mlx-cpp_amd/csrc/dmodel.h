@@ -37,6 +37,11 @@ enum { MAX_DOF_DEPTH = 64 };
 // indices per row. Up to it the rows stay dense for the Hessian's matrix-core
 // tiles (32 x 32).
 enum { SPARSE_ROWS_NV = 32 };
+// beyond this many dofs qM and its LDL^T factor are stored as compact rows
+// (MuJoCo's sparse qM): row k holds its diagonal, then its dof ancestors
+// nearest first, at dof_madr[k]; entry (k, j) of an ancestor j sits at
+// dof_madr[k] + dof_depth[k] - dof_depth[j]. Up to it both are dense nv x nv.
+enum { COMPACT_M_NV = 64 };
 
 // Device passes see the model's arrays as global memory (address space 1), so
 // loads through them stay global loads wherever the struct itself is read
@@ -58,6 +63,7 @@ struct DModel {
   int nfeat;  // learned warm-start input width: nq + nv + nu + FEAT_NCON * FEAT_PER_CON
   int nsensor, nsensordata, nsens_acc;  // sensors, sensordata width, accelerometers
   int jw;  // entries per compressed constraint row (0: dense rows of nv)
+  int nM;  // stored entries of qM (and of its factor): nv * nv, or the compact rows' total
   float magnetic[3];
   // pointers only from here on: Blob::bind relocates every word after body_parentid
   // bodies
@@ -71,7 +77,7 @@ struct DModel {
   const MJK_G float *qpos0, *qpos_spring;
   // dofs
   const MJK_G int *dof_bodyid, *dof_jntid, *dof_parentid, *dof_depth, *dof_subtree_end, *doflevel_adr,
-      *doflevel_dof, *dof_chain, *dof_nchain, *dof_act_adr, *dof_act_num, *dof_act_list;
+      *doflevel_dof, *dof_chain, *dof_nchain, *dof_act_adr, *dof_act_num, *dof_act_list, *dof_madr;
   // level-parallel LDL of qM: level l updates entries [flev_adr[l], flev_adr[l+1]),
   // entry e is (flev_i[e], flev_j[e]) and sums over dofs flev_k[flev_kadr[e] .. + flev_kcnt[e])
   const MJK_G int *flev_adr, *flev_i, *flev_j, *flev_kadr, *flev_kcnt, *flev_k;
@@ -110,6 +116,7 @@ struct Layout {
   // con_cap: contacts counted per env (maxcon <= con_cap of them are stored:
   // at least every contact the row budget can admit)
   int maxcon, maxefc, efc_cap, con_cap, nv, total;
+  int nM;    // floats of qM and of qLD (DModel::nM)
   int rand;  // per-env physical parameters (rand_size floats), or -1 without domain randomisation
   int qpos, qvel, ctrl, qacc_ws, qacc, qM, qLD, qfrc_smooth, qacc_smooth, qfrc_constraint, act_force, misc;
   int xpos, xquat, xmat, xipos, xanchor, xaxis, gxpos, gxmat, scom, cinert, cdof, cvel, cdofdot, crb, cacc, cfrc, buf6;
@@ -155,11 +162,13 @@ MJK_HD inline int rand_size(int nbody, int nu, int nfree) { return 4 * nbody + 1
 // SPARSE_ROWS_NV dofs); subtree com and cdof then stay live through the solve
 // (J^T f is accumulated from per-body contact wrenches).
 MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, int ngeom, int maxcon, int maxefc,
-                                 bool qld_shared = false, int nrand = 0, int jw = 0) {
+                                 bool qld_shared = false, int nrand = 0, int jw = 0, int nM = 0) {
   Layout L;
   L.maxcon = maxcon; L.maxefc = maxefc; L.efc_cap = maxefc; L.con_cap = maxcon; L.nv = nv;
   L.jw = jw;
   int nvp = nv > 0 ? nv : 1, nup = nu > 0 ? nu : 1, njp = njnt > 0 ? njnt : 1, ngp = ngeom > 0 ? ngeom : 1;
+  if (nM <= 0) nM = nvp * nvp;
+  L.nM = nM;
   int ncon = maxcon > 0 ? maxcon : 1;
   int o = 0;
   // persistent across the substep
@@ -168,8 +177,8 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   L.ctrl = o; o += al2(nup);
   L.qacc_ws = o; o += al2(nvp);
   L.qacc = o; o += al2(nvp);
-  L.qM = o; o += al2(nvp * nvp);
-  if (!qld_shared) { L.qLD = o; o += al2(nvp * nvp); }
+  L.qM = o; o += al2(nM);
+  if (!qld_shared) { L.qLD = o; o += al2(nM); }
   L.qfrc_smooth = o; o += al2(nvp);
   L.qacc_smooth = o; o += al2(nvp);
   L.qfrc_constraint = o; o += al2(nvp);
@@ -231,7 +240,7 @@ MJK_HD inline Layout make_layout(int nq, int nv, int nu, int nbody, int njnt, in
   int rows_end = al2(o);
   o = al2(kc_end > rows_end ? kc_end : rows_end);
   // the FP64 Newton Hessian, its lower triangle packed row by row (Newton
-  // only; its nv (nv + 1) floats also hold the nv x nv LDL factor of qM)
+  // only; its nv (nv + 1) floats also hold the LDL factor of qM, nM <= nv (nv + 1))
   L.H = o;
   if (qld_shared) { o += al2(nvp * (nvp + 1)); L.qLD = L.H; }
   int nvec = al2(8 * nvp) + 4 * nvp;  // eight FP32 vectors, then two FP64 ones

@@ -52,7 +52,7 @@ __attribute__((visibility("default"))) void* mjemu_create(const char* xml, int s
     s->caps = caps;
     auto& m = s->model;
     s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, caps.maxcon, caps.maxefc,
-                            m.solver == mjh::SOL_NEWTON, 0, s->dm.jw);
+                            m.solver == mjh::SOL_NEWTON, 0, s->dm.jw, s->dm.nM);
     s->L.efc_cap = caps.efc_cap;
     s->L.con_cap = caps.con_cap;
     s->fields = mjh::layout_fields(m, s->L);
@@ -83,7 +83,7 @@ __attribute__((visibility("default"))) int mjemu_set_rand(void* p, const float* 
   s->nrand = blocks ? mjk::rand_size(m.nbody, m.nu, nfree) : 0;
   s->rand.assign(blocks, blocks ? blocks + (size_t)s->B * s->nrand : blocks);
   s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, s->caps.maxcon, s->caps.maxefc,
-                          m.solver == mjh::SOL_NEWTON, s->nrand, s->dm.jw);
+                          m.solver == mjh::SOL_NEWTON, s->nrand, s->dm.jw, s->dm.nM);
   s->L.efc_cap = s->caps.efc_cap;
   s->L.con_cap = s->caps.con_cap;
   s->fields = mjh::layout_fields(m, s->L);

@@ -593,7 +593,7 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     for (int j = 0; j < m.njnt; j++) s->nfree += m.jnt_type[j] == mjk::JFREE;
     if (config->domain_rand) s->nrand = mjk::rand_size(m.nbody, m.nu, s->nfree);
     s->L = mjk::make_layout(m.nq, m.nv, m.nu, m.nbody, m.njnt, m.ngeom, caps.maxcon, caps.maxefc,
-                            m.solver == mjh::SOL_NEWTON, s->nrand, s->blob.proto.jw);
+                            m.solver == mjh::SOL_NEWTON, s->nrand, s->blob.proto.jw, s->blob.proto.nM);
     s->L.efc_cap = caps.efc_cap;
     s->L.con_cap = caps.con_cap;
     s->fields = mjh::layout_fields(m, s->L);

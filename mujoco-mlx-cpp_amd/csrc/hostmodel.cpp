@@ -189,6 +189,15 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
     }
     nchain[k] = n;
   }
+  // qM storage: dense up to COMPACT_M_NV dofs, else compact rows
+  std::vector<int> madr(std::max(nv, 1), 0);
+  int nM = nv * nv;
+  if (nv > COMPACT_M_NV) {
+    nM = 0;
+    for (int k = 0; k < nv; k++) { madr[k] = nM; nM += nchain[k]; }
+  } else {
+    for (int k = 0; k < nv; k++) madr[k] = k * nv;
+  }
   for (int k = 0; k < nv; k++) dsend[k] = k + 1;
   for (int k = nv - 1; k >= 0; k--) { int p = m.dof_parentid[k]; if (p >= 0) dsend[p] = std::max(dsend[p], dsend[k]); }
   for (int k = 0; k < nv; k++)
@@ -315,6 +324,7 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   Builder B;
   DModel& d = blob.proto;
   std::memset(&d, 0, sizeof(d));
+  d.nM = nM;
   d.nq = m.nq; d.nv = nv; d.nu = m.nu; d.nbody = nb; d.njnt = m.njnt; d.ngeom = m.ngeom; d.npair = np;
   d.nlevel = maxd; d.ndoflevel = ndl; d.nflev = (int)flev_adr.size() - 1; d.maxdepth = maxdd; d.nfloss = (int)floss_dof.size(); d.nlim = (int)lim_jnt.size();
   d.iterations = m.iterations; d.ls_iterations = m.ls_iterations; d.solver = m.solver; d.cone = m.cone;
@@ -344,7 +354,7 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   ADDP(dof_bodyid, m.dof_bodyid); ADDP(dof_jntid, m.dof_jntid); ADDP(dof_parentid, m.dof_parentid);
   ADDP(dof_depth, ddepth); ADDP(dof_subtree_end, dsend); ADDP(doflevel_adr, dl_adr); ADDP(doflevel_dof, dl_dof);
   ADDP(dof_chain, chain); ADDP(dof_nchain, nchain); ADDP(dof_act_adr, dof_act_adr); ADDP(dof_act_num, dof_act_num);
-  ADDP(dof_act_list, dof_act_list);
+  ADDP(dof_act_list, dof_act_list); ADDP(dof_madr, madr);
   ADDP(flev_adr, flev_adr); ADDP(flev_i, flev_i); ADDP(flev_j, flev_j); ADDP(flev_kadr, flev_kadr);
   ADDP(flev_kcnt, flev_kcnt); ADDP(flev_k, flev_k);
   ADDP(dof_armature, tof(m.dof_armature)); ADDP(dof_damping, tof(m.dof_damping)); ADDP(dof_frictionloss, tof(m.dof_frictionloss));
@@ -372,8 +382,8 @@ std::vector<LayoutField> layout_fields(const Model& m, const mjk::Layout& L) {
   int nb = m.nbody, nv = m.nv, nj = m.njnt, ng = m.ngeom;
   std::vector<LayoutField> f = {
       {"qpos", L.qpos, m.nq, 0}, {"qvel", L.qvel, nv, 0}, {"ctrl", L.ctrl, m.nu, 0},
-      {"qacc_warmstart", L.qacc_ws, nv, 0}, {"qacc", L.qacc, nv, 0}, {"qM", L.qM, nv * nv, 0},
-      {"qLD", L.qLD, nv * nv, 0}, {"qfrc_smooth", L.qfrc_smooth, nv, 0}, {"qacc_smooth", L.qacc_smooth, nv, 0},
+      {"qacc_warmstart", L.qacc_ws, nv, 0}, {"qacc", L.qacc, nv, 0}, {"qM", L.qM, L.nM, 0},
+      {"qLD", L.qLD, L.nM, 0}, {"qfrc_smooth", L.qfrc_smooth, nv, 0}, {"qacc_smooth", L.qacc_smooth, nv, 0},
       {"qfrc_constraint", L.qfrc_constraint, nv, 0}, {"actuator_force", L.act_force, m.nu, 0},
       {"misc", L.misc, mjk::MISC_SIZE, 1},
       {"xpos", L.xpos, 3 * nb, 0}, {"xquat", L.xquat, 4 * nb, 0}, {"xmat", L.xmat, 9 * nb, 0},

@@ -562,6 +562,20 @@ KDEV void st_com_pos(const DModel& m, const Layout& L, float* w) {
   KSYNC();
 }
 
+// qM / qLD storage: compact rows (DModel::nM entries) beyond COMPACT_M_NV
+// dofs, else dense nv x nv
+KDEV bool m_compact(const DModel& m) { return MJK_BIG && m.nv > COMPACT_M_NV; }
+// index of entry (i, j), j == i or a dof ancestor of i (any j when dense)
+KDEV int mxi(const DModel& m, int i, int j) {
+  return m_compact(m) ? m.dof_madr[i] + m.dof_depth[i] - m.dof_depth[j] : i * m.nv + j;
+}
+
+// entry (i, j), j <= i, of qM in either storage (zero off the dof tree)
+KDEV float m_entry(const DModel& m, const float* M, int i, int j) {
+  if (!m_compact(m)) return M[i * m.nv + j];
+  return i < m.dof_subtree_end[j] ? M[mxi(m, i, j)] : 0.f;  // j is i or an ancestor
+}
+
 // mj_crb: composite inertia by subtree ranges, then qM entries along each dof's
 // ancestor chain (reference forward.hpp:240-279).
 KDEV void st_crb(const DModel& m, const Layout& L, float* w) {
@@ -576,7 +590,9 @@ KDEV void st_crb(const DModel& m, const Layout& L, float* w) {
       for (int k = b; k < m.body_subtree_end[b]; k++) s += cinert[10 * k + q];
     crb[t] = s;
   }
-  for (int t = LANE; t < nv * nv; t += WAVE) M[t] = 0.f;
+  bool compact = m_compact(m);
+  if (!compact)  // compact rows hold no zeros: every entry is written below
+    for (int t = LANE; t < nv * nv; t += WAVE) M[t] = 0.f;
   KSYNC();
   for (int i = LANE; i < nv; i += WAVE) mul_inert(crb + 10 * m.dof_bodyid[i], cdof + 6 * i, buf + 6 * i);
   KSYNC();
@@ -588,10 +604,14 @@ KDEV void st_crb(const DModel& m, const Layout& L, float* w) {
       int j = ch[a];
       const float* cj = cdof + 6 * j;
       float s = cj[0] * bi[0] + cj[1] * bi[1] + cj[2] * bi[2] + cj[3] * bi[3] + cj[4] * bi[4] + cj[5] * bi[5];
-      M[i * nv + j] = s;
-      M[j * nv + i] = s;
+      if (compact) {
+        M[m.dof_madr[i] + a] = s;  // the chain's a-th entry is the row's a-th
+      } else {
+        M[i * nv + j] = s;
+        M[j * nv + i] = s;
+      }
     }
-    M[i * nv + i] += m.dof_armature[i];
+    M[mxi(m, i, i)] += m.dof_armature[i];
   }
   KSYNC();
 }
@@ -610,10 +630,9 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
   // beyond 64 dofs only the tree's entries are copied and scaled (row k:
   // its diagonal and its ancestors), all its readers touch; up to 64 the
   // dense matrix, whose zeros the register solve reads
-  bool sparse = MJK_BIG && nv > 64;
+  bool sparse = m_compact(m);  // compact rows: every stored entry is a tree entry
   if (sparse) {
-    for (int k = LANE; k < nv; k += WAVE)
-      for (int i = k; i >= 0; i = i == k ? m.dof_parentid[k] : m.dof_parentid[i]) LD[k * nv + i] = A[k * nv + i];
+    for (int t = LANE; t < m.nM; t += WAVE) LD[t] = A[t];
   } else {
     for (int t = LANE; t < nv * nv; t += WAVE) LD[t] = A[t];
   }
@@ -694,19 +713,20 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
       float acc = 0.f;
       for (int q = m.flev_kadr[e]; q < m.flev_kadr[e] + m.flev_kcnt[e]; q++) {
         int k = m.flev_k[q];
-        float dk = LD[k * nv + k];
+        float dk = LD[mxi(m, k, k)];
         if (!(dk > KMINVAL)) { bad = 1; dk = KMINVAL; }
-        acc += LD[k * nv + i] * LD[k * nv + j] / dk;
+        acc += LD[mxi(m, k, i)] * LD[mxi(m, k, j)] / dk;
       }
-      LD[i * nv + j] -= acc;
+      LD[mxi(m, i, j)] -= acc;
     }
     KSYNC();
   }
   if (sparse) {
     for (int k = LANE; k < nv; k += WAVE) {
-      float dk = LD[k * nv + k];
+      float* row = LD + m.dof_madr[k];  // diagonal, then the ancestors
+      float dk = row[0];
       if (!(dk > KMINVAL)) { bad = 1; dk = KMINVAL; }
-      for (int i = m.dof_parentid[k]; i >= 0; i = m.dof_parentid[i]) LD[k * nv + i] /= dk;
+      for (int a = 1, n = m.dof_nchain[k]; a < n; a++) row[a] /= dk;
     }
   } else {
     for (int t = LANE; t < nv * nv; t += WAVE) {
@@ -763,18 +783,18 @@ KDEV void solve_ld(const DModel& m, const float* LD, float* x) {
     for (int t = m.doflevel_adr[lv] + LANE; t < m.doflevel_adr[lv + 1]; t += WAVE) {
       int i = m.doflevel_dof[t];
       float acc = 0.f;
-      for (int k = i + 1; k < m.dof_subtree_end[i]; k++) acc += LD[k * nv + i] * x[k];
+      for (int k = i + 1; k < m.dof_subtree_end[i]; k++) acc += LD[mxi(m, k, i)] * x[k];
       x[i] -= acc;
     }
     KSYNC();
   }
-  for (int i = LANE; i < nv; i += WAVE) x[i] /= LD[i * nv + i];
+  for (int i = LANE; i < nv; i += WAVE) x[i] /= LD[mxi(m, i, i)];
   KSYNC();
   for (int lv = 1; lv < m.ndoflevel; lv++) {
     for (int t = m.doflevel_adr[lv] + LANE; t < m.doflevel_adr[lv + 1]; t += WAVE) {
       int k = m.doflevel_dof[t];
       float acc = 0.f;
-      for (int i = m.dof_parentid[k]; i >= 0; i = m.dof_parentid[i]) acc += LD[k * nv + i] * x[i];
+      for (int i = m.dof_parentid[k]; i >= 0; i = m.dof_parentid[i]) acc += LD[mxi(m, k, i)] * x[i];
       x[k] -= acc;
     }
     KSYNC();
@@ -1983,11 +2003,12 @@ KDEV int st_constraints(const DModel& m, const Layout& L, float* w) {
 // the diagonal and the descendants (a contiguous dof range)
 KDEV float mulM_row(const DModel& m, const float* M, const float* x, int i) {
   int nv = m.nv;
-  const float* Mi = M + i * nv;
-  if (!MJK_BIG || nv <= 64) return kdot(Mi, 1, x, 1, nv);
+  if (!m_compact(m)) return kdot(M + i * nv, 1, x, 1, nv);
   float acc = 0.f;
-  for (int j = m.dof_parentid[i]; j >= 0; j = m.dof_parentid[j]) acc += Mi[j] * x[j];
-  for (int k = i; k < m.dof_subtree_end[i]; k++) acc += Mi[k] * x[k];
+  const float* row = M + m.dof_madr[i];  // the diagonal, then the ancestors
+  int a = 1;
+  for (int j = m.dof_parentid[i]; j >= 0; j = m.dof_parentid[j], a++) acc += row[a] * x[j];
+  for (int k = i; k < m.dof_subtree_end[i]; k++) acc += M[mxi(m, k, i)] * x[k];
   return acc;
 }
 
@@ -2395,12 +2416,20 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     const int jw = L.jw;
     const int* ix = iptr(w, L.efc_Jidx);
     const int* etype = iptr(w, L.efc_type);
+    bool compact = m_compact(m);
     for (int t = LANE; t < ntri; t += WAVE) {
       int i, j;
       tri_rc(t, &i, &j);
-      H[hx(i, j)] = (double)M[i * nv + j];
+      H[hx(i, j)] = compact ? 0.0 : (double)M[i * nv + j];
     }
     KSYNC();
+    if (compact) {  // the tree entries of the compact rows
+      for (int i = LANE; i < nv; i += WAVE) {
+        const MJK_G int* ch = m.dof_chain + i * MAX_DOF_DEPTH;
+        for (int a = 0, n = m.dof_nchain[i]; a < n; a++) H[hx(i, ch[a])] = (double)M[m.dof_madr[i] + a];
+      }
+      KSYNC();
+    }
     const int* misc = iptr(w, L.misc);
     const float* con = w + L.con;
     int ncon = misc[MISC_NCON], npr = jw * (jw + 1) / 2;
@@ -2434,7 +2463,7 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     for (int t = LANE; t < ntri; t += WAVE) {
       int i, j;
       tri_rc(t, &i, &j);
-      H[hx(i, j)] = (double)(M[i * nv + j] + kdot3(Dact, J + i, nv, J + j, nv, nefc));
+      H[hx(i, j)] = (double)(m_entry(m, M, i, j) + kdot3(Dact, J + i, nv, J + j, nv, nefc));
     }
     KSYNC();
   }
@@ -3063,7 +3092,7 @@ KDEV int st_integrate(const DModel& m, const Layout& L, float* w) {
   if (implicit || damp) {
     float* A = w + L.qLD;  // scratch: qLD of M is dead after the solver
     float* Bm = w + L.qM;   // M + h*D built in place of qM (dead after the solver)
-    for (int k = LANE; k < nv; k += WAVE) Bm[k * nv + k] += h * m.dof_damping[k];
+    for (int k = LANE; k < nv; k += WAVE) Bm[mxi(m, k, k)] += h * m.dof_damping[k];
     KSYNC();
     if (implicit) {
       const float* af = w + L.act_force;
@@ -3076,7 +3105,7 @@ KDEV int st_integrate(const DModel& m, const Layout& L, float* w) {
           if (m.act_forcelimited[u] && (f <= m.act_forcerange[2 * u] || f >= m.act_forcerange[2 * u + 1])) continue;
           add -= h * m.act_gear[u] * m.act_gear[u] * m.act_bias[3 * u + 2];
         }
-        Bm[k * nv + k] += add;
+        Bm[mxi(m, k, k)] += add;
       }
       KSYNC();
     }
