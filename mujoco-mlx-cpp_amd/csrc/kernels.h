@@ -711,13 +711,24 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
     for (int e = m.flev_adr[l] + LANE; e < m.flev_adr[l + 1]; e += WAVE) {
       int i = m.flev_i[e], j = m.flev_j[e];
       float acc = 0.f;
-      for (int q = m.flev_kadr[e]; q < m.flev_kadr[e] + m.flev_kcnt[e]; q++) {
-        int k = m.flev_k[q];
-        float dk = LD[mxi(m, k, k)];
-        if (!(dk > KMINVAL)) { bad = 1; dk = KMINVAL; }
-        acc += LD[mxi(m, k, i)] * LD[mxi(m, k, j)] / dk;
+      if (sparse) {  // row k: its diagonal, then its ancestors by decreasing depth
+        for (int q = m.flev_kadr[e]; q < m.flev_kadr[e] + m.flev_kcnt[e]; q++) {
+          int k = m.flev_k[q], d = m.dof_depth[k];
+          const float* rk = LD + m.dof_madr[k];
+          float dk = rk[0];
+          if (!(dk > KMINVAL)) { bad = 1; dk = KMINVAL; }
+          acc += rk[d - m.dof_depth[i]] * rk[d - m.dof_depth[j]] / dk;
+        }
+        LD[mxi(m, i, j)] -= acc;
+      } else {
+        for (int q = m.flev_kadr[e]; q < m.flev_kadr[e] + m.flev_kcnt[e]; q++) {
+          int k = m.flev_k[q];
+          float dk = LD[k * nv + k];
+          if (!(dk > KMINVAL)) { bad = 1; dk = KMINVAL; }
+          acc += LD[k * nv + i] * LD[k * nv + j] / dk;
+        }
+        LD[i * nv + j] -= acc;
       }
-      LD[mxi(m, i, j)] -= acc;
     }
     KSYNC();
   }
