@@ -2061,6 +2061,39 @@ KDEV float jrow_dot(const DModel& m, const Layout& L, const float* w, int r, con
   return acc;
 }
 
+// y[r] = J_r x - sub[r] (sub may be null) for every row. Compressed rows
+// (in device memory beyond 144 KB workspaces) go two per lane at a time with
+// every entry's dof index and value loaded before the gathers of x: the
+// loads of a row do not wait for each other (a row's entries past its end,
+// index -1, add 0 exactly).
+KDEV void jrows_dot(const DModel& m, const Layout& L, const float* w, int nefc, const float* x, float* y,
+                    const float* sub) {
+  if (MJK_BIG && L.jw > 0) {
+    const int jw = L.jw;
+    const float* J = w + L.efc_J;
+    const int* ix = iptr(w, L.efc_Jidx);
+    for (int r = LANE; r < nefc; r += 2 * WAVE) {
+      int r2 = r + WAVE < nefc ? r + WAVE : r;
+      const float *Ja = J + r * jw, *Jb = J + r2 * jw;
+      const int *ia = ix + r * jw, *ib = ix + r2 * jw;
+      float a = 0.f, b = 0.f;
+#pragma unroll 4
+      for (int q = 0; q < jw; q++) {
+        int ka = ia[q], kb = ib[q];
+        a = fmaf(ka >= 0 ? Ja[q] : 0.f, x[ka >= 0 ? ka : 0], a);
+        b = fmaf(kb >= 0 ? Jb[q] : 0.f, x[kb >= 0 ? kb : 0], b);
+      }
+      y[r] = sub ? a - sub[r] : a;
+      if (r + WAVE < nefc) y[r + WAVE] = sub ? b - sub[r + WAVE] : b;
+    }
+    return;
+  }
+  for (int r = LANE; r < nefc; r += WAVE) {
+    float a = jrow_dot(m, L, w, r, x);
+    y[r] = sub ? a - sub[r] : a;
+  }
+}
+
 // Per-body contact lists for mul_jt_sparse, built once per substep after the
 // rows: body b's entries are 2 c + side (side 0: b is the contact's body2,
 // 1: its body1) in contact order, CSR in bcon_adr / bcon.
@@ -2233,7 +2266,7 @@ KDEV void set_qacc(const DModel& m, const Layout& L, float* w, const float* x, i
   float *qacc = w + L.qacc, *Ma = w + L.Ma, *Jaref = w + L.efc_Jaref;
   const float *aref = w + L.efc_aref, *M = w + L.qM;
   for (int i = LANE; i < nv; i += WAVE) Ma[i] = Mx ? Mx[i] : mulM_row(m, M, x, i);
-  for (int r = LANE; r < nefc; r += WAVE) Jaref[r] = jrow_dot(m, L, w, r, x) - aref[r];
+  jrows_dot(m, L, w, nefc, x, Jaref, aref);
   KSYNC();
   for (int i = LANE; i < nv; i += WAVE) qacc[i] = x[i];
   KSYNC();
@@ -2900,7 +2933,7 @@ KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int
   float scale = 1.f / (m.meaninertia * (float)(nv > 1 ? nv : 1));
   float gtol = m.tolerance * m.ls_tolerance * snorm / scale;
   mulM(m, M, s, Mv);
-  for (int r = LANE; r < nefc; r += WAVE) Jv[r] = jrow_dot(m, L, w, r, s);
+  jrows_dot(m, L, w, nefc, s, Jv, nullptr);
   KSYNC();
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   for (int i = LANE; i < nv; i += WAVE) {
