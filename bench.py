@@ -209,6 +209,7 @@ def run_boxes(B, warmup, steps):
                                "dt 0.005 x 4 substeps; no row budget (up to 512 contacts per env, all with rows)",
                    "num_envs": B, "substeps_per_env_step": 4, "groups": len(groups),
                    "parallelism": "single GPU, 1 wavefront per env, one sim (stream) per N",
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                    "workspace": "LDS or device memory per group, picked by the autotuner (mjmlx_batched_autotune)"},
         # the groups' launches overlap on their streams: the control step's
         # time is the wall time per step of the queued groups
@@ -488,7 +489,13 @@ def main():
     ap.add_argument("--nbox", type=int, default=16, help="boxes per env for --stages --scene boxes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 3 x 256 MiB device copy (HBM counter calibration)")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (HIP's default is 4): the boxes workload's 29 "
+                         "sims step on 29 streams, which HIP maps onto this many hardware queues")
     args = ap.parse_args()
+    # read by the HIP runtime when it starts, which is after this point (the
+    # engine library starts it at its first call)
+    os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     if args.copy_calibration:
         copy_calibration()
         return
