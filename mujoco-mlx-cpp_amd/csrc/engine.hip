@@ -69,6 +69,32 @@ __global__ void k_reset(const int* mask, float* qpos, float* qvel, float* qws, c
   for (int i = 0; i < nv; i++) { qvel[(size_t)e * nv + i] = 0.f; qws[(size_t)e * nv + i] = 0.f; }
 }
 
+// Launch order of the step's workgroups: envs by decreasing solver iterations
+// of the previous call (a counting sort on min(niter, ORDER_KEYS - 1), one
+// workgroup), so the envs that ran longest start first and the launch ends
+// on short ones. Envs of equal count keep no particular order: each env's
+// result does not depend on when it runs.
+constexpr int ORDER_KEYS = 128;
+__global__ void __launch_bounds__(1024) k_order(const int* __restrict__ niter, int* __restrict__ order, int B) {
+  __shared__ int start[ORDER_KEYS];
+  int t = (int)threadIdx.x;
+  if (t < ORDER_KEYS) start[t] = 0;
+  __syncthreads();
+  auto key = [&](int e) {  // bucket 0 holds the longest envs
+    int k = niter[e];
+    k = k < 0 ? 0 : (k > ORDER_KEYS - 1 ? ORDER_KEYS - 1 : k);
+    return ORDER_KEYS - 1 - k;
+  };
+  for (int e = t; e < B; e += (int)blockDim.x) atomicAdd(&start[key(e)], 1);
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int k = 0; k < ORDER_KEYS; k++) { int c = start[k]; start[k] = acc; acc += c; }
+  }
+  __syncthreads();
+  for (int e = t; e < B; e += (int)blockDim.x) order[atomicAdd(&start[key(e)], 1)] = e;
+}
+
 // packed readback: [qpos, qvel, |cfrc_ext force| of chosen bodies, sum sq and sum abs of qfrc_actuator range]
 // (reference src/batched.cpp:1683-1708)
 __global__ void k_pack(const float* qpos, const float* qvel, const float* cfrc, const float* qfa, const int* bodies,
@@ -110,6 +136,13 @@ struct MjmlxBatchedSim {
   char* d_blob = nullptr;
   float *d_qpos = nullptr, *d_qvel = nullptr, *d_qws = nullptr, *d_ctrl = nullptr, *d_qpos0 = nullptr;
   int *d_niter = nullptr, *d_status = nullptr, *d_ncon = nullptr, *d_nefc = nullptr, *d_queue = nullptr, *d_mask = nullptr;
+  int* d_order = nullptr;  // workgroup slot -> env of the fused step launch (k_order)
+  int* d_ident = nullptr;  // the identity order of every other launch
+  // longest envs first (k_order), for batches of at least 16 envs per CU:
+  // smaller ones (the boxes workload's groups, ~565 envs on concurrent
+  // streams) lose more to the sort's launch than the shorter tail saves.
+  // MJMLX_HIP_ORDER=0/1 forces it off / on.
+  int order_mode = -1;
   float* d_aux[7] = {};  // xpos xquat subtree_com cinert cvel qfrc_actuator cfrc_ext
   float* h_aux[7] = {};
   bool aux_on = false;
@@ -180,7 +213,7 @@ void free_sim(MjmlxBatchedSim* s) {
   void* dev[] = {s->d_blob, s->d_qpos, s->d_qvel, s->d_qws, s->d_ctrl, s->d_qpos0, s->d_niter, s->d_status,
                  s->d_ncon, s->d_nefc, s->d_queue, s->d_mask, s->d_pack_bodies, s->d_pack, s->d_dump1, s->d_dump2,
                  s->d_feat[0], s->d_feat[1], s->d_pred, s->d_gws, s->d_rand, s->d_rays, s->d_rdist,
-                 s->d_rgid, s->d_gmask, s->d_sensordata, s->d_sens_prev, s->d_kp};
+                 s->d_rgid, s->d_gmask, s->d_sensordata, s->d_sens_prev, s->d_kp, s->d_order, s->d_ident};
   for (void* p : dev) if (p) hipFree(p);
   for (int i = 0; i < 7; i++) { if (s->d_aux[i]) hipFree(s->d_aux[i]); if (s->h_aux[i]) hipHostFree(s->h_aux[i]); }
   if (s->h_qpos) hipHostFree(s->h_qpos);
@@ -233,6 +266,7 @@ mjk::StepIO make_io(MjmlxBatchedSim* s, float* dump1, float* dump2) {
   io.aux.cfrc_ext = s->d_aux[6];
   io.rand = s->d_rand; io.nrand = s->nrand;
   io.sensordata = s->d_sensordata; io.sens_prev = s->d_sens_prev;
+  io.order = s->d_ident;
   return io;
 }
 
@@ -255,6 +289,11 @@ void upload_ctrl(MjmlxBatchedSim* s, const float* ctrl) {
 void launch(MjmlxBatchedSim* s, int nsub, float* dump1, float* dump2) {
   mjk::StepIO io = make_io(s, dump1, dump2);
   HIPOK(hipEventRecord(s->ev0, s->stream));
+  bool order = s->order_mode >= 0 ? s->order_mode == 1 : s->B >= 16 * std::max(1, s->ncu);
+  if (order && nsub > 0 && s->B > 1) {
+    hipLaunchKernelGGL(k_order, dim3(1), dim3(1024), 0, s->stream, s->d_niter, s->d_order, s->B);
+    io.order = s->d_order;
+  }
   launch_step(s, io, nsub);
   HIPOK(hipGetLastError());
   HIPOK(hipEventRecord(s->ev1, s->stream));
@@ -654,6 +693,14 @@ MJMLX_API MjmlxBatchedSim* mjmlx_batched_create(const MjmlxModel* model, const M
     HIPOK(hipMalloc(&s->d_ncon, B * sizeof(int)));
     HIPOK(hipMalloc(&s->d_nefc, B * sizeof(int)));
     HIPOK(hipMalloc(&s->d_queue, sizeof(int)));
+    HIPOK(hipMalloc(&s->d_order, B * sizeof(int)));
+    HIPOK(hipMalloc(&s->d_ident, B * sizeof(int)));
+    {
+      std::vector<int> iota(B);
+      for (size_t e = 0; e < B; e++) iota[e] = (int)e;
+      HIPOK(hipMemcpy(s->d_ident, iota.data(), B * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (const char* v = std::getenv("MJMLX_HIP_ORDER")) s->order_mode = std::atoi(v) != 0;
     HIPOK(hipMalloc(&s->d_mask, B * sizeof(int)));
     HIPOK(hipHostMalloc(&s->h_qpos, B * std::max(1, m.nq) * sizeof(float), hipHostMallocDefault));
     HIPOK(hipHostMalloc(&s->h_qvel, B * std::max(1, m.nv) * sizeof(float), hipHostMallocDefault));

@@ -3333,6 +3333,7 @@ struct StepIO {
   int nrand;
   float* sensordata;         // sensors: [env * nsensordata] of the last substep's start state, or null
   float* sens_prev;          // accelerometers' previous frame velocities [env * (3 * nsens_acc + 1)], last: valid
+  const int* order;          // env of each workgroup slot (a permutation of the batch; the identity by default)
 };
 
 // Sensors of the substep's start state (reference src/sensor.cpp:265-359),

@@ -12,6 +12,10 @@
 #error "step_np.hip is compiled with -DMJK_BIG=0 (models of at most SPARSE_ROWS_NV = 32 dofs) and -DMJK_BIG=1 (any size)"
 #endif
 
+// env stepped by workgroup slot `slot` (the launch's order: the identity
+// unless the envs are sorted by their previous work)
+__device__ __forceinline__ int slot_env(const mjk::StepIO& io, int slot) { return io.order[slot]; }
+
 #define MJK_CAT2(a, b) a##b
 #define MJK_CAT(a, b) MJK_CAT2(a, b)
 #if defined(MJK_HALF) && MJK_HALF
@@ -35,7 +39,7 @@ k_step(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0, int
   const mjk::KParams& P = mjk::kparams(kp);
   float* w = GWS ? gws + (size_t)blockIdx.x * P.L.total : lds;
   mjk::LsCache lc{GWS ? reinterpret_cast<mjk::LsRow4*>(lds) : nullptr, GWS ? ls_cap : 0};
-  mjk::run_env<NPSET>(P.m, P.L, w, io, env0 + (int)blockIdx.x, nsub, lc);
+  mjk::run_env<NPSET>(P.m, P.L, w, io, slot_env(io, env0 + (int)blockIdx.x), nsub, lc);
 }
 
 // The same step capped at 168 VGPRs (three waves per SIMD, a few hundred
@@ -50,7 +54,7 @@ k_step_occ(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0,
   const mjk::KParams& P = mjk::kparams(kp);
   float* w = GWS ? gws + (size_t)blockIdx.x * P.L.total : lds;
   mjk::LsCache lc{GWS ? reinterpret_cast<mjk::LsRow4*>(lds) : nullptr, GWS ? ls_cap : 0};
-  mjk::run_env<NPSET | 8>(P.m, P.L, w, io, env0 + (int)blockIdx.x, nsub, lc);
+  mjk::run_env<NPSET | 8>(P.m, P.L, w, io, slot_env(io, env0 + (int)blockIdx.x), nsub, lc);
 }
 
 // The step with the whole register file of a SIMD lane (one wave per SIMD),
@@ -61,7 +65,7 @@ k_step_big(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0,
   extern __shared__ __align__(16) float lds[];
   if ((int)blockIdx.x >= B) return;
   const mjk::KParams& P = mjk::kparams(kp);
-  mjk::run_env<NPSET>(P.m, P.L, lds, io, env0 + (int)blockIdx.x, nsub);
+  mjk::run_env<NPSET>(P.m, P.L, lds, io, slot_env(io, env0 + (int)blockIdx.x), nsub);
 }
 
 #if defined(MJK_HALF) && MJK_HALF
@@ -76,7 +80,7 @@ k_step_half(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int env0
   int h = (int)(threadIdx.x >> 5), e = 2 * (int)blockIdx.x + h;
   if (e >= B) return;
   const mjk::KParams& P = mjk::kparams(kp);
-  mjk::run_env<NPSET>(P.m, P.L, lds + h * P.L.total, io, env0 + e, nsub);
+  mjk::run_env<NPSET>(P.m, P.L, lds + h * P.L.total, io, slot_env(io, env0 + e), nsub);
 }
 
 template <int NPSET>
@@ -86,7 +90,7 @@ k_step_half_occ(const mjk::KParams* __restrict__ kp, mjk::StepIO io, int B, int 
   int h = (int)(threadIdx.x >> 5), e = 2 * (int)blockIdx.x + h;
   if (e >= B) return;
   const mjk::KParams& P = mjk::kparams(kp);
-  mjk::run_env<NPSET | 8>(P.m, P.L, lds + h * P.L.total, io, env0 + e, nsub);
+  mjk::run_env<NPSET | 8>(P.m, P.L, lds + h * P.L.total, io, slot_env(io, env0 + e), nsub);
 }
 
 namespace mjk {
