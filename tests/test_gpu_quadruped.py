@@ -309,6 +309,39 @@ def test_duplicate_envs_bitwise(model):
         assert np.array_equal(vf[:8], vf[8 * k: 8 * k + 8])
 
 
+@pytest.mark.parametrize("scene", ["quadruped", "humanoid"])
+def test_launch_order_keeps_results(model, monkeypatch, scene):
+    """The fused launch stepping envs longest first (MJMLX_HIP_ORDER=1, read
+    at creation) against the batch order (=0): bitwise the same states,
+    iteration counts and contact counts after every control step of a
+    rollout whose per-env iteration counts differ, so the sort permutes."""
+    if scene == "humanoid":
+        model = capi.Model(scenes.humanoid_xml())
+    B = 1000
+    rng = np.random.default_rng(23)
+    q = np.tile(model.field("qpos0"), (B, 1))
+    q[:, 2] += rng.uniform(0.0, 0.3, B)
+    qv = rng.normal(0, 0.5, (B, model.nv))
+    cs = rng.uniform(-0.5, 0.5, (12, B, model.nu)).astype(np.float32)
+    sims = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MJMLX_HIP_ORDER", mode)
+        sim = capi.BatchedSim(model, B)
+        sim.set_state(q.astype(np.float32), qv.astype(np.float32))
+        sims.append(sim)
+    seen = set()
+    for k in range(12):
+        out = []
+        for sim in sims:
+            sim.step(cs[k], 4)
+            st = sim.stats()
+            out.append((*sim.state(), st["niter"].copy(), st["ncon"].copy()))
+        for a, b in zip(out[0], out[1]):
+            assert np.array_equal(a, b), k
+        seen.update(np.unique(out[0][2]).tolist())
+    assert len(seen) > 3, seen  # iteration counts that differ between envs
+
+
 def test_cfrc_ext_from_contact_forces(model):
     """cfrc_ext after a one-substep step against the FP64 oracle's contacts and
     row forces at the same state: each contact's normal force (sum of its
