@@ -42,6 +42,10 @@ enum { SPARSE_ROWS_NV = 32 };
 // nearest first, at dof_madr[k]; entry (k, j) of an ancestor j sits at
 // dof_madr[k] + dof_depth[k] - dof_depth[j]. Up to it both are dense nv x nv.
 enum { COMPACT_M_NV = 64 };
+// compact qM whose dof trees are all paths (each dof's parent the dof
+// before it) of at most this many dofs: one lane solves and multiplies a
+// whole tree in registers (free bodies: 6 dofs each)
+enum { PATH_TREE_MAX = 8 };
 
 // Device passes see the model's arrays as global memory (address space 1), so
 // loads through them stay global loads wherever the struct itself is read
@@ -64,6 +68,7 @@ struct DModel {
   int nsensor, nsensordata, nsens_acc;  // sensors, sensordata width, accelerometers
   int jw;  // entries per compressed constraint row (0: dense rows of nv)
   int nM;  // stored entries of qM (and of its factor): nv * nv, or the compact rows' total
+  int ntree;  // dof trees when every tree is a path of at most PATH_TREE_MAX dofs, else 0
   float magnetic[3];
   // pointers only from here on: Blob::bind relocates every word after body_parentid
   // bodies
@@ -77,7 +82,8 @@ struct DModel {
   const MJK_G float *qpos0, *qpos_spring;
   // dofs
   const MJK_G int *dof_bodyid, *dof_jntid, *dof_parentid, *dof_depth, *dof_subtree_end, *doflevel_adr,
-      *doflevel_dof, *dof_chain, *dof_nchain, *dof_act_adr, *dof_act_num, *dof_act_list, *dof_madr;
+      *doflevel_dof, *dof_chain, *dof_nchain, *dof_act_adr, *dof_act_num, *dof_act_list, *dof_madr,
+      *tree_adr;  // first dof of each path tree [ntree + 1]
   // level-parallel LDL of qM: level l updates entries [flev_adr[l], flev_adr[l+1]),
   // entry e is (flev_i[e], flev_j[e]) and sums over dofs flev_k[flev_kadr[e] .. + flev_kcnt[e])
   const MJK_G int *flev_adr, *flev_i, *flev_j, *flev_kadr, *flev_kcnt, *flev_k;

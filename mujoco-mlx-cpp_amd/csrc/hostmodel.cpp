@@ -200,6 +200,23 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   }
   for (int k = 0; k < nv; k++) dsend[k] = k + 1;
   for (int k = nv - 1; k >= 0; k--) { int p = m.dof_parentid[k]; if (p >= 0) dsend[p] = std::max(dsend[p], dsend[k]); }
+  // dof trees (a root dof's subtree, a contiguous range) that are all paths of
+  // at most PATH_TREE_MAX dofs
+  std::vector<int> tree_adr;
+  {
+    bool paths = nv > 0;
+    for (int k = 0; k < nv && paths; k++) {
+      int p = m.dof_parentid[k];
+      if (p < 0) {
+        tree_adr.push_back(k);
+        paths = dsend[k] - k <= PATH_TREE_MAX;
+      } else {
+        paths = p == k - 1;
+      }
+    }
+    if (paths) tree_adr.push_back(nv);
+    else tree_adr.assign(1, 0);
+  }
   for (int k = 0; k < nv; k++)
     for (int q = k + 1; q < dsend[k]; q++) {
       bool desc = false;
@@ -325,6 +342,7 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   DModel& d = blob.proto;
   std::memset(&d, 0, sizeof(d));
   d.nM = nM;
+  d.ntree = (int)tree_adr.size() - 1;
   d.nq = m.nq; d.nv = nv; d.nu = m.nu; d.nbody = nb; d.njnt = m.njnt; d.ngeom = m.ngeom; d.npair = np;
   d.nlevel = maxd; d.ndoflevel = ndl; d.nflev = (int)flev_adr.size() - 1; d.maxdepth = maxdd; d.nfloss = (int)floss_dof.size(); d.nlim = (int)lim_jnt.size();
   d.iterations = m.iterations; d.ls_iterations = m.ls_iterations; d.solver = m.solver; d.cone = m.cone;
@@ -354,7 +372,7 @@ Blob build_blob(const Model& m, std::vector<CollisionPair>* pairs_out) {
   ADDP(dof_bodyid, m.dof_bodyid); ADDP(dof_jntid, m.dof_jntid); ADDP(dof_parentid, m.dof_parentid);
   ADDP(dof_depth, ddepth); ADDP(dof_subtree_end, dsend); ADDP(doflevel_adr, dl_adr); ADDP(doflevel_dof, dl_dof);
   ADDP(dof_chain, chain); ADDP(dof_nchain, nchain); ADDP(dof_act_adr, dof_act_adr); ADDP(dof_act_num, dof_act_num);
-  ADDP(dof_act_list, dof_act_list); ADDP(dof_madr, madr);
+  ADDP(dof_act_list, dof_act_list); ADDP(dof_madr, madr); ADDP(tree_adr, tree_adr);
   ADDP(flev_adr, flev_adr); ADDP(flev_i, flev_i); ADDP(flev_j, flev_j); ADDP(flev_kadr, flev_kadr);
   ADDP(flev_kcnt, flev_kcnt); ADDP(flev_k, flev_k);
   ADDP(dof_armature, tof(m.dof_armature)); ADDP(dof_damping, tof(m.dof_damping)); ADDP(dof_frictionloss, tof(m.dof_frictionloss));

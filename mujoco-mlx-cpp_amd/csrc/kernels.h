@@ -754,6 +754,77 @@ KDEV int factor_ld(const DModel& m, const float* A, float* LD) {
   return wori(bad) ? ST_M_NOT_PD : 0;
 }
 
+// Path trees (DModel::ntree > 0, compact rows): dof j of a tree starting at
+// dof d0 has the ancestors j - 1, .., 0 of the same tree, so its compact row
+// (diagonal, then the ancestors nearest first) is entry (j, i) at
+// j (j + 1) / 2 + j - i of the tree's block, which starts at dof_madr[d0].
+KDEV bool path_trees(const DModel& m) { return m_compact(m) && m.ntree > 0; }
+KDEV int ptx(int j, int i) { return j * (j + 1) / 2 + j - i; }
+
+// x <- (L^T D L)^-1 x on path trees: one lane per tree, the tree's entries of
+// x in registers. The same operations in the same order as the level-parallel
+// solve below (each dof's sum over its descendants, then over its ancestors
+// nearest first), so the results are the same bits.
+KDEV void solve_ld_trees(const DModel& m, const float* LD, float* x) {
+  for (int t = LANE; t < m.ntree; t += WAVE) {
+    int d0 = m.tree_adr[t], n = m.tree_adr[t + 1] - d0;
+    const float* T = LD + m.dof_madr[d0];
+    float v[PATH_TREE_MAX];
+#pragma unroll
+    for (int j = 0; j < PATH_TREE_MAX; j++) v[j] = j < n ? x[d0 + j] : 0.f;
+#pragma unroll
+    for (int i = PATH_TREE_MAX - 2; i >= 0; i--) {  // L^T y = x, deepest first
+      if (i + 1 < n) {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = i + 1; k < PATH_TREE_MAX; k++)
+          if (k < n) acc += T[ptx(k, i)] * v[k];
+        v[i] -= acc;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < PATH_TREE_MAX; j++)
+      if (j < n) v[j] /= T[ptx(j, j)];
+#pragma unroll
+    for (int k = 1; k < PATH_TREE_MAX; k++) {  // L z = y, from the root
+      if (k < n) {
+        float acc = 0.f;
+#pragma unroll
+        for (int i = k - 1; i >= 0; i--) acc += T[ptx(k, i)] * v[i];
+        v[k] -= acc;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < PATH_TREE_MAX; j++)
+      if (j < n) x[d0 + j] = v[j];
+  }
+  KSYNC();
+}
+
+// y = M x on path trees, one lane per tree; per dof the order of mulM_row
+// (ancestors nearest first, then the diagonal and the descendants)
+KDEV void mulM_trees(const DModel& m, const float* M, const float* x, float* y) {
+  for (int t = LANE; t < m.ntree; t += WAVE) {
+    int d0 = m.tree_adr[t], n = m.tree_adr[t + 1] - d0;
+    const float* T = M + m.dof_madr[d0];
+    float v[PATH_TREE_MAX];
+#pragma unroll
+    for (int j = 0; j < PATH_TREE_MAX; j++) v[j] = j < n ? x[d0 + j] : 0.f;
+#pragma unroll
+    for (int i = 0; i < PATH_TREE_MAX; i++) {
+      if (i < n) {
+        float acc = 0.f;
+#pragma unroll
+        for (int j = i - 1; j >= 0; j--) acc += T[ptx(i, j)] * v[j];
+#pragma unroll
+        for (int k = i; k < PATH_TREE_MAX; k++)
+          if (k < n) acc += T[ptx(k, i)] * v[k];
+        y[d0 + i] = acc;
+      }
+    }
+  }
+}
+
 // x <- (L^T D L)^-1 x with L unit lower (entries outside the dof tree are 0).
 // GPU, nv <= 64: lane i holds x_i; each column step broadcasts one solved entry
 // with readlane and updates the other lanes, with no LDS round trip per step.
@@ -790,6 +861,10 @@ KDEV void solve_ld(const DModel& m, const float* LD, float* x) {
     return;
   }
 #endif
+  if (path_trees(m)) {
+    solve_ld_trees(m, LD, x);
+    return;
+  }
   for (int lv = m.ndoflevel - 2; lv >= 0; lv--) {
     for (int t = m.doflevel_adr[lv] + LANE; t < m.doflevel_adr[lv + 1]; t += WAVE) {
       int i = m.doflevel_dof[t];
@@ -2024,6 +2099,10 @@ KDEV float mulM_row(const DModel& m, const float* M, const float* x, int i) {
 }
 
 KDEV void mulM(const DModel& m, const float* M, const float* x, float* y) {
+  if (path_trees(m)) {
+    mulM_trees(m, M, x, y);
+    return;
+  }
   for (int i = LANE; i < m.nv; i += WAVE) y[i] = mulM_row(m, M, x, i);
 }
 
@@ -2265,7 +2344,9 @@ KDEV void set_qacc(const DModel& m, const Layout& L, float* w, const float* x, i
   int nv = m.nv;
   float *qacc = w + L.qacc, *Ma = w + L.Ma, *Jaref = w + L.efc_Jaref;
   const float *aref = w + L.efc_aref, *M = w + L.qM;
-  for (int i = LANE; i < nv; i += WAVE) Ma[i] = Mx ? Mx[i] : mulM_row(m, M, x, i);
+  if (!Mx && path_trees(m)) mulM_trees(m, M, x, Ma);
+  else
+    for (int i = LANE; i < nv; i += WAVE) Ma[i] = Mx ? Mx[i] : mulM_row(m, M, x, i);
   jrows_dot(m, L, w, nefc, x, Jaref, aref);
   KSYNC();
   for (int i = LANE; i < nv; i += WAVE) qacc[i] = x[i];
