@@ -2432,7 +2432,11 @@ KDEV void grad_only(const DModel& m, const Layout& L, float* w, int nefc) {
 
 // with_grad false: forces, active flags and cost only (comparing warm-start
 // candidates), no gradient
-KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* nchanged, bool with_grad = true) {
+// chmask (when not null): the rows whose activity changed, bit r % 64 of word
+// r / 64, for pyramidal / frictionless / limit / friction-loss rows of at most
+// 128 rows on 64 lanes (or the serial interpreter); otherwise both words ~0
+KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* nchanged, bool with_grad = true,
+                     unsigned long long* chmask = nullptr) {
   int nv = m.nv;
   const float *Jaref = w + L.efc_Jaref, *D = w + L.efc_D;
   float *force = w + L.efc_force, *Dact = w + L.efc_Dact;
@@ -2440,7 +2444,9 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
   const int* eid = iptr(w, L.efc_id);
   float part = 0.f;
   int nch = 0;
-  for (int r = LANE; r < nefc; r += WAVE) {
+  unsigned long long cm[2] = {0ull, 0ull};
+  int q = 0;
+  for (int r = LANE; r < nefc; r += WAVE, q++) {
     float x = Jaref[r], d = D[r];
     int ty = etype[r];
     if (ty == EFC_CONTACT_ELLIPTIC) continue;  // per contact, below
@@ -2455,8 +2461,17 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
       if (x < 0.f) { act[r] = 1; force[r] = -d * x; part += 0.5f * d * x * x; }
       else { act[r] = 0; force[r] = 0.f; }
     }
-    nch += act[r] != old;
+    bool chg = act[r] != old;
+    nch += chg;
     Dact[r] = act[r] ? d : 0.f;
+#if WAVE == 64
+    unsigned long long b = __ballot(chg);
+    if (q < 2) cm[q] = b;
+#elif WAVE == 1
+    if (chg && r < 128) cm[r >> 6] |= 1ull << (r & 63);
+#else
+    (void)q;
+#endif
   }
   if (m.cone != 0) {
     const int* misc = reinterpret_cast<const int*>(w + L.misc);
@@ -2482,6 +2497,18 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
   for (int i = LANE; i < nv; i += WAVE) part += 0.5f * (Ma[i] - qfs[i]) * (qacc[i] - qas[i]);
   float cost = wsum(part);
   *nchanged = wsumi(nch);
+  if (chmask) {
+    bool known = (WAVE == 64 || WAVE == 1) && nefc <= 128 && m.cone == 0;
+#if WAVE == 64
+    // lane 0 takes part in every round that any lane does
+    for (int k = 0; k < 2; k++) {
+      unsigned lo = __builtin_amdgcn_readlane((unsigned)cm[k], 0), hi = __builtin_amdgcn_readlane((unsigned)(cm[k] >> 32), 0);
+      cm[k] = ((unsigned long long)hi << 32) | lo;
+    }
+#endif
+    chmask[0] = known ? cm[0] : ~0ull;
+    chmask[1] = known ? cm[1] : ~0ull;
+  }
   KSYNC();
   if (with_grad) grad_only(m, L, w, nefc);
   return cost;
@@ -2726,6 +2753,74 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
   }
   KSYNC();
   return wori(bad) ? ST_NOT_PD : 0;
+}
+
+// Rank-1 changes of the factored Hessian for rows whose activity changed
+// (H += D_r J_r^T J_r for a row that became active, -= for one that left):
+// the LDL^T update of Gill, Golub, Murray and Saunders (1974, method C1) on
+// newton_factor's storage (H_ij = L_ij d_j below the diagonal, H_jj = d_j,
+// dvec = 1 / d), one column step per dof with the row's J entries in lanes.
+// Returns false (the caller factors anew) when a pivot would lose more than
+// ten digits, which a downdate of M + J^T D J should not do.
+KDEV bool newton_update(const DModel& m, const Layout& L, float* w, const unsigned long long* chmask) {
+  int nv = m.nv;
+  double* H = dptr(w, L.H);
+  double* idg = dptr(w, L.dvec);
+  const float *J = w + L.efc_J, *D = w + L.efc_D;
+  const int* act = iptr(w, L.efc_act);
+  if (chmask[0] == ~0ull && chmask[1] == ~0ull) return false;  // rows not known
+  for (int q = 0; q < 2; q++) {
+    unsigned long long bits = chmask[q];
+    while (bits) {
+      int r = 64 * q + __builtin_ctzll(bits);
+      bits &= bits - 1;
+      double a = act[r] ? (double)D[r] : -(double)D[r];
+#if WAVE == 64
+      double z = LANE < nv ? (double)J[r * nv + LANE] : 0.0;
+      double d = LANE < nv ? H[hx(LANE, LANE)] : 1.0, id = LANE < nv ? idg[LANE] : 1.0;
+      for (int j = 0; j < nv; j++) {
+        double p = rl_d(z, j), dj = rl_d(d, j), idj = rl_d(id, j);
+        double dn = dj + a * p * p;
+        if (!(dn > 1e-10 * dj)) return false;
+        double idn = krcp(dn), beta = p * a * idn;
+        a *= dj * idn;
+        if (LANE == j) { d = dn; id = idn; }
+        if (LANE > j && LANE < nv) {
+          double* hp = H + hx(LANE, j);
+          double lij = *hp * idj;
+          z -= p * lij;
+          lij += beta * z;
+          *hp = lij * dn;
+        }
+      }
+      if (LANE < nv) { H[hx(LANE, LANE)] = d; idg[LANE] = id; }
+      KSYNC();
+#elif WAVE == 1
+      double* z = dptr(w, L.dvec) + nv;
+      for (int i = 0; i < nv; i++) z[i] = (double)J[r * nv + i];
+      for (int j = 0; j < nv; j++) {
+        double p = z[j], dj = H[hx(j, j)], idj = idg[j];
+        double dn = dj + a * p * p;
+        if (!(dn > 1e-10 * dj)) return false;
+        double idn = krcp(dn), beta = p * a * idn;
+        a *= dj * idn;
+        for (int i = j + 1; i < nv; i++) {
+          double* hp = H + hx(i, j);
+          double lij = *hp * idj;
+          z[i] -= p * lij;
+          lij += beta * z[i];
+          *hp = lij * dn;
+        }
+        H[hx(j, j)] = dn;
+        idg[j] = idn;
+      }
+#else
+      (void)a; (void)J; (void)H; (void)idg;
+      return false;
+#endif
+    }
+  }
+  return true;
 }
 
 // Mgrad = H^-1 grad with the factor of newton_factor, in FP64; on the GPU the
@@ -3155,7 +3250,8 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     KSYNC();
     float oldcost = cost;
     KPROF_MARK(pa, KP_SOLVE_REST);
-    cost = cost_grad(kfresh(m), kfresh(L), w, nefc, &nch);
+    unsigned long long chm[2];
+    cost = cost_grad(kfresh(m), kfresh(L), w, nefc, &nch, true, chm);
     KPROF_MARK(pa, KP_COSTGRAD);
     float gp = 0.f;
     for (int i = LANE; i < nv; i += WAVE) gp += grad[i] * grad[i];
@@ -3168,7 +3264,10 @@ KDEV int st_solve(const DModel& m, const Layout& L, float* w, int* niter_out, KP
     if (newton && exact) break;
     if (newton) {
       // H depends on qacc only through the active set: refactor only when it changed
-      if (nch) st |= newton_factor<TRI_CACHE>(kfresh(m), kfresh(L), w, nefc, pa);
+      // one or two rows changed activity: rank-1 updates of the factor
+      // (dense rows only: the compressed-row Hessian is factored anew)
+      if (nch && !(nch <= 2 && (!MJK_BIG || L.jw == 0) && newton_update(kfresh(m), kfresh(L), w, chm)))
+        st |= newton_factor<TRI_CACHE>(kfresh(m), kfresh(L), w, nefc, pa);
       KPROF_MARK(pa, KP_DIR);
       newton_solve(kfresh(m), kfresh(L), w);
       KPROF_MARK(pa, KP_NSOLVE);
