@@ -2514,6 +2514,63 @@ KDEV float cost_grad(const DModel& m, const Layout& L, float* w, int nefc, int* 
   return cost;
 }
 
+// The cone Hessian of the elliptic contacts in their cone's middle zone as
+// weighted virtual rows after the constraint rows (dense rows only), so that
+// J^T diag(Dact) J over all the rows includes it. With f_j the scaled
+// tangential frictions, T = |(f_j x_j)|, u_j = f_j x_j / T (a unit vector),
+// s = sum_j f_j u_j J_j and the contact's rows J_0 (normal) .. J_(dim-1):
+//   J_c^T Hc J_c = Dm mu^2 a a^T - Dm cg Y^T (I - u u^T) Y,  a = J_0 - s,
+// Y_j = f_j J_j, and I - u u^T = sum_k w_k w_k^T with w_k = e_k - u_k u, so
+// its rows are v_0 = J_0 - s (weight Dm mu^2) and v_k = f_k J_k - u_k s
+// (weight -Dm cg, positive in the middle zone, where g = N - mu T < 0).
+// Returns the row count, or -1 when the virtual rows do not fit in the row
+// buffer (the caller then adds the cone terms entry by entry).
+KDEV int cone_virtual_rows(const DModel& m, const Layout& L, float* w, int nefc) {
+  int nv = m.nv;
+  const int* misc = iptr(w, L.misc);
+  const int* act = iptr(w, L.efc_act);
+  const float *Jaref = w + L.efc_Jaref, *D = w + L.efc_D, *con = w + L.con;
+  float *J = w + L.efc_J, *Dact = w + L.efc_Dact;
+  int ncon = misc[MISC_NCON], nr = nefc;
+  for (int c = 0; c < ncon; c++) {
+    const float* cr = con + CON_STRIDE * c;
+    int dim = iptr(cr, CON_NROW)[0], r0 = iptr(cr, CON_ROW0)[0];
+    if (dim >= 2 && r0 + dim <= nefc && act[r0] == 2) nr += dim;
+  }
+  if (nr > L.maxefc) return -1;
+  nr = nefc;
+  float sc = fric_scale(m, L, w);
+  for (int c = 0; c < ncon; c++) {
+    const float* cr = con + CON_STRIDE * c;
+    int dim = iptr(cr, CON_NROW)[0], r0 = iptr(cr, CON_ROW0)[0];
+    if (dim < 2 || r0 + dim > nefc || act[r0] != 2) continue;
+    const float* f5 = m.pair_friction + 5 * iptr(cr, CON_PAIR)[0];
+    const float* x = Jaref + r0;
+    float mu = f5[0] * sc, T = 0.f;
+    for (int j = 1; j < dim; j++) { float U = (f5[j - 1] * sc) * x[j]; T += U * U; }
+    T = sqrtf(T);
+    float Dm = D[r0] / (mu * mu * (1.f + mu * mu)), g = mu * x[0] - mu * T, cg = g * mu / T;
+    const float* Jc = J + r0 * nv;
+    float* V = J + nr * nv;
+    for (int i = LANE; i < nv; i += WAVE) {
+      float s = 0.f;
+      for (int j = 1; j < dim; j++) { float fj = f5[j - 1] * sc; s += fj * (fj * x[j] / T) * Jc[j * nv + i]; }
+      V[i] = Jc[i] - s;
+      for (int k = 1; k < dim; k++) {
+        float fk = f5[k - 1] * sc;
+        V[k * nv + i] = fk * Jc[k * nv + i] - (fk * x[k] / T) * s;
+      }
+    }
+    if (LANE == 0) {
+      Dact[nr] = Dm * mu * mu;
+      for (int k = 1; k < dim; k++) Dact[nr + k] = -Dm * cg;
+    }
+    nr += dim;
+  }
+  KSYNC();
+  return nr;
+}
+
 // H = M + J^T diag(D act) J (FP32 accumulate), factored in FP64 as
 // H = L D L^T (right-looking, one LDS phase per column, the column left
 // unscaled); the reciprocal pivots go to dvec.
@@ -2528,6 +2585,14 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
   double* idg = dptr(w, L.dvec);
   const float *M = w + L.qM, *J = w + L.efc_J, *Dact = w + L.efc_Dact;
   int ntri = nv * (nv + 1) / 2;
+  // rows of J^T diag(Dact) J: the constraint rows and, with dense rows, the
+  // elliptic cones' virtual rows
+  int nrow = nefc;
+  bool cone_rows = false;
+  if (m.cone != 0 && !(MJK_BIG && L.jw > 0)) {
+    int nr = cone_virtual_rows(m, L, w, nefc);
+    if (nr >= 0) { nrow = nr; cone_rows = true; }
+  }
 #if WAVE == 64
   if (nv <= 32) {
     // J^T diag(Dact) J on the matrix cores: 16 x 16 x 4 FP32 MFMA tiles over the
@@ -2537,9 +2602,9 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     typedef float v4f_t __attribute__((ext_vector_type(4)));
     v4f_t t00 = {0.f, 0.f, 0.f, 0.f}, t10 = t00, t11 = t00;
     int lr = LANE & 15, lk = LANE >> 4;
-    for (int r0 = 0; r0 < nefc; r0 += 4) {
+    for (int r0 = 0; r0 < nrow; r0 += 4) {
       int r = r0 + lk;
-      bool in = r < nefc;
+      bool in = r < nrow;
       float d = in ? Dact[r] : 0.f;
       float a0 = (in && lr < nv) ? J[r * nv + lr] : 0.f;
       float a1 = (in && lr + 16 < nv) ? J[r * nv + 16 + lr] : 0.f;
@@ -2615,11 +2680,11 @@ KDEV int newton_factor(const DModel& m, const Layout& L, float* w, int nefc, KPr
     for (int t = LANE; t < ntri; t += WAVE) {
       int i, j;
       tri_rc(t, &i, &j);
-      H[hx(i, j)] = (double)(m_entry(m, M, i, j) + kdot3(Dact, J + i, nv, J + j, nv, nefc));
+      H[hx(i, j)] = (double)(m_entry(m, M, i, j) + kdot3(Dact, J + i, nv, J + j, nv, nrow));
     }
     KSYNC();
   }
-  if (m.cone != 0) {
+  if (m.cone != 0 && !cone_rows) {
     // contacts in the middle zone of the elliptic cone: H += J_c^T Hc J_c with
     // Hc = Dm (a a^T + g B), a = (mu, -mu f_j u_j), g = N - mu T,
     // B_jk = -mu f_j f_k (delta_jk - u_j u_k) / T (u = U_t / T); per entry
