@@ -3173,8 +3173,17 @@ KDEV float linesearch_(const DModel& m, const Layout& L, float* w, int nefc, int
   if (!(snorm >= KMINVAL)) return 0.f;
   float scale = 1.f / (m.meaninertia * (float)(nv > 1 ? nv : 1));
   float gtol = m.tolerance * m.ls_tolerance * snorm / scale;
+  // compressed rows in device memory: J s gathers s from a copy in the line
+  // search's LDS cache, which the rows are staged into only afterwards
+  const float* sx = s;
+  if (MJK_BIG && L.jw > 0 && lc.p && 4 * lc.cap >= nv) {
+    float* xs = reinterpret_cast<float*>(lc.p);
+    for (int i = LANE; i < nv; i += WAVE) xs[i] = s[i];
+    KSYNC();
+    sx = xs;
+  }
   mulM(m, M, s, Mv);
-  jrows_dot(m, L, w, nefc, s, Jv, nullptr);
+  jrows_dot(m, L, w, nefc, sx, Jv, nullptr);
   KSYNC();
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   for (int i = LANE; i < nv; i += WAVE) {
