@@ -25,6 +25,13 @@ MJMLX_API int mjmlx_hip_device_count(void);
  * friction,solref,solimp,margin,gap}. */
 MJMLX_API int mjmlx_model_field(const MjmlxModel* model, const char* name, double* out, int cap);
 
+/* Name of object `id` of MuJoCo type obj_type (1 body, 2 xbody, 3 joint,
+ * 5 geom, 6 site, 19 actuator, 20 sensor), or NULL for an unnamed object, an
+ * id out of range or another type; the inverse of mjmlx_name2id. No mjmlx.h
+ * counterpart: it serves mjb_id2name, which the reference forwards to
+ * MuJoCo's mj_id2name (src/mjb.cpp:241-245). */
+MJMLX_API const char* mjmlx_id2name(const MjmlxModel* model, int obj_type, int id);
+
 /* Warm start of the constraint solver:
  *   0  previous step's solver qacc, kept if its total cost is not above the
  *      cost at qacc_smooth (MuJoCo's warm start; default)

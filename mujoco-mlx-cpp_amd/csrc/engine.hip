@@ -529,6 +529,9 @@ MJMLX_API int mjmlx_name2id(const MjmlxModel* model, int obj_type, const char* n
   if (!model || !name) return -1;
   return model->m.name2id(obj_type, name);
 }
+MJMLX_API const char* mjmlx_id2name(const MjmlxModel* model, int obj_type, int id) {
+  return model ? model->m.id2name(obj_type, id) : nullptr;
+}
 // sensors of the compiled model (reference src/sensor.cpp:362-377)
 MJMLX_API int mjmlx_model_nsensor(const MjmlxModel* model) { return model ? model->m.nsensor : 0; }
 MJMLX_API int mjmlx_model_nsensordata(const MjmlxModel* model) { return model ? model->m.nsensordata : 0; }

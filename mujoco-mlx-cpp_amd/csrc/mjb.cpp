@@ -73,7 +73,9 @@ MJB_API float mjb_model_opt_timestep(const MjbModel* m) { return m ? mjmlx_model
 MJB_API void mjb_model_set_opt_timestep(MjbModel* m, float dt) { if (m) mjmlx_model_set_opt_timestep(m->mlx, dt); }
 MJB_API float mjb_model_body_mass(const MjbModel* m, int id) { return m ? mjmlx_model_body_mass(m->mlx, id) : 0.f; }
 MJB_API int mjb_name2id(const MjbModel* m, int t, const char* name) { return m ? mjmlx_name2id(m->mlx, t, name) : -1; }
-MJB_API const char* mjb_id2name(const MjbModel*, int, int) { return nullptr; }
+MJB_API const char* mjb_id2name(const MjbModel* m, int t, int id) {
+  return m && m->mlx ? mjmlx_id2name(m->mlx, t, id) : nullptr;
+}
 MJB_API int mjb_model_jnt_qposadr(const MjbModel* m, int j) { return (int)field1(m, "jnt_qposadr", j, -1); }
 MJB_API int mjb_model_jnt_dofadr(const MjbModel* m, int j) { return (int)field1(m, "jnt_dofadr", j, -1); }
 MJB_API int mjb_model_jnt_type(const MjbModel* m, int j) { return (int)field1(m, "jnt_type", j, -1); }

@@ -917,6 +917,21 @@ int Model::name2id(int objtype, const std::string& nm) const {
   return -1;
 }
 
+const char* Model::id2name(int objtype, int id) const {
+  const std::vector<std::string>* v = nullptr;
+  switch (objtype) {
+    case 1: case 2: v = &body_name; break;
+    case 3: v = &jnt_name; break;
+    case 5: v = &geom_name; break;
+    case 6: v = &site_name; break;
+    case 19: v = &actuator_name; break;
+    case 20: v = &sensor_name; break;
+    default: return nullptr;
+  }
+  if (id < 0 || id >= (int)v->size() || (*v)[id].empty()) return nullptr;
+  return (*v)[id].c_str();
+}
+
 Model compile_mjcf(const std::string& xml, const LoadOptions& opt) {
   Compiler c(xml, opt);
   return c.run();

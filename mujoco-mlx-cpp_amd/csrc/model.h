@@ -91,6 +91,8 @@ struct Model {
 
   // MuJoCo mjtObj ids: 1 body (2 xbody), 3 joint, 5 geom, 6 site, 19 actuator, 20 sensor
   int name2id(int objtype, const std::string& nm) const;
+  // the name of object id, or nullptr (unnamed, out of range, other types)
+  const char* id2name(int objtype, int id) const;
 };
 
 // One candidate geom pair of the static broadphase table, with the mixed

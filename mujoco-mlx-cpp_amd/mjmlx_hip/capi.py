@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
             "mjmlx_model_opt_timestep": (C.c_float, [C.c_void_p]),
             "mjmlx_model_body_mass": (C.c_float, [C.c_void_p, C.c_int]),
             "mjmlx_name2id": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
+            "mjmlx_id2name": (C.c_char_p, [C.c_void_p, C.c_int, C.c_int]),
             "mjmlx_model_field": (C.c_int, [C.c_void_p, C.c_char_p, _D, C.c_int]),
             "mjmlx_batched_create": (C.c_void_p, [C.c_void_p, C.POINTER(BatchedConfig)]),
             "mjmlx_batched_free": (None, [C.c_void_p]),

@@ -54,6 +54,14 @@ def test_host_only_calls_without_gpu():
     assert L.mjmlx_name2id(m.h, 3, b"FL_calf_joint") == 3  # mjOBJ_JOINT
     assert L.mjmlx_name2id(m.h, 2, b"torso") == 1  # mjOBJ_XBODY
     assert L.mjmlx_name2id(m.h, 5, b"nope") == -1
+    # id2name inverts name2id for every named body, joint and geom (mjb_id2name's source)
+    for t, n in ((1, m.info.nbody), (3, m.info.njnt), (5, m.info.ngeom)):
+        for i in range(n):
+            nm = L.mjmlx_id2name(m.h, t, i)
+            if nm is not None:
+                assert L.mjmlx_name2id(m.h, t, nm) == i, (t, i, nm)
+    assert L.mjmlx_id2name(m.h, 3, 3) == b"FL_calf_joint"
+    assert L.mjmlx_id2name(m.h, 1, m.info.nbody) is None and L.mjmlx_id2name(m.h, 7, 0) is None
     assert L.mjmlx_model_opt_timestep(m.h) == pytest.approx(0.005)
     assert L.mjmlx_load_model_from_string(b"<mujoco><worldbody><geom type='mesh'/></worldbody></mujoco>") is None
     assert not L.mjmlx_get_mj_model(m.h)
@@ -73,5 +81,8 @@ def test_mjb_cpu_backend_reports_unavailable():
     L.mjb_free_model.argtypes = [C.c_void_p]
     m = L.mjb_load_model_from_string(b, scenes.quadruped_xml().encode())
     assert m
+    L.mjb_id2name.restype = C.c_char_p
+    L.mjb_id2name.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    assert L.mjb_id2name(m, 1, 1) == b"torso" and L.mjb_id2name(m, 3, 3) == b"FL_calf_joint"
     L.mjb_free_model(m)
     L.mjb_free_backend(b)
