@@ -2928,11 +2928,7 @@ KDEV void newton_solve(const DModel& m, const Layout& L, float* w) {
   for (int k = 0; k < nv - 1; k++) {          // L y = g, L_ik = H_ik / H_kk
     double h = hn;
     if (k + 1 < nv - 1) hn = Hr[k + 1];
-#if WAVE == 64
     double yk = rl_d(v * id, k);  // lane k's final v_k / d_k: one broadcast instead of two
-#else
-    double yk = rl_d(v, k) * rl_d(id, k);
-#endif
     if (LANE > k && LANE < nv) v -= h * yk;
   }
   v *= id;                                      // D z = y
